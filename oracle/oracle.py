@@ -1,0 +1,302 @@
+"""ctypes bindings for the CPU oracle (oracle/bin/liboracle.so) and, when it has
+been built, the real reference (oracle/_ref/libskidref.so).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker -- never as the product path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_ORACLE = os.path.join(HERE, "bin", "liboracle.so")
+LIB_REF = os.path.join(HERE, "_ref", "libskidref.so")
+
+_f = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_i = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+_d = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+class OrParams(C.Structure):
+    _fields_ = [
+        ("nSmooth", C.c_int), ("nBucket", C.c_int),
+        ("fPeriod", C.c_float), ("fCenter", C.c_float * 3), ("bPeriodic", C.c_int),
+        ("fTau", C.c_float), ("fCvg", C.c_float), ("fScoop", C.c_float),
+        ("fDensMin", C.c_float), ("fTempMax", C.c_float), ("fMassMax", C.c_float),
+        ("nMembers", C.c_int), ("nMaxMembers", C.c_int),
+        ("bGasAndDark", C.c_int), ("bGasOnly", C.c_int), ("bForceInitialCut", C.c_int),
+        ("bNoUnbind", C.c_int), ("iSoftType", C.c_int),
+        ("G", C.c_float), ("Omega0", C.c_float), ("Lambda", C.c_float),
+        ("fQuintess", C.c_float), ("H0", C.c_float), ("z", C.c_float),
+        ("fEps", C.c_float), ("bOutDens", C.c_int), ("bQuiet", C.c_int),
+    ]
+
+
+class RefParams(C.Structure):
+    _fields_ = [
+        ("nSmooth", C.c_int), ("nBucket", C.c_int),
+        ("fPeriod", C.c_float), ("fCenter", C.c_float * 3), ("bPeriodic", C.c_int),
+        ("fTau", C.c_float), ("fCvg", C.c_float), ("fScoop", C.c_float),
+        ("fDensMin", C.c_float), ("fTempMax", C.c_float), ("fMassMax", C.c_float),
+        ("nMembers", C.c_int), ("nMaxMembers", C.c_int),
+        ("bGasAndDark", C.c_int), ("bGasOnly", C.c_int), ("bForceInitialCut", C.c_int),
+        ("bNoUnbind", C.c_int), ("iSoftType", C.c_int),
+        ("G", C.c_float), ("Omega0", C.c_float), ("Lambda", C.c_float),
+        ("fQuintess", C.c_float), ("H0", C.c_float), ("z", C.c_float),
+        ("fEps", C.c_float), ("bOutDens", C.c_int),
+    ]
+
+
+class RefOut(C.Structure):
+    _fields_ = [
+        ("fBall2", C.c_void_p), ("fDensity", C.c_void_p), ("rMoved", C.c_void_p),
+        ("rMicro", C.c_void_p), ("iFofGroup", C.c_void_p), ("iGroup", C.c_void_p),
+        ("nExtraScat", C.c_int), ("nMove", C.c_int), ("nFofGroup", C.c_int),
+        ("nGroup", C.c_int), ("nIttr", C.c_int), ("trace", C.c_void_p), ("maxTrace", C.c_int),
+        ("grpMass", C.c_void_p), ("grpCenter", C.c_void_p), ("grpVcm", C.c_void_p),
+        ("maxGroup", C.c_int),
+    ]
+
+
+def params(**kw) -> dict:
+    """SKID defaults (main.c:82-136) overridden by kw; keys as in or_params."""
+    p = dict(nSmooth=64, nBucket=16, fPeriod=3.4028234663852886e38, fCenter=(0.0, 0.0, 0.0),
+             bPeriodic=0, fTau=0.0, fCvg=0.0, fScoop=0.0, fDensMin=0.0,
+             fTempMax=3.4028234663852886e38, fMassMax=3.4028234663852886e38, nMembers=8,
+             nMaxMembers=0, bGasAndDark=0, bGasOnly=0, bForceInitialCut=0, bNoUnbind=0,
+             iSoftType=2, G=1.0, Omega0=1.0, Lambda=0.0, fQuintess=0.0, H0=0.0, z=0.0,
+             fEps=0.0, bOutDens=0, bQuiet=1)
+    for k, v in kw.items():
+        if k not in p:
+            raise KeyError(k)
+        p[k] = v
+    if "fPeriod" in kw and "bPeriodic" not in kw:
+        p["bPeriodic"] = 1
+    return p
+
+
+def _fill(struct, p: dict):
+    s = struct()
+    for name, _ in struct._fields_:
+        v = p[name]
+        if name == "fCenter":
+            s.fCenter[:] = list(v)
+        else:
+            setattr(s, name, v)
+    return s
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_ORACLE):
+            raise RuntimeError(f"oracle not built: {LIB_ORACLE} (run `make -C oracle`)")
+        L = C.CDLL(LIB_ORACLE)
+        L.or_create.restype = C.c_void_p
+        L.or_create.argtypes = [C.POINTER(OrParams)]
+        L.or_destroy.argtypes = [C.c_void_p]
+        L.or_load.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_int, _f, _f, _f, _f, _f]
+        for fn in ("or_density",):
+            getattr(L, fn).argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_init_move.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.or_move_first.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_move_block.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_fof.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
+        L.or_microsteps.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.or_get_moved.argtypes = [C.c_void_p, _f]
+        L.or_centers.argtypes = [C.c_void_p]
+        L.or_unbind.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.or_too_small.argtypes = [C.c_void_p]
+        L.or_get_groups.argtypes = [C.c_void_p, _i, C.POINTER(C.c_int)]
+        L.or_get_catalogue.argtypes = [C.c_void_p] + [C.c_void_p] * 6
+        L.or_run.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.POINTER(C.c_int)]
+        L.or_fof_points.argtypes = [C.c_int, _f, C.c_float, C.c_float, C.c_int, C.c_int, _i]
+        L.or_knn_brute.argtypes = [C.c_int, _f, C.c_int, C.c_float, C.c_int, C.c_void_p, C.c_void_p]
+        L.or_fof_brute.argtypes = [C.c_int, _f, C.c_float, C.c_float, C.c_int, _i]
+        L.or_density_f64.argtypes = [C.c_int, _f, _f, _f, C.c_float, C.c_int, _d]
+        L.or_gradient_f64.argtypes = [C.c_int, _f, _f, _f, C.c_int, _f, C.c_float, C.c_int, _d]
+        L.or_potential_f64.argtypes = [C.c_int, _f, _f, _f, C.c_float, C.c_int, _d]
+        L.or_spline_pot.argtypes = [C.c_float, C.c_float]
+        L.or_spline_pot.restype = C.c_float
+        _lib = L
+    return _lib
+
+
+def have_ref() -> bool:
+    return os.path.exists(LIB_REF)
+
+
+def ref():
+    global _ref
+    if _ref is None:
+        if not have_ref():
+            raise RuntimeError("reference harness not built (make -f oracle/ref.mk)")
+        R = C.CDLL(LIB_REF)
+        R.ref_density.argtypes = [C.POINTER(RefParams), C.c_int, C.c_int, C.c_int, _f, _f, _f, _f,
+                                  _f, _f, C.POINTER(C.c_int)]
+        R.ref_pipeline.argtypes = [C.POINTER(RefParams), C.c_int, C.c_int, C.c_int, _f, _f, _f,
+                                   _f, _f, C.POINTER(RefOut)]
+        _ref = R
+    return _ref
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+class Oracle:
+    """One oracle context (or_ctx) over a loaded box."""
+
+    def __init__(self, box, **kw):
+        self.p = params(**kw)
+        self.L = lib()
+        self.box = box
+        self.ctx = self.L.or_create(C.byref(_fill(OrParams, self.p)))
+        b = box
+        rc = self.L.or_load(self.ctx, b.n_gas, b.n_dark, b.n_star, b.pos.reshape(-1), b.vel.reshape(-1),
+                            b.mass, b.soft, b.temp)
+        if rc:
+            raise RuntimeError("or_load failed")
+
+    def __del__(self):
+        if getattr(self, "ctx", None):
+            self.L.or_destroy(self.ctx)
+            self.ctx = None
+
+    def density(self):
+        n = self.box.n
+        b2 = np.empty(n, np.float32)
+        rho = np.empty(n, np.float32)
+        ne = C.c_int()
+        rc = self.L.or_density(self.ctx, _ptr(b2), _ptr(rho), C.byref(ne))
+        if rc:
+            raise RuntimeError(f"or_density failed {rc}")
+        return b2, rho, ne.value
+
+    def run(self, max_trace=100000):
+        tr = np.zeros(2 * max_trace, np.int32)
+        nt = C.c_int()
+        rc = self.L.or_run(self.ctx, _ptr(tr), max_trace, C.byref(nt))
+        if rc:
+            raise RuntimeError(f"or_run failed {rc}")
+        return tr[: 2 * nt.value].reshape(-1, 2)
+
+    def groups(self):
+        g = np.empty(self.box.n, np.int32)
+        ng = C.c_int()
+        self.L.or_get_groups(self.ctx, g, C.byref(ng))
+        return g, ng.value
+
+
+def ref_density(box, **kw):
+    p = params(**kw)
+    R = ref()
+    n = box.n
+    b2 = np.empty(n, np.float32)
+    rho = np.empty(n, np.float32)
+    ne = C.c_int()
+    R.ref_density(C.byref(_fill(RefParams, p)), box.n_gas, box.n_dark, box.n_star, box.pos.reshape(-1),
+                  box.mass, box.soft, box.temp, b2, rho, C.byref(ne))
+    return b2, rho, ne.value
+
+
+def ref_pipeline(box, max_trace=100000, max_group=1 << 20, **kw):
+    p = params(**kw)
+    R = ref()
+    n = box.n
+    out = {k: np.empty(n, np.float32) for k in ("fBall2", "fDensity")}
+    out.update({k: np.empty(3 * n, np.float32) for k in ("rMoved", "rMicro")})
+    out.update({k: np.empty(n, np.int32) for k in ("iFofGroup", "iGroup")})
+    tr = np.zeros(2 * max_trace, np.int32)
+    gm = np.zeros(max_group, np.float32)
+    gc = np.zeros(3 * max_group, np.float32)
+    gv = np.zeros(3 * max_group, np.float32)
+    o = RefOut()
+    for k in ("fBall2", "fDensity", "rMoved", "rMicro", "iFofGroup", "iGroup"):
+        setattr(o, k, _ptr(out[k]))
+    o.trace = _ptr(tr)
+    o.maxTrace = max_trace
+    o.grpMass, o.grpCenter, o.grpVcm = _ptr(gm), _ptr(gc), _ptr(gv)
+    o.maxGroup = max_group
+    R.ref_pipeline(C.byref(_fill(RefParams, p)), box.n_gas, box.n_dark, box.n_star,
+                   box.pos.reshape(-1), box.vel.reshape(-1), box.mass, box.soft, box.temp, C.byref(o))
+    ng = o.nGroup
+    out.update(nExtraScat=o.nExtraScat, nMove=o.nMove, nFofGroup=o.nFofGroup, nGroup=ng,
+               trace=tr[: 2 * o.nIttr].reshape(-1, 2), grpMass=gm[:ng], grpCenter=gc[: 3 * ng].reshape(-1, 3),
+               grpVcm=gv[: 3 * ng].reshape(-1, 3))
+    out["rMoved"] = out["rMoved"].reshape(-1, 3)
+    out["rMicro"] = out["rMicro"].reshape(-1, 3)
+    return out
+
+
+def knn_brute(pos, k, period=1.0, periodic=True, lists=False):
+    n = pos.shape[0]
+    b2 = np.empty(n, np.float32)
+    nb = np.empty(n * k, np.int32) if lists else None
+    rc = lib().or_knn_brute(n, np.ascontiguousarray(pos, np.float32).reshape(-1), k, period, int(periodic),
+                            _ptr(b2), _ptr(nb))
+    if rc:
+        raise RuntimeError("knn_brute failed")
+    return (b2, nb.reshape(n, k)) if lists else b2
+
+
+def fof_brute(pos, tau, period=1.0, periodic=True):
+    n = pos.shape[0]
+    g = np.empty(n, np.int32)
+    ng = lib().or_fof_brute(n, np.ascontiguousarray(pos, np.float32).reshape(-1), tau, period, int(periodic), g)
+    return g, ng
+
+
+def fof_points(pos, tau, period=1.0, periodic=True, nbucket=16):
+    n = pos.shape[0]
+    g = np.empty(n, np.int32)
+    ng = lib().or_fof_points(n, np.ascontiguousarray(pos, np.float32).reshape(-1), tau, period, int(periodic),
+                             nbucket, g)
+    return g, ng
+
+
+def density_f64(pos, mass, ball2, period=1.0, periodic=True):
+    n = pos.shape[0]
+    rho = np.empty(n, np.float64)
+    lib().or_density_f64(n, np.ascontiguousarray(pos, np.float32).reshape(-1), np.ascontiguousarray(mass, np.float32),
+                         np.ascontiguousarray(ball2, np.float32), period, int(periodic), rho)
+    return rho
+
+
+def gradient_f64(spos, smass, sball2, qpos, period=1.0, periodic=True):
+    nq = qpos.shape[0]
+    acc = np.empty(3 * nq, np.float64)
+    lib().or_gradient_f64(spos.shape[0], np.ascontiguousarray(spos, np.float32).reshape(-1),
+                          np.ascontiguousarray(smass, np.float32), np.ascontiguousarray(sball2, np.float32), nq,
+                          np.ascontiguousarray(qpos, np.float32).reshape(-1), period, int(periodic), acc)
+    return acc.reshape(nq, 3)
+
+
+def potential_f64(pos, mass, soft, G=1.0, soft_type=2):
+    n = pos.shape[0]
+    phi = np.empty(n, np.float64)
+    lib().or_potential_f64(n, np.ascontiguousarray(pos, np.float32).reshape(-1), np.ascontiguousarray(mass, np.float32),
+                           np.ascontiguousarray(soft, np.float32), G, soft_type, phi)
+    return phi
+
+
+def relabel_min_index(labels: np.ndarray) -> np.ndarray:
+    """Renumber nonzero labels 1..G in order of each group's smallest member index."""
+    lab = np.asarray(labels)
+    out = np.zeros_like(lab)
+    nz = np.nonzero(lab)[0]
+    if nz.size == 0:
+        return out
+    uniq, first = np.unique(lab[nz], return_index=True)
+    order = np.argsort(nz[first], kind="stable")
+    newid = np.empty(uniq.size, np.int64)
+    newid[order] = np.arange(1, uniq.size + 1)
+    out[nz] = newid[np.searchsorted(uniq, lab[nz])]
+    return out
