@@ -1,0 +1,169 @@
+/*
+ * skg.h -- C ABI of the MI355X-native SKID group-finding library (libskg.so).
+ *
+ * The library replaces the stage functions that SKID's main.c calls between
+ * reading the tipsy snapshot and writing the group files.  Each entry point
+ * below names the reference interface it replaces.  Conventions:
+ *
+ *   - Every call returns an int status: SKG_OK (0) or a negative SKG_E_* code
+ *     (skg_strerror() describes it).  The library never aborts, exits or
+ *     enables floating-point traps.
+ *   - Host arrays belong to the caller; the library copies in and out.  Per-
+ *     particle host arrays are indexed by iOrder (tipsy input order: gas, dark,
+ *     star), as SKID's outputs are.
+ *   - Device memory belongs to the context.  Calls are synchronous at return;
+ *     work inside a call is stream-ordered on the context's HIP stream.
+ *   - One host thread per context; separate contexts are independent.
+ *
+ * Plain C: pointers and sizes only.
+ */
+#ifndef SKG_H
+#define SKG_H
+#include <stddef.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SKG_OK           0
+#define SKG_E_NODEVICE  -1   /* no usable GPU / HIP runtime error at init */
+#define SKG_E_PARAM     -2   /* invalid parameter */
+#define SKG_E_NOMEM     -3   /* device or host allocation failed */
+#define SKG_E_STATE     -4   /* stage called out of order */
+#define SKG_E_HIP       -5   /* HIP runtime error during a stage */
+#define SKG_E_NSMOOTH   -6   /* nSmooth > number of scatterers (smooth1.c:12) */
+
+#define SKG_DARK 1
+#define SKG_GAS  2
+#define SKG_STAR 4
+#define SKG_PLUMMER 1
+#define SKG_SPLINE  2
+
+/* Accumulation modes of the density / gradient sums. */
+#define SKG_ACC_PLAIN        0   /* fp32 running sums (the reference's arithmetic) */
+#define SKG_ACC_COMPENSATED  1   /* fp32 two-sum compensated sums (default) */
+
+typedef struct skg_params {
+	int nSmooth;            /* -s, default 64 (main.c:103) */
+	int nBucket;            /* leaf size, 16 in SKID (main.c:82) */
+	float fPeriod[3];       /* -p; FLT_MAX when not periodic (main.c:125-128) */
+	float fCenter[3];       /* -c/-cx/-cy/-cz */
+	int bPeriodic;
+	float fTau;             /* -tau */
+	float fCvg;             /* -cvg, default tau/2 (main.c:343) */
+	float fStep;            /* fCvg/2 (main.c:345) */
+	float fScoop;           /* -scoop, default 2*tau (main.c:344) */
+	float fDensMin, fTempMax, fMassMax;   /* -d -t -M */
+	int nMembers;           /* -m */
+	int nMaxMembers;        /* -maxgroup (<= 0: unlimited) */
+	int bGasAndDark, bGasOnly, bForceInitialCut, bNoUnbind;   /* -gd -go -fic -nu */
+	int iSoftType;          /* SKG_SPLINE or SKG_PLUMMER */
+	float G;                /* -G */
+	float z;                /* -z */
+	float fShift;           /* 1/(1+z), from the host (kd.c:1317) */
+	float fCosmo;           /* fShift*H(fShift), host FP64 cosmology (kd.c:1318) */
+	int accumulate;         /* SKG_ACC_* */
+	int device;             /* HIP device ordinal */
+} skg_params;
+
+typedef struct skg_ctx skg_ctx;
+
+/* Fill *P with SKID's defaults (main.c:82-136); the caller then sets fTau etc. */
+void skg_default_params(skg_params *P);
+
+/* Replaces kdInit (kd.c:46-81) + smInit (smooth1.c:8-29). */
+int skg_create(const skg_params *P, skg_ctx **out);
+/* Replaces kdFinish (kd.c:1842-1850) + smFinish (smooth1.c:32-38). */
+int skg_destroy(skg_ctx *ctx);
+const char *skg_strerror(int status);
+
+/* Particle input in tipsy species order, n = nGas + nDark + nStar; replaces the
+   array fill of kdReadTipsy (kd.c:122-222).  pos/vel are n*3 floats; vel, temp
+   may be NULL. */
+int skg_load(skg_ctx *ctx, int nGas, int nDark, int nStar, const float *pos,
+		const float *vel, const float *mass, const float *soft, const float *temp);
+
+/* Scatterer selection, tree build, kNN radii, symmetric M4 density and
+   periodic replicas: kdScatterActive + kdBuildTree + smDensityInit
+   (main.c:374-378; kd.c:669-693, 371-460; smooth1.c:132-335).  Outputs by
+   iOrder (NULL to skip): fBall2 (-2 for non-scatterers), density (0 for
+   non-scatterers). */
+int skg_density(skg_ctx *ctx, float *fBall2, float *rho, int *nExtraScat);
+
+/* k-nearest neighbour lists of the last skg_density, for inspection: nbr is
+   n*nSmooth iOrders, each row sorted by (d2, iOrder); rows of non-scatterers
+   are -1.  d2 (optional) holds the matching squared distances.  No reference
+   counterpart: the reference keeps these lists only inside smDensityInit. */
+int skg_neighbors(skg_ctx *ctx, int *nbr, float *d2);
+
+/* Moving-particle selection: kdInitMove (kd.c:630-666). */
+int skg_init_move(skg_ctx *ctx, int *nMove);
+
+/* One Denmax step at iteration 0 (main.c:395-404): kdBuildMoveTree +
+   smAccDensity(bInitial) + kdMoveParticles. */
+int skg_move_first(skg_ctx *ctx, int *nActive, int *nScatter);
+/* PRUNE_STEPS=5 steps + kdPruneInactive (main.c:410-416). */
+int skg_move_block(skg_ctx *ctx, int *nActive, int *nScatter);
+
+/* Friends-of-friends of the converged positions: kdFoF (kd.c:802-917).
+   grp (by iOrder, optional) receives group ids 1..nGroup-1 numbered in order
+   of each group's smallest iOrder; particles that do not move get 0. */
+int skg_fof(skg_ctx *ctx, int *grp, int *nGroup);
+
+/* The 5 micro-steps at 0.1*fStep (main.c:431-438). */
+int skg_microsteps(skg_ctx *ctx, int *nScatter);
+
+/* Positions of the moving particles (3n by iOrder; NaN for non-moving). */
+int skg_get_moved(skg_ctx *ctx, float *r);
+
+/* kdInitpGroup + kdCalcCenter (kd.c:969-1075). */
+int skg_centers(skg_ctx *ctx);
+
+/* Iterative unbinding: kdUnbind (kd.c:1299-1466) with grav.c:8-135. */
+int skg_unbind(skg_ctx *ctx, int *nUnbound);
+
+/* kdTooSmall (kd.c:1251-1293). */
+int skg_too_small(skg_ctx *ctx);
+
+/* Group ids by iOrder and the group count (groups + 1, as kd->nGroup). */
+int skg_get_groups(skg_ctx *ctx, int *grp, int *nGroup);
+
+/* Per-group catalogue for groups 0..nGroup-1 (what kdWriteGroup writes,
+   kd.c:1612-1687): mass, rCenter (3), vcm (3), rBound (3), radius, nMembers.
+   Any pointer may be NULL. */
+int skg_get_catalogue(skg_ctx *ctx, float *mass, float *center, float *vcm,
+		float *rbound, float *radius, int *nMembers);
+
+/* Vector-Jacobian product of the M4 density with frozen neighbour sets and
+   smoothing lengths h_i = sqrt(fBall2_i)/2 (no reference counterpart: the
+   reference only forms -grad of the frozen scatter field, smooth1.c:447-459).
+   gbar (by iOrder) weights each density; outputs by iOrder:
+     dx[3i+c] = sum_j gbar_j d rho_j / d x_i,c     (targets and neighbours)
+     dh[i]    = sum_j gbar_j d rho_j / d h_i
+   parts selects the gather (1), scatter (2) or both (3) halves of rho. */
+int skg_density_adjoint(skg_ctx *ctx, const float *gbar, int parts, float *dx, float *dh);
+
+/* Frozen-field gradient a(x) = sum_s (r_s - x) m_s g(q)/(pi h_s^5) at nq
+   arbitrary points (the quantity smAccDensity accumulates, smooth1.c:447-459),
+   over all scatterers and replicas of the last skg_density. */
+int skg_field_gradient(skg_ctx *ctx, int nq, const float *qpos, float *acc);
+
+/* FOF of an arbitrary point set (the link rule of kd.c:860-897 on the given
+   positions), labels numbered by smallest index.  Standalone: needs only a
+   context for the device and stream. */
+int skg_fof_points(skg_ctx *ctx, int n, const float *pos, float fTau, int *grp, int *nGroup);
+
+/* Per-stage device time in ms since skg_create, indexed by SKG_T_*. */
+#define SKG_T_TREE 0
+#define SKG_T_KNN 1
+#define SKG_T_DENSITY 2
+#define SKG_T_MOVE 3
+#define SKG_T_FOF 4
+#define SKG_T_GROUP 5
+#define SKG_T_UNBIND 6
+#define SKG_T_N 8
+int skg_stage_times(skg_ctx *ctx, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,204 @@
+// skg_common.h -- shared state and device helpers of libskg (MI355X / gfx950).
+//
+// Layout in HBM (structure of arrays, 16-byte records where a kernel reads
+// several fields of one item):
+//   particles by iOrder : pos f32x3, vel f32x3, mass, soft, temp, type
+//   scatterers (S)       : Morton-sorted float4 (x,y,z,m), iOrder, fBall2, rho parts
+//   entities (S')        : S plus periodic replicas, Morton-sorted;
+//                          float4 (x,y,z,m) + float4 (fBall2 live, ih2, grad norm, density norm)
+//   movers               : persistent float4 pos / old pos by mover index, plus the
+//                          active list (mover indices) re-sorted by Morton key each prune
+//   trees                : complete binary tree over leaves of nBucket consecutive
+//                          sorted items, heap-indexed, float4 lo (w: max fBall2) / hi.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <math.h>
+#include "../../include/skg.h"
+
+#define SKG_WAVE 64
+
+struct DevTree {
+	int n = 0;         // items
+	int B = 16;        // bucket (leaf) size
+	int nLeaf = 0;     // ceil(n/B)
+	int nLeaf2 = 0;    // next power of two >= nLeaf (>= 1)
+	float4 *lo = nullptr;   // [2*nLeaf2], index 0 unused; lo.w = max live fBall2 below
+	float4 *hi = nullptr;
+};
+
+struct skg_ctx {
+	skg_params P;
+	int dev = 0;
+	hipStream_t st = nullptr;
+	hipEvent_t e0 = nullptr, e1 = nullptr;
+	double tms[SKG_T_N] = {0};
+	int stage = 0;            // 0 created, 1 loaded, 2 density, 3 movers, 4 moving done, 5 fof, 6 micro, 7 centers, 8 unbound, 9 final
+	int n = 0, nGas = 0, nDark = 0, nStar = 0, inType = 0;
+
+	// by iOrder
+	float *d_pos = nullptr, *d_vel = nullptr, *d_mass = nullptr, *d_soft = nullptr, *d_temp = nullptr;
+	float *d_rho = nullptr, *d_ball2 = nullptr;
+	int *d_grp = nullptr;
+
+	// scatterers
+	int nS = 0;
+	float4 *s_pos = nullptr;
+	int *s_ord = nullptr;
+	float *s_ball2 = nullptr, *s_rhoG = nullptr, *s_rhoS = nullptr;
+	unsigned long long *s_heap = nullptr;   // n_S * k keys of the last kNN (optional)
+	DevTree ts;
+
+	// entities S' = S + replicas
+	int nE = 0, nRep = 0;
+	float4 *e_pos = nullptr, *e_aux = nullptr;
+	float *e_b2 = nullptr;       // fBall2 as built (all entities live)
+	float *e_rho = nullptr;      // effective density (0 once zeroed by the initial cut)
+	int *e_src = nullptr;        // scatterer (sorted S) index
+	int *e_live = nullptr;       // 1 active, 0 cut
+	int *e_hit = nullptr;        // step stamp of the last hit
+	DevTree te;
+
+	// movers
+	int nMove = 0, nActive = 0, step = 0;
+	float4 *m_pos = nullptr, *m_old = nullptr;
+	int *m_ord = nullptr;
+	int *m_list = nullptr, *m_list2 = nullptr;
+	float *d_thr = nullptr;      // [2]: threshold scratch
+	int *d_cnt = nullptr;        // small counters
+
+	// groups
+	int nGroup = 0;
+	struct GroupState *gs = nullptr;
+
+	// scratch
+	void *tmp = nullptr; size_t tmpBytes = 0;
+	unsigned long long *k64a = nullptr, *k64b = nullptr; size_t kcap = 0;
+	int *v32a = nullptr, *v32b = nullptr;
+	size_t vcap = 0;
+};
+
+// ---------------------------------------------------------------- errors
+#define SKG_HIP(x) do { hipError_t _e = (x); if (_e != hipSuccess) return SKG_E_HIP; } while (0)
+
+// ---------------------------------------------------------------- wave helpers
+__device__ __forceinline__ float wave_min(float v) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
+	return v;
+}
+__device__ __forceinline__ float wave_max(float v) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
+	return v;
+}
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ---------------------------------------------------------------- exact fp32 ops
+// The path's distance arithmetic must match the reference's (no contraction);
+// all HIP sources are built with -ffp-contract=off, and these make it explicit.
+__device__ __forceinline__ float fsub(float a, float b) { return __fsub_rn(a, b); }
+__device__ __forceinline__ float fmul(float a, float b) { return __fmul_rn(a, b); }
+__device__ __forceinline__ float fadd(float a, float b) { return __fadd_rn(a, b); }
+__device__ __forceinline__ float sq3(float dx, float dy, float dz) {
+	return fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz));
+}
+
+// Compensated (two-sum) accumulator.
+struct Acc {
+	float s, c;
+	__device__ __forceinline__ void init() { s = 0.f; c = 0.f; }
+	__device__ __forceinline__ void add(float x) {
+		float t = fadd(s, x);
+		float bp = fsub(t, s);
+		float e = fadd(fsub(s, fsub(t, bp)), fsub(x, bp));
+		c = fadd(c, e);
+		s = t;
+	}
+	__device__ __forceinline__ float val() const { return fadd(s, c); }
+};
+template <bool COMP> struct Sum;
+template <> struct Sum<true> : Acc {};
+template <> struct Sum<false> {
+	float s;
+	__device__ __forceinline__ void init() { s = 0.f; }
+	__device__ __forceinline__ void add(float x) { s = fadd(s, x); }
+	__device__ __forceinline__ float val() const { return s; }
+};
+
+// ---------------------------------------------------------------- box distances
+// All lower bounds are computed from the same fp32 operands as the per-pair
+// deltas, so by monotonicity of rounding they never exceed a member's d2.
+__device__ __forceinline__ float gap1(float alo, float ahi, float blo, float bhi) {
+	float g = fmaxf(fsub(alo, bhi), fsub(blo, ahi));
+	return fmaxf(g, 0.f);
+}
+template <bool PER>
+__device__ __forceinline__ float gap_axis(float nlo, float nhi, float blo, float bhi, float L) {
+	float g = gap1(nlo, nhi, blo, bhi);
+	if (PER) {
+		g = fminf(g, gap1(nlo, nhi, fadd(blo, L), fadd(bhi, L)));
+		g = fminf(g, gap1(nlo, nhi, fsub(blo, L), fsub(bhi, L)));
+	}
+	return g;
+}
+template <bool PER>
+__device__ __forceinline__ float box_box_d2(float4 nlo, float4 nhi, float3 blo, float3 bhi, float L) {
+	float gx = gap_axis<PER>(nlo.x, nhi.x, blo.x, bhi.x, L);
+	float gy = gap_axis<PER>(nlo.y, nhi.y, blo.y, bhi.y, L);
+	float gz = gap_axis<PER>(nlo.z, nhi.z, blo.z, bhi.z, L);
+	return sq3(gx, gy, gz);
+}
+
+// Query-side nearest-image delta: d = q - p, re-evaluated from q -/+ L rounded
+// first when |d| exceeds half a period (the reference's INTERSECT shift).
+template <bool PER>
+__device__ __forceinline__ float img_delta(float q, float qm, float qp, float p, float hL) {
+	float d = fsub(q, p);
+	if (PER) {
+		if (d > hL) d = fsub(qm, p);
+		else if (d < -hL) d = fsub(qp, p);
+	}
+	return d;
+}
+
+// M4 spline density term in the reference's float/double mix (smooth1.c:257-262).
+__device__ __forceinline__ float m4_term(float d2, float ih2, float fNorm) {
+	float r2 = fmul(d2, ih2);
+	float rs = (float)(2.0 - sqrt((double)r2));
+	if (r2 < 1.0f) rs = (float)(1.0 - 0.75 * (double)rs * (double)r2);
+	else rs = (float)(0.25 * (double)rs * (double)rs * (double)rs);
+	return fmul(rs, fNorm);
+}
+// Gradient factor g(q)*norm in the reference's float/double mix
+// (smooth1.c:450-454): sqrt of a float in double rounds like a correctly
+// rounded fp32 sqrt, and 2.25*rs is exact in double, so the fma is the
+// reference's multiply-then-add.
+__device__ __forceinline__ float grad_factor(float d2, float ih2, float gnorm) {
+	float r2 = fmul(d2, ih2);
+	float rs = __fsqrt_rn(r2);
+	if (r2 < 1.0f) rs = (float)fma(2.25, (double)rs, -3.0);
+	else rs = (float)((-3.0 / (double)rs + 3.0) - 0.75 * (double)rs);
+	return fmul(rs, gnorm);
+}
+
+// ---------------------------------------------------------------- host helpers (skg_core.hip)
+int skg_ensure_tmp(skg_ctx *c, size_t bytes);
+int skg_ensure_keys(skg_ctx *c, size_t n);
+int skg_sort_pairs(skg_ctx *c, unsigned long long *kin, unsigned long long *kout, int *vin, int *vout,
+		int n, int endBit);
+int skg_build_tree(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, const float *ball2live);
+int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out);
+int skg_exclusive_scan(skg_ctx *c, const int *in, int *out, int n);
+void skg_free_tree(DevTree *t);
+int skg_gradient_points(skg_ctx *c, const float4 *q, int nq, float *out);   // skg_adjoint.hip
+void skg_timer_start(skg_ctx *c);
+void skg_timer_stop(skg_ctx *c, int slot);
+template <typename T> static inline int dalloc(T **p, size_t count) {
+	if (*p) { (void)hipFree(*p); *p = nullptr; }
+	if (count == 0) count = 1;
+	return hipMalloc((void **)p, count * sizeof(T)) == hipSuccess ? 0 : SKG_E_NOMEM;
+}
+template <typename T> static inline void dfree(T **p) {
+	if (*p) { (void)hipFree(*p); *p = nullptr; }
+}

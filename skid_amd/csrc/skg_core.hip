@@ -1,0 +1,1109 @@
+// skg_core.hip -- context, trees, kNN + density, and the Denmax move loop.
+//
+// Stage map to the reference (SKID v1.4.1):
+//   skg_density     kdScatterActive + kdBuildTree + smDensityInit (main.c:374-378)
+//   skg_init_move   kdInitMove (kd.c:630-666)
+//   skg_move_first  main.c:395-404;  skg_move_block  main.c:410-416
+//   skg_microsteps  main.c:431-438
+#include "skg_common.h"
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
+#include <algorithm>
+#include <cfloat>
+#include <cstring>
+#include <vector>
+
+static const unsigned long long KEY_SENT = 0x7F800000FFFFFFFFull;   // d2 = +inf, index = max
+
+// =====================================================================
+// scratch, timers, sorting
+// =====================================================================
+int skg_ensure_tmp(skg_ctx *c, size_t bytes) {
+	if (bytes <= c->tmpBytes && c->tmp) return 0;
+	if (c->tmp) { (void)hipFree(c->tmp); c->tmp = nullptr; c->tmpBytes = 0; }
+	bytes = std::max<size_t>(bytes, 1 << 20);
+	if (hipMalloc(&c->tmp, bytes) != hipSuccess) return SKG_E_NOMEM;
+	c->tmpBytes = bytes;
+	return 0;
+}
+
+int skg_ensure_keys(skg_ctx *c, size_t n) {
+	if (n <= c->kcap && c->k64a) return 0;
+	size_t cap = std::max<size_t>(n, 1024);
+	if (dalloc(&c->k64a, cap) || dalloc(&c->k64b, cap) || dalloc(&c->v32a, cap) || dalloc(&c->v32b, cap))
+		return SKG_E_NOMEM;
+	c->kcap = cap;
+	c->vcap = cap;
+	return 0;
+}
+
+int skg_sort_pairs(skg_ctx *c, unsigned long long *kin, unsigned long long *kout, int *vin, int *vout,
+		int n, int endBit) {
+	size_t bytes = 0;
+	if (n <= 0) return 0;
+	SKG_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)endBit, c->st));
+	if (skg_ensure_tmp(c, bytes)) return SKG_E_NOMEM;
+	bytes = c->tmpBytes;
+	SKG_HIP(rocprim::radix_sort_pairs(c->tmp, bytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)endBit, c->st));
+	return 0;
+}
+
+int skg_exclusive_scan(skg_ctx *c, const int *in, int *out, int n) {
+	size_t bytes = 0;
+	if (n <= 0) return 0;
+	SKG_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, 0, (size_t)n, rocprim::plus<int>(), c->st));
+	if (skg_ensure_tmp(c, bytes)) return SKG_E_NOMEM;
+	bytes = c->tmpBytes;
+	SKG_HIP(rocprim::exclusive_scan(c->tmp, bytes, in, out, 0, (size_t)n, rocprim::plus<int>(), c->st));
+	return 0;
+}
+
+void skg_timer_start(skg_ctx *c) { (void)hipEventRecord(c->e0, c->st); }
+void skg_timer_stop(skg_ctx *c, int slot) {
+	float ms = 0.f;
+	(void)hipEventRecord(c->e1, c->st);
+	(void)hipEventSynchronize(c->e1);
+	(void)hipEventElapsedTime(&ms, c->e0, c->e1);
+	c->tms[slot] += ms;
+}
+
+static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+// =====================================================================
+// Morton ordering
+// =====================================================================
+__device__ __forceinline__ unsigned ord_f(float f) {
+	unsigned u = __float_as_uint(f);
+	return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float unord_f(unsigned u) {
+	return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
+}
+
+__global__ void k_bbox(const float4 *__restrict__ pos, const int *__restrict__ idx, int n, unsigned *bb) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	float4 p = make_float4(FLT_MAX, FLT_MAX, FLT_MAX, 0.f);
+	float4 q = make_float4(-FLT_MAX, -FLT_MAX, -FLT_MAX, 0.f);
+	if (i < n) { p = pos[idx ? idx[i] : i]; q = p; }
+	float a = wave_min(p.x), b = wave_min(p.y), cz = wave_min(p.z);
+	float d = wave_max(q.x), e = wave_max(q.y), f = wave_max(q.z);
+	if ((threadIdx.x & 63) == 0) {
+		atomicMin(&bb[0], ord_f(a)); atomicMin(&bb[1], ord_f(b)); atomicMin(&bb[2], ord_f(cz));
+		atomicMax(&bb[3], ord_f(d)); atomicMax(&bb[4], ord_f(e)); atomicMax(&bb[5], ord_f(f));
+	}
+}
+
+__global__ void k_bbox_init(unsigned *bb) {
+	int i = threadIdx.x;
+	if (i < 6) bb[i] = i < 3 ? 0xFFFFFFFFu : 0u;
+}
+
+__device__ __forceinline__ unsigned long long spread21(unsigned v) {
+	unsigned long long x = v & 0x1fffffu;
+	x = (x | x << 32) & 0x1f00000000ffffull;
+	x = (x | x << 16) & 0x1f0000ff0000ffull;
+	x = (x | x << 8) & 0x100f00f00f00f00full;
+	x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+	x = (x | x << 2) & 0x1249249249249249ull;
+	return x;
+}
+
+__global__ void k_morton(const float4 *__restrict__ pos, const int *__restrict__ idx, int n,
+		const unsigned *__restrict__ bb, unsigned long long *keys, int *vals) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int j = idx ? idx[i] : i;
+	float4 p = pos[j];
+	float lo[3] = {unord_f(bb[0]), unord_f(bb[1]), unord_f(bb[2])};
+	float hi[3] = {unord_f(bb[3]), unord_f(bb[4]), unord_f(bb[5])};
+	float v[3] = {p.x, p.y, p.z};
+	unsigned q[3];
+	for (int a = 0; a < 3; ++a) {
+		float ext = hi[a] - lo[a];
+		float t = ext > 0.f ? (v[a] - lo[a]) / ext : 0.f;
+		t = fminf(fmaxf(t, 0.f), 1.f);
+		q[a] = (unsigned)fminf(t * 2097152.0f, 2097151.0f);
+	}
+	keys[i] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+	vals[i] = j;
+}
+
+// perm_out[i] = index (into pos, or idx[] entry) of the i-th point in Morton order.
+int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out) {
+	if (n <= 0) return 0;
+	if (skg_ensure_keys(c, n)) return SKG_E_NOMEM;
+	k_bbox_init<<<1, 64, 0, c->st>>>((unsigned *)(c->d_cnt + 8));
+	k_bbox<<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, (unsigned *)(c->d_cnt + 8));
+	k_morton<<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, (const unsigned *)(c->d_cnt + 8), c->k64a, c->v32a);
+	SKG_HIP(hipGetLastError());
+	return skg_sort_pairs(c, c->k64a, c->k64b, c->v32a, perm_out, n, 63);
+}
+
+// =====================================================================
+// Trees: complete binary tree over leaves of B consecutive sorted items.
+// =====================================================================
+__global__ void k_tree_leaves(const float4 *__restrict__ pos, int n, int B, int nLeaf, int nLeaf2,
+		const float *__restrict__ b2, float4 *lo, float4 *hi) {
+	int l = blockIdx.x * blockDim.x + threadIdx.x;
+	if (l >= nLeaf2) return;
+	float4 a = make_float4(INFINITY, INFINITY, INFINITY, -INFINITY);
+	float4 b = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+	if (l < nLeaf) {
+		int s = l * B, e = min(n, s + B);
+		for (int i = s; i < e; ++i) {
+			float4 p = pos[i];
+			a.x = fminf(a.x, p.x); a.y = fminf(a.y, p.y); a.z = fminf(a.z, p.z);
+			b.x = fmaxf(b.x, p.x); b.y = fmaxf(b.y, p.y); b.z = fmaxf(b.z, p.z);
+			a.w = fmaxf(a.w, b2 ? b2[i] : 0.f);
+		}
+	}
+	lo[nLeaf2 + l] = a;
+	hi[nLeaf2 + l] = b;
+}
+
+__global__ void k_tree_level(float4 *lo, float4 *hi, int first, int count) {
+	int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= count) return;
+	int i = first + t;
+	float4 a0 = lo[2 * i], a1 = lo[2 * i + 1], b0 = hi[2 * i], b1 = hi[2 * i + 1];
+	lo[i] = make_float4(fminf(a0.x, a1.x), fminf(a0.y, a1.y), fminf(a0.z, a1.z), fmaxf(a0.w, a1.w));
+	hi[i] = make_float4(fmaxf(b0.x, b1.x), fmaxf(b0.y, b1.y), fmaxf(b0.z, b1.z), 0.f);
+}
+
+void skg_free_tree(DevTree *t) { dfree(&t->lo); dfree(&t->hi); t->n = t->nLeaf = t->nLeaf2 = 0; }
+
+static int tree_fill(skg_ctx *c, DevTree *t, const float4 *pos, const float *b2) {
+	k_tree_leaves<<<nblk(t->nLeaf2, 256), 256, 0, c->st>>>(pos, t->n, t->B, t->nLeaf, t->nLeaf2, b2, t->lo, t->hi);
+	for (int m = t->nLeaf2 >> 1; m >= 1; m >>= 1)
+		k_tree_level<<<nblk(m, 256), 256, 0, c->st>>>(t->lo, t->hi, m, m);
+	SKG_HIP(hipGetLastError());
+	return 0;
+}
+
+int skg_build_tree(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, const float *b2) {
+	int nLeaf = std::max(1, (n + B - 1) / B), nLeaf2 = 1;
+	while (nLeaf2 < nLeaf) nLeaf2 <<= 1;
+	if (nLeaf2 != t->nLeaf2 || !t->lo) {
+		if (dalloc(&t->lo, 2 * (size_t)nLeaf2) || dalloc(&t->hi, 2 * (size_t)nLeaf2)) return SKG_E_NOMEM;
+	}
+	t->n = n; t->B = B; t->nLeaf = nLeaf; t->nLeaf2 = nLeaf2;
+	return tree_fill(c, t, pos, b2);
+}
+
+
+// =====================================================================
+// kNN: one wavefront per block of 64 Morton-consecutive queries, one lane
+// per query.  Each lane keeps a bounded max-heap of packed (d2, iOrder) keys
+// in LDS (slot-major so a slot of all 64 lanes is one conflict-free row).
+// The wave walks the tree once for the block (nearest child first), prunes
+// with the block's largest current radius, and streams each visited bucket
+// through the lanes as a uniform (scalar-load) candidate tile.
+// =====================================================================
+template <int KMAX>
+__device__ __forceinline__ unsigned long long heap_replace(unsigned long long *heap, int lane, int k,
+		unsigned long long key) {
+	int i = 0;
+	for (;;) {
+		int l = 2 * i + 1;
+		if (l >= k) break;
+		unsigned long long cv = heap[l * 64 + lane];
+		int ci = l;
+		if (l + 1 < k) {
+			unsigned long long cr = heap[(l + 1) * 64 + lane];
+			if (cr > cv) { cv = cr; ci = l + 1; }
+		}
+		if (cv <= key) break;
+		heap[i * 64 + lane] = cv;
+		i = ci;
+	}
+	heap[i * 64 + lane] = key;
+	return heap[lane];
+}
+
+template <bool PER>
+__device__ __forceinline__ float point_box_d2(float4 lo, float4 hi, float qx, float qy, float qz,
+		float L) {
+	float gx = gap_axis<PER>(lo.x, hi.x, qx, qx, L);
+	float gy = gap_axis<PER>(lo.y, hi.y, qy, qy, L);
+	float gz = gap_axis<PER>(lo.z, hi.z, qz, qz, L);
+	return sq3(gx, gy, gz);
+}
+
+__device__ __forceinline__ float key_d2(unsigned long long k) { return __uint_as_float((unsigned)(k >> 32)); }
+
+template <int KMAX, bool PER, bool COMP>
+__global__ __launch_bounds__(64) void k_knn(
+		const float4 *__restrict__ spos, const int *__restrict__ sord, int n,
+		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
+		float L, int k, const float *__restrict__ mass_ord,
+		float *__restrict__ ball2_out, float *__restrict__ rhoG, unsigned long long *__restrict__ heap_out) {
+	__shared__ unsigned long long heap[KMAX * 64];
+	__shared__ int stk[64];
+	__shared__ float stkd[64];
+	const int lane = threadIdx.x;
+	const int base = blockIdx.x * 64;
+	const int qi = base + lane;
+	const bool valid = qi < n;
+	const float4 q = spos[valid ? qi : base];
+	const float hL = 0.5f * L;
+	const float qxm = fsub(q.x, L), qxp = fadd(q.x, L);
+	const float qym = fsub(q.y, L), qyp = fadd(q.y, L);
+	const float qzm = fsub(q.z, L), qzp = fadd(q.z, L);
+	const float3 blo = make_float3(wave_min(q.x), wave_min(q.y), wave_min(q.z));
+	const float3 bhi = make_float3(wave_max(q.x), wave_max(q.y), wave_max(q.z));
+	for (int s = 0; s < k; ++s) heap[s * 64 + lane] = KEY_SENT;
+	unsigned long long top = KEY_SENT;
+
+#define KNN_TRY(J)                                                                        \
+	{                                                                                     \
+		const float4 cc = spos[J];                                                        \
+		const unsigned oj = (unsigned)sord[J];                                            \
+		float dx = img_delta<PER>(q.x, qxm, qxp, cc.x, hL);                              \
+		float dy = img_delta<PER>(q.y, qym, qyp, cc.y, hL);                              \
+		float dz = img_delta<PER>(q.z, qzm, qzp, cc.z, hL);                              \
+		unsigned long long key = ((unsigned long long)__float_as_uint(sq3(dx, dy, dz)) << 32) | oj; \
+		if (key < top) top = heap_replace<KMAX>(heap, lane, k, key);                     \
+	}
+
+	// Seed with the block's own 64 points (the nearest tile of candidates).
+	const int nseed = min(64, n - base);
+	if (valid)
+		for (int t = 0; t < nseed; ++t) KNN_TRY(base + t);
+	float R2 = wave_max(valid ? key_d2(top) : 0.f);
+
+	int sp = 1;
+	stk[0] = 1; stkd[0] = 0.f;
+	while (sp > 0) {
+		--sp;
+		const int node = uniform_int(stk[sp]);
+		if (stkd[sp] > R2) continue;
+		const float4 lo = nlo[node], hi = nhi[node];
+		if (node < nLeaf2) {
+			const int c0 = 2 * node;
+			const float d0 = box_box_d2<PER>(nlo[c0], nhi[c0], blo, bhi, L);
+			const float d1 = box_box_d2<PER>(nlo[c0 + 1], nhi[c0 + 1], blo, bhi, L);
+			const int nf = d0 <= d1 ? c0 : c0 + 1;
+			const float dn = d0 <= d1 ? d0 : d1, df = d0 <= d1 ? d1 : d0;
+			if (df <= R2) { stk[sp] = nf ^ 1; stkd[sp] = df; ++sp; }
+			if (dn <= R2) { stk[sp] = nf; stkd[sp] = dn; ++sp; }
+			continue;
+		}
+		const int start = (node - nLeaf2) * B;
+		if (start >= n) continue;
+		const int cnt = min(B, n - start);
+		if (start >= base && start + cnt <= base + 64) continue;     // seeded tile
+		const bool need = valid && point_box_d2<PER>(lo, hi, q.x, q.y, q.z, L) <= key_d2(top);
+		if (!__any(need)) continue;
+		if (need) {
+			for (int t = 0; t < cnt; ++t) {
+				const int j = start + t;
+				if (j >= base && j < base + 64) continue;
+				KNN_TRY(j);
+			}
+		}
+		R2 = wave_max(valid ? key_d2(top) : 0.f);
+	}
+#undef KNN_TRY
+	if (!valid) return;
+	// fBall2 = the k-th smallest d2; gather half of the symmetric density over
+	// all queue members but the head (smooth1.c:243-263).
+	const float h2 = key_d2(top);
+	const float ih2 = (float)(4.0 / (double)h2);
+	const float fNorm = (float)(0.5 * M_1_PI * sqrt((double)ih2) * (double)ih2);
+	Sum<COMP> acc;
+	acc.init();
+	for (int s = 1; s < k; ++s) {
+		const unsigned long long key = heap[s * 64 + lane];
+		acc.add(fmul(m4_term(key_d2(key), ih2, fNorm), mass_ord[(unsigned)key]));
+	}
+	ball2_out[qi] = h2;
+	rhoG[qi] = acc.val();
+	if (heap_out)
+		for (int s = 0; s < k; ++s) heap_out[(size_t)qi * k + s] = heap[s * 64 + lane];
+}
+
+// =====================================================================
+// Ball cover: for each point, every entity whose ball strictly holds it
+// (non-periodic; periodic images are explicit replica entities, as the
+// reference's smx->pp).  MODE 0 accumulates the scatter half of the density,
+// MODE 1 the density gradient and then moves the point (kdMoveParticles).
+// =====================================================================
+template <int MODE, bool COMP>
+__global__ __launch_bounds__(64) void k_cover(
+		float4 *__restrict__ ppos, const int *__restrict__ plist, int np,
+		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, int ne,
+		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
+		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
+		float fStep, float cx, float cy, float cz, float Lx, float Ly, float Lz) {
+	const int lane = threadIdx.x;
+	const int base = blockIdx.x * 64;
+	const int ii = base + lane;
+	const bool valid = ii < np;
+	const int pi = plist ? plist[valid ? ii : base] : (valid ? ii : base);
+	const float4 p = ppos[pi];
+	const float3 blo = make_float3(wave_min(p.x), wave_min(p.y), wave_min(p.z));
+	const float3 bhi = make_float3(wave_max(p.x), wave_max(p.y), wave_max(p.z));
+	Sum<COMP> s0, s1, s2;
+	s0.init(); s1.init(); s2.init();
+	int node = 1;
+	for (;;) {
+		const float4 lo = nlo[node], hi = nhi[node];
+		const bool visit = box_box_d2<false>(lo, hi, blo, bhi, 0.f) < lo.w;
+		if (visit && node < nLeaf2) { node = 2 * node; continue; }
+		if (visit) {
+			const int start = (node - nLeaf2) * B;
+			if (start < ne) {
+				const int cnt = min(B, ne - start);
+				const bool need = valid && point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
+				if (__any(need) && need) {
+					for (int t = 0; t < cnt; ++t) {
+						const int e = start + t;
+						const float4 A = epos[e], X = eaux[e];
+						const float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
+						const float d2 = sq3(dx, dy, dz);
+						if (d2 < X.x) {
+							if (MODE == 0) {
+								s0.add(fmul(m4_term(d2, X.y, X.w), A.w));
+							} else {
+								const float f = grad_factor(d2, X.y, X.z);
+								s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
+								ehit[e] = stamp;
+							}
+						}
+					}
+				}
+			}
+		}
+		// next node in pre-order after node's subtree
+		int nx = node;
+		while (nx & 1) nx >>= 1;
+		node = nx + 1;
+		if (node == 1) break;
+	}
+	if (!valid) return;
+	if (MODE == 0) { out_sum[ii] = s0.val(); return; }
+	// kdMoveParticles (kd.c:702-732)
+	const float ax = s0.val(), ay = s1.val(), az = s2.val();
+	const float ss = sq3(ax, ay, az);
+	float ai = (float)sqrt((double)ss);
+	if (ai > 0.0f) ai = (float)((double)fStep / sqrt((double)ss));
+	else ai = 0.0f;
+	float r[3] = {fsub(p.x, fmul(ai, ax)), fsub(p.y, fmul(ai, ay)), fsub(p.z, fmul(ai, az))};
+	const float C[3] = {cx, cy, cz}, LL[3] = {Lx, Ly, Lz};
+	for (int j = 0; j < 3; ++j) {
+		if ((double)r[j] > (double)C[j] + 0.5 * (double)LL[j]) r[j] = fsub(r[j], LL[j]);
+		if ((double)r[j] <= (double)C[j] - 0.5 * (double)LL[j]) r[j] = fadd(r[j], LL[j]);
+	}
+	ppos[pi] = make_float4(r[0], r[1], r[2], p.w);
+}
+
+// =====================================================================
+// Small kernels: gathers, replicas, entity setup, ScatterCut, prune.
+// =====================================================================
+__global__ void k_gather_scat(const float *__restrict__ pos, const float *__restrict__ mass,
+		const int *__restrict__ lo_hi /* up to 3 ranges */, int nr, int nS, float4 *out4, int *ord) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nS) return;
+	int o = -1, acc = 0;
+	for (int r = 0; r < nr; ++r) {
+		int a = lo_hi[2 * r], b = lo_hi[2 * r + 1];
+		if (i < acc + (b - a)) { o = a + (i - acc); break; }
+		acc += b - a;
+	}
+	out4[i] = make_float4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], mass[o]);
+	ord[i] = o;
+}
+
+__global__ void k_permute_s(const float4 *__restrict__ in4, const int *__restrict__ inord,
+		const int *__restrict__ perm, int n, float4 *out4, int *outord) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int j = perm[i];
+	out4[i] = in4[j];
+	outord[i] = inord[j];
+}
+
+// Replica scatterers: the 26 images whose ball reaches the box (smooth1.c:283-331),
+// tested with INTERSECTNP arithmetic against the float box bounds.
+__device__ __forceinline__ float box_np_d2(float lx, float ly, float lz, float hx, float hy, float hz,
+		float x, float y, float z) {
+	float dx = fsub(lx, x), dx1 = fsub(x, hx), dy = fsub(ly, y), dy1 = fsub(y, hy);
+	float dz = fsub(lz, z), dz1 = fsub(z, hz);
+	float gx = dx > 0.f ? dx : (dx1 > 0.f ? dx1 : 0.f);
+	float gy = dy > 0.f ? dy : (dy1 > 0.f ? dy1 : 0.f);
+	float gz = dz > 0.f ? dz : (dz1 > 0.f ? dz1 : 0.f);
+	return sq3(gx, gy, gz);
+}
+
+__global__ void k_rep_mask(const float4 *__restrict__ spos, const float *__restrict__ b2, int nS,
+		float lx, float ly, float lz, float hx, float hy, float hz, float Lx, float Ly, float Lz,
+		unsigned *mask, int *cnt) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nS) return;
+	float4 p = spos[i];
+	unsigned m = 0;
+	int bit = 0;
+	for (int ix = -1; ix <= 1; ++ix) {
+		float x = fadd(p.x, (float)ix * Lx);
+		for (int iy = -1; iy <= 1; ++iy) {
+			float y = fadd(p.y, (float)iy * Ly);
+			for (int iz = -1; iz <= 1; ++iz, ++bit) {
+				float z = fadd(p.z, (float)iz * Lz);
+				if (!(ix || iy || iz)) continue;
+				if (box_np_d2(lx, ly, lz, hx, hy, hz, x, y, z) < b2[i]) m |= 1u << bit;
+			}
+		}
+	}
+	mask[i] = m;
+	cnt[i] = __popc(m);
+}
+
+// Unsorted entity list: [0,nS) scatterers, then replicas in (scatterer, image) order.
+__global__ void k_rep_write(const float4 *__restrict__ spos, const float *__restrict__ b2, int nS,
+		const unsigned *__restrict__ mask, const int *__restrict__ off, float Lx, float Ly, float Lz,
+		float4 *e4, float *eb2, int *esrc) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nS) return;
+	float4 p = spos[i];
+	e4[i] = p; eb2[i] = b2[i]; esrc[i] = i;
+	unsigned m = mask[i];
+	int o = nS + off[i], bit = 0;
+	for (int ix = -1; ix <= 1; ++ix)
+		for (int iy = -1; iy <= 1; ++iy)
+			for (int iz = -1; iz <= 1; ++iz, ++bit)
+				if (m & (1u << bit)) {
+					e4[o] = make_float4(fadd(p.x, (float)ix * Lx), fadd(p.y, (float)iy * Ly),
+							fadd(p.z, (float)iz * Lz), p.w);
+					eb2[o] = b2[i]; esrc[o] = i; ++o;
+				}
+}
+
+// Entities in Morton order + per-entity constants (smooth1.c:447-448, 250-251).
+__global__ void k_entity_setup(const float4 *__restrict__ e4u, const float *__restrict__ eb2u,
+		const int *__restrict__ esrcu, const int *__restrict__ perm, int ne,
+		const float *__restrict__ srho, float4 *e4, float4 *eaux, float *erho, int *esrc, int *elive,
+		int *ehit, float *eb2s) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= ne) return;
+	int j = perm[i];
+	float4 p = e4u[j];
+	float b2 = eb2u[j];
+	float ih2 = (float)(4.0 / (double)b2);
+	double dih2 = ih2;
+	float gnorm = (float)(M_1_PI * dih2 * dih2 * sqrt(dih2) * (double)p.w);
+	float dnorm = (float)(0.5 * M_1_PI * sqrt(dih2) * dih2);
+	e4[i] = p;
+	eaux[i] = make_float4(b2, ih2, gnorm, dnorm);
+	eb2s[i] = b2;
+	esrc[i] = esrcu[j];
+	erho[i] = srho ? srho[esrcu[j]] : 0.f;
+	elive[i] = 1;
+	ehit[i] = -1;
+}
+
+__global__ void k_entity_rho(const int *__restrict__ esrc, int ne, const float *__restrict__ srho, float *erho) {
+	int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < ne) erho[e] = srho[esrc[e]];
+}
+
+__global__ void k_combine_rho(const float *__restrict__ g, const float *__restrict__ s,
+		const float *__restrict__ b2, const int *__restrict__ ord, int nS, float *rho_s,
+		float *rho_ord, float *b2_ord) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nS) return;
+	float r = fadd(g[i], s[i]);
+	rho_s[i] = r;
+	rho_ord[ord[i]] = r;
+	b2_ord[ord[i]] = b2[i];
+}
+
+__global__ void k_fill_f(float *a, int n, float v) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) a[i] = v;
+}
+__global__ void k_fill_i(int *a, int n, int v) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) a[i] = v;
+}
+__global__ void k_iota(int *a, int n) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) a[i] = i;
+}
+
+__global__ void k_step_init(unsigned *thr, int *cnt) {
+	if (threadIdx.x == 0) { *thr = 0x7F800000u; *cnt = 0; }
+}
+
+// ScatterCut threshold: smallest density among live entities that held an
+// active mover this step; on the initial step live entities that held none
+// get density 0 (smooth1.c:460-470, 497-507).
+__global__ void k_scat_thr(const int *__restrict__ elive, const int *__restrict__ ehit, float *erho,
+		int ne, int stamp, int bInitial, unsigned *thr) {
+	int e = blockIdx.x * blockDim.x + threadIdx.x;
+	float v = INFINITY;
+	if (e < ne && elive[e]) {
+		if (ehit[e] == stamp) { float r = erho[e]; if (r > 0.f) v = r; }
+		else if (bInitial) erho[e] = 0.f;
+	}
+	v = wave_min(v);
+	if ((threadIdx.x & 63) == 0 && v < INFINITY) atomicMin(thr, __float_as_uint(v));
+}
+
+// ScatterCut (smooth1.c:387-405): keep entities with density >= threshold.
+__global__ void k_scat_cut(int *elive, float4 *eaux, const float *__restrict__ erho, int ne,
+		const unsigned *__restrict__ thr, int *cnt) {
+	int e = blockIdx.x * blockDim.x + threadIdx.x;
+	int live = 0;
+	if (e < ne) {
+		unsigned tb = *thr;
+		float t = (tb == 0x7F800000u) ? 0.f : __uint_as_float(tb);
+		live = elive[e];
+		if (live && !(erho[e] >= t)) { live = 0; elive[e] = 0; eaux[e].x = -1.f; }
+	}
+	unsigned long long b = __ballot(live);
+	if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, __popcll(b));
+}
+
+__global__ void k_live_b2(const float4 *__restrict__ eaux, int ne, float *b2live) {
+	int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < ne) b2live[e] = eaux[e].x;
+}
+
+// kdPruneInactive test (kd.c:754-776): a mover stays active while it moved at
+// least fCvg over the last block; survivors restart from here.
+__global__ void k_prune(const int *__restrict__ list, int na, const float4 *__restrict__ mpos, float4 *mold,
+		float hx, float hy, float hz, float cv2, int *flag) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= na) return;
+	int m = list[i];
+	float4 p = mpos[m], o = mold[m];
+	float dx = fsub(p.x, o.x), dy = fsub(p.y, o.y), dz = fsub(p.z, o.z);
+	if (dx > hx) dx = fsub(dx, 2 * hx);
+	if (dx <= -hx) dx = fadd(dx, 2 * hx);
+	if (dy > hy) dy = fsub(dy, 2 * hy);
+	if (dy <= -hy) dy = fadd(dy, 2 * hy);
+	if (dz > hz) dz = fsub(dz, 2 * hz);
+	if (dz <= -hz) dz = fadd(dz, 2 * hz);
+	int keep = sq3(dx, dy, dz) >= cv2;
+	flag[i] = keep;
+	if (keep) mold[m] = p;
+}
+
+__global__ void k_compact(const int *__restrict__ list, const int *__restrict__ flag,
+		const int *__restrict__ off, int n, int *out) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && flag[i]) out[off[i]] = list ? list[i] : i;
+}
+
+// CutCriterion (kd.c:555-597) per particle by iOrder.
+__global__ void k_mover_flags(int n, int nGas, int nDark, int inType, const float *__restrict__ rho,
+		const float *__restrict__ mass, const float *__restrict__ temp, float fDensMin, float fTempMax,
+		float fMassMax, int gd, int go, int *flag) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int t = i < nGas ? SKG_GAS : (i < nGas + nDark ? SKG_DARK : SKG_STAR);
+	int ok = 0;
+	float r = rho[i];
+	if (!(mass[i] > fMassMax)) {
+		switch (inType) {
+		case SKG_DARK: ok = r >= fDensMin; break;
+		case SKG_GAS: case SKG_DARK | SKG_GAS:
+			ok = (gd && t == SKG_DARK && r >= fDensMin) ||
+			     (t == SKG_GAS && r >= fDensMin && temp[i] <= fTempMax);
+			break;
+		case SKG_STAR: case SKG_DARK | SKG_STAR: ok = t == SKG_STAR; break;
+		case SKG_GAS | SKG_STAR: case SKG_DARK | SKG_GAS | SKG_STAR:
+			if (gd && t == SKG_DARK && r >= fDensMin) ok = 1;
+			else if (t == SKG_GAS) ok = r >= fDensMin && temp[i] <= fTempMax;
+			else ok = t == SKG_STAR && !go;
+			break;
+		}
+	}
+	flag[i] = ok;
+}
+
+__global__ void k_mover_init(const int *__restrict__ ord, int nm, const float *__restrict__ pos,
+		float4 *mpos, float4 *mold) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nm) return;
+	int o = ord[i];
+	float4 p = make_float4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], 0.f);
+	mpos[i] = p; mold[i] = p;
+}
+
+__global__ void k_scatter_moved(const int *__restrict__ ord, int nm, const float4 *__restrict__ mpos, float *r) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nm) return;
+	int o = ord[i];
+	float4 p = mpos[i];
+	r[3 * o] = p.x; r[3 * o + 1] = p.y; r[3 * o + 2] = p.z;
+}
+
+// =====================================================================
+// Host API
+// =====================================================================
+extern "C" void skg_default_params(skg_params *P) {
+	memset(P, 0, sizeof(*P));
+	P->nSmooth = 64;
+	P->nBucket = 16;
+	for (int j = 0; j < 3; ++j) { P->fPeriod[j] = FLT_MAX; P->fCenter[j] = 0.f; }
+	P->fTempMax = FLT_MAX;
+	P->fMassMax = FLT_MAX;
+	P->nMembers = 8;
+	P->iSoftType = SKG_SPLINE;
+	P->G = 1.f;
+	P->fShift = 1.f;
+	P->accumulate = SKG_ACC_COMPENSATED;
+}
+
+extern "C" const char *skg_strerror(int s) {
+	switch (s) {
+	case SKG_OK: return "ok";
+	case SKG_E_NODEVICE: return "no usable GPU";
+	case SKG_E_PARAM: return "invalid parameter";
+	case SKG_E_NOMEM: return "allocation failed";
+	case SKG_E_STATE: return "stage called out of order";
+	case SKG_E_HIP: return "HIP runtime error";
+	case SKG_E_NSMOOTH: return "nSmooth exceeds the number of scatterers";
+	}
+	return "unknown status";
+}
+
+extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
+	if (!P || !out) return SKG_E_PARAM;
+	*out = nullptr;
+	if (P->nSmooth < 2 || P->nSmooth > 256 || P->nBucket < 1 || P->nBucket > 1024) return SKG_E_PARAM;
+	if (!(P->fTau > 0.f)) return SKG_E_PARAM;
+	int nd = 0;
+	if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0) return SKG_E_NODEVICE;
+	if (P->device < 0 || P->device >= nd) return SKG_E_PARAM;
+	if (hipSetDevice(P->device) != hipSuccess) return SKG_E_NODEVICE;
+	skg_ctx *c = new (std::nothrow) skg_ctx();
+	if (!c) return SKG_E_NOMEM;
+	c->P = *P;
+	c->dev = P->device;
+	if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+	    hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
+		delete c;
+		return SKG_E_NODEVICE;
+	}
+	if (dalloc(&c->d_cnt, 64) || dalloc(&c->d_thr, 8)) { skg_destroy(c); return SKG_E_NOMEM; }
+	*out = c;
+	return SKG_OK;
+}
+
+void skg_group_free(skg_ctx *c);
+
+extern "C" int skg_destroy(skg_ctx *c) {
+	if (!c) return SKG_OK;
+	(void)hipSetDevice(c->dev);
+	if (c->st) (void)hipStreamSynchronize(c->st);
+	skg_group_free(c);
+	dfree(&c->d_pos); dfree(&c->d_vel); dfree(&c->d_mass); dfree(&c->d_soft); dfree(&c->d_temp);
+	dfree(&c->d_rho); dfree(&c->d_ball2); dfree(&c->d_grp);
+	dfree(&c->s_pos); dfree(&c->s_ord); dfree(&c->s_ball2); dfree(&c->s_rhoG); dfree(&c->s_rhoS);
+	dfree(&c->s_heap);
+	skg_free_tree(&c->ts); skg_free_tree(&c->te);
+	dfree(&c->e_b2); dfree(&c->e_pos); dfree(&c->e_aux); dfree(&c->e_rho); dfree(&c->e_src); dfree(&c->e_live); dfree(&c->e_hit);
+	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2);
+	dfree(&c->d_thr); dfree(&c->d_cnt);
+	if (c->tmp) (void)hipFree(c->tmp);
+	dfree(&c->k64a); dfree(&c->k64b); dfree(&c->v32a); dfree(&c->v32b);
+	if (c->e0) (void)hipEventDestroy(c->e0);
+	if (c->e1) (void)hipEventDestroy(c->e1);
+	if (c->st) (void)hipStreamDestroy(c->st);
+	delete c;
+	return SKG_OK;
+}
+
+extern "C" int skg_load(skg_ctx *c, int nGas, int nDark, int nStar, const float *pos, const float *vel,
+		const float *mass, const float *soft, const float *temp) {
+	if (!c || !pos || !mass || !soft || nGas < 0 || nDark < 0 || nStar < 0) return SKG_E_PARAM;
+	long n = (long)nGas + nDark + nStar;
+	if (n <= 0 || n > 0x7FFFFFF0L) return SKG_E_PARAM;
+	(void)hipSetDevice(c->dev);
+	c->nGas = nGas; c->nDark = nDark; c->nStar = nStar; c->n = (int)n;
+	c->inType = (nDark ? SKG_DARK : 0) | (nGas ? SKG_GAS : 0) | (nStar ? SKG_STAR : 0);
+	if (dalloc(&c->d_pos, 3 * n) || dalloc(&c->d_vel, 3 * n) || dalloc(&c->d_mass, n) ||
+	    dalloc(&c->d_soft, n) || dalloc(&c->d_temp, n) || dalloc(&c->d_rho, n) || dalloc(&c->d_ball2, n) ||
+	    dalloc(&c->d_grp, n))
+		return SKG_E_NOMEM;
+	SKG_HIP(hipMemcpyAsync(c->d_pos, pos, 12 * n, hipMemcpyHostToDevice, c->st));
+	if (vel) SKG_HIP(hipMemcpyAsync(c->d_vel, vel, 12 * n, hipMemcpyHostToDevice, c->st));
+	else SKG_HIP(hipMemsetAsync(c->d_vel, 0, 12 * n, c->st));
+	SKG_HIP(hipMemcpyAsync(c->d_mass, mass, 4 * n, hipMemcpyHostToDevice, c->st));
+	SKG_HIP(hipMemcpyAsync(c->d_soft, soft, 4 * n, hipMemcpyHostToDevice, c->st));
+	if (temp) {
+		SKG_HIP(hipMemcpyAsync(c->d_temp, temp, 4 * n, hipMemcpyHostToDevice, c->st));
+		if (nGas < n) SKG_HIP(hipMemsetAsync(c->d_temp + nGas, 0, 4 * (n - nGas), c->st));
+	} else SKG_HIP(hipMemsetAsync(c->d_temp, 0, 4 * n, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	c->stage = 1;
+	return SKG_OK;
+}
+
+// ScatterCriterion (kd.c:600-627) depends only on the species, so the
+// scatterer set is a union of iOrder ranges.
+static int scatter_ranges(const skg_ctx *c, int *r) {
+	int gd = c->P.bGasAndDark, go = c->P.bGasOnly, nr = 0;
+	int g0 = 0, g1 = c->nGas, d0 = g1, d1 = g1 + c->nDark, s0 = d1, s1 = c->n;
+	bool sg = false, sd = false, ss = false;
+	switch (c->inType) {
+	case SKG_DARK: sd = true; break;
+	case SKG_GAS: case SKG_DARK | SKG_GAS: sg = true; sd = gd; break;
+	case SKG_STAR: case SKG_DARK | SKG_STAR: ss = true; break;
+	case SKG_GAS | SKG_STAR: case SKG_DARK | SKG_GAS | SKG_STAR:
+		if (gd) { sg = sd = ss = true; } else { sg = true; ss = !go; }
+		break;
+	}
+	if (sg && g1 > g0) { r[2 * nr] = g0; r[2 * nr + 1] = g1; ++nr; }
+	if (sd && d1 > d0) { r[2 * nr] = d0; r[2 * nr + 1] = d1; ++nr; }
+	if (ss && s1 > s0) { r[2 * nr] = s0; r[2 * nr + 1] = s1; ++nr; }
+	return nr;
+}
+
+template <int KMAX>
+static void launch_knn(skg_ctx *c, bool per, bool comp) {
+	const int nb = nblk(c->nS, 64);
+	const float L = per ? c->P.fPeriod[0] : FLT_MAX;
+#define KL(P_, C_)                                                                           \
+	k_knn<KMAX, P_, C_><<<nb, 64, 0, c->st>>>(c->s_pos, c->s_ord, c->nS, c->ts.lo, c->ts.hi,   \
+			c->ts.nLeaf2, c->ts.B, L, c->P.nSmooth, c->d_mass, c->s_ball2, c->s_rhoG, c->s_heap)
+	if (per) { if (comp) KL(true, true); else KL(true, false); }
+	else { if (comp) KL(false, true); else KL(false, false); }
+#undef KL
+}
+
+static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, int np, float *out,
+		int stamp, float fStep) {
+	if (np <= 0) return 0;
+	const int nb = nblk(np, 64);
+	const bool comp = c->P.accumulate == SKG_ACC_COMPENSATED;
+	const float *C = c->P.fCenter, *L = c->P.fPeriod;
+#define CL(M_, C_)                                                                             \
+	k_cover<M_, C_><<<nb, 64, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->nE, c->te.lo, \
+			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, fStep, C[0], C[1], C[2], L[0], L[1], L[2])
+	if (mode == 0) { if (comp) CL(0, true); else CL(0, false); }
+	else { if (comp) CL(1, true); else CL(1, false); }
+#undef CL
+	SKG_HIP(hipGetLastError());
+	return 0;
+}
+
+extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraScat) {
+	if (!c) return SKG_E_PARAM;
+	if (c->stage < 1) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	int rc;
+	const int k = c->P.nSmooth, B = c->P.nBucket;
+	const bool per = c->P.bPeriodic != 0;
+	if (per && !(c->P.fPeriod[0] == c->P.fPeriod[1] && c->P.fPeriod[1] == c->P.fPeriod[2]))
+		return SKG_E_PARAM;   // -p sets one length for all axes (main.c:273-281)
+	int r[6];
+	int nr = scatter_ranges(c, r);
+	int nS = 0;
+	for (int i = 0; i < nr; ++i) nS += r[2 * i + 1] - r[2 * i];
+	c->nS = nS;
+	SKG_HIP(hipMemsetAsync(c->d_rho, 0, 4 * (size_t)c->n, c->st));
+	k_fill_f<<<nblk(c->n, 256), 256, 0, c->st>>>(c->d_ball2, c->n, -2.f);
+	if (nS > 0 && k > nS) return SKG_E_NSMOOTH;
+	skg_timer_start(c);
+	if (nS > 0) {
+		// scatterers in Morton order
+		float4 *tmp4 = nullptr;
+		int *tmpo = nullptr, *perm = nullptr, *rng = nullptr;
+		if (dalloc(&c->s_pos, nS) || dalloc(&c->s_ord, nS) || dalloc(&c->s_ball2, nS) ||
+		    dalloc(&c->s_rhoG, nS) || dalloc(&c->s_rhoS, nS) || dalloc(&tmp4, nS) || dalloc(&tmpo, nS) ||
+		    dalloc(&perm, nS) || dalloc(&rng, 6))
+			return SKG_E_NOMEM;
+		SKG_HIP(hipMemcpyAsync(rng, r, sizeof(r), hipMemcpyHostToDevice, c->st));
+		k_gather_scat<<<nblk(nS, 256), 256, 0, c->st>>>(c->d_pos, c->d_mass, rng, nr, nS, tmp4, tmpo);
+		if ((rc = skg_morton_sort(c, tmp4, nullptr, nS, perm))) return rc;
+		k_permute_s<<<nblk(nS, 256), 256, 0, c->st>>>(tmp4, tmpo, perm, nS, c->s_pos, c->s_ord);
+		if ((rc = skg_build_tree(c, &c->ts, c->s_pos, nS, B, nullptr))) return rc;
+		SKG_HIP(hipStreamSynchronize(c->st));
+		dfree(&tmp4); dfree(&tmpo); dfree(&perm); dfree(&rng);
+		skg_timer_stop(c, SKG_T_TREE);
+
+		// kNN radii + gather half of the density
+		skg_timer_start(c);
+		if (dalloc(&c->s_heap, (size_t)nS * k)) return SKG_E_NOMEM;
+		const bool comp = c->P.accumulate == SKG_ACC_COMPENSATED;
+		if (k <= 32) launch_knn<32>(c, per, comp);
+		else if (k <= 64) launch_knn<64>(c, per, comp);
+		else if (k <= 128) launch_knn<128>(c, per, comp);
+		else launch_knn<256>(c, per, comp);
+		SKG_HIP(hipGetLastError());
+		skg_timer_stop(c, SKG_T_KNN);
+
+		// replicas (periodic) and the entity set S'
+		skg_timer_start(c);
+		unsigned *mask = nullptr;
+		int *cnt = nullptr, *off = nullptr;
+		if (dalloc(&mask, nS) || dalloc(&cnt, nS) || dalloc(&off, nS)) return SKG_E_NOMEM;
+		int nRep = 0;
+		if (per) {
+			const float *L = c->P.fPeriod, *C = c->P.fCenter;
+			float lo[3], hi[3];
+			for (int j = 0; j < 3; ++j) { lo[j] = C[j] - 0.5 * L[j]; hi[j] = C[j] + 0.5 * L[j]; }
+			k_rep_mask<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_pos, c->s_ball2, nS, lo[0], lo[1], lo[2],
+					hi[0], hi[1], hi[2], L[0], L[1], L[2], mask, cnt);
+			if ((rc = skg_exclusive_scan(c, cnt, off, nS))) return rc;
+			int last[2];
+			SKG_HIP(hipMemcpyAsync(&last[0], off + nS - 1, 4, hipMemcpyDeviceToHost, c->st));
+			SKG_HIP(hipMemcpyAsync(&last[1], cnt + nS - 1, 4, hipMemcpyDeviceToHost, c->st));
+			SKG_HIP(hipStreamSynchronize(c->st));
+			nRep = last[0] + last[1];
+		} else {
+			SKG_HIP(hipMemsetAsync(mask, 0, 4 * (size_t)nS, c->st));
+			SKG_HIP(hipMemsetAsync(off, 0, 4 * (size_t)nS, c->st));
+		}
+		c->nRep = nRep;
+		c->nE = nS + nRep;
+		const int nE = c->nE;
+		float4 *e4u = nullptr;
+		float *eb2u = nullptr, *eb2s = nullptr;
+		int *esrcu = nullptr, *perm2 = nullptr;
+		if (dalloc(&e4u, nE) || dalloc(&eb2u, nE) || dalloc(&esrcu, nE) || dalloc(&perm2, nE) ||
+		    dalloc(&eb2s, nE) || dalloc(&c->e_pos, nE) || dalloc(&c->e_aux, nE) || dalloc(&c->e_rho, nE) ||
+		    dalloc(&c->e_src, nE) || dalloc(&c->e_live, nE) || dalloc(&c->e_hit, nE))
+			return SKG_E_NOMEM;
+		const float *L = c->P.fPeriod;
+		k_rep_write<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_pos, c->s_ball2, nS, mask, off,
+				per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, e4u, eb2u, esrcu);
+		if ((rc = skg_morton_sort(c, e4u, nullptr, nE, perm2))) return rc;
+		k_entity_setup<<<nblk(nE, 256), 256, 0, c->st>>>(e4u, eb2u, esrcu, perm2, nE, nullptr, c->e_pos,
+				c->e_aux, c->e_rho, c->e_src, c->e_live, c->e_hit, eb2s);
+		if ((rc = skg_build_tree(c, &c->te, c->e_pos, nE, B, eb2s))) return rc;
+		SKG_HIP(hipStreamSynchronize(c->st));
+		dfree(&mask); dfree(&cnt); dfree(&off); dfree(&e4u); dfree(&eb2u); dfree(&esrcu); dfree(&perm2);
+		dfree(&c->e_b2);
+		c->e_b2 = eb2s;
+
+		// scatter half of the density: balls of others holding each scatterer
+		if ((rc = cover_launch(c, 0, c->s_pos, nullptr, nS, c->s_rhoS, 0, 0.f))) return rc;
+		float *rhoS = nullptr;
+		if (dalloc(&rhoS, nS)) return SKG_E_NOMEM;
+		k_combine_rho<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_rhoG, c->s_rhoS, c->s_ball2, c->s_ord, nS, rhoS,
+				c->d_rho, c->d_ball2);
+		// entity densities for ScatterCut: each entity carries its source's density
+		k_entity_rho<<<nblk(nE, 256), 256, 0, c->st>>>(c->e_src, nE, rhoS, c->e_rho);
+		SKG_HIP(hipGetLastError());
+		SKG_HIP(hipStreamSynchronize(c->st));
+		dfree(&rhoS);
+		skg_timer_stop(c, SKG_T_DENSITY);
+	}
+	if (fBall2) SKG_HIP(hipMemcpyAsync(fBall2, c->d_ball2, 4 * (size_t)c->n, hipMemcpyDeviceToHost, c->st));
+	if (rho) SKG_HIP(hipMemcpyAsync(rho, c->d_rho, 4 * (size_t)c->n, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	if (nExtraScat) *nExtraScat = c->nRep;
+	c->stage = 2;
+	return SKG_OK;
+}
+
+
+extern "C" int skg_neighbors(skg_ctx *c, int *nbr, float *d2) {
+	if (!c || !nbr) return SKG_E_PARAM;
+	if (c->stage < 2 || !c->s_heap) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	const int k = c->P.nSmooth;
+	std::vector<unsigned long long> h((size_t)c->nS * k);
+	std::vector<int> ord(c->nS);
+	SKG_HIP(hipMemcpyAsync(h.data(), c->s_heap, h.size() * 8, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(ord.data(), c->s_ord, 4 * (size_t)c->nS, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	for (size_t i = 0; i < (size_t)c->n * k; ++i) { nbr[i] = -1; if (d2) d2[i] = -1.f; }
+	for (int i = 0; i < c->nS; ++i) {
+		unsigned long long *row = &h[(size_t)i * k];
+		std::sort(row, row + k);
+		for (int s = 0; s < k; ++s) {
+			nbr[(size_t)ord[i] * k + s] = (int)(unsigned)row[s];
+			if (d2) d2[(size_t)ord[i] * k + s] = __builtin_bit_cast(float, (unsigned)(row[s] >> 32));
+		}
+	}
+	return SKG_OK;
+}
+
+extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
+	if (!c) return SKG_E_PARAM;
+	if (c->stage < 2) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	int rc, n = c->n;
+	int *flag = nullptr, *off = nullptr;
+	if (dalloc(&flag, n) || dalloc(&off, n)) return SKG_E_NOMEM;
+	k_mover_flags<<<nblk(n, 256), 256, 0, c->st>>>(n, c->nGas, c->nDark, c->inType, c->d_rho, c->d_mass,
+			c->d_temp, c->P.fDensMin, c->P.fTempMax, c->P.fMassMax, c->P.bGasAndDark, c->P.bGasOnly, flag);
+	if ((rc = skg_exclusive_scan(c, flag, off, n))) return rc;
+	int last[2];
+	SKG_HIP(hipMemcpyAsync(&last[0], off + n - 1, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(&last[1], flag + n - 1, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	const int nm = last[0] + last[1];
+	c->nMove = nm;
+	if (dalloc(&c->m_ord, nm) || dalloc(&c->m_pos, nm) || dalloc(&c->m_old, nm) || dalloc(&c->m_list, nm) ||
+	    dalloc(&c->m_list2, nm))
+		return SKG_E_NOMEM;
+	k_compact<<<nblk(n, 256), 256, 0, c->st>>>(nullptr, flag, off, n, c->m_ord);
+	k_mover_init<<<nblk(nm, 256), 256, 0, c->st>>>(c->m_ord, nm, c->d_pos, c->m_pos, c->m_old);
+	c->nActive = nm;
+	if (nm > 0 && (rc = skg_morton_sort(c, c->m_pos, nullptr, nm, c->m_list))) return rc;
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&flag); dfree(&off);
+	c->step = 0;
+	if (nMove) *nMove = nm;
+	c->stage = 3;
+	return SKG_OK;
+}
+
+// One Denmax step: gradient + move of the active movers, then ScatterCut.
+static int denmax_step(skg_ctx *c, float fStep, int bInitial) {
+	int rc;
+	const int stamp = ++c->step;
+	if ((rc = cover_launch(c, 1, c->m_pos, c->m_list, c->nActive, nullptr, stamp, fStep))) return rc;
+	k_step_init<<<1, 64, 0, c->st>>>((unsigned *)c->d_thr, c->d_cnt);
+	k_scat_thr<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->nE, stamp, bInitial,
+			(unsigned *)c->d_thr);
+	k_scat_cut<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE,
+			(const unsigned *)c->d_thr, c->d_cnt);
+	SKG_HIP(hipGetLastError());
+	return 0;
+}
+
+static int refit_entities(skg_ctx *c) {
+	float *b2 = nullptr;
+	if (dalloc(&b2, c->nE)) return SKG_E_NOMEM;
+	k_live_b2<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_aux, c->nE, b2);
+	int rc = skg_build_tree(c, &c->te, c->e_pos, c->nE, c->te.B, b2);
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&b2);
+	return rc;
+}
+
+static int prune(skg_ctx *c) {
+	int rc, na = c->nActive;
+	if (na == 0) return 0;
+	int *flag = nullptr, *off = nullptr;
+	if (dalloc(&flag, na) || dalloc(&off, na)) return SKG_E_NOMEM;
+	const float *L = c->P.fPeriod;
+	const float hx = 0.5f * L[0], hy = 0.5f * L[1], hz = 0.5f * L[2];
+	const float cv2 = c->P.fCvg * c->P.fCvg;
+	k_prune<<<nblk(na, 256), 256, 0, c->st>>>(c->m_list, na, c->m_pos, c->m_old, hx, hy, hz, cv2, flag);
+	if ((rc = skg_exclusive_scan(c, flag, off, na))) return rc;
+	int last[2];
+	SKG_HIP(hipMemcpyAsync(&last[0], off + na - 1, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(&last[1], flag + na - 1, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	const int nn = last[0] + last[1];
+	k_compact<<<nblk(na, 256), 256, 0, c->st>>>(c->m_list, flag, off, na, c->m_list2);
+	if (nn > 0 && (rc = skg_morton_sort(c, c->m_pos, c->m_list2, nn, c->m_list))) return rc;
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&flag); dfree(&off);
+	c->nActive = nn;
+	return 0;
+}
+
+static int read_count(skg_ctx *c, int *v) {
+	SKG_HIP(hipMemcpyAsync(v, c->d_cnt, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	return 0;
+}
+
+extern "C" int skg_move_first(skg_ctx *c, int *nActive, int *nScatter) {
+	if (!c) return SKG_E_PARAM;
+	if (c->stage != 3) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	int rc, ns = c->nE;
+	skg_timer_start(c);
+	const int bInit = (c->inType == SKG_DARK) || c->P.bForceInitialCut;   // kd.c:696-699
+	if (c->nActive > 0 && c->nE > 0) {
+		if ((rc = denmax_step(c, c->P.fStep, bInit))) return rc;
+		if ((rc = read_count(c, &ns))) return rc;
+		if ((rc = refit_entities(c))) return rc;
+	}
+	skg_timer_stop(c, SKG_T_MOVE);
+	if (nActive) *nActive = c->nActive;
+	if (nScatter) *nScatter = ns;
+	c->stage = 4;
+	return SKG_OK;
+}
+
+extern "C" int skg_move_block(skg_ctx *c, int *nActive, int *nScatter) {
+	if (!c) return SKG_E_PARAM;
+	if (c->stage != 4) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	int rc, ns = c->nE;
+	skg_timer_start(c);
+	if (c->nActive > 0 && c->nE > 0) {
+		for (int i = 0; i < 5; ++i)
+			if ((rc = denmax_step(c, c->P.fStep, 0))) return rc;
+		if ((rc = read_count(c, &ns))) return rc;
+		if ((rc = prune(c))) return rc;
+		if ((rc = refit_entities(c))) return rc;
+	} else {
+		c->nActive = 0;
+	}
+	skg_timer_stop(c, SKG_T_MOVE);
+	if (nActive) *nActive = c->nActive;
+	if (nScatter) *nScatter = ns;
+	return SKG_OK;
+}
+
+extern "C" int skg_microsteps(skg_ctx *c, int *nScatter) {
+	if (!c) return SKG_E_PARAM;
+	if (c->stage != 5) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	int rc, ns = c->nE;
+	skg_timer_start(c);
+	c->nActive = c->nMove;                                  // kdReactivateMove (kd.c:796-799)
+	if (c->nMove > 0 && (rc = skg_morton_sort(c, c->m_pos, nullptr, c->nMove, c->m_list))) return rc;
+	if (c->nMove > 0 && c->nE > 0) {
+		const float micro = (float)(0.1 * (double)c->P.fStep);   // MICRO_STEP*fStep (main.c:435)
+		for (int i = 0; i < 5; ++i)
+			if ((rc = denmax_step(c, micro, 0))) return rc;
+		if ((rc = read_count(c, &ns))) return rc;
+	}
+	SKG_HIP(hipStreamSynchronize(c->st));
+	skg_timer_stop(c, SKG_T_MOVE);
+	if (nScatter) *nScatter = ns;
+	c->stage = 6;
+	return SKG_OK;
+}
+
+extern "C" int skg_get_moved(skg_ctx *c, float *r) {
+	if (!c || !r) return SKG_E_PARAM;
+	if (c->stage < 3) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	float *d = nullptr;
+	if (dalloc(&d, 3 * (size_t)c->n)) return SKG_E_NOMEM;
+	k_fill_f<<<nblk(3 * c->n, 256), 256, 0, c->st>>>(d, 3 * c->n, NAN);
+	if (c->nMove > 0) k_scatter_moved<<<nblk(c->nMove, 256), 256, 0, c->st>>>(c->m_ord, c->nMove, c->m_pos, d);
+	SKG_HIP(hipMemcpyAsync(r, d, 12 * (size_t)c->n, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&d);
+	return SKG_OK;
+}
+
+extern "C" int skg_field_gradient(skg_ctx *c, int nq, const float *qpos, float *acc) {
+	if (!c || nq < 0 || (nq && (!qpos || !acc))) return SKG_E_PARAM;
+	if (c->stage < 2) return SKG_E_STATE;
+	if (nq == 0) return SKG_OK;
+	(void)hipSetDevice(c->dev);
+	float4 *q4 = nullptr;
+	float *out = nullptr;
+	if (dalloc(&q4, nq) || dalloc(&out, 3 * (size_t)nq)) return SKG_E_NOMEM;
+	std::vector<float4> h(nq);
+	for (int i = 0; i < nq; ++i) h[i] = make_float4(qpos[3 * i], qpos[3 * i + 1], qpos[3 * i + 2], 0.f);
+	SKG_HIP(hipMemcpyAsync(q4, h.data(), 16 * (size_t)nq, hipMemcpyHostToDevice, c->st));
+	int rc = skg_gradient_points(c, q4, nq, out);
+	if (rc) return rc;
+	SKG_HIP(hipMemcpyAsync(acc, out, 12 * (size_t)nq, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&q4); dfree(&out);
+	return SKG_OK;
+}
+
+extern "C" int skg_stage_times(skg_ctx *c, double *ms) {
+	if (!c || !ms) return SKG_E_PARAM;
+	for (int i = 0; i < SKG_T_N; ++i) ms[i] = c->tms[i];
+	return SKG_OK;
+}

@@ -1,0 +1,239 @@
+// skg_fof.hip -- friends-of-friends of the converged positions (kd.c:802-917).
+//
+// The reference grows each group breadth-first from the first unvisited
+// mover.  Here every mover links to every mover within tau in one parallel
+// pass over a Morton-ordered tree, merging with a lock-free union-find
+// (larger root hooked under the smaller, so roots are component minima).
+// Link rule: f32 d2 < tau^2 evaluated with the query-side image shift
+// (kd.h:217-335), from either end of the pair.  A tree node lying wholly
+// inside the ball of a point (the reference's ContainedCell, kd.c:888-897)
+// is merged without per-pair tests; the bound is computed from the same f32
+// operands as the pair deltas, so it never admits a pair the test rejects.
+// Group ids are numbered in order of each group's smallest input index.
+#include "skg_common.h"
+#include <climits>
+#include <cfloat>
+
+static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+__device__ __forceinline__ int uf_find(int *par, int x) {
+	for (;;) {
+		int p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (p == x) return x;
+		int gp = __hip_atomic_load(&par[p], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (gp != p) __hip_atomic_store(&par[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		x = gp;
+	}
+}
+
+__device__ __forceinline__ void uf_union(int *par, int a, int b) {
+	for (;;) {
+		a = uf_find(par, a);
+		b = uf_find(par, b);
+		if (a == b) return;
+		if (a > b) { int t = a; a = b; b = t; }
+		if (atomicCAS(&par[b], b, a) == b) return;
+	}
+}
+
+// Largest |delta| over [lo,hi] on one axis under the query-side rule, or -1
+// when points of the range would take different image shifts.
+template <bool PER>
+__device__ __forceinline__ float max_delta(float q, float qm, float qp, float lo, float hi, float hL) {
+	float a = fsub(q, lo), b = fsub(q, hi);          // a >= b
+	if (!PER) return fmaxf(fabsf(a), fabsf(b));
+	if (a <= hL && b >= -hL) return fmaxf(fabsf(a), fabsf(b));
+	if (b > hL) { float x = fsub(qm, lo), y = fsub(qm, hi); return fmaxf(fabsf(x), fabsf(y)); }
+	if (a < -hL) { float x = fsub(qp, lo), y = fsub(qp, hi); return fmaxf(fabsf(x), fabsf(y)); }
+	return -1.f;
+}
+
+template <bool PER>
+__global__ __launch_bounds__(64) void k_fof_links(const float4 *__restrict__ fp, int n,
+		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int depth, int B,
+		float L, float tau2, int *par, int *united) {
+	const int lane = threadIdx.x;
+	const int base = blockIdx.x * 64;
+	const int i = base + lane;
+	const bool valid = i < n;
+	const float4 q = fp[valid ? i : base];
+	const float hL = 0.5f * L;
+	const float qxm = fsub(q.x, L), qxp = fadd(q.x, L), qym = fsub(q.y, L), qyp = fadd(q.y, L);
+	const float qzm = fsub(q.z, L), qzp = fadd(q.z, L);
+	const float3 blo = make_float3(wave_min(q.x), wave_min(q.y), wave_min(q.z));
+	const float3 bhi = make_float3(wave_max(q.x), wave_max(q.y), wave_max(q.z));
+	int node = 1, lev = 0;
+	for (;;) {
+		const float4 lo = nlo[node], hi = nhi[node];
+		bool descend = false;
+		if (box_box_d2<PER>(lo, hi, blo, bhi, L) < tau2) {
+			// per-lane: contained / overlapping / disjoint
+			const int first = ((node << (depth - lev)) - nLeaf2) * B;
+			int last = (((node + 1) << (depth - lev)) - nLeaf2) * B;
+			last = min(last, n);
+			bool overlap = false, contained = false;
+			if (valid && first < n) {
+				float gx = gap_axis<PER>(lo.x, hi.x, q.x, q.x, L), gy = gap_axis<PER>(lo.y, hi.y, q.y, q.y, L);
+				float gz = gap_axis<PER>(lo.z, hi.z, q.z, q.z, L);
+				overlap = sq3(gx, gy, gz) < tau2;
+				if (overlap && last - 1 > i) {
+					float mx = max_delta<PER>(q.x, qxm, qxp, lo.x, hi.x, hL);
+					float my = max_delta<PER>(q.y, qym, qyp, lo.y, hi.y, hL);
+					float mz = max_delta<PER>(q.z, qzm, qzp, lo.z, hi.z, hL);
+					contained = mx >= 0.f && my >= 0.f && mz >= 0.f && sq3(mx, my, mz) < tau2;
+				}
+			}
+			if (contained) {
+				// every member is within tau of q: merge q with the node, and the
+				// node's members with each other once.
+				uf_union(par, i, first);
+				if (atomicCAS(&united[node], 0, 1) == 0)
+					for (int j = first + 1; j < last; ++j) uf_union(par, first, j);
+			}
+			const bool want = overlap && !contained && (last - 1 > i);
+			if (__any(want)) {
+				if (node < nLeaf2) descend = true;
+				else if (want) {
+					for (int j = max(first, i + 1); j < last; ++j) {
+						const float4 c = fp[j];
+						float dx = img_delta<PER>(q.x, qxm, qxp, c.x, hL);
+						float dy = img_delta<PER>(q.y, qym, qyp, c.y, hL);
+						float dz = img_delta<PER>(q.z, qzm, qzp, c.z, hL);
+						bool link = sq3(dx, dy, dz) < tau2;
+						if (PER && !link) {
+							// the pair seen from j: shift applied on j's side
+							float ex = img_delta<PER>(c.x, fsub(c.x, L), fadd(c.x, L), q.x, hL);
+							float ey = img_delta<PER>(c.y, fsub(c.y, L), fadd(c.y, L), q.y, hL);
+							float ez = img_delta<PER>(c.z, fsub(c.z, L), fadd(c.z, L), q.z, hL);
+							link = sq3(ex, ey, ez) < tau2;
+						}
+						if (link) uf_union(par, i, j);
+					}
+				}
+			}
+		}
+		if (descend) { node = 2 * node; ++lev; continue; }
+		while (node & 1) { node >>= 1; --lev; }
+		node += 1;
+		if (node == 1) break;
+	}
+}
+
+__global__ void k_fof_init(int *par, int n) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) par[i] = i;
+}
+__global__ void k_zero(int *a, int n) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) a[i] = 0;
+}
+__global__ void k_fill(int *a, int n, int v) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) a[i] = v;
+}
+
+__global__ void k_fof_root_min(int *par, int n, const int *__restrict__ orig, int *cmin) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int r = uf_find(par, i);
+	par[i] = r;
+	atomicMin(&cmin[r], orig[i]);
+}
+__global__ void k_fof_first(const int *__restrict__ par, int n, const int *__restrict__ cmin, int *isFirst) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && par[i] == i) isFirst[cmin[i]] = 1;
+}
+__global__ void k_fof_label(const int *__restrict__ par, int n, const int *__restrict__ orig,
+		const int *__restrict__ cmin, const int *__restrict__ rank, const int *__restrict__ outmap,
+		int *grp) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int o = orig[i];
+	grp[outmap ? outmap[o] : o] = rank[cmin[par[i]]] + 1;
+}
+__global__ void k_gather4(const float4 *__restrict__ in, const int *__restrict__ perm, int n, float4 *out) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) out[i] = in[perm[i]];
+}
+
+// FOF of pos[0,n) (float4); writes grp[outmap ? outmap[o] : o] = 1-based id for
+// every point o, numbered by smallest o.  Returns the number of groups.
+static int fof_run(skg_ctx *c, const float4 *pos, int n, float tau, const int *outmap, int *grp, int *ngOut) {
+	int rc;
+	const bool per = c->P.bPeriodic != 0;
+	const float L = per ? c->P.fPeriod[0] : FLT_MAX;
+	int *perm = nullptr, *par = nullptr, *united = nullptr, *cmin = nullptr, *isFirst = nullptr, *rank = nullptr;
+	float4 *fp = nullptr;
+	DevTree t;
+	if (dalloc(&perm, n) || dalloc(&par, n) || dalloc(&cmin, n) || dalloc(&isFirst, n) || dalloc(&rank, n) ||
+	    dalloc(&fp, n))
+		return SKG_E_NOMEM;
+	if ((rc = skg_morton_sort(c, pos, nullptr, n, perm))) return rc;
+	k_gather4<<<nblk(n, 256), 256, 0, c->st>>>(pos, perm, n, fp);
+	if ((rc = skg_build_tree(c, &t, fp, n, c->P.nBucket, nullptr))) return rc;
+	if (dalloc(&united, 2 * (size_t)t.nLeaf2)) return SKG_E_NOMEM;
+	int depth = 0;
+	while ((1 << depth) < t.nLeaf2) ++depth;
+	k_zero<<<nblk(2 * t.nLeaf2, 256), 256, 0, c->st>>>(united, 2 * t.nLeaf2);
+	k_fof_init<<<nblk(n, 256), 256, 0, c->st>>>(par, n);
+	const float tau2 = tau * tau;
+	if (per) k_fof_links<true><<<nblk(n, 64), 64, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2, par, united);
+	else k_fof_links<false><<<nblk(n, 64), 64, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2, par, united);
+	k_fill<<<nblk(n, 256), 256, 0, c->st>>>(cmin, n, INT_MAX);
+	k_zero<<<nblk(n, 256), 256, 0, c->st>>>(isFirst, n);
+	k_fof_root_min<<<nblk(n, 256), 256, 0, c->st>>>(par, n, perm, cmin);
+	k_fof_first<<<nblk(n, 256), 256, 0, c->st>>>(par, n, cmin, isFirst);
+	if ((rc = skg_exclusive_scan(c, isFirst, rank, n))) return rc;
+	k_fof_label<<<nblk(n, 256), 256, 0, c->st>>>(par, n, perm, cmin, rank, outmap, grp);
+	SKG_HIP(hipGetLastError());
+	int last[2];
+	SKG_HIP(hipMemcpyAsync(&last[0], rank + n - 1, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(&last[1], isFirst + n - 1, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	*ngOut = last[0] + last[1];
+	dfree(&perm); dfree(&par); dfree(&united); dfree(&cmin); dfree(&isFirst); dfree(&rank); dfree(&fp);
+	skg_free_tree(&t);
+	return 0;
+}
+
+extern "C" int skg_fof(skg_ctx *c, int *grp, int *nGroup) {
+	if (!c) return SKG_E_PARAM;
+	if (c->stage != 4) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	int rc, ng = 0;
+	skg_timer_start(c);
+	k_zero<<<nblk(c->n, 256), 256, 0, c->st>>>(c->d_grp, c->n);
+	if (c->nMove > 0) {
+		if ((rc = fof_run(c, c->m_pos, c->nMove, c->P.fTau, c->m_ord, c->d_grp, &ng))) return rc;
+	}
+	skg_timer_stop(c, SKG_T_FOF);
+	c->nGroup = ng + 1;                                     // kd.c:905
+	if (grp) {
+		SKG_HIP(hipMemcpyAsync(grp, c->d_grp, 4 * (size_t)c->n, hipMemcpyDeviceToHost, c->st));
+		SKG_HIP(hipStreamSynchronize(c->st));
+	}
+	if (nGroup) *nGroup = c->nGroup;
+	c->stage = 5;
+	return SKG_OK;
+}
+
+extern "C" int skg_fof_points(skg_ctx *c, int n, const float *pos, float fTau, int *grp, int *nGroup) {
+	if (!c || n < 0 || (n && (!pos || !grp)) || !(fTau > 0.f)) return SKG_E_PARAM;
+	if (n == 0) { if (nGroup) *nGroup = 1; return SKG_OK; }
+	(void)hipSetDevice(c->dev);
+	float4 *p4 = nullptr;
+	int *g = nullptr, rc, ng = 0;
+	if (dalloc(&p4, n) || dalloc(&g, n)) return SKG_E_NOMEM;
+	float4 *h = (float4 *)malloc(16 * (size_t)n);
+	if (!h) return SKG_E_NOMEM;
+	for (int i = 0; i < n; ++i) h[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.f);
+	SKG_HIP(hipMemcpyAsync(p4, h, 16 * (size_t)n, hipMemcpyHostToDevice, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	free(h);
+	if ((rc = fof_run(c, p4, n, fTau, nullptr, g, &ng))) return rc;
+	SKG_HIP(hipMemcpyAsync(grp, g, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&p4); dfree(&g);
+	if (nGroup) *nGroup = ng + 1;
+	return SKG_OK;
+}

@@ -1,0 +1,254 @@
+"""ctypes mirror of the libskg C ABI (include/skg.h).
+
+The host side of the GPU path: a thin Python view of the same stage calls the
+C driver (skid_amd/driver/skid.c) makes, used by tests and bench.py.  It loads
+the in-tree skid_amd/libskg.so and raises if that library is missing -- there
+is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libskg.so")
+
+SKG_OK = 0
+SKG_DARK, SKG_GAS, SKG_STAR = 1, 2, 4
+SKG_PLUMMER, SKG_SPLINE = 1, 2
+SKG_ACC_PLAIN, SKG_ACC_COMPENSATED = 0, 1
+STAGES = ("tree", "knn", "density", "move", "fof", "group", "unbind", "spare")
+FLT_MAX = 3.4028234663852886e38
+
+
+class SkgParams(C.Structure):
+    _fields_ = [
+        ("nSmooth", C.c_int), ("nBucket", C.c_int),
+        ("fPeriod", C.c_float * 3), ("fCenter", C.c_float * 3), ("bPeriodic", C.c_int),
+        ("fTau", C.c_float), ("fCvg", C.c_float), ("fStep", C.c_float), ("fScoop", C.c_float),
+        ("fDensMin", C.c_float), ("fTempMax", C.c_float), ("fMassMax", C.c_float),
+        ("nMembers", C.c_int), ("nMaxMembers", C.c_int),
+        ("bGasAndDark", C.c_int), ("bGasOnly", C.c_int), ("bForceInitialCut", C.c_int),
+        ("bNoUnbind", C.c_int), ("iSoftType", C.c_int),
+        ("G", C.c_float), ("z", C.c_float), ("fShift", C.c_float), ("fCosmo", C.c_float),
+        ("accumulate", C.c_int), ("device", C.c_int),
+    ]
+
+
+class SkgError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise SkgError(f"HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        vp, ip, fp = C.c_void_p, C.POINTER(C.c_int), C.c_void_p
+        L.skg_default_params.argtypes = [C.POINTER(SkgParams)]
+        L.skg_create.argtypes = [C.POINTER(SkgParams), C.POINTER(C.c_void_p)]
+        L.skg_destroy.argtypes = [vp]
+        L.skg_strerror.argtypes = [C.c_int]
+        L.skg_strerror.restype = C.c_char_p
+        L.skg_load.argtypes = [vp, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, fp]
+        L.skg_density.argtypes = [vp, fp, fp, ip]
+        L.skg_neighbors.argtypes = [vp, fp, fp]
+        L.skg_init_move.argtypes = [vp, ip]
+        L.skg_move_first.argtypes = [vp, ip, ip]
+        L.skg_move_block.argtypes = [vp, ip, ip]
+        L.skg_fof.argtypes = [vp, fp, ip]
+        L.skg_microsteps.argtypes = [vp, ip]
+        L.skg_get_moved.argtypes = [vp, fp]
+        L.skg_centers.argtypes = [vp]
+        L.skg_unbind.argtypes = [vp, ip]
+        L.skg_too_small.argtypes = [vp]
+        L.skg_get_groups.argtypes = [vp, fp, ip]
+        L.skg_get_catalogue.argtypes = [vp, fp, fp, fp, fp, fp, fp]
+        L.skg_density_adjoint.argtypes = [vp, fp, C.c_int, fp, fp]
+        L.skg_field_gradient.argtypes = [vp, C.c_int, fp, fp]
+        L.skg_fof_points.argtypes = [vp, C.c_int, fp, C.c_float, fp, ip]
+        L.skg_stage_times.argtypes = [vp, C.POINTER(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _check(rc: int, what: str):
+    if rc != SKG_OK:
+        raise SkgError(f"{what}: {lib().skg_strerror(rc).decode()} ({rc})")
+
+
+def make_params(tau: float, nSmooth: int = 64, nBucket: int = 16, period: float | None = None,
+                center=(0.0, 0.0, 0.0), cvg: float | None = None, scoop: float | None = None,
+                accumulate: int = SKG_ACC_COMPENSATED, **kw) -> SkgParams:
+    """skid's command-line semantics (main.c:82-345) as a parameter block."""
+    P = SkgParams()
+    lib().skg_default_params(C.byref(P))
+    P.nSmooth, P.nBucket, P.fTau = nSmooth, nBucket, tau
+    if period is not None:
+        P.fPeriod[:] = [period] * 3
+        P.bPeriodic = 1
+    P.fCenter[:] = list(center)
+    P.fCvg = cvg if cvg is not None else 0.5 * tau
+    P.fScoop = scoop if scoop is not None else 2.0 * tau
+    P.fStep = 0.5 * np.float32(P.fCvg)
+    P.accumulate = accumulate
+    for k, v in kw.items():
+        setattr(P, k, v)
+    return P
+
+
+class Context:
+    """One libskg context (skg_ctx) with particles loaded."""
+
+    def __init__(self, box, params: SkgParams):
+        self.L = lib()
+        self.box = box
+        h = C.c_void_p()
+        _check(self.L.skg_create(C.byref(params), C.byref(h)), "skg_create")
+        self.h = h
+        self.n = box.n
+        self.k = params.nSmooth
+        b = box
+        self._keep = [np.ascontiguousarray(a, np.float32) for a in
+                      (b.pos.reshape(-1), b.vel.reshape(-1), b.mass, b.soft, b.temp)]
+        _check(self.L.skg_load(h, b.n_gas, b.n_dark, b.n_star, *[_p(a) for a in self._keep]), "skg_load")
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.skg_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def density(self):
+        b2 = np.empty(self.n, np.float32)
+        rho = np.empty(self.n, np.float32)
+        ne = C.c_int()
+        _check(self.L.skg_density(self.h, _p(b2), _p(rho), C.byref(ne)), "skg_density")
+        return b2, rho, ne.value
+
+    def neighbors(self):
+        nb = np.empty(self.n * self.k, np.int32)
+        d2 = np.empty(self.n * self.k, np.float32)
+        _check(self.L.skg_neighbors(self.h, _p(nb), _p(d2)), "skg_neighbors")
+        return nb.reshape(self.n, self.k), d2.reshape(self.n, self.k)
+
+    def init_move(self) -> int:
+        n = C.c_int()
+        _check(self.L.skg_init_move(self.h, C.byref(n)), "skg_init_move")
+        return n.value
+
+    def move_first(self):
+        a, s = C.c_int(), C.c_int()
+        _check(self.L.skg_move_first(self.h, C.byref(a), C.byref(s)), "skg_move_first")
+        return a.value, s.value
+
+    def move_block(self):
+        a, s = C.c_int(), C.c_int()
+        _check(self.L.skg_move_block(self.h, C.byref(a), C.byref(s)), "skg_move_block")
+        return a.value, s.value
+
+    def move_loop(self, max_blocks: int = 100000):
+        """The Denmax loop of main.c:393-419; returns the Ittr trace (nActive, nScatter).
+
+        max_blocks bounds the number of 5-step blocks (the reference loops until
+        every mover has converged); exceeding it raises instead of spinning."""
+        self.init_move()
+        tr = [self.move_first()]
+        while tr[-1][0]:
+            if len(tr) > max_blocks:
+                raise SkgError(f"move loop did not converge in {max_blocks} blocks")
+            tr.append(self.move_block())
+        return np.array(tr, np.int32)
+
+    def fof(self):
+        g = np.empty(self.n, np.int32)
+        ng = C.c_int()
+        _check(self.L.skg_fof(self.h, _p(g), C.byref(ng)), "skg_fof")
+        return g, ng.value
+
+    def microsteps(self) -> int:
+        s = C.c_int()
+        _check(self.L.skg_microsteps(self.h, C.byref(s)), "skg_microsteps")
+        return s.value
+
+    def moved(self):
+        r = np.empty(3 * self.n, np.float32)
+        _check(self.L.skg_get_moved(self.h, _p(r)), "skg_get_moved")
+        return r.reshape(self.n, 3)
+
+    def centers(self):
+        _check(self.L.skg_centers(self.h), "skg_centers")
+
+    def unbind(self) -> int:
+        n = C.c_int()
+        _check(self.L.skg_unbind(self.h, C.byref(n)), "skg_unbind")
+        return n.value
+
+    def too_small(self):
+        _check(self.L.skg_too_small(self.h), "skg_too_small")
+
+    def groups(self):
+        g = np.empty(self.n, np.int32)
+        ng = C.c_int()
+        _check(self.L.skg_get_groups(self.h, _p(g), C.byref(ng)), "skg_get_groups")
+        return g, ng.value
+
+    def catalogue(self, ngroup: int):
+        out = dict(mass=np.empty(ngroup, np.float32), center=np.empty(3 * ngroup, np.float32),
+                   vcm=np.empty(3 * ngroup, np.float32), rbound=np.empty(3 * ngroup, np.float32),
+                   radius=np.empty(ngroup, np.float32), nmembers=np.empty(ngroup, np.int32))
+        _check(self.L.skg_get_catalogue(self.h, *[_p(out[k]) for k in
+                                                  ("mass", "center", "vcm", "rbound", "radius", "nmembers")]),
+               "skg_get_catalogue")
+        for k in ("center", "vcm", "rbound"):
+            out[k] = out[k].reshape(-1, 3)
+        return out
+
+    def adjoint(self, gbar, parts: int = 3):
+        g = np.ascontiguousarray(gbar, np.float32)
+        dx = np.empty(3 * self.n, np.float32)
+        dh = np.empty(self.n, np.float32)
+        _check(self.L.skg_density_adjoint(self.h, _p(g), parts, _p(dx), _p(dh)), "skg_density_adjoint")
+        return dx.reshape(self.n, 3), dh
+
+    def field_gradient(self, q):
+        q = np.ascontiguousarray(q, np.float32).reshape(-1)
+        a = np.empty_like(q)
+        _check(self.L.skg_field_gradient(self.h, q.size // 3, _p(q), _p(a)), "skg_field_gradient")
+        return a.reshape(-1, 3)
+
+    def fof_points(self, pos, tau: float):
+        p = np.ascontiguousarray(pos, np.float32).reshape(-1)
+        g = np.empty(p.size // 3, np.int32)
+        ng = C.c_int()
+        _check(self.L.skg_fof_points(self.h, p.size // 3, _p(p), tau, _p(g), C.byref(ng)), "skg_fof_points")
+        return g, ng.value
+
+    def stage_times(self) -> dict:
+        t = (C.c_double * 8)()
+        _check(self.L.skg_stage_times(self.h, t), "skg_stage_times")
+        return {STAGES[i]: t[i] for i in range(8)}
+
+    def run(self):
+        """main.c:374-471 through the GPU stages; returns (trace, groups, nGroup)."""
+        self.density()
+        tr = self.move_loop()
+        self.fof()
+        self.microsteps()
+        self.centers()
+        self.unbind()
+        self.too_small()
+        g, ng = self.groups()
+        return tr, g, ng
