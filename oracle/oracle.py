@@ -194,6 +194,37 @@ class Oracle:
         self.L.or_get_groups(self.ctx, g, C.byref(ng))
         return g, ng.value
 
+    def move_loop(self):
+        """main.c:393-419 after density(); returns the Ittr trace."""
+        a, s = C.c_int(), C.c_int()
+        self.L.or_init_move(self.ctx, C.byref(a))
+        self.L.or_move_first(self.ctx, C.byref(a), C.byref(s))
+        tr = [(a.value, s.value)]
+        while tr[-1][0]:
+            self.L.or_move_block(self.ctx, C.byref(a), C.byref(s))
+            tr.append((a.value, s.value))
+        return np.array(tr, np.int32)
+
+    def moved(self):
+        r = np.empty(3 * self.box.n, np.float32)
+        self.L.or_get_moved(self.ctx, r)
+        return r.reshape(-1, 3)
+
+    def fof(self):
+        g = np.empty(self.box.n, np.int32)
+        ng = C.c_int()
+        self.L.or_fof(self.ctx, _ptr(g), C.byref(ng))
+        return g, ng.value
+
+    def finish(self):
+        """micro-steps, centres, unbinding and kdTooSmall after fof()."""
+        s = C.c_int()
+        self.L.or_microsteps(self.ctx, C.byref(s))
+        self.L.or_centers(self.ctx)
+        self.L.or_unbind(self.ctx, C.byref(s))
+        self.L.or_too_small(self.ctx)
+        return self.groups()
+
 
 def ref_density(box, **kw):
     p = params(**kw)
@@ -300,3 +331,46 @@ def relabel_min_index(labels: np.ndarray) -> np.ndarray:
     newid[order] = np.arange(1, uniq.size + 1)
     out[nz] = newid[np.searchsorted(uniq, lab[nz])]
     return out
+
+
+def overlap_pairing(a: np.ndarray, b: np.ndarray) -> dict:
+    """Greedy mutual-best pairing of group labels by member overlap, 0 <-> 0.
+
+    Pairs (label_a, label_b) are taken in order of decreasing overlap count,
+    each label used at most once; group 0 (not in a group) always maps to 0.
+    Returns {label_a: label_b}.
+    """
+    a = np.asarray(a, np.int64)
+    b = np.asarray(b, np.int64)
+    m = (a != 0) & (b != 0)
+    key = a[m] * (int(b.max()) + 1) + b[m]
+    uk, cnt = np.unique(key, return_counts=True)
+    la, lb = uk // (int(b.max()) + 1), uk % (int(b.max()) + 1)
+    order = np.lexsort((lb, la, -cnt))
+    pa, used_b = {0: 0}, {0}
+    for k in order:
+        x, y = int(la[k]), int(lb[k])
+        if x in pa or y in used_b:
+            continue
+        pa[x] = y
+        used_b.add(y)
+    return pa
+
+
+def match_fraction(a: np.ndarray, b: np.ndarray) -> float:
+    """(1/N) #{p : pi(a_p) == b_p} under overlap_pairing(a, b); unpaired labels never match."""
+    pa = overlap_pairing(a, b)
+    a = np.asarray(a)
+    mapped = np.array([pa.get(int(x), -1) for x in np.unique(a)])
+    lut = dict(zip(np.unique(a).tolist(), mapped.tolist()))
+    am = np.vectorize(lut.get, otypes=[np.int64])(a) if a.size else a
+    return float(np.mean(am == np.asarray(b))) if a.size else 1.0
+
+
+def same_partition(a: np.ndarray, b: np.ndarray) -> bool:
+    """True iff a and b are the same labelling up to renaming (0 fixed to 0)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    if a.shape != b.shape or not np.array_equal(a == 0, b == 0):
+        return False
+    return np.array_equal(relabel_min_index(a), relabel_min_index(b))

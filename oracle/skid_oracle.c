@@ -1236,20 +1236,49 @@ static int kd2_cmp(const void *a, const void *b)
 	return x->j - y->j;
 }
 
+static inline int kd2_less(float a, int ia, float b, int ib)
+{
+	return a < b || (a == b && ia < ib);
+}
+
 int or_knn_brute(int n, const float *pos, int k, float fPeriod, int bPeriodic,
 		float *fBall2, int *nbr)
 {
-	KD2 *buf = (KD2 *)malloc((size_t)(n > 0 ? n : 1) * sizeof(KD2));
+	/* per query: bounded max-heap of the k smallest (d2, index), then sorted */
+	KD2 *h = (KD2 *)malloc((size_t)(k > 0 ? k : 1) * sizeof(KD2));
 	int i, j;
-	if (!buf) return -1;
-	if (k > n) { free(buf); return -2; }
+	if (!h) return -1;
+	if (k > n || k < 1) { free(h); return -2; }
 	for (i = 0; i < n; ++i) {
-		for (j = 0; j < n; ++j) { buf[j].d2 = pair_d2(&pos[3*i], &pos[3*j], fPeriod, bPeriodic); buf[j].j = j; }
-		qsort(buf, n, sizeof(KD2), kd2_cmp);
-		if (fBall2) fBall2[i] = buf[k-1].d2;
-		if (nbr) for (j = 0; j < k; ++j) nbr[(size_t)i*k + j] = buf[j].j;
+		int m = 0;
+		for (j = 0; j < n; ++j) {
+			float d2 = pair_d2(&pos[3*i], &pos[3*j], fPeriod, bPeriodic);
+			int c, p;
+			if (m < k) {                       /* sift up */
+				c = m++;
+				while (c > 0) {
+					p = (c - 1) >> 1;
+					if (!kd2_less(h[p].d2, h[p].j, d2, j)) break;
+					h[c] = h[p]; c = p;
+				}
+				h[c].d2 = d2; h[c].j = j;
+			} else if (kd2_less(d2, j, h[0].d2, h[0].j)) {   /* replace root, sift down */
+				c = 0;
+				for (;;) {
+					int l = 2*c + 1, r = l + 1, b = l;
+					if (l >= k) break;
+					if (r < k && kd2_less(h[l].d2, h[l].j, h[r].d2, h[r].j)) b = r;
+					if (!kd2_less(d2, j, h[b].d2, h[b].j)) break;
+					h[c] = h[b]; c = b;
+				}
+				h[c].d2 = d2; h[c].j = j;
+			}
+		}
+		qsort(h, k, sizeof(KD2), kd2_cmp);
+		if (fBall2) fBall2[i] = h[k-1].d2;
+		if (nbr) for (j = 0; j < k; ++j) nbr[(size_t)i*k + j] = h[j].j;
 	}
-	free(buf);
+	free(h);
 	return 0;
 }
 

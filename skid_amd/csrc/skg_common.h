@@ -52,6 +52,7 @@ struct skg_ctx {
 	// entities S' = S + replicas
 	int nE = 0, nRep = 0;
 	float4 *e_pos = nullptr, *e_aux = nullptr;
+	float4 *e_org = nullptr;     // source scatterer position; w = 1 for a periodic replica
 	float *e_b2 = nullptr;       // fBall2 as built (all entities live)
 	float *e_rho = nullptr;      // effective density (0 once zeroed by the initial cut)
 	int *e_src = nullptr;        // scatterer (sorted S) index
@@ -160,6 +161,18 @@ __device__ __forceinline__ float img_delta(float q, float qm, float qp, float p,
 		else if (d < -hL) d = fsub(qp, p);
 	}
 	return d;
+}
+
+// Periodic delta (s + shift) - p accurate to one rounding of the result: the
+// rounding error of s - p is carried by a two-sum, and adding the shift to
+// s - p (|s - p| close to |shift|) is exact.  The reference instead rounds
+// s + shift first (ulp(L/2) absolute), which near the box faces costs ~1e-5
+// relative in pair separations of ~1e-3.
+__device__ __forceinline__ float exact_delta(float s, float p, float shift) {
+	float t = fsub(s, p);
+	float bp = fsub(t, s);
+	float e = fadd(fsub(s, fsub(t, bp)), fsub(-p, bp));
+	return fadd(fadd(t, shift), e);
 }
 
 // M4 spline density term in the reference's float/double mix (smooth1.c:257-262).

@@ -235,7 +235,7 @@ template <int KMAX, bool PER, bool COMP>
 __global__ __launch_bounds__(64) void k_knn(
 		const float4 *__restrict__ spos, const int *__restrict__ sord, int n,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
-		float L, int k, const float *__restrict__ mass_ord,
+		float L, int k, const float *__restrict__ mass_ord, const float *__restrict__ pos_ord,
 		float *__restrict__ ball2_out, float *__restrict__ rhoG, unsigned long long *__restrict__ heap_out) {
 	__shared__ unsigned long long heap[KMAX * 64];
 	__shared__ int stk[64];
@@ -314,7 +314,21 @@ __global__ __launch_bounds__(64) void k_knn(
 	acc.init();
 	for (int s = 1; s < k; ++s) {
 		const unsigned long long key = heap[s * 64 + lane];
-		acc.add(fmul(m4_term(key_d2(key), ih2, fNorm), mass_ord[(unsigned)key]));
+		const unsigned j = (unsigned)key;
+		float d2 = key_d2(key);
+		if (COMP && PER) {
+			// weights from exact periodic separations (membership stays the
+			// reference's d2 test, so the neighbour set is unchanged)
+			const float px = pos_ord[3 * j], py = pos_ord[3 * j + 1], pz = pos_ord[3 * j + 2];
+			float dx = fsub(q.x, px), dy = fsub(q.y, py), dz = fsub(q.z, pz);
+			if (fabsf(dx) > hL || fabsf(dy) > hL || fabsf(dz) > hL) {
+				dx = exact_delta(q.x, px, dx > hL ? -L : (dx < -hL ? L : 0.f));
+				dy = exact_delta(q.y, py, dy > hL ? -L : (dy < -hL ? L : 0.f));
+				dz = exact_delta(q.z, pz, dz > hL ? -L : (dz < -hL ? L : 0.f));
+				d2 = sq3(dx, dy, dz);
+			}
+		}
+		acc.add(fmul(m4_term(d2, ih2, fNorm), mass_ord[j]));
 	}
 	ball2_out[qi] = h2;
 	rhoG[qi] = acc.val();
@@ -331,8 +345,8 @@ __global__ __launch_bounds__(64) void k_knn(
 template <int MODE, bool COMP>
 __global__ __launch_bounds__(64) void k_cover(
 		float4 *__restrict__ ppos, const int *__restrict__ plist, int np,
-		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, int ne,
-		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
+		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
+		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
 		float fStep, float cx, float cy, float cz, float Lx, float Ly, float Lz) {
 	const int lane = threadIdx.x;
@@ -359,13 +373,23 @@ __global__ __launch_bounds__(64) void k_cover(
 					for (int t = 0; t < cnt; ++t) {
 						const int e = start + t;
 						const float4 A = epos[e], X = eaux[e];
-						const float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
-						const float d2 = sq3(dx, dy, dz);
+						float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
+						float d2 = sq3(dx, dy, dz);
 						if (d2 < X.x) {
+							if (COMP) {
+								// replica: separation from the source position plus the exact shift
+								const float4 O = eorg[e];
+								const int code = (int)O.w;     // 9(ix+1) + 3(iy+1) + (iz+1)
+								if (code != 13) {
+									dx = exact_delta(O.x, p.x, (float)(code / 9 - 1) * Lx);
+									dy = exact_delta(O.y, p.y, (float)((code / 3) % 3 - 1) * Ly);
+									dz = exact_delta(O.z, p.z, (float)(code % 3 - 1) * Lz);
+								}
+							}
 							if (MODE == 0) {
-								s0.add(fmul(m4_term(d2, X.y, X.w), A.w));
+								s0.add(fmul(m4_term(COMP ? sq3(dx, dy, dz) : d2, X.y, X.w), A.w));
 							} else {
-								const float f = grad_factor(d2, X.y, X.z);
+								const float f = grad_factor(COMP ? sq3(dx, dy, dz) : d2, X.y, X.z);
 								s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
 								ehit[e] = stamp;
 							}
@@ -481,8 +505,8 @@ __global__ void k_rep_write(const float4 *__restrict__ spos, const float *__rest
 // Entities in Morton order + per-entity constants (smooth1.c:447-448, 250-251).
 __global__ void k_entity_setup(const float4 *__restrict__ e4u, const float *__restrict__ eb2u,
 		const int *__restrict__ esrcu, const int *__restrict__ perm, int ne,
-		const float *__restrict__ srho, float4 *e4, float4 *eaux, float *erho, int *esrc, int *elive,
-		int *ehit, float *eb2s) {
+		const float *__restrict__ srho, const float4 *__restrict__ spos, float Lx, float Ly, float Lz,
+		float4 *e4, float4 *eaux, float4 *eorg, float *erho, int *esrc, int *elive, int *ehit, float *eb2s) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= ne) return;
 	int j = perm[i];
@@ -493,6 +517,12 @@ __global__ void k_entity_setup(const float4 *__restrict__ e4u, const float *__re
 	float gnorm = (float)(M_1_PI * dih2 * dih2 * sqrt(dih2) * (double)p.w);
 	float dnorm = (float)(0.5 * M_1_PI * sqrt(dih2) * dih2);
 	e4[i] = p;
+	const float4 o = spos[esrcu[j]];
+	int ix = 0, iy = 0, iz = 0;
+	if (Lx > 0.f) {
+		ix = (int)rintf((p.x - o.x) / Lx); iy = (int)rintf((p.y - o.y) / Ly); iz = (int)rintf((p.z - o.z) / Lz);
+	}
+	eorg[i] = make_float4(o.x, o.y, o.z, (float)(9 * (ix + 1) + 3 * (iy + 1) + (iz + 1)));
 	eaux[i] = make_float4(b2, ih2, gnorm, dnorm);
 	eb2s[i] = b2;
 	esrc[i] = esrcu[j];
@@ -704,7 +734,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->s_pos); dfree(&c->s_ord); dfree(&c->s_ball2); dfree(&c->s_rhoG); dfree(&c->s_rhoS);
 	dfree(&c->s_heap);
 	skg_free_tree(&c->ts); skg_free_tree(&c->te);
-	dfree(&c->e_b2); dfree(&c->e_pos); dfree(&c->e_aux); dfree(&c->e_rho); dfree(&c->e_src); dfree(&c->e_live); dfree(&c->e_hit);
+	dfree(&c->e_b2); dfree(&c->e_pos); dfree(&c->e_aux); dfree(&c->e_org); dfree(&c->e_rho); dfree(&c->e_src); dfree(&c->e_live); dfree(&c->e_hit);
 	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2);
 	dfree(&c->d_thr); dfree(&c->d_cnt);
 	if (c->tmp) (void)hipFree(c->tmp);
@@ -768,7 +798,7 @@ static void launch_knn(skg_ctx *c, bool per, bool comp) {
 	const float L = per ? c->P.fPeriod[0] : FLT_MAX;
 #define KL(P_, C_)                                                                           \
 	k_knn<KMAX, P_, C_><<<nb, 64, 0, c->st>>>(c->s_pos, c->s_ord, c->nS, c->ts.lo, c->ts.hi,   \
-			c->ts.nLeaf2, c->ts.B, L, c->P.nSmooth, c->d_mass, c->s_ball2, c->s_rhoG, c->s_heap)
+			c->ts.nLeaf2, c->ts.B, L, c->P.nSmooth, c->d_mass, c->d_pos, c->s_ball2, c->s_rhoG, c->s_heap)
 	if (per) { if (comp) KL(true, true); else KL(true, false); }
 	else { if (comp) KL(false, true); else KL(false, false); }
 #undef KL
@@ -781,7 +811,7 @@ static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, in
 	const bool comp = c->P.accumulate == SKG_ACC_COMPENSATED;
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
 #define CL(M_, C_)                                                                             \
-	k_cover<M_, C_><<<nb, 64, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->nE, c->te.lo, \
+	k_cover<M_, C_><<<nb, 64, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, \
 			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, fStep, C[0], C[1], C[2], L[0], L[1], L[2])
 	if (mode == 0) { if (comp) CL(0, true); else CL(0, false); }
 	else { if (comp) CL(1, true); else CL(1, false); }
@@ -866,14 +896,16 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		int *esrcu = nullptr, *perm2 = nullptr;
 		if (dalloc(&e4u, nE) || dalloc(&eb2u, nE) || dalloc(&esrcu, nE) || dalloc(&perm2, nE) ||
 		    dalloc(&eb2s, nE) || dalloc(&c->e_pos, nE) || dalloc(&c->e_aux, nE) || dalloc(&c->e_rho, nE) ||
+		    dalloc(&c->e_org, nE) ||
 		    dalloc(&c->e_src, nE) || dalloc(&c->e_live, nE) || dalloc(&c->e_hit, nE))
 			return SKG_E_NOMEM;
 		const float *L = c->P.fPeriod;
 		k_rep_write<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_pos, c->s_ball2, nS, mask, off,
 				per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, e4u, eb2u, esrcu);
 		if ((rc = skg_morton_sort(c, e4u, nullptr, nE, perm2))) return rc;
-		k_entity_setup<<<nblk(nE, 256), 256, 0, c->st>>>(e4u, eb2u, esrcu, perm2, nE, nullptr, c->e_pos,
-				c->e_aux, c->e_rho, c->e_src, c->e_live, c->e_hit, eb2s);
+		k_entity_setup<<<nblk(nE, 256), 256, 0, c->st>>>(e4u, eb2u, esrcu, perm2, nE, nullptr, c->s_pos,
+				per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, c->e_pos, c->e_aux, c->e_org, c->e_rho,
+				c->e_src, c->e_live, c->e_hit, eb2s);
 		if ((rc = skg_build_tree(c, &c->te, c->e_pos, nE, B, eb2s))) return rc;
 		SKG_HIP(hipStreamSynchronize(c->st));
 		dfree(&mask); dfree(&cnt); dfree(&off); dfree(&e4u); dfree(&eb2u); dfree(&esrcu); dfree(&perm2);

@@ -1,0 +1,362 @@
+/*
+ * skid.c -- command-line driver: SKID's options, tipsy input and group files,
+ * with every group-finding stage running on the GPU through libskg (skg.h).
+ *
+ * Follows main.c:44-497 of SKID v1.4.1 for option names, defaults, stage
+ * order and stdout counters; tipsy reading/writing follows kd.c:122-222 and
+ * kd.c:1502-1687 (native little-endian or, with -std, XDR big-endian, decoded
+ * here without libtirpc).  Host cosmology H(a) stays FP64 (cosmo.c:46-58).
+ */
+#include <float.h>
+#include <limits.h>
+#include <math.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#include "skg.h"
+
+#define PRUNE_STEPS 5       /* main.c:13 */
+
+static void usage(void)
+{
+	fprintf(stderr, "USAGE:\n");
+	fprintf(stderr, "skid -tau <fLinkLength> [OPTIONAL ARGUMENTS]\n");
+	fprintf(stderr, "	 reads TIPSY BINARY input file from stdin\n");
+	fprintf(stderr, "     [-std]\n");
+	fprintf(stderr, "COSMOLOGY and UNITS arguments:\n");
+	fprintf(stderr, "     [-z <fRedShift>] [-O <fOmega>]\n");
+	fprintf(stderr, "     [-G <fGravConst>] [-H <fHubble>]\n");
+	fprintf(stderr, "     [-Lambda <fLambda>] [-Q <fQuintessense]\n");
+	fprintf(stderr, "GROUP FINDING arguments (see man page!):\n");
+	fprintf(stderr, "     [-s <nSmooth>] [-d <fMinDensity>] [-t <fMaxTemp>]\n");
+	fprintf(stderr, "     [-cvg <fConvergeRadius>] [-scoop <fScoopRadius>]\n");
+	fprintf(stderr, "     [-m <nMinMembers>] [-nu] [-gd] [-unbind <GroupName>[.grp]]\n");
+	fprintf(stderr, "     [-M <fMaxMass>] [-fic] [-go] [-maxgroup nMaxMembers]\n");
+	fprintf(stderr, "GRAVITATIONAL SOFTENING arguments:\n");
+	fprintf(stderr, "     [-spline] [-plummer] [-e <fSoft>]\n");
+	fprintf(stderr, "PERIODIC BOX specification:\n");
+	fprintf(stderr, "     [-p <xyzPeriod>]\n");
+	fprintf(stderr, "     [-c <xyzCenter>]\n");
+	fprintf(stderr, "     [-cx <xCenter>] [-cy <yCenter>] [-cz <zCenter>]\n");
+	fprintf(stderr, "OUTPUT arguments:\n");
+	fprintf(stderr, "     [-o <Output Name>] [-ray] [-den] [-stats] [-diag]\n");
+	fprintf(stderr, "     [-plain] (fp32 running sums instead of compensated sums)\n");
+	exit(1);
+}
+
+static void die_skg(const char *what, int rc)
+{
+	fprintf(stderr, "skid: %s failed: %s (%d)\n", what, skg_strerror(rc), rc);
+	exit(2);
+}
+
+/* ---------------------------------------------------------------- tipsy I/O */
+typedef struct {
+	double time;
+	int nbodies, ndim, nsph, ndark, nstar;
+	int n;
+	float *pos, *vel, *mass, *soft, *temp;
+} Snap;
+
+static uint32_t bswap32(uint32_t x) { return (x >> 24) | ((x >> 8) & 0xff00u) | ((x << 8) & 0xff0000u) | (x << 24); }
+
+static int read_all(FILE *fp, void *buf, size_t n)
+{
+	return fread(buf, 1, n, fp) == n ? 0 : -1;
+}
+
+static float rdf(const unsigned char *p, int xdr)
+{
+	uint32_t u;
+	float f;
+	memcpy(&u, p, 4);
+	if (xdr) u = bswap32(u);
+	memcpy(&f, &u, 4);
+	return f;
+}
+
+static int rdi(const unsigned char *p, int xdr)
+{
+	uint32_t u;
+	memcpy(&u, p, 4);
+	if (xdr) u = bswap32(u);
+	return (int)u;
+}
+
+/* kd.c:122-222: gas (12 floats), dark (9), star (11) records after a 32-byte header. */
+static int read_tipsy(FILE *fp, int xdr, Snap *s)
+{
+	unsigned char h[32];
+	int i, j, nrec;
+	static const int nf[3] = {12, 9, 11};
+	if (read_all(fp, h, 32)) return -1;
+	if (xdr) {
+		uint64_t u;
+		memcpy(&u, h, 8);
+		u = ((uint64_t)bswap32((uint32_t)u) << 32) | bswap32((uint32_t)(u >> 32));
+		memcpy(&s->time, &u, 8);
+	} else memcpy(&s->time, h, 8);
+	s->nbodies = rdi(h + 8, xdr); s->ndim = rdi(h + 12, xdr); s->nsph = rdi(h + 16, xdr);
+	s->ndark = rdi(h + 20, xdr); s->nstar = rdi(h + 24, xdr);
+	if (s->nsph < 0 || s->ndark < 0 || s->nstar < 0) return -1;
+	s->n = s->nsph + s->ndark + s->nstar;
+	s->pos = malloc(12 * (size_t)s->n); s->vel = malloc(12 * (size_t)s->n);
+	s->mass = malloc(4 * (size_t)s->n); s->soft = malloc(4 * (size_t)s->n); s->temp = malloc(4 * (size_t)s->n);
+	if (!s->pos || !s->vel || !s->mass || !s->soft || !s->temp) return -2;
+	i = 0;
+	for (nrec = 0; nrec < 3; ++nrec) {
+		int cnt = nrec == 0 ? s->nsph : (nrec == 1 ? s->ndark : s->nstar);
+		size_t bytes = 4 * (size_t)nf[nrec] * cnt;
+		unsigned char *buf = malloc(bytes ? bytes : 1);
+		if (!buf) return -2;
+		if (read_all(fp, buf, bytes)) { free(buf); return -1; }
+		for (j = 0; j < cnt; ++j, ++i) {
+			const unsigned char *r = buf + 4 * (size_t)nf[nrec] * j;
+			int k;
+			s->mass[i] = rdf(r, xdr);
+			for (k = 0; k < 3; ++k) { s->pos[3*i+k] = rdf(r + 4 + 4*k, xdr); s->vel[3*i+k] = rdf(r + 16 + 4*k, xdr); }
+			if (nrec == 0) { s->temp[i] = rdf(r + 32, xdr); s->soft[i] = rdf(r + 36, xdr); }
+			else if (nrec == 1) { s->temp[i] = 0.0f; s->soft[i] = rdf(r + 28, xdr); }
+			else { s->temp[i] = 0.0f; s->soft[i] = rdf(r + 36, xdr); }
+		}
+		free(buf);
+	}
+	return 0;
+}
+
+static void wrf(FILE *fp, float f, int xdr)
+{
+	uint32_t u;
+	memcpy(&u, &f, 4);
+	if (xdr) u = bswap32(u);
+	fwrite(&u, 4, 1, fp);
+}
+
+static void wri(FILE *fp, int v, int xdr)
+{
+	uint32_t u = (uint32_t)v;
+	if (xdr) u = bswap32(u);
+	fwrite(&u, 4, 1, fp);
+}
+
+/* ---------------------------------------------------------------- cosmology */
+/* H(a) for the flat/curved Lambda + quintessence model (cosmo.c:46-58). */
+static double exp2hub(double H0, double om, double lam, double q, double a)
+{
+	double rad = 0.0, curv = 1.0 - om - lam - rad - q;
+	return H0*sqrt(om*a + curv*a*a + rad + q*a*a*sqrt(a) + lam*a*a*a*a)/(a*a);
+}
+
+static double now_s(void)
+{
+	struct timespec t;
+	clock_gettime(CLOCK_MONOTONIC, &t);
+	return t.tv_sec + 1e-9*t.tv_nsec;
+}
+
+int main(int argc, char **argv)
+{
+	skg_params P;
+	skg_ctx *ctx;
+	Snap S;
+	int bTau = 0, bCvg = 0, bScoop = 0, bEps = 0, bOutRay = 0, bOutDens = 0, bOutStats = 0, bStandard = 0;
+	int bUnbindOnly = 0, bPlain = 0, i, rc, nActive, nScat, nIttr, nGroup, nUnb, nExtra;
+	float fTau = 0, z = 0, Omega0 = 1, G = 1, H0 = 0, Lambda = 0, fQuintess = 0, fCvg = 0, fScoop = 0, fEps = 0;
+	float fStep;
+	char achName[256] = "skid", achFile[512];
+	double t0, t1, tDen = 0, tMove = 0, tFof = 0, tMicro = 0, tUnb = 0;
+	float *rho = NULL, *r = NULL;
+	int *grp;
+
+	printf("SKID v1.4.1 group finder (MI355X GPU path)\n");
+	skg_default_params(&P);
+	for (i = 1; i < argc; ) {
+#define ARG_F(flag, var) else if (!strcmp(argv[i], flag)) { if (++i >= argc) usage(); var = atof(argv[i++]); }
+#define ARG_I(flag, var) else if (!strcmp(argv[i], flag)) { if (++i >= argc) usage(); var = atoi(argv[i++]); }
+		if (!strcmp(argv[i], "-tau")) { if (++i >= argc) usage(); fTau = atof(argv[i++]); bTau = 1; }
+		ARG_F("-z", z) ARG_F("-O", Omega0) ARG_F("-Lambda", Lambda) ARG_F("-Q", fQuintess)
+		ARG_F("-G", G) ARG_F("-H", H0) ARG_I("-s", P.nSmooth) ARG_F("-d", P.fDensMin)
+		ARG_F("-t", P.fTempMax) ARG_F("-M", P.fMassMax) ARG_I("-m", P.nMembers)
+		ARG_I("-maxgroup", P.nMaxMembers)
+		else if (!strcmp(argv[i], "-fic")) { P.bForceInitialCut = 1; ++i; }
+		else if (!strcmp(argv[i], "-cvg")) { if (++i >= argc) usage(); fCvg = atof(argv[i++]); bCvg = 1; }
+		else if (!strcmp(argv[i], "-scoop")) { if (++i >= argc) usage(); fScoop = atof(argv[i++]); bScoop = 1; }
+		else if (!strcmp(argv[i], "-nu")) { P.bNoUnbind = 1; ++i; }
+		else if (!strcmp(argv[i], "-gd")) { P.bGasAndDark = 1; ++i; }
+		else if (!strcmp(argv[i], "-go")) { P.bGasOnly = 1; ++i; }
+		else if (!strcmp(argv[i], "-unbind")) { if (++i >= argc) usage(); bUnbindOnly = 1; ++i; }
+		else if (!strcmp(argv[i], "-spline")) { P.iSoftType = SKG_SPLINE; ++i; }
+		else if (!strcmp(argv[i], "-plummer")) { P.iSoftType = SKG_PLUMMER; ++i; }
+		else if (!strcmp(argv[i], "-e")) { if (++i >= argc) usage(); fEps = atof(argv[i++]); bEps = 1; }
+		else if (!strcmp(argv[i], "-p")) {
+			if (++i >= argc) usage();
+			P.fPeriod[0] = P.fPeriod[1] = P.fPeriod[2] = atof(argv[i++]);
+			P.bPeriodic = 1;
+		}
+		else if (!strcmp(argv[i], "-c")) {
+			if (++i >= argc) usage();
+			P.fCenter[0] = P.fCenter[1] = P.fCenter[2] = atof(argv[i++]);
+		}
+		ARG_F("-cx", P.fCenter[0]) ARG_F("-cy", P.fCenter[1]) ARG_F("-cz", P.fCenter[2])
+		else if (!strcmp(argv[i], "-o")) { if (++i >= argc) usage(); snprintf(achName, sizeof(achName), "%s", argv[i++]); }
+		else if (!strcmp(argv[i], "-ray")) { bOutRay = 1; ++i; }
+		else if (!strcmp(argv[i], "-den")) { bOutDens = 1; ++i; }
+		else if (!strcmp(argv[i], "-stats")) { bOutStats = 1; ++i; }
+		else if (!strcmp(argv[i], "-diag")) { ++i; }
+		else if (!strcmp(argv[i], "-std")) { bStandard = 1; ++i; }
+		else if (!strcmp(argv[i], "-plain")) { bPlain = 1; ++i; }
+		else usage();
+#undef ARG_F
+#undef ARG_I
+	}
+	if (!bTau) usage();                                     /* main.c:339 */
+	if (bUnbindOnly) {
+		fprintf(stderr, "skid: -unbind re-entry is not provided by this build\n");
+		return 1;
+	}
+	if (!bCvg) fCvg = 0.5*fTau;                             /* main.c:343-345 */
+	if (!bScoop) fScoop = 2.0*fTau;
+	fStep = 0.5*fCvg;
+	P.fTau = fTau; P.fCvg = fCvg; P.fScoop = fScoop; P.fStep = fStep;
+	P.G = G; P.z = z;
+	P.fShift = 1.0/(1.0 + z);                               /* kd.c:1317-1318 */
+	P.fCosmo = P.fShift*exp2hub(H0, Omega0, Lambda, fQuintess, P.fShift);
+	P.accumulate = bPlain ? SKG_ACC_PLAIN : SKG_ACC_COMPENSATED;
+
+	memset(&S, 0, sizeof(S));
+	if (read_tipsy(stdin, bStandard, &S)) { fprintf(stderr, "skid: cannot read tipsy input\n"); return 1; }
+	printf("nDark:%d nGas:%d nStar:%d\n", S.ndark, S.nsph, S.nstar);
+	fflush(stdout);
+	if (bEps) for (i = 0; i < S.n; ++i) S.soft[i] = fEps;  /* kdSetSoft (main.c:464) */
+
+	if ((rc = skg_create(&P, &ctx))) die_skg("skg_create", rc);
+	if ((rc = skg_load(ctx, S.nsph, S.ndark, S.nstar, S.pos, S.vel, S.mass, S.soft, S.temp))) die_skg("skg_load", rc);
+	rho = malloc(4*(size_t)S.n);
+	grp = malloc(4*(size_t)S.n);
+	if (!rho || !grp) { fprintf(stderr, "skid: out of memory\n"); return 1; }
+
+	t0 = now_s();
+	if ((rc = skg_density(ctx, NULL, rho, &nExtra))) die_skg("skg_density", rc);
+	t1 = now_s(); tDen = t1 - t0;
+	if (P.bPeriodic) printf("nExtraScat:%d\n", nExtra);
+	if (bOutDens) {                                         /* kdOutDensity (kd.c:1526-1547) */
+		FILE *fp;
+		snprintf(achFile, sizeof(achFile), "%s.den", achName);
+		fp = fopen(achFile, "w");
+		if (!fp) { perror(achFile); return 1; }
+		fprintf(fp, "%d\n", S.n);
+		for (i = 0; i < S.n; ++i) fprintf(fp, "%.10g\n", rho[i]);
+		fclose(fp);
+	}
+	t0 = now_s();
+	if ((rc = skg_init_move(ctx, &nActive))) die_skg("skg_init_move", rc);
+	if ((rc = skg_move_first(ctx, &nActive, &nScat))) die_skg("skg_move_first", rc);
+	printf("Ittr:%d nActive:%d nScatter:%d\n", 0, nActive, nScat);
+	fflush(stdout);
+	nIttr = 1;
+	while (nActive) {
+		if ((rc = skg_move_block(ctx, &nActive, &nScat))) die_skg("skg_move_block", rc);
+		printf("Ittr:%d nActive:%d nScatter:%d\n", nIttr, nActive, nScat);
+		fflush(stdout);
+		++nIttr;
+	}
+	t1 = now_s(); tMove = t1 - t0;
+	t0 = now_s();
+	if ((rc = skg_fof(ctx, NULL, &nGroup))) die_skg("skg_fof", rc);
+	t1 = now_s(); tFof = t1 - t0;
+	t0 = now_s();
+	if ((rc = skg_microsteps(ctx, &nScat))) die_skg("skg_microsteps", rc);
+	t1 = now_s(); tMicro = t1 - t0;
+	for (i = 0; i < PRUNE_STEPS; ++i) printf("Microstep:%d nScatter:%d\n", i + 1, nScat);
+	if (bOutRay) {                                          /* kdOutVector (kd.c:1550-1606) */
+		FILE *fp;
+		int j;
+		float h = 0.5f*P.fPeriod[0];
+		r = malloc(12*(size_t)S.n);
+		if (!r) { fprintf(stderr, "skid: out of memory\n"); return 1; }
+		if ((rc = skg_get_moved(ctx, r))) die_skg("skg_get_moved", rc);
+		snprintf(achFile, sizeof(achFile), "%s.ray", achName);
+		fp = fopen(achFile, "w");
+		if (!fp) { perror(achFile); return 1; }
+		fprintf(fp, "%d\n", S.n);
+		for (j = 0; j < 3; ++j) {
+			float hj = 0.5f*P.fPeriod[j];
+			for (i = 0; i < S.n; ++i) {
+				float d;
+				if (isnan(r[3*i+j])) { fprintf(fp, "0\n"); continue; }
+				d = r[3*i+j] - S.pos[3*i+j];
+				if (d > hj) d -= 2*hj;
+				if (d <= -hj) d += 2*hj;
+				fprintf(fp, "%g\n", d);
+			}
+		}
+		(void)h;
+		fclose(fp);
+	}
+	if ((rc = skg_centers(ctx))) die_skg("skg_centers", rc);
+	printf("Groups before Unbind:%d\n", nGroup - 1);
+	t0 = now_s();
+	if ((rc = skg_unbind(ctx, &nUnb))) die_skg("skg_unbind", rc);
+	t1 = now_s(); tUnb = t1 - t0;
+	printf("Number of particles Unbound:%d\n", nUnb);
+	if ((rc = skg_too_small(ctx))) die_skg("skg_too_small", rc);
+	if ((rc = skg_get_groups(ctx, grp, &nGroup))) die_skg("skg_get_groups", rc);
+	printf("Number of Groups:%d\n", nGroup - 1);
+	{                                                       /* kdOutGroup (kd.c:1502-1523) */
+		FILE *fp;
+		snprintf(achFile, sizeof(achFile), "%s.grp", achName);
+		fp = fopen(achFile, "w");
+		if (!fp) { perror(achFile); return 1; }
+		fprintf(fp, "%d\n", S.n);
+		for (i = 0; i < S.n; ++i) fprintf(fp, "%d\n", grp[i]);
+		fclose(fp);
+	}
+	{                                                       /* kdWriteGroup (kd.c:1609-1687) */
+		FILE *fp;
+		int ng = nGroup;
+		float *m = malloc(4*(size_t)ng), *cen = malloc(12*(size_t)ng), *vcm = malloc(12*(size_t)ng),
+		      *rad = malloc(4*(size_t)ng);
+		unsigned char hdr[32];
+		int k;
+		if (!m || !cen || !vcm || !rad) { fprintf(stderr, "skid: out of memory\n"); return 1; }
+		if ((rc = skg_get_catalogue(ctx, m, cen, vcm, NULL, rad, NULL))) die_skg("skg_get_catalogue", rc);
+		snprintf(achFile, sizeof(achFile), "%s.gtp", achName);
+		fp = fopen(achFile, "wb");
+		if (!fp) { perror(achFile); return 1; }
+		memset(hdr, 0, sizeof(hdr));
+		if (bStandard) {
+			uint64_t u;
+			double t = (double)(float)S.time;           /* kd->fTime is a float (kd.c:155) */
+			memcpy(&u, &t, 8);
+			u = ((uint64_t)bswap32((uint32_t)u) << 32) | bswap32((uint32_t)(u >> 32));
+			fwrite(&u, 8, 1, fp);
+		} else { double t = (double)(float)S.time; fwrite(&t, 8, 1, fp); }
+		wri(fp, ng - 1, bStandard); wri(fp, 3, bStandard); wri(fp, 0, bStandard); wri(fp, 0, bStandard);
+		wri(fp, ng - 1, bStandard); wri(fp, 0, bStandard);
+		for (i = 1; i < ng; ++i) {
+			wrf(fp, m[i], bStandard);
+			for (k = 0; k < 3; ++k) wrf(fp, cen[3*i+k], bStandard);
+			for (k = 0; k < 3; ++k) wrf(fp, vcm[3*i+k], bStandard);
+			wrf(fp, 0.0f, bStandard);                       /* metals */
+			wrf(fp, (float)S.time, bStandard);              /* tform */
+			wrf(fp, rad[i], bStandard);                     /* eps = group radius */
+			wrf(fp, 0.0f, bStandard);                       /* phi */
+		}
+		fclose(fp);
+		free(m); free(cen); free(vcm); free(rad);
+	}
+	if (bOutStats) fprintf(stderr, "skid: -stats output is not provided by this build\n");
+	printf("SKID GPU wall time:\n");
+	printf("   Initial Density:    %.6f\n", tDen);
+	printf("   Moving Particles:   %.6f\n", tMove);
+	printf("   Friends of Friends: %.6f\n", tFof);
+	printf("   Microstepping:      %.6f\n", tMicro);
+	if (!P.bNoUnbind) printf("   Unbinding:          %.6f\n", tUnb);
+	fflush(stdout);
+	skg_destroy(ctx);
+	free(rho); free(grp); free(r);
+	free(S.pos); free(S.vel); free(S.mass); free(S.soft); free(S.temp);
+	return 0;
+}

@@ -1,0 +1,100 @@
+"""GPU kNN radii and M4 density against the oracle (and through it the
+reference).  Bar: fBall2 and neighbour sets bit-exact; densities within 1e-5
+relative of the reference arithmetic and of the FP64 oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from skid_amd import gen, skg
+
+pytestmark = pytest.mark.gpu
+RHO_TOL = 1e-5
+
+
+def gpu_density(box, k, period=1.0, **kw):
+    P = skg.make_params(0.005, k, period=period, **kw)
+    ctx = skg.Context(box, P)
+    out = ctx.density()
+    return ctx, out
+
+
+@pytest.mark.parametrize("k,per", [(32, True), (64, True), (32, False), (64, False)])
+def test_knn_density_config1(box1, k, per):
+    """fBall2 bit-exact in both modes; the plain mode (the reference's per-term
+    arithmetic) within 1e-5 of the reference, the compensated mode (exact
+    periodic separations, two-sum sums) within 1e-5 of the FP64 oracle."""
+    kw = dict(nSmooth=k, fTau=0.005)
+    if per:
+        kw["fPeriod"] = 1.0
+    ob2, orho, one = O.Oracle(box1, **kw).density()
+    f64 = O.density_f64(box1.pos, box1.mass, ob2, period=1.0, periodic=per)
+    for mode in (skg.SKG_ACC_PLAIN, skg.SKG_ACC_COMPENSATED):
+        ctx, (b2, rho, ne) = gpu_density(box1, k, 1.0 if per else None, accumulate=mode)
+        assert np.array_equal(b2, ob2), int((b2 != ob2).sum())
+        assert ne == one
+        rel = np.abs(rho - orho) / orho
+        rel64 = np.abs(rho - f64) / f64
+        print(f"k={k} per={per} mode={mode} rho max rel: vs reference {rel.max():.2e}, vs FP64 {rel64.max():.2e}")
+        if mode == skg.SKG_ACC_PLAIN:
+            assert rel.max() < RHO_TOL
+        else:
+            assert rel64.max() < RHO_TOL
+    ref64 = np.abs(orho - f64) / f64
+    print(f"  (reference arithmetic itself vs FP64: {ref64.max():.2e})")
+
+
+def test_neighbor_lists_equal_brute_force(box1):
+    ctx, _ = gpu_density(box1, 32)
+    nb, d2 = ctx.neighbors()
+    bb, bl = O.knn_brute(box1.pos, 32, lists=True)
+    assert np.array_equal(nb, bl)
+    assert np.array_equal(d2[:, -1], bb)
+
+
+def test_compensated_is_closer_to_fp64(box1):
+    _, (b2, rho, _) = gpu_density(box1, 32, accumulate=skg.SKG_ACC_PLAIN)
+    _, (_, rhoc, _) = gpu_density(box1, 32)
+    f64 = O.density_f64(box1.pos, box1.mass, b2)
+    ep, ec = np.abs(rho - f64) / f64, np.abs(rhoc - f64) / f64
+    print(f"vs FP64: plain max {ep.max():.2e} mean {ep.mean():.2e}; compensated max {ec.max():.2e} mean {ec.mean():.2e}")
+    assert ec.max() < ep.max() and ec.mean() < ep.mean()
+
+
+@pytest.mark.parametrize("gd", [0, 1])
+def test_two_species(gd):
+    b = gen.config3(n_each=8192, seed=5, n_halos=30)
+    ctx, (b2, rho, ne) = gpu_density(b, 32, bGasAndDark=gd)
+    ob2, orho, one = O.Oracle(b, nSmooth=32, fTau=0.005, fPeriod=1.0, bGasAndDark=gd).density()
+    assert np.array_equal(b2, ob2)
+    assert ne == one
+    s = ob2 > 0
+    assert (rho[~s] == 0).all()
+    _, (_, rhop, _) = gpu_density(b, 32, bGasAndDark=gd, accumulate=skg.SKG_ACC_PLAIN)
+    assert (np.abs(rhop[s] - orho[s]) / orho[s]).max() < RHO_TOL
+
+
+def test_density_2pow20_k64():
+    """Config-2 size: 2^20 particles, 500 halos, k=64."""
+    b = gen.config2()
+    ctx, (b2, rho, ne) = gpu_density(b, 64, accumulate=skg.SKG_ACC_PLAIN)
+    ob2, orho, one = O.Oracle(b, nSmooth=64, fTau=0.002, fPeriod=1.0).density()
+    assert np.array_equal(b2, ob2), int((b2 != ob2).sum())
+    assert ne == one
+    rel = np.abs(rho - orho) / orho
+    print(f"2^20 k=64 plain: rho max rel vs reference {rel.max():.2e}")
+    assert rel.max() < RHO_TOL
+
+
+def test_edge_cases():
+    # k equal to the number of scatterers, coincident points, points on the box faces
+    pos = np.array([[0.5, 0.5, 0.5], [-0.4999, 0.0, 0.0], [0.5, 0.0, 0.0], [0.1, 0.1, 0.1],
+                    [0.1, 0.1, 0.1], [0.2, -0.3, 0.5], [0.0, 0.0, 0.0], [-0.25, 0.25, 0.0]], np.float32)
+    n = pos.shape[0]
+    b = gen.Box(pos=pos, vel=np.zeros_like(pos), mass=np.full(n, 1 / n, np.float32),
+                soft=np.full(n, 1e-3, np.float32), temp=np.zeros(n, np.float32), n_gas=0, n_dark=n)
+    for k in (2, 5, 8):
+        ctx, (b2, rho, _) = gpu_density(b, k)
+        bb = O.knn_brute(pos, k)
+        assert np.array_equal(b2, bb)
+    with pytest.raises(skg.SkgError):
+        gpu_density(b, 9)

@@ -1,0 +1,67 @@
+"""GPU move loop, FOF and unbinding against the oracle."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from skid_amd import gen, skg
+
+pytestmark = pytest.mark.gpu
+
+
+def test_fof_points_equals_brute_force():
+    rng = np.random.default_rng(4)
+    b = gen.config1(n=8192, seed=21)
+    sets = [b.pos,
+            rng.uniform(-0.5, 0.5, (5000, 3)).astype(np.float32),
+            # clumps collapsed well inside tau (contained-node path) and points on the faces
+            np.concatenate([rng.normal(0.0, 1e-4, (3000, 3)), rng.normal(0.49, 1e-3, (500, 3)),
+                            rng.uniform(-0.5, 0.5, (500, 3))]).astype(np.float32)]
+    sets[2] = (sets[2] - np.floor(sets[2] + 0.5)).astype(np.float32)
+    ctx = skg.Context(b, skg.make_params(0.005, 32, period=1.0))
+    for pos in sets:
+        for tau in (0.002, 0.01):
+            g, ng = ctx.fof_points(pos, tau)
+            bg, bng = O.fof_brute(pos, tau)
+            assert ng == bng + 1
+            assert np.array_equal(g, bg)        # same numbering: by smallest index
+
+
+def test_fof_on_reference_converged_positions(box1):
+    """The FOF stage on the reference's own converged positions reproduces the
+    reference's partition (numbering differs: here by smallest iOrder)."""
+    o = O.Oracle(box1, nSmooth=32, fTau=0.005, fPeriod=1.0)
+    o.density()
+    o.move_loop()
+    pos = o.moved()
+    fg, _ = o.fof()
+    ctx = skg.Context(box1, skg.make_params(0.005, 32, period=1.0))
+    g, _ = ctx.fof_points(pos, 0.005)
+    assert O.same_partition(g, fg)
+    assert np.array_equal(g, O.relabel_min_index(fg))
+
+
+def test_move_trace_and_catalogue_config1(box1):
+    P = skg.make_params(0.005, 32, period=1.0)
+    ctx = skg.Context(box1, P)
+    ctx.density()
+    tr = ctx.move_loop(max_blocks=2000)
+    fg, nfg = ctx.fof()
+    ctx.microsteps()
+    ctx.centers()
+    nunb = ctx.unbind()
+    ctx.too_small()
+    g, ng = ctx.groups()
+    o = O.Oracle(box1, nSmooth=32, fTau=0.005, fPeriod=1.0)
+    o.density()
+    otr = o.move_loop()
+    ofg, onfg = o.fof()
+    og, ong = o.finish()
+    print("trace gpu", tr[:4].tolist(), len(tr), "oracle", otr[:4].tolist(), len(otr))
+    print("fof groups", nfg, onfg, "final", ng, ong, "unbound", nunb)
+    ffrac = O.match_fraction(fg, ofg)
+    frac = O.match_fraction(g, og)
+    print(f"fof match {ffrac:.5f} final match {frac:.5f}")
+    assert abs(len(tr) - len(otr)) <= 3
+    assert np.all(np.abs(tr[:5, 0] - otr[:5, 0]) <= 0.002 * otr[0, 0] + 2)
+    assert ffrac >= 0.999
+    assert frac >= 0.999
