@@ -163,6 +163,23 @@ int skg_fof_points(skg_ctx *ctx, int n, const float *pos, float fTau, int *grp, 
 #define SKG_T_N 8
 int skg_stage_times(skg_ctx *ctx, double *ms);
 
+/* Kernel-level timing (no reference counterpart): when enabled, each launch of
+   the instrumented kernels is bracketed by HIP events on the context's stream.
+   skg_kernel_stats() returns, for kernel class `which` (SKG_K_*), the summed
+   device time in ms, the launch count and the summed algorithmic work of those
+   launches (bytes for the search, density, move and FOF classes, FP32 flops
+   for SKG_K_UNB_POT). */
+#define SKG_K_KNN 0          /* 16 B per query */
+#define SKG_K_COVER_DENS 1   /* 24 B per scatterer */
+#define SKG_K_COVER_MOVE 2   /* 24 B per active mover + 24 B per live scatterer, per step */
+#define SKG_K_FOF 3          /* 16 B per mover */
+#define SKG_K_UNB_POT 4      /* 20 flops per ordered pair */
+#define SKG_K_UNB_LOOP 5     /* removal loops of groups < 32768 members: 20 flops per kdSubPot pair */
+#define SKG_K_UNB_BIG 6      /* removal loops of larger groups (cooperative): 20 flops per kdSubPot pair */
+#define SKG_K_N 7
+int skg_set_profiling(skg_ctx *ctx, int on);
+int skg_kernel_stats(skg_ctx *ctx, int which, double *ms, long long *launches, double *work);
+
 #ifdef __cplusplus
 }
 #endif

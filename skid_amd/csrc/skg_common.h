@@ -50,7 +50,7 @@ struct skg_ctx {
 	DevTree ts;
 
 	// entities S' = S + replicas
-	int nE = 0, nRep = 0;
+	int nE = 0, nRep = 0, nLive = 0;
 	float4 *e_pos = nullptr, *e_aux = nullptr;
 	float4 *e_org = nullptr;     // source scatterer position; w = 1 for a periodic replica
 	float *e_b2 = nullptr;       // fBall2 as built (all entities live)
@@ -69,8 +69,16 @@ struct skg_ctx {
 	int *d_cnt = nullptr;        // small counters
 
 	// groups
-	int nGroup = 0;
+	int nGroup = 0, nBigGroups = 0;
+	long long nFullPasses = 0;
 	struct GroupState *gs = nullptr;
+
+	// kernel timing: (class, start, stop, work) per instrumented launch
+	int prof = 0;
+	struct ProfRec { int k; hipEvent_t a, b; double work; };
+	ProfRec *prec = nullptr; int nrec = 0, caprec = 0;
+	double kms[SKG_K_N] = {0}, kwork[SKG_K_N] = {0};
+	long long klaunch[SKG_K_N] = {0};
 
 	// scratch
 	void *tmp = nullptr; size_t tmpBytes = 0;
@@ -206,6 +214,9 @@ int skg_exclusive_scan(skg_ctx *c, const int *in, int *out, int n);
 void skg_free_tree(DevTree *t);
 int skg_gradient_points(skg_ctx *c, const float4 *q, int nq, float *out);   // skg_adjoint.hip
 void skg_timer_start(skg_ctx *c);
+int skg_prof_begin(skg_ctx *c);                        // returns a record slot or -1
+void skg_prof_end(skg_ctx *c, int slot, int which, double work);
+void skg_prof_flush(skg_ctx *c);
 void skg_timer_stop(skg_ctx *c, int slot);
 template <typename T> static inline int dalloc(T **p, size_t count) {
 	if (*p) { (void)hipFree(*p); *p = nullptr; }

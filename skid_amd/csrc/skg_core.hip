@@ -69,6 +69,56 @@ void skg_timer_stop(skg_ctx *c, int slot) {
 
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
+int skg_prof_begin(skg_ctx *c) {
+	if (!c->prof) return -1;
+	if (c->nrec == c->caprec) {
+		int nc = c->caprec ? 2 * c->caprec : 256;
+		skg_ctx::ProfRec *p = (skg_ctx::ProfRec *)realloc(c->prec, nc * sizeof(skg_ctx::ProfRec));
+		if (!p) return -1;
+		for (int i = c->caprec; i < nc; ++i) {
+			if (hipEventCreate(&p[i].a) != hipSuccess || hipEventCreate(&p[i].b) != hipSuccess) { c->prec = p; return -1; }
+		}
+		c->prec = p; c->caprec = nc;
+	}
+	const int slot = c->nrec++;
+	(void)hipEventRecord(c->prec[slot].a, c->st);
+	return slot;
+}
+void skg_prof_end(skg_ctx *c, int slot, int which, double work) {
+	if (slot < 0) return;
+	(void)hipEventRecord(c->prec[slot].b, c->st);
+	c->prec[slot].k = which;
+	c->prec[slot].work = work;
+}
+void skg_prof_flush(skg_ctx *c) {
+	if (!c->nrec) return;
+	(void)hipStreamSynchronize(c->st);
+	for (int i = 0; i < c->nrec; ++i) {
+		float ms = 0.f;
+		(void)hipEventElapsedTime(&ms, c->prec[i].a, c->prec[i].b);
+		const int k = c->prec[i].k;
+		c->kms[k] += ms; c->kwork[k] += c->prec[i].work; c->klaunch[k] += 1;
+	}
+	c->nrec = 0;
+}
+
+extern "C" int skg_set_profiling(skg_ctx *c, int on) {
+	if (!c) return SKG_E_PARAM;
+	skg_prof_flush(c);
+	c->prof = on;
+	for (int i = 0; i < SKG_K_N; ++i) { c->kms[i] = 0; c->kwork[i] = 0; c->klaunch[i] = 0; }
+	return SKG_OK;
+}
+
+extern "C" int skg_kernel_stats(skg_ctx *c, int which, double *ms, long long *launches, double *work) {
+	if (!c || which < 0 || which >= SKG_K_N) return SKG_E_PARAM;
+	skg_prof_flush(c);
+	if (ms) *ms = c->kms[which];
+	if (launches) *launches = c->klaunch[which];
+	if (work) *work = c->kwork[which];
+	return SKG_OK;
+}
+
 // =====================================================================
 // Morton ordering
 // =====================================================================
@@ -359,17 +409,20 @@ __global__ __launch_bounds__(64) void k_cover(
 	const float3 bhi = make_float3(wave_max(p.x), wave_max(p.y), wave_max(p.z));
 	Sum<COMP> s0, s1, s2;
 	s0.init(); s1.init(); s2.init();
+	// A node is opened when its largest live ball reaches the block's box and
+	// at least one lane's own point (so scattered blocks late in the move loop
+	// walk only the union of their lanes' paths).
 	int node = 1;
 	for (;;) {
 		const float4 lo = nlo[node], hi = nhi[node];
-		const bool visit = box_box_d2<false>(lo, hi, blo, bhi, 0.f) < lo.w;
+		const bool need = valid && point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
+		const bool visit = box_box_d2<false>(lo, hi, blo, bhi, 0.f) < lo.w && __any(need);
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit) {
 			const int start = (node - nLeaf2) * B;
 			if (start < ne) {
 				const int cnt = min(B, ne - start);
-				const bool need = valid && point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
-				if (__any(need) && need) {
+				if (need) {
 					for (int t = 0; t < cnt; ++t) {
 						const int e = start + t;
 						const float4 A = epos[e], X = eaux[e];
@@ -739,6 +792,8 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->d_thr); dfree(&c->d_cnt);
 	if (c->tmp) (void)hipFree(c->tmp);
 	dfree(&c->k64a); dfree(&c->k64b); dfree(&c->v32a); dfree(&c->v32b);
+	for (int i = 0; i < c->caprec; ++i) { (void)hipEventDestroy(c->prec[i].a); (void)hipEventDestroy(c->prec[i].b); }
+	free(c->prec);
 	if (c->e0) (void)hipEventDestroy(c->e0);
 	if (c->e1) (void)hipEventDestroy(c->e1);
 	if (c->st) (void)hipStreamDestroy(c->st);
@@ -805,8 +860,9 @@ static void launch_knn(skg_ctx *c, bool per, bool comp) {
 }
 
 static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, int np, float *out,
-		int stamp, float fStep) {
+		int stamp, float fStep, double work) {
 	if (np <= 0) return 0;
+	const int pslot = skg_prof_begin(c);
 	const int nb = nblk(np, 64);
 	const bool comp = c->P.accumulate == SKG_ACC_COMPENSATED;
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
@@ -816,6 +872,7 @@ static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, in
 	if (mode == 0) { if (comp) CL(0, true); else CL(0, false); }
 	else { if (comp) CL(1, true); else CL(1, false); }
 #undef CL
+	skg_prof_end(c, pslot, mode == 0 ? SKG_K_COVER_DENS : SKG_K_COVER_MOVE, work);
 	SKG_HIP(hipGetLastError());
 	return 0;
 }
@@ -859,10 +916,12 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		skg_timer_start(c);
 		if (dalloc(&c->s_heap, (size_t)nS * k)) return SKG_E_NOMEM;
 		const bool comp = c->P.accumulate == SKG_ACC_COMPENSATED;
+		const int pslot = skg_prof_begin(c);
 		if (k <= 32) launch_knn<32>(c, per, comp);
 		else if (k <= 64) launch_knn<64>(c, per, comp);
 		else if (k <= 128) launch_knn<128>(c, per, comp);
 		else launch_knn<256>(c, per, comp);
+		skg_prof_end(c, pslot, SKG_K_KNN, 16.0 * nS);
 		SKG_HIP(hipGetLastError());
 		skg_timer_stop(c, SKG_T_KNN);
 
@@ -890,6 +949,7 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		}
 		c->nRep = nRep;
 		c->nE = nS + nRep;
+		c->nLive = c->nE;
 		const int nE = c->nE;
 		float4 *e4u = nullptr;
 		float *eb2u = nullptr, *eb2s = nullptr;
@@ -913,7 +973,7 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		c->e_b2 = eb2s;
 
 		// scatter half of the density: balls of others holding each scatterer
-		if ((rc = cover_launch(c, 0, c->s_pos, nullptr, nS, c->s_rhoS, 0, 0.f))) return rc;
+		if ((rc = cover_launch(c, 0, c->s_pos, nullptr, nS, c->s_rhoS, 0, 0.f, 24.0 * nS))) return rc;
 		float *rhoS = nullptr;
 		if (dalloc(&rhoS, nS)) return SKG_E_NOMEM;
 		k_combine_rho<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_rhoG, c->s_rhoS, c->s_ball2, c->s_ord, nS, rhoS,
@@ -991,7 +1051,8 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 static int denmax_step(skg_ctx *c, float fStep, int bInitial) {
 	int rc;
 	const int stamp = ++c->step;
-	if ((rc = cover_launch(c, 1, c->m_pos, c->m_list, c->nActive, nullptr, stamp, fStep))) return rc;
+	if ((rc = cover_launch(c, 1, c->m_pos, c->m_list, c->nActive, nullptr, stamp, fStep,
+			24.0 * c->nActive + 24.0 * c->nLive))) return rc;
 	k_step_init<<<1, 64, 0, c->st>>>((unsigned *)c->d_thr, c->d_cnt);
 	k_scat_thr<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->nE, stamp, bInitial,
 			(unsigned *)c->d_thr);
@@ -1037,6 +1098,7 @@ static int prune(skg_ctx *c) {
 static int read_count(skg_ctx *c, int *v) {
 	SKG_HIP(hipMemcpyAsync(v, c->d_cnt, 4, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
+	c->nLive = *v;
 	return 0;
 }
 

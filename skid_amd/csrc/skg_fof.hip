@@ -177,8 +177,10 @@ static int fof_run(skg_ctx *c, const float4 *pos, int n, float tau, const int *o
 	k_zero<<<nblk(2 * t.nLeaf2, 256), 256, 0, c->st>>>(united, 2 * t.nLeaf2);
 	k_fof_init<<<nblk(n, 256), 256, 0, c->st>>>(par, n);
 	const float tau2 = tau * tau;
+	const int pslot = skg_prof_begin(c);
 	if (per) k_fof_links<true><<<nblk(n, 64), 64, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2, par, united);
 	else k_fof_links<false><<<nblk(n, 64), 64, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2, par, united);
+	skg_prof_end(c, pslot, SKG_K_FOF, 16.0 * n);
 	k_fill<<<nblk(n, 256), 256, 0, c->st>>>(cmin, n, INT_MAX);
 	k_zero<<<nblk(n, 256), 256, 0, c->st>>>(isFirst, n);
 	k_fof_root_min<<<nblk(n, 256), 256, 0, c->st>>>(par, n, perm, cmin);

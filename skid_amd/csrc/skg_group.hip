@@ -11,9 +11,11 @@
 // as uniform loads); the energies, the per-group centre-of-mass updates and
 // the bound/unbound sign test run in fp64.
 #include "skg_common.h"
+#include <hip/hip_cooperative_groups.h>
 #include <algorithm>
 #include <cfloat>
 #include <climits>
+#include <cstdlib>
 #include <vector>
 
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
@@ -185,26 +187,568 @@ __global__ void k_unb_setup(const int *__restrict__ gs, const int *__restrict__ 
 	(void)nMax;
 }
 
-// phi_i = sum_{j != i} G m_j K(r_ij; eps_i + eps_j) over the member's group
-// (kdCellPot, grav.c:8-36; both orders of each pair evaluated, no atomics).
-__global__ __launch_bounds__(64) void k_unb_pot(const int2 *__restrict__ tiles, const int *__restrict__ gs,
+// phi_i += sum_{j != i} G m_j K(r_ij; eps_i + eps_j) over the member's group
+// (kdCellPot, grav.c:8-36): a workgroup holds 256 targets in registers and
+// streams the group's members through LDS in tiles of 256 (both orders of each
+// pair are evaluated; no atomics, fixed summation order).
+#define POT_T 256
+__global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tiles, const int *__restrict__ gs,
 		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv,
 		float2 *uphi, float G, int soft) {
+	__shared__ float4 sq[POT_T];
+	__shared__ float se[POT_T];
 	const int2 tl = tiles[blockIdx.x];   // (group, first target)
 	const int g = tl.x, s = gs[g], e = ge[g];
 	const int i = tl.y + threadIdx.x;
 	const bool valid = i < e;
-	const float4 qi = uq[valid ? i : s], vi = uv[valid ? i : s];
+	const float4 qi = uq[valid ? i : s];
+	const float ei = uv[valid ? i : s].w;
 	Acc acc;
 	acc.init();
-	for (int j = s; j < e; ++j) {
-		const float4 qj = uq[j];
-		const float ej = uv[j].w;
-		const float dx = fsub(qi.x, qj.x), dy = fsub(qi.y, qj.y), dz = fsub(qi.z, qj.z);
-		const float k = pot_kernel(soft, sq3(dx, dy, dz), fadd(vi.w, ej));
-		if (j != i) acc.add(fmul(fmul(G, qj.w), k));
+	for (int j0 = s; j0 < e; j0 += POT_T) {
+		const int nt = min(POT_T, e - j0);
+		__syncthreads();
+		if (threadIdx.x < nt) { sq[threadIdx.x] = uq[j0 + threadIdx.x]; se[threadIdx.x] = uv[j0 + threadIdx.x].w; }
+		__syncthreads();
+		for (int t = 0; t < nt; ++t) {
+			const float4 qj = sq[t];
+			const float dx = fsub(qi.x, qj.x), dy = fsub(qi.y, qj.y), dz = fsub(qi.z, qj.z);
+			const float k = pot_kernel(soft, sq3(dx, dy, dz), fadd(ei, se[t]));
+			if (j0 + t != i) acc.add(fmul(fmul(G, qj.w), k));
+		}
 	}
-	if (valid) uphi[i] = make_float2(acc.s, acc.c);
+	if (valid) {
+		float2 ph = uphi[i];
+		Acc a; a.s = ph.x; a.c = ph.y;
+		a.add(acc.s); a.add(acc.c);
+		uphi[i] = make_float2(a.s, a.c);
+	}
+}
+
+// Groups of at most 64 members: one wavefront per group, member i in lane i.
+// Potential by shuffles, then the removal loop entirely in registers
+// (kd.c:1360-1457); energies, centre of mass and the sign test in fp64.
+__global__ __launch_bounds__(64) void k_unb_small(const int *__restrict__ glist, const int *__restrict__ gs,
+		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv,
+		const float2 *__restrict__ uphi, const int *__restrict__ uord, int *grp, int *gcount, float *gmass,
+		float4 *gvcm, float4 *grbound, const float4 *__restrict__ grel, float fShift, float fCosmo, float z,
+		float G, int soft, int bSub, int bNoUnbind, float hx, int *nunb, unsigned long long *npairs) {
+	const int g = glist[blockIdx.x], lane = threadIdx.x;
+	unsigned long long pairs = 0;
+	const int s = gs[g], n0 = ge[g] - s;
+	const bool mine = lane < n0;
+	const float4 q = uq[s + (mine ? lane : 0)], v = uv[s + (mine ? lane : 0)];
+	const float2 ph0 = uphi[s + (mine ? lane : 0)];
+	Acc phi; phi.s = ph0.x; phi.c = ph0.y;
+	for (int j = 0; j < n0; ++j) {
+		const float qx = __shfl(q.x, j, 64), qy = __shfl(q.y, j, 64), qz = __shfl(q.z, j, 64);
+		const float qm = __shfl(q.w, j, 64), qe = __shfl(v.w, j, 64);
+		const float dx = fsub(q.x, qx), dy = fsub(q.y, qy), dz = fsub(q.z, qz);
+		const float k = pot_kernel(soft, sq3(dx, dy, dz), fadd(v.w, qe));
+		if (mine && j != lane) phi.add(fmul(fmul(G, qm), k));
+	}
+	double cm[7];
+	{
+		double a[7] = {mine ? (double)q.w : 0.0,
+			mine ? (double)fmul(q.w, q.x) : 0.0, mine ? (double)fmul(q.w, q.y) : 0.0, mine ? (double)fmul(q.w, q.z) : 0.0,
+			mine ? (double)fmul(q.w, v.x) : 0.0, mine ? (double)fmul(q.w, v.y) : 0.0, mine ? (double)fmul(q.w, v.z) : 0.0};
+		for (int k = 0; k < 7; ++k) {
+			for (int o = 32; o > 0; o >>= 1) a[k] += __shfl_xor(a[k], o, 64);
+			cm[k] = a[k];
+		}
+		for (int k = 1; k < 7; ++k) cm[k] /= cm[0];
+	}
+	bool alive = mine;
+	int n = n0, removed = 0, imin = 0;
+	const double zf = 1.0 + (double)z;
+	for (;;) {
+		double E = -INFINITY;
+		if (alive) {
+			const double d0 = (double)fShift * ((double)v.x - cm[4]) + (double)fCosmo * ((double)q.x - cm[1]);
+			const double d1 = (double)fShift * ((double)v.y - cm[5]) + (double)fCosmo * ((double)q.y - cm[2]);
+			const double d2 = (double)fShift * ((double)v.z - cm[6]) + (double)fCosmo * ((double)q.z - cm[3]);
+			E = 0.5 * (d0 * d0 + d1 * d1 + d2 * d2) - ((double)phi.s + (double)phi.c) * zf;
+		}
+		double bE = E, mE = alive ? E : INFINITY;
+		int bi = alive ? lane : 64, mi = alive ? lane : 64;
+		for (int o = 32; o > 0; o >>= 1) {
+			const double oE = __shfl_xor(bE, o, 64), omE = __shfl_xor(mE, o, 64);
+			const int oi = __shfl_xor(bi, o, 64), omi = __shfl_xor(mi, o, 64);
+			if (oE > bE || (oE == bE && oi < bi)) { bE = oE; bi = oi; }
+			if (omE < mE || (omE == mE && omi < mi)) { mE = omE; mi = omi; }
+		}
+		imin = mi;
+		if (!(bE >= 0.0) || bNoUnbind || n == 0) break;
+		const float bx = __shfl(q.x, bi, 64), by = __shfl(q.y, bi, 64), bz = __shfl(q.z, bi, 64);
+		const float bm = __shfl(q.w, bi, 64), be = __shfl(v.w, bi, 64);
+		const float bvx = __shfl(v.x, bi, 64), bvy = __shfl(v.y, bi, 64), bvz = __shfl(v.z, bi, 64);
+		if (lane == bi) { alive = false; grp[uord[s + lane]] = 0; }
+		--n; ++removed;
+		if (n == 0) { cm[0] = 0.0; cm[4] = cm[5] = cm[6] = 0.0; break; }
+		cm[0] -= (double)bm;
+		const double f = (double)bm / cm[0];
+		cm[1] += f * (cm[1] - (double)bx); cm[2] += f * (cm[2] - (double)by); cm[3] += f * (cm[3] - (double)bz);
+		cm[4] += f * (cm[4] - (double)bvx); cm[5] += f * (cm[5] - (double)bvy); cm[6] += f * (cm[6] - (double)bvz);
+		if (bSub) pairs += n;
+		if (bSub && alive) {
+			const float dx = fsub(bx, q.x), dy = fsub(by, q.y), dz = fsub(bz, q.z);
+			phi.add(-fmul(fmul(G, bm), pot_kernel(soft, sq3(dx, dy, dz), fadd(be, v.w))));
+		}
+	}
+	const float rbx = __shfl(q.x, imin < 64 ? imin : 0, 64), rby = __shfl(q.y, imin < 64 ? imin : 0, 64);
+	const float rbz = __shfl(q.z, imin < 64 ? imin : 0, 64);
+	if (lane == 0) {
+		gcount[g] = n;
+		gmass[g] = (float)cm[0];
+		gvcm[g] = make_float4((float)cm[4], (float)cm[5], (float)cm[6], 0.f);
+		const float4 r = grel[g];
+		float b[3] = {fadd(rbx, r.x), fadd(rby, r.y), fadd(rbz, r.z)};
+		for (int j = 0; j < 3; ++j) {                                    // hx on all axes (kd.c:1452-1456)
+			if (b[j] > hx) b[j] = fsub(b[j], 2 * hx);
+			if (b[j] <= -hx) b[j] = fadd(b[j], 2 * hx);
+		}
+		grbound[g] = make_float4(b[0], b[1], b[2], 0.f);
+		atomicAdd(nunb, removed);
+		atomicAdd(npairs, pairs);
+	}
+}
+
+// All large groups at once (one cooperative launch): group gi owns
+// workgroups [wg0[gi], wg0[gi]+nwgG[gi]), each workgroup a fixed slice of the
+// group's members.  Per removal round, one grid barrier publishes the slice
+// maxima (double-buffered per round); each workgroup of a group reduces its
+// group's maxima identically and applies the removal to its own slice
+// (kd.c:1385-1447).  A group that has finished idles; all workgroups leave
+// together once the count of unfinished groups, read right after a barrier,
+// is zero.
+#define COOP_T 256
+__global__ __launch_bounds__(COOP_T) void k_unb_coop(const int *__restrict__ bg, const int *__restrict__ wg0,
+		const int *__restrict__ nwgG, const int *__restrict__ wgGroup, const int *__restrict__ gs,
+		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv, float2 *uphi,
+		int *uord, int *grp, int *gcount, float *gmass, float4 *gvcm, float4 *grbound,
+		const float4 *__restrict__ grel, float fShift, float fCosmo, float z, float G, int soft, int bSub,
+		int bNoUnbind, float hx, int *nunb, double *part /* 2 * gridDim * 4 */,
+		const double *__restrict__ cmInit /* 8 per big group */, int *active, unsigned long long *npairs) {
+	namespace cg = cooperative_groups;
+	cg::grid_group grid = cg::this_grid();
+	__shared__ double rE[COOP_T], rM[COOP_T];
+	__shared__ int rI[COOP_T], rJ[COOP_T];
+	__shared__ int sBig, sMin;
+	__shared__ double sBigE;
+	const int tid = threadIdx.x, nwgAll = gridDim.x, wg = blockIdx.x;
+	const int gi = wgGroup[wg], g = bg[gi], lw = wg - wg0[gi], nwg = nwgG[gi];
+	const int s = gs[g], n0 = ge[g] - s;
+	const int chunk = (n0 + nwg - 1) / nwg;
+	const int a0 = min(n0, lw * chunk), a1 = min(n0, a0 + chunk);
+	double cm[7];
+	for (int k = 0; k < 7; ++k) cm[k] = cmInit[8 * gi + k];
+	const double zf = 1.0 + (double)z;
+	int n = n0, removed = 0, prev = -1, imin = 0;
+	bool done = false;
+	unsigned long long pairs = 0;
+	for (int iter = 0;; ++iter) {
+		double *P = part + (size_t)(iter & 1) * nwgAll * 4;
+		if (!done) {
+			float4 pq = make_float4(0.f, 0.f, 0.f, 0.f);
+			float pe = 0.f;
+			if (prev >= 0) { pq = uq[s + prev]; pe = uv[s + prev].w; }
+			double bE = -INFINITY, mE = INFINITY;
+			int bi = INT_MAX, mi = INT_MAX;
+			for (int i = a0 + tid; i < a1; i += COOP_T) {
+				if (uord[s + i] < 0) continue;
+				const float4 q = uq[s + i], v = uv[s + i];
+				float2 ph = uphi[s + i];
+				if (prev >= 0 && bSub) {                              // kdSubPot (grav.c:39-60)
+					const float dx = fsub(pq.x, q.x), dy = fsub(pq.y, q.y), dz = fsub(pq.z, q.z);
+					Acc ac; ac.s = ph.x; ac.c = ph.y;
+					ac.add(-fmul(fmul(G, pq.w), pot_kernel(soft, sq3(dx, dy, dz), fadd(pe, v.w))));
+					ph = make_float2(ac.s, ac.c);
+					uphi[s + i] = ph;
+				}
+				const double d0 = (double)fShift * ((double)v.x - cm[4]) + (double)fCosmo * ((double)q.x - cm[1]);
+				const double d1 = (double)fShift * ((double)v.y - cm[5]) + (double)fCosmo * ((double)q.y - cm[2]);
+				const double d2 = (double)fShift * ((double)v.z - cm[6]) + (double)fCosmo * ((double)q.z - cm[3]);
+				const double E = 0.5 * (d0 * d0 + d1 * d1 + d2 * d2) - ((double)ph.x + (double)ph.y) * zf;
+				if (E > bE || (E == bE && i < bi)) { bE = E; bi = i; }
+				if (E < mE || (E == mE && i < mi)) { mE = E; mi = i; }
+			}
+			rE[tid] = bE; rI[tid] = bi; rM[tid] = mE; rJ[tid] = mi;
+			__syncthreads();
+			for (int o = COOP_T / 2; o > 0; o >>= 1) {
+				if (tid < o) {
+					if (rE[tid + o] > rE[tid] || (rE[tid + o] == rE[tid] && rI[tid + o] < rI[tid])) { rE[tid] = rE[tid + o]; rI[tid] = rI[tid + o]; }
+					if (rM[tid + o] < rM[tid] || (rM[tid + o] == rM[tid] && rJ[tid + o] < rJ[tid])) { rM[tid] = rM[tid + o]; rJ[tid] = rJ[tid + o]; }
+				}
+				__syncthreads();
+			}
+			if (tid == 0) {
+				P[4 * wg] = rE[0]; P[4 * wg + 1] = (double)rI[0]; P[4 * wg + 2] = rM[0]; P[4 * wg + 3] = (double)rJ[0];
+			}
+			__threadfence();
+		}
+		grid.sync();
+		if (!done) {
+			// the group's workgroups reduce its slice maxima identically
+			double BE = -INFINITY, ME = INFINITY;
+			int BI = INT_MAX, MI = INT_MAX;
+			for (int w = tid; w < nwg; w += COOP_T) {
+				const int ww = wg0[gi] + w;
+				const double e1 = __hip_atomic_load(&P[4 * ww], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				const double e2 = __hip_atomic_load(&P[4 * ww + 2], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				const int i1 = (int)__hip_atomic_load(&P[4 * ww + 1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				const int i2 = (int)__hip_atomic_load(&P[4 * ww + 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+				if (e1 > BE || (e1 == BE && i1 < BI)) { BE = e1; BI = i1; }
+				if (e2 < ME || (e2 == ME && i2 < MI)) { ME = e2; MI = i2; }
+			}
+			rE[tid] = BE; rI[tid] = BI; rM[tid] = ME; rJ[tid] = MI;
+			__syncthreads();
+			for (int o = COOP_T / 2; o > 0; o >>= 1) {
+				if (tid < o) {
+					if (rE[tid + o] > rE[tid] || (rE[tid + o] == rE[tid] && rI[tid + o] < rI[tid])) { rE[tid] = rE[tid + o]; rI[tid] = rI[tid + o]; }
+					if (rM[tid + o] < rM[tid] || (rM[tid + o] == rM[tid] && rJ[tid + o] < rJ[tid])) { rM[tid] = rM[tid + o]; rJ[tid] = rJ[tid + o]; }
+				}
+				__syncthreads();
+			}
+			if (tid == 0) { sBig = rI[0]; sBigE = rE[0]; sMin = rJ[0]; }
+			__syncthreads();
+		}
+		const int ib = sBig;
+		bool stop = done;
+		if (!done) {
+		imin = sMin;
+		stop = !(sBigE >= 0.0) || bNoUnbind || n == 0 || ib == INT_MAX;
+		if (!stop) {
+			const float4 qb = uq[s + ib], vb = uv[s + ib];
+			if (ib >= a0 && ib < a1 && tid == 0) {
+				const int ob = uord[s + ib];
+				grp[ob] = 0;
+				uord[s + ib] = ~ob;
+			}
+			--n; ++removed;
+			if (n == 0) {
+				cm[0] = 0.0; cm[4] = cm[5] = cm[6] = 0.0;
+				stop = true;
+			} else {
+				cm[0] -= (double)qb.w;
+				const double f = (double)qb.w / cm[0];
+				cm[1] += f * (cm[1] - (double)qb.x); cm[2] += f * (cm[2] - (double)qb.y); cm[3] += f * (cm[3] - (double)qb.z);
+				cm[4] += f * (cm[4] - (double)vb.x); cm[5] += f * (cm[5] - (double)vb.y); cm[6] += f * (cm[6] - (double)vb.z);
+				prev = ib;
+				if (bSub) pairs += n;
+			}
+		}
+		}
+		if (stop && !done) {
+			done = true;
+			if (lw == 0 && tid == 0) {
+				gcount[g] = n;
+				gmass[g] = (float)cm[0];
+				gvcm[g] = make_float4((float)cm[4], (float)cm[5], (float)cm[6], 0.f);
+				const float4 q = uq[s + (imin < n0 ? imin : 0)], r = grel[g];
+				float b[3] = {fadd(q.x, r.x), fadd(q.y, r.y), fadd(q.z, r.z)};
+				for (int j = 0; j < 3; ++j) {
+					if (b[j] > hx) b[j] = fsub(b[j], 2 * hx);
+					if (b[j] <= -hx) b[j] = fadd(b[j], 2 * hx);
+				}
+				grbound[g] = make_float4(b[0], b[1], b[2], 0.f);
+				atomicAdd(nunb, removed);
+				atomicAdd(npairs, pairs);
+				__threadfence();
+				atomicSub(active, 1);
+			}
+		}
+		__syncthreads();
+		// Termination is read only between two barriers: no group finishes
+		// (decrements) between this barrier and the next round's, so every
+		// workgroup sees the same count and they all leave on the same round.
+		if ((iter & 15) == 15) {
+			grid.sync();
+			if (__hip_atomic_load(active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+		}
+	}
+}
+
+// Exact sequential removal with a bounded candidate set, one workgroup per
+// group (kd.c:1385-1447).  With dv_i = u_i - w (u_i = fShift v_i + fCosmo r_i,
+// w = fShift vcm + fCosmo rcm), E_i = |u_i - w|^2/2 - phi_i (1+z).  A full pass
+// at w0 keeps each thread's CT most energetic members as candidates and
+// records, over all other members, the largest energy E_cut and the largest
+// |u_i - w0| = R.  Until the next full pass every non-candidate satisfies
+//   E_i(w) <= E_cut + R |w - w0| + |w - w0|^2/2 + growth,
+// where growth sums the largest possible potential drop (1+z) G m_b K_max of
+// each removal (kdSubPot).  While the best candidate beats that bound strictly
+// it is the global argmax and is removed without touching other members; the
+// potentials of non-candidates catch up at the next full pass.  The sequence
+// of removals is therefore the reference's (ties aside).
+#define CAND_T 256
+#define CT 8
+__device__ __forceinline__ float kernel_max(int soft, float twoh) {
+	return soft == SKG_PLUMMER ? 2.0f / twoh : 2.8f / twoh;     // K(0): spline 1.4*2/twoh, Plummer 2/twoh
+}
+__global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ glist, const int *__restrict__ gs,
+		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv, float2 *uphi,
+		int *uord, int *grp, int *gcount, float *gmass, float4 *gvcm, float4 *grbound,
+		const float4 *__restrict__ grel, float fShift, float fCosmo, float z, float G, int soft, int bSub,
+		int bNoUnbind, float hx, int *nunb, unsigned long long *npairs, unsigned long long *nfull) {
+	__shared__ double rA[CAND_T], rB[CAND_T];
+	__shared__ int rI[CAND_T], rJ[CAND_T];
+	__shared__ int pend[CAND_T * CT];
+	__shared__ float4 bq, bv;
+	__shared__ double cmS[7];
+	const int g = glist[blockIdx.x], tid = threadIdx.x;
+	const int s = gs[g], n0 = ge[g] - s;
+	const double zf = 1.0 + (double)z;
+	// fp64 mass, centre of mass and velocity (kd.c:1360-1375)
+	{
+		double a[7] = {0, 0, 0, 0, 0, 0, 0};
+		for (int i = tid; i < n0; i += CAND_T) {
+			const float4 q = uq[s + i], v = uv[s + i];
+			a[0] += (double)q.w;
+			a[1] += (double)fmul(q.w, q.x); a[2] += (double)fmul(q.w, q.y); a[3] += (double)fmul(q.w, q.z);
+			a[4] += (double)fmul(q.w, v.x); a[5] += (double)fmul(q.w, v.y); a[6] += (double)fmul(q.w, v.z);
+		}
+		for (int k = 0; k < 7; ++k) {
+			rA[tid] = a[k];
+			__syncthreads();
+			for (int o = CAND_T / 2; o > 0; o >>= 1) { if (tid < o) rA[tid] += rA[tid + o]; __syncthreads(); }
+			if (tid == 0) cmS[k] = rA[0];
+			__syncthreads();
+		}
+		if (tid == 0) for (int k = 1; k < 7; ++k) cmS[k] /= cmS[0];
+		__syncthreads();
+	}
+	double cm[7];
+	for (int k = 0; k < 7; ++k) cm[k] = cmS[k];
+	int cI[CT];
+	float4 cQ[CT], cV[CT];
+	double cP[CT];
+	int n = n0, removed = 0, npend = 0, imin = 0;
+	unsigned long long pairs = 0, fulls = 0;
+	double w0x = 0, w0y = 0, w0z = 0, Ecut = -INFINITY, Rmax = 0, grow = 0;
+	float epsMin = 0.f;
+	bool needFull = true, stop = false, lastWasFull = false;
+	while (!stop) {
+		if (needFull) {
+			++fulls;
+			// potentials catch up with the removals since the last pass (kdSubPot, grav.c:39-60)
+			const double wx = (double)fShift * cm[4] + (double)fCosmo * cm[1];
+			const double wy = (double)fShift * cm[5] + (double)fCosmo * cm[2];
+			const double wz = (double)fShift * cm[6] + (double)fCosmo * cm[3];
+			double eCut = -INFINITY, rMax = 0.0, mE = INFINITY;
+			int mi = INT_MAX;
+			float eMin = INFINITY;
+			double cE[CT];
+			#pragma unroll
+			for (int c = 0; c < CT; ++c) { cI[c] = -1; cE[c] = -INFINITY; }
+			for (int i = tid; i < n0; i += CAND_T) {
+				if (uord[s + i] < 0) continue;
+				const float4 q = uq[s + i], v = uv[s + i];
+				float2 ph = uphi[s + i];
+				if (bSub && npend) {
+					Acc ac; ac.s = ph.x; ac.c = ph.y;
+					for (int k = 0; k < npend; ++k) {
+						const int b = pend[k];
+						const float4 qb2 = uq[s + b];
+						const float eb = uv[s + b].w;
+						const float dx = fsub(qb2.x, q.x), dy = fsub(qb2.y, q.y), dz = fsub(qb2.z, q.z);
+						ac.add(-fmul(fmul(G, qb2.w), pot_kernel(soft, sq3(dx, dy, dz), fadd(eb, v.w))));
+					}
+					ph = make_float2(ac.s, ac.c);
+					uphi[s + i] = ph;
+				}
+				const double d0 = (double)fShift * ((double)v.x - cm[4]) + (double)fCosmo * ((double)q.x - cm[1]);
+				const double d1 = (double)fShift * ((double)v.y - cm[5]) + (double)fCosmo * ((double)q.y - cm[2]);
+				const double d2 = (double)fShift * ((double)v.z - cm[6]) + (double)fCosmo * ((double)q.z - cm[3]);
+				const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+				const double phi = (double)ph.x + (double)ph.y;
+				const double E = 0.5 * dd - phi * zf;
+				if (E < mE || (E == mE && i < mi)) { mE = E; mi = i; }
+				eMin = fminf(eMin, v.w);
+				// keep the CT largest; what is displaced feeds the bound
+				int lo = 0;
+				double loE = cE[0];
+				#pragma unroll
+				for (int c = 1; c < CT; ++c) if (cE[c] < loE) { loE = cE[c]; lo = c; }
+				double outE = E, outR = sqrt(dd);
+				if (E > loE) {
+					#pragma unroll
+					for (int c = 0; c < CT; ++c) {
+						if (c == lo) {
+							if (cI[c] >= 0) {
+								const double ox = (double)fShift * (double)cV[c].x + (double)fCosmo * (double)cQ[c].x - wx;
+								const double oy = (double)fShift * (double)cV[c].y + (double)fCosmo * (double)cQ[c].y - wy;
+								const double oz = (double)fShift * (double)cV[c].z + (double)fCosmo * (double)cQ[c].z - wz;
+								outE = cE[c]; outR = sqrt(ox * ox + oy * oy + oz * oz);
+							} else { outE = -INFINITY; outR = 0.0; }
+							cI[c] = i; cQ[c] = q; cV[c] = v; cP[c] = phi; cE[c] = E;
+						}
+					}
+				}
+				if (outE > eCut) eCut = outE;
+				if (outR > rMax) rMax = outR;
+			}
+			// block reductions: E_cut, R, smallest softening, argmin E
+			rA[tid] = eCut; rB[tid] = mE; rI[tid] = mi; rJ[tid] = __float_as_int(eMin);
+			__syncthreads();
+			for (int o = CAND_T / 2; o > 0; o >>= 1) {
+				if (tid < o) {
+					rA[tid] = fmax(rA[tid], rA[tid + o]);
+					if (rB[tid + o] < rB[tid] || (rB[tid + o] == rB[tid] && rI[tid + o] < rI[tid])) { rB[tid] = rB[tid + o]; rI[tid] = rI[tid + o]; }
+					rJ[tid] = __float_as_int(fminf(__int_as_float(rJ[tid]), __int_as_float(rJ[tid + o])));
+				}
+				__syncthreads();
+			}
+			Ecut = rA[0]; imin = rI[0]; epsMin = __int_as_float(rJ[0]);
+			__syncthreads();
+			rA[tid] = rMax;
+			__syncthreads();
+			for (int o = CAND_T / 2; o > 0; o >>= 1) { if (tid < o) rA[tid] = fmax(rA[tid], rA[tid + o]); __syncthreads(); }
+			Rmax = rA[0];
+			__syncthreads();
+			w0x = wx; w0y = wy; w0z = wz;
+			grow = 0.0; npend = 0; needFull = false; lastWasFull = true;
+			if (bNoUnbind || n == 0) break;
+		}
+		// best candidate at the current centre of mass
+		double bE = -INFINITY;
+		int bi = INT_MAX, bc = -1;
+		#pragma unroll
+		for (int c = 0; c < CT; ++c) {
+			if (cI[c] < 0) continue;
+			const double d0 = (double)fShift * ((double)cV[c].x - cm[4]) + (double)fCosmo * ((double)cQ[c].x - cm[1]);
+			const double d1 = (double)fShift * ((double)cV[c].y - cm[5]) + (double)fCosmo * ((double)cQ[c].y - cm[2]);
+			const double d2 = (double)fShift * ((double)cV[c].z - cm[6]) + (double)fCosmo * ((double)cQ[c].z - cm[3]);
+			const double E = 0.5 * (d0 * d0 + d1 * d1 + d2 * d2) - cP[c] * zf;
+			if (E > bE || (E == bE && cI[c] < bi)) { bE = E; bi = cI[c]; bc = c; }
+		}
+		rA[tid] = bE; rI[tid] = bi;
+		__syncthreads();
+		for (int o = CAND_T / 2; o > 0; o >>= 1) {
+			if (tid < o && (rA[tid + o] > rA[tid] || (rA[tid + o] == rA[tid] && rI[tid + o] < rI[tid]))) {
+				rA[tid] = rA[tid + o]; rI[tid] = rI[tid + o];
+			}
+			__syncthreads();
+		}
+		const double BE = rA[0];
+		const int BI = rI[0];
+		__syncthreads();
+		const double wx = (double)fShift * cm[4] + (double)fCosmo * cm[1];
+		const double wy = (double)fShift * cm[5] + (double)fCosmo * cm[2];
+		const double wz = (double)fShift * cm[6] + (double)fCosmo * cm[3];
+		const double dl = sqrt((wx - w0x) * (wx - w0x) + (wy - w0y) * (wy - w0y) + (wz - w0z) * (wz - w0z));
+		// right after a full pass every energy is exact and BE >= E_cut holds;
+		// later the bound carries a relative slack for fp64 rounding
+		const double slack = 1e-12 * (fabs(Ecut) + Rmax * Rmax + 1.0);
+		const double bound = lastWasFull ? Ecut : Ecut + Rmax * dl + 0.5 * dl * dl + grow + slack;
+		if (!(lastWasFull ? BE >= bound : BE > bound)) {
+			if (BE < 0.0 && bound < 0.0) { stop = true; break; }      // every member is bound
+			needFull = true;
+			continue;
+		}
+		lastWasFull = false;
+		if (BE < 0.0) { stop = true; break; }                         // the global maximum is negative
+		// remove BI (owned by one thread's candidate slot)
+		if (bc >= 0 && bi == BI) {
+			#pragma unroll
+			for (int c = 0; c < CT; ++c) if (c == bc) { bq = cQ[c]; bv = cV[c]; cI[c] = -1; }
+			const int ob = uord[s + BI];
+			grp[ob] = 0;
+			uord[s + BI] = ~ob;
+		}
+		__syncthreads();
+		const float4 qb = bq, vb = bv;
+		--n; ++removed;
+		if (n == 0) { cm[0] = 0.0; cm[4] = cm[5] = cm[6] = 0.0; stop = true; break; }
+		cm[0] -= (double)qb.w;
+		const double f = (double)qb.w / cm[0];
+		cm[1] += f * (cm[1] - (double)qb.x); cm[2] += f * (cm[2] - (double)qb.y); cm[3] += f * (cm[3] - (double)qb.z);
+		cm[4] += f * (cm[4] - (double)vb.x); cm[5] += f * (cm[5] - (double)vb.y); cm[6] += f * (cm[6] - (double)vb.z);
+		if (bSub) {
+			pairs += n;
+			#pragma unroll
+			for (int c = 0; c < CT; ++c) {
+				if (cI[c] < 0) continue;
+				const float dx = fsub(qb.x, cQ[c].x), dy = fsub(qb.y, cQ[c].y), dz = fsub(qb.z, cQ[c].z);
+				cP[c] -= (double)fmul(fmul(G, qb.w), pot_kernel(soft, sq3(dx, dy, dz), fadd(vb.w, cV[c].w)));
+			}
+			grow += zf * (double)fmul(fmul(G, qb.w), kernel_max(soft, fadd(vb.w, epsMin)));
+			if (tid == 0) pend[npend] = BI;
+			++npend;
+			if (npend >= CAND_T * CT) needFull = true;
+		}
+		__syncthreads();
+	}
+	// iMin over all members in the final state (kd.c:1400-1408 of the last pass)
+	if (!lastWasFull && n > 0) {
+		double mE = INFINITY;
+		int mi = INT_MAX;
+		for (int i = tid; i < n0; i += CAND_T) {
+			if (uord[s + i] < 0) continue;
+			const float4 q = uq[s + i], v = uv[s + i];
+			const float2 ph = uphi[s + i];
+			double phi = (double)ph.x + (double)ph.y;
+			if (bSub)
+				for (int k = 0; k < npend; ++k) {
+					const int b = pend[k];
+					const float4 qb2 = uq[s + b];
+					const float dx = fsub(qb2.x, q.x), dy = fsub(qb2.y, q.y), dz = fsub(qb2.z, q.z);
+					phi -= (double)fmul(fmul(G, qb2.w), pot_kernel(soft, sq3(dx, dy, dz), fadd(uv[s + b].w, v.w)));
+				}
+			const double d0 = (double)fShift * ((double)v.x - cm[4]) + (double)fCosmo * ((double)q.x - cm[1]);
+			const double d1 = (double)fShift * ((double)v.y - cm[5]) + (double)fCosmo * ((double)q.y - cm[2]);
+			const double d2 = (double)fShift * ((double)v.z - cm[6]) + (double)fCosmo * ((double)q.z - cm[3]);
+			const double E = 0.5 * (d0 * d0 + d1 * d1 + d2 * d2) - phi * zf;
+			if (E < mE || (E == mE && i < mi)) { mE = E; mi = i; }
+		}
+		rB[tid] = mE; rI[tid] = mi;
+		__syncthreads();
+		for (int o = CAND_T / 2; o > 0; o >>= 1) {
+			if (tid < o && (rB[tid + o] < rB[tid] || (rB[tid + o] == rB[tid] && rI[tid + o] < rI[tid]))) {
+				rB[tid] = rB[tid + o]; rI[tid] = rI[tid + o];
+			}
+			__syncthreads();
+		}
+		imin = rI[0];
+	}
+	if (tid == 0) {
+		gcount[g] = n;
+		gmass[g] = (float)cm[0];
+		gvcm[g] = make_float4((float)cm[4], (float)cm[5], (float)cm[6], 0.f);
+		const float4 q = uq[s + (imin < n0 ? imin : 0)], r = grel[g];
+		float b[3] = {fadd(q.x, r.x), fadd(q.y, r.y), fadd(q.z, r.z)};
+		for (int j = 0; j < 3; ++j) {
+			if (b[j] > hx) b[j] = fsub(b[j], 2 * hx);
+			if (b[j] <= -hx) b[j] = fadd(b[j], 2 * hx);
+		}
+		grbound[g] = make_float4(b[0], b[1], b[2], 0.f);
+		atomicAdd(nunb, removed);
+		atomicAdd(npairs, pairs);
+		atomicAdd(nfull, fulls);
+	}
+}
+
+// Mass, centre of mass and velocity of one group in fp64 (kd.c:1360-1375).
+__global__ __launch_bounds__(256) void k_group_cm(const int *__restrict__ bg, const int *__restrict__ gs,
+		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv, double *outAll) {
+	__shared__ double sd[256];
+	const int g = bg[blockIdx.x];
+	double *out = outAll + 8 * blockIdx.x;
+	const int s = gs[g], n = ge[g] - s, tid = threadIdx.x;
+	double a[7] = {0, 0, 0, 0, 0, 0, 0};
+	for (int i = tid; i < n; i += 256) {
+		const float4 q = uq[s + i], v = uv[s + i];
+		a[0] += (double)q.w;
+		a[1] += (double)fmul(q.w, q.x); a[2] += (double)fmul(q.w, q.y); a[3] += (double)fmul(q.w, q.z);
+		a[4] += (double)fmul(q.w, v.x); a[5] += (double)fmul(q.w, v.y); a[6] += (double)fmul(q.w, v.z);
+	}
+	for (int k = 0; k < 7; ++k) {
+		sd[tid] = a[k];
+		__syncthreads();
+		for (int o = 128; o > 0; o >>= 1) { if (tid < o) sd[tid] += sd[tid + o]; __syncthreads(); }
+		if (tid == 0) out[k] = k == 0 ? sd[0] : sd[0];
+		__syncthreads();
+	}
+	if (tid == 0) for (int k = 1; k < 7; ++k) out[k] /= out[0];
 }
 
 // Scoop: group-0 particles within fScoop of the group centre add their
@@ -281,7 +825,8 @@ __global__ __launch_bounds__(64) void k_unb_scoop(const int *__restrict__ gs, co
 __global__ __launch_bounds__(UNB_T) void k_unb_loop(const int *__restrict__ glist, const int *__restrict__ gs,
 		const int *__restrict__ ge, float4 *uq, float4 *uv, float2 *uphi, int *uord, int *grp,
 		int *gcount, float *gmass, float4 *gvcm, float4 *grbound, const float4 *__restrict__ grel,
-		float fShift, float fCosmo, float z, float G, int soft, int bSub, int bNoUnbind, float hx, int *nunb) {
+		float fShift, float fCosmo, float z, float G, int soft, int bSub, int bNoUnbind, float hx, int *nunb,
+		unsigned long long *npairs) {
 	__shared__ double sd[UNB_T];
 	__shared__ int si[UNB_T];
 	__shared__ double smin[UNB_T];
@@ -368,6 +913,7 @@ __global__ __launch_bounds__(UNB_T) void k_unb_loop(const int *__restrict__ glis
 		}
 		__syncthreads();
 		if (n == 0) { if (tid == 0) smn = 0; break; }
+		if (bSub && tid == 0) atomicAdd(npairs, (unsigned long long)n);
 		if (bSub) {                                                    // kdSubPot (grav.c:39-60)
 			for (int i = tid; i < n; i += UNB_T) {
 				const float4 q = uq[s + i];
@@ -513,25 +1059,47 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		return SKG_E_NOMEM;
 	const float *L = c->P.fPeriod;
 	const float hx = 0.5f * L[0], hy = 0.5f * L[1], hz = 0.5f * L[2];
+	// groups by size class: <= 64 members one wavefront each, up to BIG one
+	// workgroup each, larger ones one cooperative launch each
+	// SKG_UNBIND_COOP_MIN overrides the size from which a group is unbound by
+	// the cooperative kernel (tests use it to drive that path on small boxes).
+	// Groups of at most SMALL members: one wavefront each.  Larger: the bounded
+	// candidate kernel, one workgroup each.  SKG_UNBIND_COOP_MIN routes groups
+	// from that size to the cooperative all-GPU kernel instead, and
+	// SKG_UNBIND_SMALL_MAX lowers SMALL (tests drive both paths with these).
+	int BIG = INT_MAX, SMALL = 64;
+	if (const char *e = getenv("SKG_UNBIND_COOP_MIN")) BIG = std::max(65, atoi(e));
+	if (const char *e = getenv("SKG_UNBIND_SMALL_MAX")) SMALL = std::min(64, std::max(1, atoi(e)));
 	std::vector<int2> tiles;
-	std::vector<int> glist;
+	std::vector<int> gsmall, gmed, gbig;
 	for (int g = 1; g < nG; ++g) {
 		const int m = he[g] - hs[g];
 		if (m <= 0 || m >= nMax) continue;
-		glist.push_back(g);
-		for (int t = hs[g]; t < he[g]; t += 64) tiles.push_back(make_int2(g, t));
+		if (m <= SMALL) { gsmall.push_back(g); continue; }
+		(m < BIG ? gmed : gbig).push_back(g);
+		for (int t = hs[g]; t < he[g]; t += POT_T) tiles.push_back(make_int2(g, t));
+	}
+	if (gbig.size() > 128) {   // keep the 128 largest cooperative; the rest one workgroup each
+		std::sort(gbig.begin(), gbig.end(), [&](int a, int b) { return he[a] - hs[a] > he[b] - hs[b]; });
+		gmed.insert(gmed.end(), gbig.begin() + 128, gbig.end());
+		gbig.resize(128);
 	}
 	SKG_HIP(hipMemsetAsync(G->d_nunb, 0, 4, c->st));
-	if (!glist.empty()) {
+	const bool any = !gsmall.empty() || !gmed.empty() || !gbig.empty();
+	if (any) {
 		int2 *dt = nullptr;
-		int *dg = nullptr;
-		if (dalloc(&dt, tiles.size()) || dalloc(&dg, glist.size())) return SKG_E_NOMEM;
-		SKG_HIP(hipMemcpyAsync(dt, tiles.data(), 8 * tiles.size(), hipMemcpyHostToDevice, c->st));
-		SKG_HIP(hipMemcpyAsync(dg, glist.data(), 4 * glist.size(), hipMemcpyHostToDevice, c->st));
+		int *dgs = nullptr, *dgm = nullptr;
+		double *part = nullptr, *cmv = nullptr;
+		unsigned long long *npair = nullptr;
+		if (dalloc(&dt, std::max<size_t>(1, tiles.size())) || dalloc(&dgs, std::max<size_t>(1, gsmall.size())) ||
+		    dalloc(&dgm, std::max<size_t>(1, gmed.size())) || dalloc(&cmv, 8) || dalloc(&npair, 3))
+			return SKG_E_NOMEM;
+		SKG_HIP(hipMemsetAsync(npair, 0, 24, c->st));
+		if (!tiles.empty()) SKG_HIP(hipMemcpyAsync(dt, tiles.data(), 8 * tiles.size(), hipMemcpyHostToDevice, c->st));
+		if (!gsmall.empty()) SKG_HIP(hipMemcpyAsync(dgs, gsmall.data(), 4 * gsmall.size(), hipMemcpyHostToDevice, c->st));
+		if (!gmed.empty()) SKG_HIP(hipMemcpyAsync(dgm, gmed.data(), 4 * gmed.size(), hipMemcpyHostToDevice, c->st));
 		k_unb_setup<<<nblk(nTot, 256), 256, 0, c->st>>>(G->gstart, G->gend, nG, G->mem, c->d_pos, c->d_vel,
 				c->d_mass, c->d_soft, G->rel, hx, hy, hz, nMax, G->uq, G->uv, G->uphi, G->uord, nTot);
-		k_unb_pot<<<(int)tiles.size(), 64, 0, c->st>>>(dt, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G,
-				c->P.iSoftType);
 		// scoop potential from group 0 (always applied by the reference: kd.c:1379-1384)
 		if (n0 > 0) {
 			float4 *zp = nullptr, *zs = nullptr;
@@ -553,13 +1121,78 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 			dfree(&zp); dfree(&zs); dfree(&perm); dfree(&ze); dfree(&zes);
 			skg_free_tree(&t);
 		}
+		double pairs = 0;
+		for (int gg : gmed) pairs += (double)(he[gg] - hs[gg]) * (he[gg] - hs[gg] - 1);
+		for (int gg : gbig) pairs += (double)(he[gg] - hs[gg]) * (he[gg] - hs[gg] - 1);
+		const int pslot = skg_prof_begin(c);
+		if (!tiles.empty())
+			k_unb_pot<<<(int)tiles.size(), POT_T, 0, c->st>>>(dt, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G,
+					c->P.iSoftType);
+		skg_prof_end(c, pslot, SKG_K_UNB_POT, 20.0 * pairs);
 		const int bSub = (c->inType == SKG_DARK || c->inType == SKG_STAR);   // kd.c:1441
-		k_unb_loop<<<(int)glist.size(), UNB_T, 0, c->st>>>(dg, G->gstart, G->gend, G->uq, G->uv, G->uphi, G->uord,
-				c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel, c->P.fShift, c->P.fCosmo, c->P.z,
-				c->P.G, c->P.iSoftType, bSub, c->P.bNoUnbind, hx, G->d_nunb);
-		SKG_HIP(hipGetLastError());
+		const int lslot = skg_prof_begin(c);
+		if (!gsmall.empty())
+			k_unb_small<<<(int)gsmall.size(), 64, 0, c->st>>>(dgs, G->gstart, G->gend, G->uq, G->uv, G->uphi, G->uord,
+					c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel, c->P.fShift, c->P.fCosmo, c->P.z,
+					c->P.G, c->P.iSoftType, bSub, c->P.bNoUnbind, hx, G->d_nunb, npair);
+		if (!gmed.empty())
+			k_unb_cand<<<(int)gmed.size(), CAND_T, 0, c->st>>>(dgm, G->gstart, G->gend, G->uq, G->uv, G->uphi, G->uord,
+					c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel, c->P.fShift, c->P.fCosmo, c->P.z,
+					c->P.G, c->P.iSoftType, bSub, c->P.bNoUnbind, hx, G->d_nunb, npair, npair + 2);
+		unsigned long long hp[3] = {0, 0, 0};
+		SKG_HIP(hipMemcpyAsync(&hp[0], npair, 8, hipMemcpyDeviceToHost, c->st));
+		SKG_HIP(hipMemcpyAsync(&hp[2], npair + 2, 8, hipMemcpyDeviceToHost, c->st));
 		SKG_HIP(hipStreamSynchronize(c->st));
-		dfree(&dt); dfree(&dg);
+		skg_prof_end(c, lslot, SKG_K_UNB_LOOP, 20.0 * (double)hp[0]);
+		const int bslot = skg_prof_begin(c);
+		if (!gbig.empty()) {
+			int per = 0, ncu = 0;
+			SKG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_unb_coop, COOP_T, 0));
+			SKG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
+			const int nb = (int)gbig.size();
+			int nwgAll = std::max(nb, std::max(1, std::min(per, 2)) * ncu);
+			// workgroups per group in proportion to its members (at least one each)
+			double tot = 0;
+			for (int gg : gbig) tot += he[gg] - hs[gg];
+			std::vector<int> w0(nb), wn(nb), wmap;
+			int used = 0;
+			for (int i = 0; i < nb; ++i) {
+				const double m = he[gbig[i]] - hs[gbig[i]];
+				wn[i] = std::max(1, (int)((nwgAll - nb) * m / tot) + 1);
+				used += wn[i];
+			}
+			nwgAll = used;
+			for (int i = 0, acc = 0; i < nb; ++i) { w0[i] = acc; acc += wn[i]; for (int j = 0; j < wn[i]; ++j) wmap.push_back(i); }
+			int *dbg = nullptr, *dw0 = nullptr, *dwn = nullptr, *dwm = nullptr, *dact = nullptr;
+			double *cmAll = nullptr;
+			if (dalloc(&dbg, nb) || dalloc(&dw0, nb) || dalloc(&dwn, nb) || dalloc(&dwm, nwgAll) || dalloc(&dact, 1) ||
+			    dalloc(&cmAll, 8 * (size_t)nb) || dalloc(&part, 8 * (size_t)nwgAll))
+				return SKG_E_NOMEM;
+			SKG_HIP(hipMemcpyAsync(dbg, gbig.data(), 4 * nb, hipMemcpyHostToDevice, c->st));
+			SKG_HIP(hipMemcpyAsync(dw0, w0.data(), 4 * nb, hipMemcpyHostToDevice, c->st));
+			SKG_HIP(hipMemcpyAsync(dwn, wn.data(), 4 * nb, hipMemcpyHostToDevice, c->st));
+			SKG_HIP(hipMemcpyAsync(dwm, wmap.data(), 4 * (size_t)nwgAll, hipMemcpyHostToDevice, c->st));
+			SKG_HIP(hipMemcpyAsync(dact, &nb, 4, hipMemcpyHostToDevice, c->st));
+			k_group_cm<<<nb, 256, 0, c->st>>>(dbg, G->gstart, G->gend, G->uq, G->uv, cmAll);
+			SKG_HIP(hipStreamSynchronize(c->st));
+			if (nwgAll > std::max(1, per) * ncu) return SKG_E_HIP;   // must be co-resident
+			float fS = c->P.fShift, fC = c->P.fCosmo, zz = c->P.z, GG = c->P.G, hxx = hx;
+			int sft = c->P.iSoftType, sub = bSub, nou = c->P.bNoUnbind;
+			unsigned long long *npair1 = npair + 1;
+			void *args[] = {&dbg, &dw0, &dwn, &dwm, &G->gstart, &G->gend, &G->uq, &G->uv, &G->uphi, &G->uord,
+					&c->d_grp, &G->count, &G->mass, &G->vcm, &G->rbound, &G->rel, &fS, &fC, &zz, &GG, &sft, &sub,
+					&nou, &hxx, &G->d_nunb, &part, &cmAll, &dact, &npair1};
+			SKG_HIP(hipLaunchCooperativeKernel((const void *)k_unb_coop, dim3(nwgAll), dim3(COOP_T), args, 0, c->st));
+			SKG_HIP(hipStreamSynchronize(c->st));
+			dfree(&dbg); dfree(&dw0); dfree(&dwn); dfree(&dwm); dfree(&dact); dfree(&cmAll);
+		}
+		SKG_HIP(hipGetLastError());
+		SKG_HIP(hipMemcpyAsync(&hp[1], npair + 1, 8, hipMemcpyDeviceToHost, c->st));
+		SKG_HIP(hipStreamSynchronize(c->st));
+		skg_prof_end(c, bslot, SKG_K_UNB_BIG, 20.0 * (double)hp[1]);
+		c->nBigGroups = (int)gbig.size();
+		c->nFullPasses = (long long)hp[2];
+		dfree(&dt); dfree(&dgs); dfree(&dgm); dfree(&part); dfree(&cmv); dfree(&npair);
 	}
 	int nu = 0;
 	SKG_HIP(hipMemcpy(&nu, G->d_nunb, 4, hipMemcpyDeviceToHost));

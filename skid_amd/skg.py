@@ -20,6 +20,7 @@ SKG_DARK, SKG_GAS, SKG_STAR = 1, 2, 4
 SKG_PLUMMER, SKG_SPLINE = 1, 2
 SKG_ACC_PLAIN, SKG_ACC_COMPENSATED = 0, 1
 STAGES = ("tree", "knn", "density", "move", "fof", "group", "unbind", "spare")
+KERNELS = ("knn", "cover_density", "cover_move", "fof", "unbind_potential", "unbind_loop", "unbind_big")
 FLT_MAX = 3.4028234663852886e38
 
 
@@ -74,6 +75,9 @@ def lib():
         L.skg_field_gradient.argtypes = [vp, C.c_int, fp, fp]
         L.skg_fof_points.argtypes = [vp, C.c_int, fp, C.c_float, fp, ip]
         L.skg_stage_times.argtypes = [vp, C.POINTER(C.c_double)]
+        L.skg_set_profiling.argtypes = [vp, C.c_int]
+        L.skg_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                                       C.POINTER(C.c_double)]
         _lib = L
     return _lib
 
@@ -240,6 +244,17 @@ class Context:
         t = (C.c_double * 8)()
         _check(self.L.skg_stage_times(self.h, t), "skg_stage_times")
         return {STAGES[i]: t[i] for i in range(8)}
+
+    def set_profiling(self, on: bool = True):
+        _check(self.L.skg_set_profiling(self.h, int(on)), "skg_set_profiling")
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for i, name in enumerate(KERNELS):
+            ms, n, w = C.c_double(), C.c_longlong(), C.c_double()
+            _check(self.L.skg_kernel_stats(self.h, i, C.byref(ms), C.byref(n), C.byref(w)), "skg_kernel_stats")
+            out[name] = dict(ms=ms.value, launches=n.value, work=w.value)
+        return out
 
     def run(self):
         """main.c:374-471 through the GPU stages; returns (trace, groups, nGroup)."""

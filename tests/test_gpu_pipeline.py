@@ -65,3 +65,44 @@ def test_move_trace_and_catalogue_config1(box1):
     assert np.all(np.abs(tr[:5, 0] - otr[:5, 0]) <= 0.002 * otr[0, 0] + 2)
     assert ffrac >= 0.999
     assert frac >= 0.999
+
+
+def test_unbinding_paths_agree(box1):
+    """The three removal kernels -- one wavefront per small group, the bounded
+    candidate kernel (one workgroup per group) and the cooperative all-GPU
+    kernel -- give the same catalogue (SKG_UNBIND_* route groups between them)."""
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, '.');"
+        "from skid_amd import gen, skg;"
+        "b = gen.config1();"
+        "ctx = skg.Context(b, skg.make_params(0.005, 32, period=1.0));"
+        "tr, g, ng = ctx.run(); np.save(sys.argv[1], g)"
+    )
+    outs = []
+    for tag, env_add in (("default", {}), ("cand", {"SKG_UNBIND_SMALL_MAX": "1"}),
+                         ("coop", {"SKG_UNBIND_COOP_MIN": "100"})):
+        env = dict(os.environ)
+        env.update(env_add)
+        path = f"/tmp/skg_unb_{tag}.npy"
+        subprocess.run([sys.executable, "-c", code, path], check=True, env=env, timeout=240,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+        outs.append(np.load(path))
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], outs[2])
+
+
+def test_two_species_catalogue():
+    """Gas+dark with -gd (no potential updates during removal, kd.c:1441)."""
+    b = gen.config3(n_each=8192, seed=5, n_halos=30)
+    tau = 0.001 * (2 ** ((24 - 13) / 3))
+    ctx = skg.Context(b, skg.make_params(tau, 64, period=1.0, bGasAndDark=1))
+    tr, g, ng = ctx.run()
+    o = O.Oracle(b, nSmooth=64, fTau=tau, fPeriod=1.0, bGasAndDark=1)
+    o.run()
+    og, ong = o.groups()
+    frac = O.match_fraction(g, og)
+    print(f"two species: groups gpu {ng - 1} oracle {ong - 1} match {frac:.5f}")
+    assert frac >= 0.999
