@@ -85,6 +85,7 @@ struct skg_ctx {
 	unsigned long long *k64a = nullptr, *k64b = nullptr; size_t kcap = 0;
 	int *v32a = nullptr, *v32b = nullptr;
 	size_t vcap = 0;
+	float *part = nullptr; size_t partCap = 0;   // per-block reduction partials
 };
 
 // ---------------------------------------------------------------- errors
@@ -206,6 +207,7 @@ __device__ __forceinline__ float grad_factor(float d2, float ih2, float gnorm) {
 // ---------------------------------------------------------------- host helpers (skg_core.hip)
 int skg_ensure_tmp(skg_ctx *c, size_t bytes);
 int skg_ensure_keys(skg_ctx *c, size_t n);
+int skg_ensure_part(skg_ctx *c, size_t n);
 int skg_sort_pairs(skg_ctx *c, unsigned long long *kin, unsigned long long *kout, int *vin, int *vout,
 		int n, int endBit);
 int skg_build_tree(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, const float *ball2live);

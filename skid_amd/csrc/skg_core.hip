@@ -27,6 +27,14 @@ int skg_ensure_tmp(skg_ctx *c, size_t bytes) {
 	return 0;
 }
 
+int skg_ensure_part(skg_ctx *c, size_t n) {
+	if (n <= c->partCap && c->part) return 0;
+	size_t cap = std::max<size_t>(n, 4096);
+	if (dalloc(&c->part, cap)) return SKG_E_NOMEM;
+	c->partCap = cap;
+	return 0;
+}
+
 int skg_ensure_keys(skg_ctx *c, size_t n) {
 	if (n <= c->kcap && c->k64a) return 0;
 	size_t cap = std::max<size_t>(n, 1024);
@@ -130,22 +138,42 @@ __device__ __forceinline__ float unord_f(unsigned u) {
 	return __uint_as_float((u & 0x80000000u) ? (u & 0x7FFFFFFFu) : ~u);
 }
 
-__global__ void k_bbox(const float4 *__restrict__ pos, const int *__restrict__ idx, int n, unsigned *bb) {
+// Bounding box of the points: block minima/maxima into partials, then one
+// block folds the partials (no contended atomics).
+__global__ __launch_bounds__(256) void k_bbox(const float4 *__restrict__ pos, const int *__restrict__ idx, int n,
+		float *part) {
+	__shared__ float sm[6][4];
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	float4 p = make_float4(FLT_MAX, FLT_MAX, FLT_MAX, 0.f);
 	float4 q = make_float4(-FLT_MAX, -FLT_MAX, -FLT_MAX, 0.f);
 	if (i < n) { p = pos[idx ? idx[i] : i]; q = p; }
-	float a = wave_min(p.x), b = wave_min(p.y), cz = wave_min(p.z);
-	float d = wave_max(q.x), e = wave_max(q.y), f = wave_max(q.z);
-	if ((threadIdx.x & 63) == 0) {
-		atomicMin(&bb[0], ord_f(a)); atomicMin(&bb[1], ord_f(b)); atomicMin(&bb[2], ord_f(cz));
-		atomicMax(&bb[3], ord_f(d)); atomicMax(&bb[4], ord_f(e)); atomicMax(&bb[5], ord_f(f));
+	float v[6] = {wave_min(p.x), wave_min(p.y), wave_min(p.z), wave_max(q.x), wave_max(q.y), wave_max(q.z)};
+	const int w = threadIdx.x >> 6;
+	if ((threadIdx.x & 63) == 0) for (int k = 0; k < 6; ++k) sm[k][w] = v[k];
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int k = 0; k < 6; ++k) {
+			float r = sm[k][0];
+			for (int j = 1; j < 4; ++j) r = k < 3 ? fminf(r, sm[k][j]) : fmaxf(r, sm[k][j]);
+			part[6 * blockIdx.x + k] = r;
+		}
 	}
 }
-
-__global__ void k_bbox_init(unsigned *bb) {
-	int i = threadIdx.x;
-	if (i < 6) bb[i] = i < 3 ? 0xFFFFFFFFu : 0u;
+__global__ __launch_bounds__(256) void k_bbox_final(const float *__restrict__ part, int nb, unsigned *bb) {
+	float v[6] = {FLT_MAX, FLT_MAX, FLT_MAX, -FLT_MAX, -FLT_MAX, -FLT_MAX};
+	for (int b = threadIdx.x; b < nb; b += 256)
+		for (int k = 0; k < 6; ++k) v[k] = k < 3 ? fminf(v[k], part[6 * b + k]) : fmaxf(v[k], part[6 * b + k]);
+	__shared__ float sm[6][4];
+	for (int k = 0; k < 6; ++k) v[k] = k < 3 ? wave_min(v[k]) : wave_max(v[k]);
+	const int w = threadIdx.x >> 6;
+	if ((threadIdx.x & 63) == 0) for (int k = 0; k < 6; ++k) sm[k][w] = v[k];
+	__syncthreads();
+	if (threadIdx.x == 0)
+		for (int k = 0; k < 6; ++k) {
+			float r = sm[k][0];
+			for (int j = 1; j < 4; ++j) r = k < 3 ? fminf(r, sm[k][j]) : fmaxf(r, sm[k][j]);
+			bb[k] = ord_f(r);
+		}
 }
 
 __device__ __forceinline__ unsigned long long spread21(unsigned v) {
@@ -182,8 +210,10 @@ __global__ void k_morton(const float4 *__restrict__ pos, const int *__restrict__
 int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out) {
 	if (n <= 0) return 0;
 	if (skg_ensure_keys(c, n)) return SKG_E_NOMEM;
-	k_bbox_init<<<1, 64, 0, c->st>>>((unsigned *)(c->d_cnt + 8));
-	k_bbox<<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, (unsigned *)(c->d_cnt + 8));
+	const int nb = nblk(n, 256);
+	if (skg_ensure_part(c, 6 * (size_t)nb)) return SKG_E_NOMEM;
+	k_bbox<<<nb, 256, 0, c->st>>>(pos, idx, n, c->part);
+	k_bbox_final<<<1, 256, 0, c->st>>>(c->part, nb, (unsigned *)(c->d_cnt + 8));
 	k_morton<<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, (const unsigned *)(c->d_cnt + 8), c->k64a, c->v32a);
 	SKG_HIP(hipGetLastError());
 	return skg_sort_pairs(c, c->k64a, c->k64b, c->v32a, perm_out, n, 63);
@@ -613,15 +643,12 @@ __global__ void k_iota(int *a, int n) {
 	if (i < n) a[i] = i;
 }
 
-__global__ void k_step_init(unsigned *thr, int *cnt) {
-	if (threadIdx.x == 0) { *thr = 0x7F800000u; *cnt = 0; }
-}
-
 // ScatterCut threshold: smallest density among live entities that held an
 // active mover this step; on the initial step live entities that held none
-// get density 0 (smooth1.c:460-470, 497-507).
-__global__ void k_scat_thr(const int *__restrict__ elive, const int *__restrict__ ehit, float *erho,
-		int ne, int stamp, int bInitial, unsigned *thr) {
+// get density 0 (smooth1.c:460-470, 497-507).  Block minima go to partials.
+__global__ __launch_bounds__(256) void k_scat_thr(const int *__restrict__ elive, const int *__restrict__ ehit,
+		float *erho, int ne, int stamp, int bInitial, float *part) {
+	__shared__ float sm[4];
 	int e = blockIdx.x * blockDim.x + threadIdx.x;
 	float v = INFINITY;
 	if (e < ne && elive[e]) {
@@ -629,22 +656,47 @@ __global__ void k_scat_thr(const int *__restrict__ elive, const int *__restrict_
 		else if (bInitial) erho[e] = 0.f;
 	}
 	v = wave_min(v);
-	if ((threadIdx.x & 63) == 0 && v < INFINITY) atomicMin(thr, __float_as_uint(v));
+	if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) part[blockIdx.x] = fminf(fminf(sm[0], sm[1]), fminf(sm[2], sm[3]));
+}
+__global__ __launch_bounds__(256) void k_min_final(const float *__restrict__ part, int nb, float *out) {
+	__shared__ float sm[4];
+	float v = INFINITY;
+	for (int b = threadIdx.x; b < nb; b += 256) v = fminf(v, part[b]);
+	v = wave_min(v);
+	if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) out[0] = fminf(fminf(sm[0], sm[1]), fminf(sm[2], sm[3]));
 }
 
-// ScatterCut (smooth1.c:387-405): keep entities with density >= threshold.
-__global__ void k_scat_cut(int *elive, float4 *eaux, const float *__restrict__ erho, int ne,
-		const unsigned *__restrict__ thr, int *cnt) {
+// ScatterCut (smooth1.c:387-405): keep entities with density >= threshold
+// (no hit anywhere: threshold 0, everything kept).  Block live counts go to
+// partials.
+__global__ __launch_bounds__(256) void k_scat_cut(int *elive, float4 *eaux, const float *__restrict__ erho, int ne,
+		const float *__restrict__ thr, int *part) {
+	__shared__ int sm[4];
 	int e = blockIdx.x * blockDim.x + threadIdx.x;
 	int live = 0;
 	if (e < ne) {
-		unsigned tb = *thr;
-		float t = (tb == 0x7F800000u) ? 0.f : __uint_as_float(tb);
+		const float t0 = thr[0];
+		const float t = (t0 == INFINITY) ? 0.f : t0;
 		live = elive[e];
 		if (live && !(erho[e] >= t)) { live = 0; elive[e] = 0; eaux[e].x = -1.f; }
 	}
-	unsigned long long b = __ballot(live);
-	if ((threadIdx.x & 63) == 0 && b) atomicAdd(cnt, __popcll(b));
+	const int c = __popcll(__ballot(live));
+	if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = c;
+	__syncthreads();
+	if (threadIdx.x == 0) part[blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+}
+__global__ __launch_bounds__(256) void k_sum_final(const int *__restrict__ part, int nb, int *out) {
+	__shared__ int sm[4];
+	int v = 0;
+	for (int b = threadIdx.x; b < nb; b += 256) v += part[b];
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) out[0] = sm[0] + sm[1] + sm[2] + sm[3];
 }
 
 __global__ void k_live_b2(const float4 *__restrict__ eaux, int ne, float *b2live) {
@@ -791,7 +843,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2);
 	dfree(&c->d_thr); dfree(&c->d_cnt);
 	if (c->tmp) (void)hipFree(c->tmp);
-	dfree(&c->k64a); dfree(&c->k64b); dfree(&c->v32a); dfree(&c->v32b);
+	dfree(&c->k64a); dfree(&c->k64b); dfree(&c->v32a); dfree(&c->v32b); dfree(&c->part);
 	for (int i = 0; i < c->caprec; ++i) { (void)hipEventDestroy(c->prec[i].a); (void)hipEventDestroy(c->prec[i].b); }
 	free(c->prec);
 	if (c->e0) (void)hipEventDestroy(c->e0);
@@ -1053,11 +1105,12 @@ static int denmax_step(skg_ctx *c, float fStep, int bInitial) {
 	const int stamp = ++c->step;
 	if ((rc = cover_launch(c, 1, c->m_pos, c->m_list, c->nActive, nullptr, stamp, fStep,
 			24.0 * c->nActive + 24.0 * c->nLive))) return rc;
-	k_step_init<<<1, 64, 0, c->st>>>((unsigned *)c->d_thr, c->d_cnt);
-	k_scat_thr<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->nE, stamp, bInitial,
-			(unsigned *)c->d_thr);
-	k_scat_cut<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE,
-			(const unsigned *)c->d_thr, c->d_cnt);
+	const int nb = nblk(c->nE, 256);
+	if (skg_ensure_part(c, (size_t)nb)) return SKG_E_NOMEM;
+	k_scat_thr<<<nb, 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->nE, stamp, bInitial, c->part);
+	k_min_final<<<1, 256, 0, c->st>>>(c->part, nb, c->d_thr);
+	k_scat_cut<<<nb, 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE, c->d_thr, (int *)c->part);
+	k_sum_final<<<1, 256, 0, c->st>>>((const int *)c->part, nb, c->d_cnt);
 	SKG_HIP(hipGetLastError());
 	return 0;
 }

@@ -138,25 +138,28 @@ __global__ __launch_bounds__(64) void k_centers(const int *__restrict__ gs, cons
 }
 
 // ---------------------------------------------------------------------------
-// Softened pair potentials (grav.h:11-31, grav.c:25-27) in fp32.
+// Softened pair potentials (grav.h:11-31, grav.c:25-27) in fp32.  1/r comes
+// from the hardware reciprocal square root (about 1 ulp; the potentials are
+// checked against the FP64 oracle, not for bit parity), so the far field
+// r >= eps_i + eps_j -- nearly every pair of a large group -- costs one
+// transcendental.  Coincident points (d2 = 0) take the r = 0 branch.
 __device__ __forceinline__ float pot_kernel(int soft, float d2, float twoh) {
-	if (soft == SKG_PLUMMER) return 1.0f / __fsqrt_rn(fadd(d2, fmul(fmul(0.25f, twoh), twoh)));
-	const float r = __fsqrt_rn(d2);
-	if (r < twoh) {
-		const float dih = 2.0f / twoh, u = fmul(r, dih), u2 = fmul(u, u);
-		if (u < 1.0f) {
-			float p = fmaf(-0.1f, u, 0.3f);
-			p = fmaf(p, u2, -2.0f / 3.0f);
-			p = fmaf(p, u2, 1.4f);
-			return fmul(dih, p);
-		}
-		float p = fmaf(1.0f / 30.0f, u, -0.3f);
-		p = fmaf(p, u, 1.0f);
-		p = fmaf(p, u, -4.0f / 3.0f);
-		p = fmaf(p, u2, 1.6f);
-		return fsub(fmul(dih, p), 1.0f / (15.0f * r));
+	if (soft == SKG_PLUMMER) return __builtin_amdgcn_rsqf(fmaf(0.25f * twoh, twoh, d2));
+	const float ir = __builtin_amdgcn_rsqf(d2);
+	const float r = d2 > 0.f ? d2 * ir : 0.f;
+	if (r >= twoh) return ir;
+	const float dih = 2.0f * __builtin_amdgcn_rcpf(twoh), u = r * dih, u2 = u * u;
+	if (u < 1.0f) {
+		float p = fmaf(-0.1f, u, 0.3f);
+		p = fmaf(p, u2, -2.0f / 3.0f);
+		p = fmaf(p, u2, 1.4f);
+		return dih * p;
 	}
-	return 1.0f / r;
+	float p = fmaf(1.0f / 30.0f, u, -0.3f);
+	p = fmaf(p, u, 1.0f);
+	p = fmaf(p, u, -4.0f / 3.0f);
+	p = fmaf(p, u2, 1.6f);
+	return fmaf(dih, p, (-1.0f / 15.0f) * ir);
 }
 
 // Relative coordinates and unbinding workspace for groups to unbind (kd.c:1341-1355).
@@ -195,7 +198,7 @@ __global__ void k_unb_setup(const int *__restrict__ gs, const int *__restrict__ 
 __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tiles, const int *__restrict__ gs,
 		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv,
 		float2 *uphi, float G, int soft) {
-	__shared__ float4 sq[POT_T];
+	__shared__ float4 sq[POT_T];      // x, y, z, G*m
 	__shared__ float se[POT_T];
 	const int2 tl = tiles[blockIdx.x];   // (group, first target)
 	const int g = tl.x, s = gs[g], e = ge[g];
@@ -208,13 +211,19 @@ __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tile
 	for (int j0 = s; j0 < e; j0 += POT_T) {
 		const int nt = min(POT_T, e - j0);
 		__syncthreads();
-		if (threadIdx.x < nt) { sq[threadIdx.x] = uq[j0 + threadIdx.x]; se[threadIdx.x] = uv[j0 + threadIdx.x].w; }
+		if (threadIdx.x < nt) {
+			const float4 q = uq[j0 + threadIdx.x];
+			sq[threadIdx.x] = make_float4(q.x, q.y, q.z, fmul(G, q.w));
+			se[threadIdx.x] = uv[j0 + threadIdx.x].w;
+		}
 		__syncthreads();
+		const int self = i - j0;
 		for (int t = 0; t < nt; ++t) {
 			const float4 qj = sq[t];
-			const float dx = fsub(qi.x, qj.x), dy = fsub(qi.y, qj.y), dz = fsub(qi.z, qj.z);
-			const float k = pot_kernel(soft, sq3(dx, dy, dz), fadd(ei, se[t]));
-			if (j0 + t != i) acc.add(fmul(fmul(G, qj.w), k));
+			const float dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
+			const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+			const float term = qj.w * pot_kernel(soft, d2, ei + se[t]);
+			acc.add(t == self ? 0.f : term);
 		}
 	}
 	if (valid) {
@@ -243,9 +252,9 @@ __global__ __launch_bounds__(64) void k_unb_small(const int *__restrict__ glist,
 	for (int j = 0; j < n0; ++j) {
 		const float qx = __shfl(q.x, j, 64), qy = __shfl(q.y, j, 64), qz = __shfl(q.z, j, 64);
 		const float qm = __shfl(q.w, j, 64), qe = __shfl(v.w, j, 64);
-		const float dx = fsub(q.x, qx), dy = fsub(q.y, qy), dz = fsub(q.z, qz);
-		const float k = pot_kernel(soft, sq3(dx, dy, dz), fadd(v.w, qe));
-		if (mine && j != lane) phi.add(fmul(fmul(G, qm), k));
+		const float dx = q.x - qx, dy = q.y - qy, dz = q.z - qz;
+		const float k = pot_kernel(soft, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), v.w + qe);
+		if (mine && j != lane) phi.add(fmul(G, qm) * k);
 	}
 	double cm[7];
 	{
@@ -292,7 +301,7 @@ __global__ __launch_bounds__(64) void k_unb_small(const int *__restrict__ glist,
 		if (bSub) pairs += n;
 		if (bSub && alive) {
 			const float dx = fsub(bx, q.x), dy = fsub(by, q.y), dz = fsub(bz, q.z);
-			phi.add(-fmul(fmul(G, bm), pot_kernel(soft, sq3(dx, dy, dz), fadd(be, v.w))));
+			phi.add(-fmul(fmul(G, bm), pot_kernel(soft, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), fadd(be, v.w))));
 		}
 	}
 	const float rbx = __shfl(q.x, imin < 64 ? imin : 0, 64), rby = __shfl(q.y, imin < 64 ? imin : 0, 64);
@@ -361,7 +370,7 @@ __global__ __launch_bounds__(COOP_T) void k_unb_coop(const int *__restrict__ bg,
 				if (prev >= 0 && bSub) {                              // kdSubPot (grav.c:39-60)
 					const float dx = fsub(pq.x, q.x), dy = fsub(pq.y, q.y), dz = fsub(pq.z, q.z);
 					Acc ac; ac.s = ph.x; ac.c = ph.y;
-					ac.add(-fmul(fmul(G, pq.w), pot_kernel(soft, sq3(dx, dy, dz), fadd(pe, v.w))));
+					ac.add(-fmul(fmul(G, pq.w), pot_kernel(soft, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), fadd(pe, v.w))));
 					ph = make_float2(ac.s, ac.c);
 					uphi[s + i] = ph;
 				}
@@ -551,7 +560,7 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 						const float4 qb2 = uq[s + b];
 						const float eb = uv[s + b].w;
 						const float dx = fsub(qb2.x, q.x), dy = fsub(qb2.y, q.y), dz = fsub(qb2.z, q.z);
-						ac.add(-fmul(fmul(G, qb2.w), pot_kernel(soft, sq3(dx, dy, dz), fadd(eb, v.w))));
+						ac.add(-fmul(fmul(G, qb2.w), pot_kernel(soft, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), fadd(eb, v.w))));
 					}
 					ph = make_float2(ac.s, ac.c);
 					uphi[s + i] = ph;
@@ -669,7 +678,7 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 			for (int c = 0; c < CT; ++c) {
 				if (cI[c] < 0) continue;
 				const float dx = fsub(qb.x, cQ[c].x), dy = fsub(qb.y, cQ[c].y), dz = fsub(qb.z, cQ[c].z);
-				cP[c] -= (double)fmul(fmul(G, qb.w), pot_kernel(soft, sq3(dx, dy, dz), fadd(vb.w, cV[c].w)));
+				cP[c] -= (double)fmul(fmul(G, qb.w), pot_kernel(soft, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), fadd(vb.w, cV[c].w)));
 			}
 			grow += zf * (double)fmul(fmul(G, qb.w), kernel_max(soft, fadd(vb.w, epsMin)));
 			if (tid == 0) pend[npend] = BI;
@@ -692,7 +701,7 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 					const int b = pend[k];
 					const float4 qb2 = uq[s + b];
 					const float dx = fsub(qb2.x, q.x), dy = fsub(qb2.y, q.y), dz = fsub(qb2.z, q.z);
-					phi -= (double)fmul(fmul(G, qb2.w), pot_kernel(soft, sq3(dx, dy, dz), fadd(uv[s + b].w, v.w)));
+					phi -= (double)fmul(fmul(G, qb2.w), pot_kernel(soft, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), fadd(uv[s + b].w, v.w)));
 				}
 			const double d0 = (double)fShift * ((double)v.x - cm[4]) + (double)fCosmo * ((double)q.x - cm[1]);
 			const double d1 = (double)fShift * ((double)v.y - cm[5]) + (double)fCosmo * ((double)q.y - cm[2]);
@@ -802,7 +811,7 @@ __global__ __launch_bounds__(64) void k_unb_scoop(const int *__restrict__ gs, co
 					for (int i = s + lane; i < e; i += 64) {
 						const float4 q = uq[i];
 						const float dx = fsub(nx, q.x), dy = fsub(ny, q.y), dz = fsub(nz2, q.z);
-						const float k = pot_kernel(soft, sq3(dx, dy, dz), fadd(peps, uv[i].w));
+						const float k = pot_kernel(soft, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), fadd(peps, uv[i].w));
 						float2 ph = uphi[i];
 						Acc a; a.s = ph.x; a.c = ph.y;
 						a.add(fmul(fmul(G, pm), k));
@@ -918,7 +927,7 @@ __global__ __launch_bounds__(UNB_T) void k_unb_loop(const int *__restrict__ glis
 			for (int i = tid; i < n; i += UNB_T) {
 				const float4 q = uq[s + i];
 				const float dx = fsub(qb.x, q.x), dy = fsub(qb.y, q.y), dz = fsub(qb.z, q.z);
-				const float k = pot_kernel(soft, sq3(dx, dy, dz), fadd(vb.w, uv[s + i].w));
+				const float k = pot_kernel(soft, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), fadd(vb.w, uv[s + i].w));
 				float2 ph = uphi[s + i];
 				Acc ac; ac.s = ph.x; ac.c = ph.y;
 				ac.add(-fmul(fmul(G, qb.w), k));
