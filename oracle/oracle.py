@@ -108,6 +108,7 @@ def lib():
         L.or_init_move.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.or_move_first.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.or_move_block.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.or_first_gradient.argtypes = [C.c_void_p, _f]
         L.or_fof.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
         L.or_microsteps.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
         L.or_get_moved.argtypes = [C.c_void_p, _f]
@@ -204,6 +205,13 @@ class Oracle:
             self.L.or_move_block(self.ctx, C.byref(a), C.byref(s))
             tr.append((a.value, s.value))
         return np.array(tr, np.int32)
+
+    def first_gradient(self):
+        """The reference's density gradient a of the first Denmax step (after density())."""
+        a = np.empty(3 * self.box.n, np.float32)
+        if self.L.or_first_gradient(self.ctx, a):
+            raise RuntimeError("or_first_gradient failed")
+        return a.reshape(-1, 3)
 
     def moved(self):
         r = np.empty(3 * self.box.n, np.float32)

@@ -748,6 +748,20 @@ int or_move_first(or_ctx *c, int *nActive, int *nScatter)
 	return 0;
 }
 
+int or_first_gradient(or_ctx *c, float *acc)
+{
+	/* smAccDensity's accumulation of the initial iteration (main.c:394-401),
+	   before any move: a at every mover, by iOrder (NaN for non-movers). */
+	int i, j, nm;
+	if (or_init_move(c, &nm)) return -1;
+	if (build_move_tree(c)) return -1;
+	if (acc_density(c, (c->inType == OR_DARK) || c->P.bForceInitialCut) < 0) return -1;
+	for (i = 0; i < 3*c->n; ++i) acc[i] = NAN;
+	for (i = 0; i < c->nMove; ++i)
+		for (j = 0; j < 3; ++j) acc[3*c->m[i].ord+j] = c->m[i].a[j];
+	return 0;
+}
+
 int or_move_block(or_ctx *c, int *nActive, int *nScatter)
 {
 	int i, ns = 0;

@@ -58,6 +58,9 @@ int or_density(or_ctx *c, float *fBall2, float *fDensity, int *nExtraScat);
 int or_init_move(or_ctx *c, int *nMove);
 int or_move_first(or_ctx *c, int *nActive, int *nScatter);      /* main.c:395-404 */
 int or_move_block(or_ctx *c, int *nActive, int *nScatter);      /* main.c:410-416 */
+/* The reference's own density gradient a of the initial iteration (smooth1.c:408-518
+   called at main.c:396-401), by iOrder; replaces or_init_move + or_move_first. */
+int or_first_gradient(or_ctx *c, float *acc);
 int or_fof(or_ctx *c, int *iGroup, int *nGroup);                /* main.c:425 */
 int or_microsteps(or_ctx *c, int *nScatter);                     /* main.c:431-438 */
 int or_get_moved(or_ctx *c, float *r);                           /* 3N by iOrder, NaN if not moving */

@@ -1,11 +1,225 @@
-// skg_adjoint.hip -- density adjoint and frozen-field gradient at arbitrary points.
+// skg_adjoint.hip -- reverse-mode derivative of the M4 density and the frozen
+// density field's gradient at arbitrary points.
+//
+// Density (smooth1.c:249-263): rho_j = sum_{i in N_j} m_i W(r_ij, h_j)/2
+//                                    + sum_{i : j in N_i} m_i W(r_ij, h_i)/2,
+// W(r, h) = w(r/h)/(pi h^3), h = sqrt(fBall2)/2, N_i the kNN set of i.  With
+// the sets and radii frozen, every term is a pair (i, j in N_i) evaluated with
+// h_i, so the vector-Jacobian product for weights gbar is
+//   c_ij  = dW/dr(r_ij, h_i)/2 * (gbar_i m_j [gather] + gbar_j m_i [scatter])
+//   dx_i += c_ij (r_i - r_j)/r,   dx_j -= c_ij (r_i - r_j)/r,
+//   dh_i += dW/dh(r_ij, h_i)/2 * (gbar_i m_j + gbar_j m_i),
+// with dW/dr = w'(q)/(pi h^4) and dW/dh = -(3 w(q) + q w'(q))/(pi h^4).  The
+// first two sums run per query over its kNN list; the third (each point's
+// share as a neighbour) runs per point over the balls that hold it, so
+// neither pass needs atomics.  Since g(q) = w'(q)/q, the reference's gradient
+// (smooth1.c:447-459) is a = -2 d(rho_scatter)/dx at a point.
 #include "skg_common.h"
+#include <cfloat>
+#include <vector>
+
+static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+__device__ __forceinline__ void m4_derivs(float q, float &w, float &wp) {
+	if (q < 1.0f) {
+		w = fmaf(fmaf(0.75f, q, -1.5f), q * q, 1.0f);
+		wp = q * fmaf(2.25f, q, -3.0f);
+	} else if (q < 2.0f) {
+		const float t = 2.0f - q;
+		w = 0.25f * t * t * t;
+		wp = -0.75f * t * t;
+	} else {
+		w = 0.f; wp = 0.f;
+	}
+}
+
+// exact periodic separation a - b under the nearest-image rule
+__device__ __forceinline__ float per_delta(float a, float b, float L, int per) {
+	float d = fsub(a, b);
+	if (per) {
+		const float hL = 0.5f * L;
+		if (d > hL) d = exact_delta(a, b, -L);
+		else if (d < -hL) d = exact_delta(a, b, L);
+	}
+	return d;
+}
+
+// Pass 1: per query i, over its kNN list (heap keys; slot 0 is the head).
+__global__ __launch_bounds__(256) void k_adj_gather(const unsigned long long *__restrict__ heap, int k,
+		const int *__restrict__ sord, const float *__restrict__ sb2, int nS, const float *__restrict__ pos,
+		const float *__restrict__ mass, const float *__restrict__ gbar, int parts, float L, int per,
+		float *dx, float *dh) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nS) return;
+	const int oi = sord[i];
+	const float xi = pos[3 * oi], yi = pos[3 * oi + 1], zi = pos[3 * oi + 2], mi = mass[oi], gi = gbar[oi];
+	const float h = 0.5f * sqrtf(sb2[i]), ih = 1.0f / h, norm = (float)(M_1_PI) * ih * ih * ih * ih;
+	Acc ax, ay, az, ahh;
+	ax.init(); ay.init(); az.init(); ahh.init();
+	for (int s = 1; s < k; ++s) {
+		const unsigned long long key = heap[(size_t)i * k + s];
+		const int j = (int)(unsigned)key;
+		float dxv = 0.f, dyv = 0.f, dzv = 0.f, r = 0.f;
+		if (j != oi) {
+			dxv = per_delta(xi, pos[3 * j], L, per); dyv = per_delta(yi, pos[3 * j + 1], L, per);
+			dzv = per_delta(zi, pos[3 * j + 2], L, per);
+			r = sqrtf(sq3(dxv, dyv, dzv));
+		}
+		float w, wp;
+		m4_derivs(r * ih, w, wp);
+		const float coef = ((parts & 1) ? gi * mass[j] : 0.f) + ((parts & 2) ? gbar[j] * mi : 0.f);
+		// the self pair (r = 0, W = 1/(pi h^3)) moves with h but not with x
+		ahh.add(-0.5f * (3.0f * w + r * ih * wp) * norm * coef);
+		if (!(r > 0.f)) continue;
+		const float c = 0.5f * wp * norm * coef / r;
+		ax.add(c * dxv); ay.add(c * dyv); az.add(c * dzv);
+	}
+	dx[3 * oi] = ax.val(); dx[3 * oi + 1] = ay.val(); dx[3 * oi + 2] = az.val();
+	dh[oi] = ahh.val();
+}
+
+// Pass 2 (and the field gradient): per point, every entity (scatterer or
+// periodic replica) whose ball strictly holds it, over the entity tree with
+// the radii as built.  MODE 0: dx_j -= c_ij r_hat; MODE 1: a(x).
+template <int MODE>
+__global__ __launch_bounds__(64) void k_adj_cover(const float4 *__restrict__ pts, int np,
+		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float *__restrict__ eb2,
+		const float4 *__restrict__ eorg, const int *__restrict__ esrc, const int *__restrict__ sord, int ne,
+		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
+		const float *__restrict__ mass, const float *__restrict__ gbar, const int *__restrict__ pord,
+		int parts, float Lx, float Ly, float Lz, float *out) {
+	const int lane = threadIdx.x, base = blockIdx.x * 64, ii = base + lane;
+	const bool valid = ii < np;
+	const float4 p = pts[valid ? ii : base];
+	const float3 blo = make_float3(wave_min(p.x), wave_min(p.y), wave_min(p.z));
+	const float3 bhi = make_float3(wave_max(p.x), wave_max(p.y), wave_max(p.z));
+	float mj = 0.f, gj = 0.f;
+	if (MODE == 0 && valid) { const int oj = pord[ii]; mj = mass[oj]; gj = gbar[oj]; }
+	Acc s0, s1, s2;
+	s0.init(); s1.init(); s2.init();
+	int node = 1;
+	for (;;) {
+		const float4 lo = nlo[node], hi = nhi[node];
+		const bool need = valid && point_box_d2_np(lo, hi, p) < lo.w;
+		const bool visit = box_box_d2<false>(lo, hi, blo, bhi, 0.f) < lo.w && __any(need);
+		if (visit && node < nLeaf2) { node = 2 * node; continue; }
+		if (visit && need) {
+			const int start = (node - nLeaf2) * B;
+			for (int e = start; e < min(ne, start + B); ++e) {
+				const float4 A = epos[e];
+				float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
+				const float d2 = sq3(dx, dy, dz);
+				if (!(d2 < eb2[e])) continue;
+				const float4 O = eorg[e];
+				const int code = (int)O.w;
+				if (code != 13) {
+					dx = exact_delta(O.x, p.x, (float)(code / 9 - 1) * Lx);
+					dy = exact_delta(O.y, p.y, (float)((code / 3) % 3 - 1) * Ly);
+					dz = exact_delta(O.z, p.z, (float)(code % 3 - 1) * Lz);
+				}
+				const float4 X = eaux[e];
+				if (MODE == 1) {
+					const float f = grad_factor(sq3(dx, dy, dz), X.y, X.z);
+					s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
+				} else {
+					const float r = sqrtf(sq3(dx, dy, dz));
+					if (!(r > 0.f)) continue;
+					const float h = 0.5f * sqrtf(eb2[e]), ih = 1.0f / h;
+					float w, wp;
+					m4_derivs(r * ih, w, wp);
+					const int oi = sord[esrc[e]];
+					const float coef = ((parts & 1) ? gbar[oi] * mj : 0.f) + ((parts & 2) ? gj * A.w : 0.f);
+					const float c = 0.5f * wp * (float)M_1_PI * ih * ih * ih * ih * coef / r;
+					// dx_j -= c (r_i - r_j)/r with (r_i - r_j) = (dx, dy, dz)
+					s0.add(-c * dx); s1.add(-c * dy); s2.add(-c * dz);
+				}
+			}
+		}
+		int nx = node;
+		while (nx & 1) nx >>= 1;
+		node = nx + 1;
+		if (node == 1) break;
+	}
+	if (!valid) return;
+	out[3 * ii] = s0.val(); out[3 * ii + 1] = s1.val(); out[3 * ii + 2] = s2.val();
+}
+
+__global__ void k_full_b2(const float *__restrict__ eb2, int ne, float *b2) {
+	int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < ne) b2[e] = eb2[e];
+}
+__global__ void k_gather_pts(const float4 *__restrict__ in, const int *__restrict__ perm, int n, float4 *out) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) out[i] = in[perm[i]];
+}
+__global__ void k_unpermute3(const float *__restrict__ in, const int *__restrict__ perm, int n, float *out) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int o = perm[i];
+	out[3 * o] = in[3 * i]; out[3 * o + 1] = in[3 * i + 1]; out[3 * o + 2] = in[3 * i + 2];
+}
+__global__ void k_add3_ord(const float *__restrict__ in, const int *__restrict__ ord, int n, float *dx) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int o = ord[i];
+	for (int c = 0; c < 3; ++c) dx[3 * o + c] = fadd(dx[3 * o + c], in[3 * i + c]);
+}
+
+// The entity tree with every radius as built (independent of later ScatterCuts).
+static int full_tree(skg_ctx *c) {
+	if (c->tfull.lo && c->tfull.n == c->nE) return 0;
+	return skg_build_tree(c, &c->tfull, c->e_pos, c->nE, c->te.B, c->e_b2);
+}
+
+template <int MODE>
+static void launch_cover(skg_ctx *c, const float4 *pts, int np, const int *pord, const float *gbar, int parts,
+		float *out) {
+	const float *L = c->P.fPeriod;
+	const bool per = c->P.bPeriodic != 0;
+	k_adj_cover<MODE><<<nblk(np, 64), 64, 0, c->st>>>(pts, np, c->e_pos, c->e_aux, c->e_b2, c->e_org, c->e_src,
+			c->s_ord, c->nE, c->tfull.lo, c->tfull.hi, c->tfull.nLeaf2, c->tfull.B, c->d_mass, gbar, pord, parts,
+			per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, out);
+}
 
 int skg_gradient_points(skg_ctx *c, const float4 *q, int nq, float *out) {
-	(void)c; (void)q; (void)nq; (void)out;
-	return SKG_E_STATE;
+	int rc;
+	if ((rc = full_tree(c))) return rc;
+	int *perm = nullptr;
+	float4 *qs = nullptr;
+	float *tmp = nullptr;
+	if (dalloc(&perm, nq) || dalloc(&qs, nq) || dalloc(&tmp, 3 * (size_t)nq)) return SKG_E_NOMEM;
+	if ((rc = skg_morton_sort(c, q, nullptr, nq, perm))) return rc;
+	k_gather_pts<<<nblk(nq, 256), 256, 0, c->st>>>(q, perm, nq, qs);
+	launch_cover<1>(c, qs, nq, nullptr, nullptr, 0, tmp);
+	k_unpermute3<<<nblk(nq, 256), 256, 0, c->st>>>(tmp, perm, nq, out);
+	SKG_HIP(hipGetLastError());
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&perm); dfree(&qs); dfree(&tmp);
+	return 0;
 }
+
 extern "C" int skg_density_adjoint(skg_ctx *c, const float *gbar, int parts, float *dx, float *dh) {
-	(void)gbar; (void)parts; (void)dx; (void)dh;
-	return c ? SKG_E_STATE : SKG_E_PARAM;
+	if (!c || !gbar || !dx || !dh || parts < 1 || parts > 3) return SKG_E_PARAM;
+	if (c->stage < 2 || !c->s_heap || c->nS == 0) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	int rc;
+	const int n = c->n, nS = c->nS;
+	float *dg = nullptr, *ddx = nullptr, *ddh = nullptr, *part2 = nullptr;
+	if (dalloc(&dg, n) || dalloc(&ddx, 3 * (size_t)n) || dalloc(&ddh, n) || dalloc(&part2, 3 * (size_t)nS))
+		return SKG_E_NOMEM;
+	SKG_HIP(hipMemcpyAsync(dg, gbar, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
+	SKG_HIP(hipMemsetAsync(ddx, 0, 12 * (size_t)n, c->st));
+	SKG_HIP(hipMemsetAsync(ddh, 0, 4 * (size_t)n, c->st));
+	const bool per = c->P.bPeriodic != 0;
+	k_adj_gather<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_heap, c->P.nSmooth, c->s_ord, c->s_ball2, nS, c->d_pos,
+			c->d_mass, dg, parts, per ? c->P.fPeriod[0] : 0.f, per, ddx, ddh);
+	if ((rc = full_tree(c))) return rc;
+	launch_cover<0>(c, c->s_pos, nS, c->s_ord, dg, parts, part2);
+	k_add3_ord<<<nblk(nS, 256), 256, 0, c->st>>>(part2, c->s_ord, nS, ddx);
+	SKG_HIP(hipGetLastError());
+	SKG_HIP(hipMemcpyAsync(dx, ddx, 12 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(dh, ddh, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&dg); dfree(&ddx); dfree(&ddh); dfree(&part2);
+	return SKG_OK;
 }

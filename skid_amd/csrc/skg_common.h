@@ -59,6 +59,7 @@ struct skg_ctx {
 	int *e_live = nullptr;       // 1 active, 0 cut
 	int *e_hit = nullptr;        // step stamp of the last hit
 	DevTree te;
+	DevTree tfull;               // entity tree with every radius as built (adjoint, field gradient)
 
 	// movers
 	int nMove = 0, nActive = 0, step = 0;
@@ -158,6 +159,10 @@ __device__ __forceinline__ float box_box_d2(float4 nlo, float4 nhi, float3 blo, 
 	float gy = gap_axis<PER>(nlo.y, nhi.y, blo.y, bhi.y, L);
 	float gz = gap_axis<PER>(nlo.z, nhi.z, blo.z, bhi.z, L);
 	return sq3(gx, gy, gz);
+}
+
+__device__ __forceinline__ float point_box_d2_np(float4 lo, float4 hi, float4 p) {
+	return sq3(gap1(lo.x, hi.x, p.x, p.x), gap1(lo.y, hi.y, p.y, p.y), gap1(lo.z, hi.z, p.z, p.z));
 }
 
 // Query-side nearest-image delta: d = q - p, re-evaluated from q -/+ L rounded

@@ -838,7 +838,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->d_rho); dfree(&c->d_ball2); dfree(&c->d_grp);
 	dfree(&c->s_pos); dfree(&c->s_ord); dfree(&c->s_ball2); dfree(&c->s_rhoG); dfree(&c->s_rhoS);
 	dfree(&c->s_heap);
-	skg_free_tree(&c->ts); skg_free_tree(&c->te);
+	skg_free_tree(&c->ts); skg_free_tree(&c->te); skg_free_tree(&c->tfull);
 	dfree(&c->e_b2); dfree(&c->e_pos); dfree(&c->e_aux); dfree(&c->e_org); dfree(&c->e_rho); dfree(&c->e_src); dfree(&c->e_live); dfree(&c->e_hit);
 	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2);
 	dfree(&c->d_thr); dfree(&c->d_cnt);
@@ -1001,6 +1001,7 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		}
 		c->nRep = nRep;
 		c->nE = nS + nRep;
+		skg_free_tree(&c->tfull);
 		c->nLive = c->nE;
 		const int nE = c->nE;
 		float4 *e4u = nullptr;
