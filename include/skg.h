@@ -82,6 +82,27 @@ const char *skg_strerror(int status);
 int skg_load(skg_ctx *ctx, int nGas, int nDark, int nStar, const float *pos,
 		const float *vel, const float *mass, const float *soft, const float *temp);
 
+/* Batched sub-volumes (no reference counterpart: the reference runs one box per
+   process, so this replaces nBox separate kdReadTipsy + main.c:374-471 runs).
+   Box b holds nGas[b] + nDark[b] + nStar[b] particles in tipsy species order;
+   the boxes are stored one after another in pos/vel/mass/soft/temp, and iOrder
+   runs over that concatenation.  All boxes share the parameter block (period,
+   centre, nSmooth, ...) and one species mix (SKG_E_PARAM otherwise); fTau
+   (optional, per box) gives each box its own tau and with it fCvg = tau/2,
+   fStep = fCvg/2, fScoop = 2 tau (main.c:343-345).  NULL uses the parameter
+   block's fTau/fCvg/fStep/fScoop for every box.  Every later stage runs all
+   boxes in the same launches; no pair, link or scoop crosses a box.  Group ids
+   stay numbered by smallest iOrder, so box b's groups are one contiguous range.
+   skg_field_gradient is not available on a batched context. */
+int skg_load_batch(skg_ctx *ctx, int nBox, const int *nGas, const int *nDark, const int *nStar,
+		const float *fTau, const float *pos, const float *vel, const float *mass,
+		const float *soft, const float *temp);
+
+/* Per-box counters after the last move call (the numbers main.c prints on each
+   Ittr: line, main.c:417, for every box): nActive[b] active movers, nScatter[b]
+   live scatterers + replicas.  Either pointer may be NULL. */
+int skg_box_counts(skg_ctx *ctx, int *nActive, int *nScatter);
+
 /* Scatterer selection, tree build, kNN radii, symmetric M4 density and
    periodic replicas: kdScatterActive + kdBuildTree + smDensityInit
    (main.c:374-378; kd.c:669-693, 371-460; smooth1.c:132-335).  Outputs by

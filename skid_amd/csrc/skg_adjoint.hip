@@ -80,17 +80,19 @@ __global__ __launch_bounds__(256) void k_adj_gather(const unsigned long long *__
 
 // Pass 2 (and the field gradient): per point, every entity (scatterer or
 // periodic replica) whose ball strictly holds it, over the entity tree with
-// the radii as built.  MODE 0: dx_j -= c_ij r_hat; MODE 1: a(x).
+// the radii as built.  MODE 0: dx_j -= c_ij r_hat; MODE 1: a(x).  pbox
+// (batched boxes) restricts each point to its own box's entities.
 template <int MODE>
 __global__ __launch_bounds__(64) void k_adj_cover(const float4 *__restrict__ pts, int np,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float *__restrict__ eb2,
 		const float4 *__restrict__ eorg, const int *__restrict__ esrc, const int *__restrict__ sord, int ne,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		const float *__restrict__ mass, const float *__restrict__ gbar, const int *__restrict__ pord,
-		int parts, float Lx, float Ly, float Lz, float *out) {
+		int parts, float Lx, float Ly, float Lz, const int *__restrict__ pbox, float *out) {
 	const int lane = threadIdx.x, base = blockIdx.x * 64, ii = base + lane;
 	const bool valid = ii < np;
 	const float4 p = pts[valid ? ii : base];
+	const int pb = pbox ? pbox[valid ? ii : base] : 0;
 	const float3 blo = make_float3(wave_min(p.x), wave_min(p.y), wave_min(p.z));
 	const float3 bhi = make_float3(wave_max(p.x), wave_max(p.y), wave_max(p.z));
 	float mj = 0.f, gj = 0.f;
@@ -100,7 +102,7 @@ __global__ __launch_bounds__(64) void k_adj_cover(const float4 *__restrict__ pts
 	int node = 1;
 	for (;;) {
 		const float4 lo = nlo[node], hi = nhi[node];
-		const bool need = valid && point_box_d2_np(lo, hi, p) < lo.w;
+		const bool need = valid && brange_has(hi.w, pb) && point_box_d2_np(lo, hi, p) < lo.w;
 		const bool visit = box_box_d2<false>(lo, hi, blo, bhi, 0.f) < lo.w && __any(need);
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit && need) {
@@ -168,17 +170,17 @@ __global__ void k_add3_ord(const float *__restrict__ in, const int *__restrict__
 // The entity tree with every radius as built (independent of later ScatterCuts).
 static int full_tree(skg_ctx *c) {
 	if (c->tfull.lo && c->tfull.n == c->nE) return 0;
-	return skg_build_tree(c, &c->tfull, c->e_pos, c->nE, c->te.B, c->e_b2);
+	return skg_build_tree(c, &c->tfull, c->e_pos, c->nE, c->te.B, c->e_b2, c->e_box);
 }
 
 template <int MODE>
 static void launch_cover(skg_ctx *c, const float4 *pts, int np, const int *pord, const float *gbar, int parts,
-		float *out) {
+		const int *pbox, float *out) {
 	const float *L = c->P.fPeriod;
 	const bool per = c->P.bPeriodic != 0;
 	k_adj_cover<MODE><<<nblk(np, 64), 64, 0, c->st>>>(pts, np, c->e_pos, c->e_aux, c->e_b2, c->e_org, c->e_src,
 			c->s_ord, c->nE, c->tfull.lo, c->tfull.hi, c->tfull.nLeaf2, c->tfull.B, c->d_mass, gbar, pord, parts,
-			per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, out);
+			per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, c->nBox > 1 ? pbox : nullptr, out);
 }
 
 int skg_gradient_points(skg_ctx *c, const float4 *q, int nq, float *out) {
@@ -188,9 +190,9 @@ int skg_gradient_points(skg_ctx *c, const float4 *q, int nq, float *out) {
 	float4 *qs = nullptr;
 	float *tmp = nullptr;
 	if (dalloc(&perm, nq) || dalloc(&qs, nq) || dalloc(&tmp, 3 * (size_t)nq)) return SKG_E_NOMEM;
-	if ((rc = skg_morton_sort(c, q, nullptr, nq, perm))) return rc;
+	if ((rc = skg_morton_sort(c, q, nullptr, nq, perm, nullptr))) return rc;
 	k_gather_pts<<<nblk(nq, 256), 256, 0, c->st>>>(q, perm, nq, qs);
-	launch_cover<1>(c, qs, nq, nullptr, nullptr, 0, tmp);
+	launch_cover<1>(c, qs, nq, nullptr, nullptr, 0, nullptr, tmp);
 	k_unpermute3<<<nblk(nq, 256), 256, 0, c->st>>>(tmp, perm, nq, out);
 	SKG_HIP(hipGetLastError());
 	SKG_HIP(hipStreamSynchronize(c->st));
@@ -214,7 +216,7 @@ extern "C" int skg_density_adjoint(skg_ctx *c, const float *gbar, int parts, flo
 	k_adj_gather<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_heap, c->P.nSmooth, c->s_ord, c->s_ball2, nS, c->d_pos,
 			c->d_mass, dg, parts, per ? c->P.fPeriod[0] : 0.f, per, ddx, ddh);
 	if ((rc = full_tree(c))) return rc;
-	launch_cover<0>(c, c->s_pos, nS, c->s_ord, dg, parts, part2);
+	launch_cover<0>(c, c->s_pos, nS, c->s_ord, dg, parts, c->s_box, part2);
 	k_add3_ord<<<nblk(nS, 256), 256, 0, c->st>>>(part2, c->s_ord, nS, ddx);
 	SKG_HIP(hipGetLastError());
 	SKG_HIP(hipMemcpyAsync(dx, ddx, 12 * (size_t)n, hipMemcpyDeviceToHost, c->st));

@@ -18,13 +18,27 @@
 
 #define SKG_WAVE 64
 
+// Per-box constants of a batched context (box b = particles [off, off+n) by
+// iOrder).  A single-box context has one entry holding the parameter block's
+// values.
+struct DevBox {
+	int off, n;
+	float fStep, fMicro;     // Denmax step and the micro-step 0.1*fStep (main.c:345, 435)
+	float cvg2, tau2;        // fCvg^2 (kd.c:762), tau^2 (kd.c:875)
+	float scoop2, pad;       // fScoop^2 (grav.c:63-135)
+};
+struct HostBox {
+	int off, n, nGas, nDark, nStar;
+	float tau, cvg, step, scoop;
+};
+
 struct DevTree {
 	int n = 0;         // items
 	int B = 16;        // bucket (leaf) size
 	int nLeaf = 0;     // ceil(n/B)
 	int nLeaf2 = 0;    // next power of two >= nLeaf (>= 1)
 	float4 *lo = nullptr;   // [2*nLeaf2], index 0 unused; lo.w = max live fBall2 below
-	float4 *hi = nullptr;
+	float4 *hi = nullptr;   // hi.w: bits of the box range [b0, b1] below (b0 | b1 << 16)
 };
 
 struct skg_ctx {
@@ -36,6 +50,17 @@ struct skg_ctx {
 	int stage = 0;            // 0 created, 1 loaded, 2 density, 3 movers, 4 moving done, 5 fof, 6 micro, 7 centers, 8 unbound, 9 final
 	int n = 0, nGas = 0, nDark = 0, nStar = 0, inType = 0;
 
+	// boxes (one unless loaded with skg_load_batch)
+	int nBox = 1;
+	HostBox *hbox = nullptr;
+	DevBox *d_bt = nullptr;
+	int *d_boxOf = nullptr;      // box by iOrder
+	unsigned char *d_type = nullptr;   // SKG_GAS / SKG_DARK / SKG_STAR by iOrder
+	int *h_eblk = nullptr;       // [nBox+1] first 256-entity block of each box (host)
+	int *d_eblk = nullptr;       // same, on the device
+	int *d_bact = nullptr, *d_bscat = nullptr;   // per-box active movers / live entities
+	float *d_bthr = nullptr;     // per-box ScatterCut threshold
+
 	// by iOrder
 	float *d_pos = nullptr, *d_vel = nullptr, *d_mass = nullptr, *d_soft = nullptr, *d_temp = nullptr;
 	float *d_rho = nullptr, *d_ball2 = nullptr;
@@ -46,6 +71,7 @@ struct skg_ctx {
 	float4 *s_pos = nullptr;
 	int *s_ord = nullptr;
 	float *s_ball2 = nullptr, *s_rhoG = nullptr, *s_rhoS = nullptr;
+	int *s_box = nullptr;
 	unsigned long long *s_heap = nullptr;   // n_S * k keys of the last kNN (optional)
 	DevTree ts;
 
@@ -58,6 +84,8 @@ struct skg_ctx {
 	int *e_src = nullptr;        // scatterer (sorted S) index
 	int *e_live = nullptr;       // 1 active, 0 cut
 	int *e_hit = nullptr;        // step stamp of the last hit
+	int *e_box = nullptr;        // box; each box's range starts on a 256-entity boundary,
+	                             // filled up with dead entities (fBall2 -1, position NaN)
 	DevTree te;
 	DevTree tfull;               // entity tree with every radius as built (adjoint, field gradient)
 
@@ -65,6 +93,7 @@ struct skg_ctx {
 	int nMove = 0, nActive = 0, step = 0;
 	float4 *m_pos = nullptr, *m_old = nullptr;
 	int *m_ord = nullptr;
+	int *m_box = nullptr;
 	int *m_list = nullptr, *m_list2 = nullptr;
 	float *d_thr = nullptr;      // [2]: threshold scratch
 	int *d_cnt = nullptr;        // small counters
@@ -104,6 +133,35 @@ __device__ __forceinline__ float wave_max(float v) {
 	return v;
 }
 __device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int wave_min_i(int v) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o, 64));
+	return v;
+}
+__device__ __forceinline__ int wave_max_i(int v) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+	return v;
+}
+
+// ---------------------------------------------------------------- box ranges
+// A tree node's hi.w holds the range [b0, b1] of boxes below it as raw bits;
+// an empty node has b0 > b1.  Single-box trees hold [0, 0].
+#define SKG_MAX_BOX 32767
+__device__ __forceinline__ float brange_pack(int b0, int b1) {
+	return __uint_as_float((unsigned)b0 | ((unsigned)b1 << 16));
+}
+__device__ __forceinline__ bool brange_has(float w, int b) {
+	const unsigned r = __float_as_uint(w);
+	return b >= (int)(r & 0xFFFFu) && b <= (int)(r >> 16);
+}
+__device__ __forceinline__ bool brange_meets(float w, int lo, int hi) {
+	const unsigned r = __float_as_uint(w);
+	return max(lo, (int)(r & 0xFFFFu)) <= min(hi, (int)(r >> 16));
+}
+__device__ __forceinline__ bool brange_is(float w, int b) {
+	return __float_as_uint(w) == ((unsigned)b | ((unsigned)b << 16));
+}
 
 // ---------------------------------------------------------------- exact fp32 ops
 // The path's distance arithmetic must match the reference's (no contraction);
@@ -215,8 +273,10 @@ int skg_ensure_keys(skg_ctx *c, size_t n);
 int skg_ensure_part(skg_ctx *c, size_t n);
 int skg_sort_pairs(skg_ctx *c, unsigned long long *kin, unsigned long long *kout, int *vin, int *vout,
 		int n, int endBit);
-int skg_build_tree(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, const float *ball2live);
-int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out);
+int skg_build_tree(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, const float *ball2live,
+		const int *ibox);
+int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box);
+int skg_box_bits(const skg_ctx *c);
 int skg_exclusive_scan(skg_ctx *c, const int *in, int *out, int n);
 void skg_free_tree(DevTree *t);
 int skg_gradient_points(skg_ctx *c, const float4 *q, int nq, float *out);   // skg_adjoint.hip

@@ -187,7 +187,8 @@ __device__ __forceinline__ unsigned long long spread21(unsigned v) {
 }
 
 __global__ void k_morton(const float4 *__restrict__ pos, const int *__restrict__ idx, int n,
-		const unsigned *__restrict__ bb, unsigned long long *keys, int *vals) {
+		const unsigned *__restrict__ bb, unsigned long long *keys, int *vals, const int *__restrict__ box,
+		int bbits) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	int j = idx ? idx[i] : i;
@@ -202,19 +203,30 @@ __global__ void k_morton(const float4 *__restrict__ pos, const int *__restrict__
 		t = fminf(fmaxf(t, 0.f), 1.f);
 		q[a] = (unsigned)fminf(t * 2097152.0f, 2097151.0f);
 	}
-	keys[i] = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+	unsigned long long key = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+	// boxes first: the box id takes the top bits, the coarse Morton digits the rest
+	if (box && bbits > 0) key = ((unsigned long long)box[j] << (63 - bbits)) | (key >> bbits);
+	keys[i] = key;
 	vals[i] = j;
 }
 
-// perm_out[i] = index (into pos, or idx[] entry) of the i-th point in Morton order.
-int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out) {
+int skg_box_bits(const skg_ctx *c) {
+	int b = 0;
+	while ((1 << b) < c->nBox) ++b;
+	return b;
+}
+
+// perm_out[i] = index (into pos, or idx[] entry) of the i-th point in Morton
+// order; with box (indexed like pos) the order is by box first.
+int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box) {
 	if (n <= 0) return 0;
 	if (skg_ensure_keys(c, n)) return SKG_E_NOMEM;
 	const int nb = nblk(n, 256);
 	if (skg_ensure_part(c, 6 * (size_t)nb)) return SKG_E_NOMEM;
 	k_bbox<<<nb, 256, 0, c->st>>>(pos, idx, n, c->part);
 	k_bbox_final<<<1, 256, 0, c->st>>>(c->part, nb, (unsigned *)(c->d_cnt + 8));
-	k_morton<<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, (const unsigned *)(c->d_cnt + 8), c->k64a, c->v32a);
+	k_morton<<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, (const unsigned *)(c->d_cnt + 8), c->k64a, c->v32a,
+			box, box ? skg_box_bits(c) : 0);
 	SKG_HIP(hipGetLastError());
 	return skg_sort_pairs(c, c->k64a, c->k64b, c->v32a, perm_out, n, 63);
 }
@@ -222,21 +234,28 @@ int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *p
 // =====================================================================
 // Trees: complete binary tree over leaves of B consecutive sorted items.
 // =====================================================================
+// Leaf bounds skip filler items (NaN position); hi.w receives the leaf's box
+// range (ibox indexed like pos; none: box 0).
 __global__ void k_tree_leaves(const float4 *__restrict__ pos, int n, int B, int nLeaf, int nLeaf2,
-		const float *__restrict__ b2, float4 *lo, float4 *hi) {
+		const float *__restrict__ b2, const int *__restrict__ ibox, float4 *lo, float4 *hi) {
 	int l = blockIdx.x * blockDim.x + threadIdx.x;
 	if (l >= nLeaf2) return;
 	float4 a = make_float4(INFINITY, INFINITY, INFINITY, -INFINITY);
 	float4 b = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
+	int b0 = 0xFFFF, b1 = 0;
 	if (l < nLeaf) {
 		int s = l * B, e = min(n, s + B);
 		for (int i = s; i < e; ++i) {
+			const int bx = ibox ? ibox[i] : 0;
+			b0 = min(b0, bx); b1 = max(b1, bx);
 			float4 p = pos[i];
+			if (p.x != p.x) continue;
 			a.x = fminf(a.x, p.x); a.y = fminf(a.y, p.y); a.z = fminf(a.z, p.z);
 			b.x = fmaxf(b.x, p.x); b.y = fmaxf(b.y, p.y); b.z = fmaxf(b.z, p.z);
 			a.w = fmaxf(a.w, b2 ? b2[i] : 0.f);
 		}
 	}
+	b.w = brange_pack(b0, b1);
 	lo[nLeaf2 + l] = a;
 	hi[nLeaf2 + l] = b;
 }
@@ -246,28 +265,31 @@ __global__ void k_tree_level(float4 *lo, float4 *hi, int first, int count) {
 	if (t >= count) return;
 	int i = first + t;
 	float4 a0 = lo[2 * i], a1 = lo[2 * i + 1], b0 = hi[2 * i], b1 = hi[2 * i + 1];
+	const unsigned r0 = __float_as_uint(b0.w), r1 = __float_as_uint(b1.w);
+	const int x0 = min((int)(r0 & 0xFFFFu), (int)(r1 & 0xFFFFu)), x1 = max((int)(r0 >> 16), (int)(r1 >> 16));
 	lo[i] = make_float4(fminf(a0.x, a1.x), fminf(a0.y, a1.y), fminf(a0.z, a1.z), fmaxf(a0.w, a1.w));
-	hi[i] = make_float4(fmaxf(b0.x, b1.x), fmaxf(b0.y, b1.y), fmaxf(b0.z, b1.z), 0.f);
+	hi[i] = make_float4(fmaxf(b0.x, b1.x), fmaxf(b0.y, b1.y), fmaxf(b0.z, b1.z), brange_pack(x0, x1));
 }
 
 void skg_free_tree(DevTree *t) { dfree(&t->lo); dfree(&t->hi); t->n = t->nLeaf = t->nLeaf2 = 0; }
 
-static int tree_fill(skg_ctx *c, DevTree *t, const float4 *pos, const float *b2) {
-	k_tree_leaves<<<nblk(t->nLeaf2, 256), 256, 0, c->st>>>(pos, t->n, t->B, t->nLeaf, t->nLeaf2, b2, t->lo, t->hi);
+static int tree_fill(skg_ctx *c, DevTree *t, const float4 *pos, const float *b2, const int *ibox) {
+	k_tree_leaves<<<nblk(t->nLeaf2, 256), 256, 0, c->st>>>(pos, t->n, t->B, t->nLeaf, t->nLeaf2, b2, ibox,
+			t->lo, t->hi);
 	for (int m = t->nLeaf2 >> 1; m >= 1; m >>= 1)
 		k_tree_level<<<nblk(m, 256), 256, 0, c->st>>>(t->lo, t->hi, m, m);
 	SKG_HIP(hipGetLastError());
 	return 0;
 }
 
-int skg_build_tree(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, const float *b2) {
+int skg_build_tree(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, const float *b2, const int *ibox) {
 	int nLeaf = std::max(1, (n + B - 1) / B), nLeaf2 = 1;
 	while (nLeaf2 < nLeaf) nLeaf2 <<= 1;
 	if (nLeaf2 != t->nLeaf2 || !t->lo) {
 		if (dalloc(&t->lo, 2 * (size_t)nLeaf2) || dalloc(&t->hi, 2 * (size_t)nLeaf2)) return SKG_E_NOMEM;
 	}
 	t->n = n; t->B = B; t->nLeaf = nLeaf; t->nLeaf2 = nLeaf2;
-	return tree_fill(c, t, pos, b2);
+	return tree_fill(c, t, pos, b2, ibox);
 }
 
 
@@ -311,12 +333,16 @@ __device__ __forceinline__ float point_box_d2(float4 lo, float4 hi, float qx, fl
 
 __device__ __forceinline__ float key_d2(unsigned long long k) { return __uint_as_float((unsigned)(k >> 32)); }
 
-template <int KMAX, bool PER, bool COMP>
+// BATCH: several boxes share the arrays; a lane accepts only candidates whose
+// iOrder lies in its own box's range, and the walk skips subtrees holding none
+// of the block's boxes.
+template <int KMAX, bool PER, bool COMP, bool BATCH>
 __global__ __launch_bounds__(64) void k_knn(
 		const float4 *__restrict__ spos, const int *__restrict__ sord, int n,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		float L, int k, const float *__restrict__ mass_ord, const float *__restrict__ pos_ord,
-		float *__restrict__ ball2_out, float *__restrict__ rhoG, unsigned long long *__restrict__ heap_out) {
+		float *__restrict__ ball2_out, float *__restrict__ rhoG, unsigned long long *__restrict__ heap_out,
+		const int *__restrict__ sbox, const DevBox *__restrict__ bt) {
 	__shared__ unsigned long long heap[KMAX * 64];
 	__shared__ int stk[64];
 	__shared__ float stkd[64];
@@ -325,6 +351,10 @@ __global__ __launch_bounds__(64) void k_knn(
 	const int qi = base + lane;
 	const bool valid = qi < n;
 	const float4 q = spos[valid ? qi : base];
+	int qb = 0;
+	unsigned boff = 0u, bn = 0xFFFFFFFFu;
+	if (BATCH) { qb = sbox[valid ? qi : base]; boff = (unsigned)bt[qb].off; bn = (unsigned)bt[qb].n; }
+	const int wb0 = BATCH ? wave_min_i(qb) : 0, wb1 = BATCH ? wave_max_i(qb) : 0;
 	const float hL = 0.5f * L;
 	const float qxm = fsub(q.x, L), qxp = fadd(q.x, L);
 	const float qym = fsub(q.y, L), qyp = fadd(q.y, L);
@@ -338,11 +368,13 @@ __global__ __launch_bounds__(64) void k_knn(
 	{                                                                                     \
 		const float4 cc = spos[J];                                                        \
 		const unsigned oj = (unsigned)sord[J];                                            \
-		float dx = img_delta<PER>(q.x, qxm, qxp, cc.x, hL);                              \
-		float dy = img_delta<PER>(q.y, qym, qyp, cc.y, hL);                              \
-		float dz = img_delta<PER>(q.z, qzm, qzp, cc.z, hL);                              \
-		unsigned long long key = ((unsigned long long)__float_as_uint(sq3(dx, dy, dz)) << 32) | oj; \
-		if (key < top) top = heap_replace<KMAX>(heap, lane, k, key);                     \
+		if (!BATCH || oj - boff < bn) {                                                   \
+			float dx = img_delta<PER>(q.x, qxm, qxp, cc.x, hL);                          \
+			float dy = img_delta<PER>(q.y, qym, qyp, cc.y, hL);                          \
+			float dz = img_delta<PER>(q.z, qzm, qzp, cc.z, hL);                          \
+			unsigned long long key = ((unsigned long long)__float_as_uint(sq3(dx, dy, dz)) << 32) | oj; \
+			if (key < top) top = heap_replace<KMAX>(heap, lane, k, key);                 \
+		}                                                                                 \
 	}
 
 	// Seed with the block's own 64 points (the nearest tile of candidates).
@@ -360,19 +392,24 @@ __global__ __launch_bounds__(64) void k_knn(
 		const float4 lo = nlo[node], hi = nhi[node];
 		if (node < nLeaf2) {
 			const int c0 = 2 * node;
-			const float d0 = box_box_d2<PER>(nlo[c0], nhi[c0], blo, bhi, L);
-			const float d1 = box_box_d2<PER>(nlo[c0 + 1], nhi[c0 + 1], blo, bhi, L);
-			const int nf = d0 <= d1 ? c0 : c0 + 1;
-			const float dn = d0 <= d1 ? d0 : d1, df = d0 <= d1 ? d1 : d0;
-			if (df <= R2) { stk[sp] = nf ^ 1; stkd[sp] = df; ++sp; }
-			if (dn <= R2) { stk[sp] = nf; stkd[sp] = dn; ++sp; }
+			const float4 h0 = nhi[c0], h1 = nhi[c0 + 1];
+			const float d0 = box_box_d2<PER>(nlo[c0], h0, blo, bhi, L);
+			const float d1 = box_box_d2<PER>(nlo[c0 + 1], h1, blo, bhi, L);
+			const bool m0 = !BATCH || brange_meets(h0.w, wb0, wb1), m1 = !BATCH || brange_meets(h1.w, wb0, wb1);
+			const bool near0 = d0 <= d1;
+			const int nf = near0 ? c0 : c0 + 1;
+			const float dn = near0 ? d0 : d1, df = near0 ? d1 : d0;
+			const bool mn = near0 ? m0 : m1, mf = near0 ? m1 : m0;
+			if (mf && df <= R2) { stk[sp] = nf ^ 1; stkd[sp] = df; ++sp; }
+			if (mn && dn <= R2) { stk[sp] = nf; stkd[sp] = dn; ++sp; }
 			continue;
 		}
 		const int start = (node - nLeaf2) * B;
 		if (start >= n) continue;
 		const int cnt = min(B, n - start);
 		if (start >= base && start + cnt <= base + 64) continue;     // seeded tile
-		const bool need = valid && point_box_d2<PER>(lo, hi, q.x, q.y, q.z, L) <= key_d2(top);
+		const bool need = valid && (!BATCH || brange_has(hi.w, qb)) &&
+				point_box_d2<PER>(lo, hi, q.x, q.y, q.z, L) <= key_d2(top);
 		if (!__any(need)) continue;
 		if (need) {
 			for (int t = 0; t < cnt; ++t) {
@@ -422,19 +459,25 @@ __global__ __launch_bounds__(64) void k_knn(
 // reference's smx->pp).  MODE 0 accumulates the scatter half of the density,
 // MODE 1 the density gradient and then moves the point (kdMoveParticles).
 // =====================================================================
+// pbox (indexed like ppos; none: box 0) gives each point's box: a lane opens
+// only nodes holding its box, and the entity array starts every box on a new
+// leaf, so every entity a lane tests is of its own box.  MODE 1 takes the
+// box's step (micro: its micro-step).
 template <int MODE, bool COMP>
 __global__ __launch_bounds__(64) void k_cover(
 		float4 *__restrict__ ppos, const int *__restrict__ plist, int np,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
 		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
-		float fStep, float cx, float cy, float cz, float Lx, float Ly, float Lz) {
+		const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
+		float cx, float cy, float cz, float Lx, float Ly, float Lz) {
 	const int lane = threadIdx.x;
 	const int base = blockIdx.x * 64;
 	const int ii = base + lane;
 	const bool valid = ii < np;
 	const int pi = plist ? plist[valid ? ii : base] : (valid ? ii : base);
 	const float4 p = ppos[pi];
+	const int pb = pbox ? pbox[pi] : 0;
 	const float3 blo = make_float3(wave_min(p.x), wave_min(p.y), wave_min(p.z));
 	const float3 bhi = make_float3(wave_max(p.x), wave_max(p.y), wave_max(p.z));
 	Sum<COMP> s0, s1, s2;
@@ -445,7 +488,7 @@ __global__ __launch_bounds__(64) void k_cover(
 	int node = 1;
 	for (;;) {
 		const float4 lo = nlo[node], hi = nhi[node];
-		const bool need = valid && point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
+		const bool need = valid && brange_has(hi.w, pb) && point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
 		const bool visit = box_box_d2<false>(lo, hi, blo, bhi, 0.f) < lo.w && __any(need);
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit) {
@@ -490,6 +533,7 @@ __global__ __launch_bounds__(64) void k_cover(
 	if (!valid) return;
 	if (MODE == 0) { out_sum[ii] = s0.val(); return; }
 	// kdMoveParticles (kd.c:702-732)
+	const float fStep = micro ? bt[pb].fMicro : bt[pb].fStep;
 	const float ax = s0.val(), ay = s1.val(), az = s2.val();
 	const float ss = sq3(ax, ay, az);
 	float ai = (float)sqrt((double)ss);
@@ -507,27 +551,50 @@ __global__ __launch_bounds__(64) void k_cover(
 // =====================================================================
 // Small kernels: gathers, replicas, entity setup, ScatterCut, prune.
 // =====================================================================
+// ScatterCriterion (kd.c:600-627) depends only on the species: flag = the
+// particle's species is in the selected set.
+__global__ void k_species_flags(const unsigned char *__restrict__ type, int n, int mask, int *flag) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) flag[i] = (type[i] & mask) != 0;
+}
+
 __global__ void k_gather_scat(const float *__restrict__ pos, const float *__restrict__ mass,
-		const int *__restrict__ lo_hi /* up to 3 ranges */, int nr, int nS, float4 *out4, int *ord) {
+		const int *__restrict__ ord, const int *__restrict__ boxOf, int nS, float4 *out4, int *obox) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= nS) return;
-	int o = -1, acc = 0;
-	for (int r = 0; r < nr; ++r) {
-		int a = lo_hi[2 * r], b = lo_hi[2 * r + 1];
-		if (i < acc + (b - a)) { o = a + (i - acc); break; }
-		acc += b - a;
-	}
+	const int o = ord[i];
 	out4[i] = make_float4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], mass[o]);
-	ord[i] = o;
+	obox[i] = boxOf[o];
 }
 
 __global__ void k_permute_s(const float4 *__restrict__ in4, const int *__restrict__ inord,
-		const int *__restrict__ perm, int n, float4 *out4, int *outord) {
+		const int *__restrict__ inbox, const int *__restrict__ perm, int n, float4 *out4, int *outord,
+		int *outbox) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	int j = perm[i];
 	out4[i] = in4[j];
 	outord[i] = inord[j];
+	outbox[i] = inbox[j];
+}
+
+// Box of each particle (binary search over box offsets) and its species from
+// the box's tipsy order (gas, dark, star).  binfo: off, nGas, nDark per box.
+__global__ void k_box_assign(int n, int nBox, const int *__restrict__ binfo, int *boxOf, unsigned char *type) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	int lo = 0, hi = nBox - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (binfo[3 * mid] <= i) lo = mid; else hi = mid - 1;
+	}
+	const int l = i - binfo[3 * lo], ng = binfo[3 * lo + 1], nd = binfo[3 * lo + 2];
+	boxOf[i] = lo;
+	type[i] = (unsigned char)(l < ng ? SKG_GAS : (l < ng + nd ? SKG_DARK : SKG_STAR));
+}
+__global__ void k_temp_nongas(const unsigned char *__restrict__ type, int n, float *temp) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && type[i] != SKG_GAS) temp[i] = 0.f;
 }
 
 // Replica scatterers: the 26 images whose ball reaches the box (smooth1.c:283-331),
@@ -585,14 +652,46 @@ __global__ void k_rep_write(const float4 *__restrict__ spos, const float *__rest
 				}
 }
 
-// Entities in Morton order + per-entity constants (smooth1.c:447-448, 250-251).
-__global__ void k_entity_setup(const float4 *__restrict__ e4u, const float *__restrict__ eb2u,
-		const int *__restrict__ esrcu, const int *__restrict__ perm, int ne,
-		const float *__restrict__ srho, const float4 *__restrict__ spos, float Lx, float Ly, float Lz,
-		float4 *e4, float4 *eaux, float4 *eorg, float *erho, int *esrc, int *elive, int *ehit, float *eb2s) {
+// Box of every unsorted entity (from its source scatterer).
+__global__ void k_entity_box(const int *__restrict__ esrcu, const int *__restrict__ sbox, int ne, int *ebu) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < ne) ebu[i] = sbox[esrcu[i]];
+}
+// First sorted entity of each box (entities are sorted by box first).
+__global__ void k_box_first(const int *__restrict__ ebu, const int *__restrict__ perm, int ne, int *first) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= ne) return;
-	int j = perm[i];
+	const int b = ebu[perm[i]];
+	if (i == 0 || ebu[perm[i - 1]] != b) first[b] = i;
+}
+// Filler entities of the padded array: dead, NaN position, never hit.
+__global__ void k_entity_fill(int ne, const int *__restrict__ eblk, int nBox, float4 *e4, float4 *eaux,
+		float4 *eorg, float *erho, int *esrc, int *elive, int *ehit, float *eb2s, int *ebox) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= ne) return;
+	int lo = 0, hi = nBox - 1;
+	while (lo < hi) {
+		const int mid = (lo + hi + 1) >> 1;
+		if (eblk[mid] * 256 <= i) lo = mid; else hi = mid - 1;
+	}
+	e4[i] = make_float4(NAN, NAN, NAN, 0.f);
+	eaux[i] = make_float4(-1.f, 0.f, 0.f, 0.f);
+	eorg[i] = make_float4(NAN, NAN, NAN, 13.f);
+	erho[i] = 0.f; esrc[i] = -1; elive[i] = 0; ehit[i] = -1; eb2s[i] = -1.f; ebox[i] = lo;
+}
+
+// Entities in Morton order + per-entity constants (smooth1.c:447-448, 250-251),
+// written to their box's padded range.
+__global__ void k_entity_setup(const float4 *__restrict__ e4u, const float *__restrict__ eb2u,
+		const int *__restrict__ esrcu, const int *__restrict__ perm, int ne,
+		const int *__restrict__ ebu, const int *__restrict__ bfirst, const int *__restrict__ eblk,
+		const float *__restrict__ srho, const float4 *__restrict__ spos, float Lx, float Ly, float Lz,
+		float4 *e4, float4 *eaux, float4 *eorg, float *erho, int *esrc, int *elive, int *ehit, float *eb2s) {
+	int ii = blockIdx.x * blockDim.x + threadIdx.x;
+	if (ii >= ne) return;
+	int j = perm[ii];
+	const int bx = ebu[j];
+	const int i = eblk[bx] * 256 + (ii - bfirst[bx]);
 	float4 p = e4u[j];
 	float b2 = eb2u[j];
 	float ih2 = (float)(4.0 / (double)b2);
@@ -616,7 +715,7 @@ __global__ void k_entity_setup(const float4 *__restrict__ e4u, const float *__re
 
 __global__ void k_entity_rho(const int *__restrict__ esrc, int ne, const float *__restrict__ srho, float *erho) {
 	int e = blockIdx.x * blockDim.x + threadIdx.x;
-	if (e < ne) erho[e] = srho[esrc[e]];
+	if (e < ne) erho[e] = esrc[e] >= 0 ? srho[esrc[e]] : 0.f;
 }
 
 __global__ void k_combine_rho(const float *__restrict__ g, const float *__restrict__ s,
@@ -670,16 +769,35 @@ __global__ __launch_bounds__(256) void k_min_final(const float *__restrict__ par
 	if (threadIdx.x == 0) out[0] = fminf(fminf(sm[0], sm[1]), fminf(sm[2], sm[3]));
 }
 
-// ScatterCut (smooth1.c:387-405): keep entities with density >= threshold
-// (no hit anywhere: threshold 0, everything kept).  Block live counts go to
-// partials.
+// One wavefront per box folds that box's block partials (a box's entities
+// fill whole 256-entity blocks).
+__global__ __launch_bounds__(64) void k_box_min(const float *__restrict__ part, const int *__restrict__ eblk,
+		float *out) {
+	const int b = blockIdx.x;
+	float v = INFINITY;
+	for (int i = eblk[b] + (int)threadIdx.x; i < eblk[b + 1]; i += 64) v = fminf(v, part[i]);
+	v = wave_min(v);
+	if (threadIdx.x == 0) out[b] = v;
+}
+__global__ __launch_bounds__(64) void k_box_sum(const int *__restrict__ part, const int *__restrict__ eblk,
+		int *out) {
+	const int b = blockIdx.x;
+	int v = 0;
+	for (int i = eblk[b] + (int)threadIdx.x; i < eblk[b + 1]; i += 64) v += part[i];
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	if (threadIdx.x == 0) out[b] = v;
+}
+
+// ScatterCut (smooth1.c:387-405): keep entities with density >= their box's
+// threshold (no hit in the box: threshold 0, everything kept).  Block live
+// counts go to partials.
 __global__ __launch_bounds__(256) void k_scat_cut(int *elive, float4 *eaux, const float *__restrict__ erho, int ne,
-		const float *__restrict__ thr, int *part) {
+		const int *__restrict__ ebox, const float *__restrict__ thr, int *part) {
 	__shared__ int sm[4];
 	int e = blockIdx.x * blockDim.x + threadIdx.x;
 	int live = 0;
 	if (e < ne) {
-		const float t0 = thr[0];
+		const float t0 = thr[ebox[e]];
 		const float t = (t0 == INFINITY) ? 0.f : t0;
 		live = elive[e];
 		if (live && !(erho[e] >= t)) { live = 0; elive[e] = 0; eaux[e].x = -1.f; }
@@ -707,10 +825,11 @@ __global__ void k_live_b2(const float4 *__restrict__ eaux, int ne, float *b2live
 // kdPruneInactive test (kd.c:754-776): a mover stays active while it moved at
 // least fCvg over the last block; survivors restart from here.
 __global__ void k_prune(const int *__restrict__ list, int na, const float4 *__restrict__ mpos, float4 *mold,
-		float hx, float hy, float hz, float cv2, int *flag) {
+		float hx, float hy, float hz, const int *__restrict__ mbox, const DevBox *__restrict__ bt, int *flag) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= na) return;
 	int m = list[i];
+	const float cv2 = bt[mbox[m]].cvg2;
 	float4 p = mpos[m], o = mold[m];
 	float dx = fsub(p.x, o.x), dy = fsub(p.y, o.y), dz = fsub(p.z, o.z);
 	if (dx > hx) dx = fsub(dx, 2 * hx);
@@ -730,13 +849,27 @@ __global__ void k_compact(const int *__restrict__ list, const int *__restrict__ 
 	if (i < n && flag[i]) out[off[i]] = list ? list[i] : i;
 }
 
+// Active movers per box (lanes of one box add once per wavefront).
+__global__ void k_box_count(const int *__restrict__ list, int n, const int *__restrict__ mbox, int *cnt) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	const int b = i < n ? mbox[list[i]] : -1;
+	const int b0 = __shfl(b, 0, 64);
+	const bool same = __all(b == b0 || b < 0);
+	if (same) {
+		const int c = __popcll(__ballot(b >= 0));
+		if ((threadIdx.x & 63) == 0 && b0 >= 0 && c) atomicAdd(&cnt[b0], c);
+	} else if (b >= 0) {
+		atomicAdd(&cnt[b], 1);
+	}
+}
+
 // CutCriterion (kd.c:555-597) per particle by iOrder.
-__global__ void k_mover_flags(int n, int nGas, int nDark, int inType, const float *__restrict__ rho,
+__global__ void k_mover_flags(int n, const unsigned char *__restrict__ type, int inType, const float *__restrict__ rho,
 		const float *__restrict__ mass, const float *__restrict__ temp, float fDensMin, float fTempMax,
 		float fMassMax, int gd, int go, int *flag) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
-	int t = i < nGas ? SKG_GAS : (i < nGas + nDark ? SKG_DARK : SKG_STAR);
+	int t = type[i];
 	int ok = 0;
 	float r = rho[i];
 	if (!(mass[i] > fMassMax)) {
@@ -758,12 +891,13 @@ __global__ void k_mover_flags(int n, int nGas, int nDark, int inType, const floa
 }
 
 __global__ void k_mover_init(const int *__restrict__ ord, int nm, const float *__restrict__ pos,
-		float4 *mpos, float4 *mold) {
+		const int *__restrict__ boxOf, float4 *mpos, float4 *mold, int *mbox) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= nm) return;
 	int o = ord[i];
 	float4 p = make_float4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], 0.f);
 	mpos[i] = p; mold[i] = p;
+	mbox[i] = boxOf[o];
 }
 
 __global__ void k_scatter_moved(const int *__restrict__ ord, int nm, const float4 *__restrict__ mpos, float *r) {
@@ -837,10 +971,14 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->d_pos); dfree(&c->d_vel); dfree(&c->d_mass); dfree(&c->d_soft); dfree(&c->d_temp);
 	dfree(&c->d_rho); dfree(&c->d_ball2); dfree(&c->d_grp);
 	dfree(&c->s_pos); dfree(&c->s_ord); dfree(&c->s_ball2); dfree(&c->s_rhoG); dfree(&c->s_rhoS);
-	dfree(&c->s_heap);
+	dfree(&c->s_heap); dfree(&c->s_box);
 	skg_free_tree(&c->ts); skg_free_tree(&c->te); skg_free_tree(&c->tfull);
 	dfree(&c->e_b2); dfree(&c->e_pos); dfree(&c->e_aux); dfree(&c->e_org); dfree(&c->e_rho); dfree(&c->e_src); dfree(&c->e_live); dfree(&c->e_hit);
-	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2);
+	dfree(&c->e_box);
+	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2); dfree(&c->m_box);
+	dfree(&c->d_bt); dfree(&c->d_boxOf); dfree(&c->d_type); dfree(&c->d_eblk); dfree(&c->d_bact); dfree(&c->d_bscat);
+	dfree(&c->d_bthr);
+	delete[] c->hbox; delete[] c->h_eblk;
 	dfree(&c->d_thr); dfree(&c->d_cnt);
 	if (c->tmp) (void)hipFree(c->tmp);
 	dfree(&c->k64a); dfree(&c->k64b); dfree(&c->v32a); dfree(&c->v32b); dfree(&c->part);
@@ -853,18 +991,57 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	return SKG_OK;
 }
 
-extern "C" int skg_load(skg_ctx *c, int nGas, int nDark, int nStar, const float *pos, const float *vel,
-		const float *mass, const float *soft, const float *temp) {
-	if (!c || !pos || !mass || !soft || nGas < 0 || nDark < 0 || nStar < 0) return SKG_E_PARAM;
-	long n = (long)nGas + nDark + nStar;
-	if (n <= 0 || n > 0x7FFFFFF0L) return SKG_E_PARAM;
+extern "C" int skg_load_batch(skg_ctx *c, int nBox, const int *nGas, const int *nDark, const int *nStar,
+		const float *fTau, const float *pos, const float *vel, const float *mass, const float *soft,
+		const float *temp) {
+	if (!c || !pos || !mass || !soft || !nGas || !nDark || !nStar || nBox < 1 || nBox > SKG_MAX_BOX)
+		return SKG_E_PARAM;
+	long n = 0;
+	int inType = -1, tg = 0, td = 0, ts = 0;
+	for (int b = 0; b < nBox; ++b) {
+		if (nGas[b] < 0 || nDark[b] < 0 || nStar[b] < 0) return SKG_E_PARAM;
+		const long nb = (long)nGas[b] + nDark[b] + nStar[b];
+		if (nb <= 0) return SKG_E_PARAM;
+		if (fTau && !(fTau[b] > 0.f)) return SKG_E_PARAM;
+		const int t = (nDark[b] ? SKG_DARK : 0) | (nGas[b] ? SKG_GAS : 0) | (nStar[b] ? SKG_STAR : 0);
+		if (inType >= 0 && t != inType) return SKG_E_PARAM;   // one species mix per batch
+		inType = t;
+		n += nb; tg += nGas[b]; td += nDark[b]; ts += nStar[b];
+	}
+	if (n > 0x7FFFFFF0L) return SKG_E_PARAM;
 	(void)hipSetDevice(c->dev);
-	c->nGas = nGas; c->nDark = nDark; c->nStar = nStar; c->n = (int)n;
-	c->inType = (nDark ? SKG_DARK : 0) | (nGas ? SKG_GAS : 0) | (nStar ? SKG_STAR : 0);
+	c->nBox = nBox; c->nGas = tg; c->nDark = td; c->nStar = ts; c->n = (int)n; c->inType = inType;
+	delete[] c->hbox;
+	c->hbox = new (std::nothrow) HostBox[nBox];
+	if (!c->hbox) return SKG_E_NOMEM;
+	std::vector<DevBox> db(nBox);
+	std::vector<int> binfo(3 * (size_t)nBox);
+	for (int b = 0, off = 0; b < nBox; ++b) {
+		HostBox &h = c->hbox[b];
+		h.off = off; h.nGas = nGas[b]; h.nDark = nDark[b]; h.nStar = nStar[b];
+		h.n = nGas[b] + nDark[b] + nStar[b];
+		if (fTau) {   // main.c:343-345 defaults from the box's tau
+			h.tau = fTau[b]; h.cvg = (float)(0.5 * (double)h.tau); h.scoop = (float)(2.0 * (double)h.tau);
+			h.step = (float)(0.5 * (double)h.cvg);
+		} else {
+			h.tau = c->P.fTau; h.cvg = c->P.fCvg; h.scoop = c->P.fScoop; h.step = c->P.fStep;
+		}
+		DevBox &d = db[b];
+		d.off = off; d.n = h.n; d.fStep = h.step; d.fMicro = (float)(0.1 * (double)h.step);
+		d.cvg2 = h.cvg * h.cvg; d.tau2 = h.tau * h.tau; d.scoop2 = h.scoop * h.scoop; d.pad = 0.f;
+		binfo[3 * b] = off; binfo[3 * b + 1] = nGas[b]; binfo[3 * b + 2] = nDark[b];
+		off += h.n;
+	}
+	int *dbi = nullptr;
 	if (dalloc(&c->d_pos, 3 * n) || dalloc(&c->d_vel, 3 * n) || dalloc(&c->d_mass, n) ||
 	    dalloc(&c->d_soft, n) || dalloc(&c->d_temp, n) || dalloc(&c->d_rho, n) || dalloc(&c->d_ball2, n) ||
-	    dalloc(&c->d_grp, n))
+	    dalloc(&c->d_grp, n) || dalloc(&c->d_boxOf, n) || dalloc(&c->d_type, n) || dalloc(&c->d_bt, nBox) ||
+	    dalloc(&c->d_bact, nBox) || dalloc(&c->d_bscat, nBox) || dalloc(&c->d_bthr, nBox) ||
+	    dalloc(&dbi, 3 * (size_t)nBox))
 		return SKG_E_NOMEM;
+	SKG_HIP(hipMemcpyAsync(c->d_bt, db.data(), sizeof(DevBox) * nBox, hipMemcpyHostToDevice, c->st));
+	SKG_HIP(hipMemcpyAsync(dbi, binfo.data(), 12 * (size_t)nBox, hipMemcpyHostToDevice, c->st));
+	k_box_assign<<<nblk(n, 256), 256, 0, c->st>>>((int)n, nBox, dbi, c->d_boxOf, c->d_type);
 	SKG_HIP(hipMemcpyAsync(c->d_pos, pos, 12 * n, hipMemcpyHostToDevice, c->st));
 	if (vel) SKG_HIP(hipMemcpyAsync(c->d_vel, vel, 12 * n, hipMemcpyHostToDevice, c->st));
 	else SKG_HIP(hipMemsetAsync(c->d_vel, 0, 12 * n, c->st));
@@ -872,47 +1049,64 @@ extern "C" int skg_load(skg_ctx *c, int nGas, int nDark, int nStar, const float 
 	SKG_HIP(hipMemcpyAsync(c->d_soft, soft, 4 * n, hipMemcpyHostToDevice, c->st));
 	if (temp) {
 		SKG_HIP(hipMemcpyAsync(c->d_temp, temp, 4 * n, hipMemcpyHostToDevice, c->st));
-		if (nGas < n) SKG_HIP(hipMemsetAsync(c->d_temp + nGas, 0, 4 * (n - nGas), c->st));
+		k_temp_nongas<<<nblk(n, 256), 256, 0, c->st>>>(c->d_type, (int)n, c->d_temp);
 	} else SKG_HIP(hipMemsetAsync(c->d_temp, 0, 4 * n, c->st));
+	SKG_HIP(hipGetLastError());
 	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&dbi);
 	c->stage = 1;
 	return SKG_OK;
 }
 
-// ScatterCriterion (kd.c:600-627) depends only on the species, so the
-// scatterer set is a union of iOrder ranges.
-static int scatter_ranges(const skg_ctx *c, int *r) {
-	int gd = c->P.bGasAndDark, go = c->P.bGasOnly, nr = 0;
-	int g0 = 0, g1 = c->nGas, d0 = g1, d1 = g1 + c->nDark, s0 = d1, s1 = c->n;
-	bool sg = false, sd = false, ss = false;
-	switch (c->inType) {
-	case SKG_DARK: sd = true; break;
-	case SKG_GAS: case SKG_DARK | SKG_GAS: sg = true; sd = gd; break;
-	case SKG_STAR: case SKG_DARK | SKG_STAR: ss = true; break;
-	case SKG_GAS | SKG_STAR: case SKG_DARK | SKG_GAS | SKG_STAR:
-		if (gd) { sg = sd = ss = true; } else { sg = true; ss = !go; }
-		break;
-	}
-	if (sg && g1 > g0) { r[2 * nr] = g0; r[2 * nr + 1] = g1; ++nr; }
-	if (sd && d1 > d0) { r[2 * nr] = d0; r[2 * nr + 1] = d1; ++nr; }
-	if (ss && s1 > s0) { r[2 * nr] = s0; r[2 * nr + 1] = s1; ++nr; }
-	return nr;
+extern "C" int skg_load(skg_ctx *c, int nGas, int nDark, int nStar, const float *pos, const float *vel,
+		const float *mass, const float *soft, const float *temp) {
+	return skg_load_batch(c, 1, &nGas, &nDark, &nStar, nullptr, pos, vel, mass, soft, temp);
 }
 
-template <int KMAX>
+extern "C" int skg_box_counts(skg_ctx *c, int *nActive, int *nScatter) {
+	if (!c) return SKG_E_PARAM;
+	if (c->stage < 3) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	if (nActive) SKG_HIP(hipMemcpyAsync(nActive, c->d_bact, 4 * (size_t)c->nBox, hipMemcpyDeviceToHost, c->st));
+	if (nScatter) SKG_HIP(hipMemcpyAsync(nScatter, c->d_bscat, 4 * (size_t)c->nBox, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	return SKG_OK;
+}
+
+// ScatterCriterion (kd.c:600-627) selects whole species: the mask of scatterer
+// species for the context's species mix.
+static int scatter_species(const skg_ctx *c) {
+	const int gd = c->P.bGasAndDark, go = c->P.bGasOnly;
+	switch (c->inType) {
+	case SKG_DARK: return SKG_DARK;
+	case SKG_GAS: case SKG_DARK | SKG_GAS: return SKG_GAS | (gd ? SKG_DARK : 0);
+	case SKG_STAR: case SKG_DARK | SKG_STAR: return SKG_STAR;
+	case SKG_GAS | SKG_STAR: case SKG_DARK | SKG_GAS | SKG_STAR:
+		return gd ? (SKG_GAS | SKG_DARK | SKG_STAR) : (SKG_GAS | (go ? 0 : SKG_STAR));
+	}
+	return 0;
+}
+
+template <int KMAX, bool BATCH>
 static void launch_knn(skg_ctx *c, bool per, bool comp) {
 	const int nb = nblk(c->nS, 64);
 	const float L = per ? c->P.fPeriod[0] : FLT_MAX;
 #define KL(P_, C_)                                                                           \
-	k_knn<KMAX, P_, C_><<<nb, 64, 0, c->st>>>(c->s_pos, c->s_ord, c->nS, c->ts.lo, c->ts.hi,   \
-			c->ts.nLeaf2, c->ts.B, L, c->P.nSmooth, c->d_mass, c->d_pos, c->s_ball2, c->s_rhoG, c->s_heap)
+	k_knn<KMAX, P_, C_, BATCH><<<nb, 64, 0, c->st>>>(c->s_pos, c->s_ord, c->nS, c->ts.lo, c->ts.hi, \
+			c->ts.nLeaf2, c->ts.B, L, c->P.nSmooth, c->d_mass, c->d_pos, c->s_ball2, c->s_rhoG, c->s_heap, \
+			c->s_box, c->d_bt)
 	if (per) { if (comp) KL(true, true); else KL(true, false); }
 	else { if (comp) KL(false, true); else KL(false, false); }
 #undef KL
 }
+template <int KMAX>
+static void launch_knn(skg_ctx *c, bool per, bool comp) {
+	if (c->nBox > 1) launch_knn<KMAX, true>(c, per, comp);
+	else launch_knn<KMAX, false>(c, per, comp);
+}
 
-static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, int np, float *out,
-		int stamp, float fStep, double work) {
+static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, int np, const int *pbox,
+		float *out, int stamp, int micro, double work) {
 	if (np <= 0) return 0;
 	const int pslot = skg_prof_begin(c);
 	const int nb = nblk(np, 64);
@@ -920,7 +1114,8 @@ static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, in
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
 #define CL(M_, C_)                                                                             \
 	k_cover<M_, C_><<<nb, 64, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, \
-			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, fStep, C[0], C[1], C[2], L[0], L[1], L[2])
+			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, \
+			C[0], C[1], C[2], L[0], L[1], L[2])
 	if (mode == 0) { if (comp) CL(0, true); else CL(0, false); }
 	else { if (comp) CL(1, true); else CL(1, false); }
 #undef CL
@@ -934,34 +1129,41 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 	if (c->stage < 1) return SKG_E_STATE;
 	(void)hipSetDevice(c->dev);
 	int rc;
-	const int k = c->P.nSmooth, B = c->P.nBucket;
+	const int k = c->P.nSmooth, B = c->P.nBucket, n = c->n, nBox = c->nBox;
 	const bool per = c->P.bPeriodic != 0;
 	if (per && !(c->P.fPeriod[0] == c->P.fPeriod[1] && c->P.fPeriod[1] == c->P.fPeriod[2]))
 		return SKG_E_PARAM;   // -p sets one length for all axes (main.c:273-281)
-	int r[6];
-	int nr = scatter_ranges(c, r);
+	if (256 % B != 0 && nBox > 1) return SKG_E_PARAM;   // boxes start on whole leaves
+	const int smask = scatter_species(c);
 	int nS = 0;
-	for (int i = 0; i < nr; ++i) nS += r[2 * i + 1] - r[2 * i];
+	for (int b = 0; b < nBox; ++b) {
+		const HostBox &h = c->hbox[b];
+		const int nb = ((smask & SKG_GAS) ? h.nGas : 0) + ((smask & SKG_DARK) ? h.nDark : 0) +
+				((smask & SKG_STAR) ? h.nStar : 0);
+		if (nb > 0 && k > nb) return SKG_E_NSMOOTH;   // smooth1.c:12, per box
+		nS += nb;
+	}
 	c->nS = nS;
-	SKG_HIP(hipMemsetAsync(c->d_rho, 0, 4 * (size_t)c->n, c->st));
-	k_fill_f<<<nblk(c->n, 256), 256, 0, c->st>>>(c->d_ball2, c->n, -2.f);
-	if (nS > 0 && k > nS) return SKG_E_NSMOOTH;
+	SKG_HIP(hipMemsetAsync(c->d_rho, 0, 4 * (size_t)n, c->st));
+	k_fill_f<<<nblk(n, 256), 256, 0, c->st>>>(c->d_ball2, n, -2.f);
 	skg_timer_start(c);
 	if (nS > 0) {
-		// scatterers in Morton order
+		// scatterers (kdScatterActive) in (box, Morton) order
 		float4 *tmp4 = nullptr;
-		int *tmpo = nullptr, *perm = nullptr, *rng = nullptr;
-		if (dalloc(&c->s_pos, nS) || dalloc(&c->s_ord, nS) || dalloc(&c->s_ball2, nS) ||
+		int *tmpo = nullptr, *tmpb = nullptr, *perm = nullptr, *flag = nullptr, *off = nullptr;
+		if (dalloc(&c->s_pos, nS) || dalloc(&c->s_ord, nS) || dalloc(&c->s_box, nS) || dalloc(&c->s_ball2, nS) ||
 		    dalloc(&c->s_rhoG, nS) || dalloc(&c->s_rhoS, nS) || dalloc(&tmp4, nS) || dalloc(&tmpo, nS) ||
-		    dalloc(&perm, nS) || dalloc(&rng, 6))
+		    dalloc(&tmpb, nS) || dalloc(&perm, nS) || dalloc(&flag, n) || dalloc(&off, n))
 			return SKG_E_NOMEM;
-		SKG_HIP(hipMemcpyAsync(rng, r, sizeof(r), hipMemcpyHostToDevice, c->st));
-		k_gather_scat<<<nblk(nS, 256), 256, 0, c->st>>>(c->d_pos, c->d_mass, rng, nr, nS, tmp4, tmpo);
-		if ((rc = skg_morton_sort(c, tmp4, nullptr, nS, perm))) return rc;
-		k_permute_s<<<nblk(nS, 256), 256, 0, c->st>>>(tmp4, tmpo, perm, nS, c->s_pos, c->s_ord);
-		if ((rc = skg_build_tree(c, &c->ts, c->s_pos, nS, B, nullptr))) return rc;
+		k_species_flags<<<nblk(n, 256), 256, 0, c->st>>>(c->d_type, n, smask, flag);
+		if ((rc = skg_exclusive_scan(c, flag, off, n))) return rc;
+		k_compact<<<nblk(n, 256), 256, 0, c->st>>>(nullptr, flag, off, n, tmpo);
+		k_gather_scat<<<nblk(nS, 256), 256, 0, c->st>>>(c->d_pos, c->d_mass, tmpo, c->d_boxOf, nS, tmp4, tmpb);
+		if ((rc = skg_morton_sort(c, tmp4, nullptr, nS, perm, nBox > 1 ? tmpb : nullptr))) return rc;
+		k_permute_s<<<nblk(nS, 256), 256, 0, c->st>>>(tmp4, tmpo, tmpb, perm, nS, c->s_pos, c->s_ord, c->s_box);
+		if ((rc = skg_build_tree(c, &c->ts, c->s_pos, nS, B, nullptr, nBox > 1 ? c->s_box : nullptr))) return rc;
 		SKG_HIP(hipStreamSynchronize(c->st));
-		dfree(&tmp4); dfree(&tmpo); dfree(&perm); dfree(&rng);
+		dfree(&tmp4); dfree(&tmpo); dfree(&tmpb); dfree(&perm); dfree(&flag); dfree(&off);
 		skg_timer_stop(c, SKG_T_TREE);
 
 		// kNN radii + gather half of the density
@@ -980,8 +1182,8 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		// replicas (periodic) and the entity set S'
 		skg_timer_start(c);
 		unsigned *mask = nullptr;
-		int *cnt = nullptr, *off = nullptr;
-		if (dalloc(&mask, nS) || dalloc(&cnt, nS) || dalloc(&off, nS)) return SKG_E_NOMEM;
+		int *cnt = nullptr, *roff = nullptr;
+		if (dalloc(&mask, nS) || dalloc(&cnt, nS) || dalloc(&roff, nS)) return SKG_E_NOMEM;
 		int nRep = 0;
 		if (per) {
 			const float *L = c->P.fPeriod, *C = c->P.fCenter;
@@ -989,44 +1191,72 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 			for (int j = 0; j < 3; ++j) { lo[j] = C[j] - 0.5 * L[j]; hi[j] = C[j] + 0.5 * L[j]; }
 			k_rep_mask<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_pos, c->s_ball2, nS, lo[0], lo[1], lo[2],
 					hi[0], hi[1], hi[2], L[0], L[1], L[2], mask, cnt);
-			if ((rc = skg_exclusive_scan(c, cnt, off, nS))) return rc;
+			if ((rc = skg_exclusive_scan(c, cnt, roff, nS))) return rc;
 			int last[2];
-			SKG_HIP(hipMemcpyAsync(&last[0], off + nS - 1, 4, hipMemcpyDeviceToHost, c->st));
+			SKG_HIP(hipMemcpyAsync(&last[0], roff + nS - 1, 4, hipMemcpyDeviceToHost, c->st));
 			SKG_HIP(hipMemcpyAsync(&last[1], cnt + nS - 1, 4, hipMemcpyDeviceToHost, c->st));
 			SKG_HIP(hipStreamSynchronize(c->st));
 			nRep = last[0] + last[1];
 		} else {
 			SKG_HIP(hipMemsetAsync(mask, 0, 4 * (size_t)nS, c->st));
-			SKG_HIP(hipMemsetAsync(off, 0, 4 * (size_t)nS, c->st));
+			SKG_HIP(hipMemsetAsync(roff, 0, 4 * (size_t)nS, c->st));
 		}
 		c->nRep = nRep;
-		c->nE = nS + nRep;
 		skg_free_tree(&c->tfull);
-		c->nLive = c->nE;
-		const int nE = c->nE;
+		const int nEu = nS + nRep;
+		c->nLive = nEu;
 		float4 *e4u = nullptr;
-		float *eb2u = nullptr, *eb2s = nullptr;
-		int *esrcu = nullptr, *perm2 = nullptr;
-		if (dalloc(&e4u, nE) || dalloc(&eb2u, nE) || dalloc(&esrcu, nE) || dalloc(&perm2, nE) ||
-		    dalloc(&eb2s, nE) || dalloc(&c->e_pos, nE) || dalloc(&c->e_aux, nE) || dalloc(&c->e_rho, nE) ||
-		    dalloc(&c->e_org, nE) ||
-		    dalloc(&c->e_src, nE) || dalloc(&c->e_live, nE) || dalloc(&c->e_hit, nE))
+		float *eb2u = nullptr;
+		int *esrcu = nullptr, *perm2 = nullptr, *ebu = nullptr, *bfirst = nullptr;
+		if (dalloc(&e4u, nEu) || dalloc(&eb2u, nEu) || dalloc(&esrcu, nEu) || dalloc(&perm2, nEu) ||
+		    dalloc(&ebu, nEu) || dalloc(&bfirst, nBox))
 			return SKG_E_NOMEM;
 		const float *L = c->P.fPeriod;
-		k_rep_write<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_pos, c->s_ball2, nS, mask, off,
+		k_rep_write<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_pos, c->s_ball2, nS, mask, roff,
 				per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, e4u, eb2u, esrcu);
-		if ((rc = skg_morton_sort(c, e4u, nullptr, nE, perm2))) return rc;
-		k_entity_setup<<<nblk(nE, 256), 256, 0, c->st>>>(e4u, eb2u, esrcu, perm2, nE, nullptr, c->s_pos,
-				per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, c->e_pos, c->e_aux, c->e_org, c->e_rho,
-				c->e_src, c->e_live, c->e_hit, eb2s);
-		if ((rc = skg_build_tree(c, &c->te, c->e_pos, nE, B, eb2s))) return rc;
+		k_entity_box<<<nblk(nEu, 256), 256, 0, c->st>>>(esrcu, c->s_box, nEu, ebu);
+		if ((rc = skg_morton_sort(c, e4u, nullptr, nEu, perm2, nBox > 1 ? ebu : nullptr))) return rc;
+		// each box's entities start on a 256-entity block (and so on a leaf)
+		k_fill_i<<<nblk(nBox, 256), 256, 0, c->st>>>(bfirst, nBox, -1);
+		k_box_first<<<nblk(nEu, 256), 256, 0, c->st>>>(ebu, perm2, nEu, bfirst);
+		std::vector<int> hf(nBox);
+		SKG_HIP(hipMemcpyAsync(hf.data(), bfirst, 4 * (size_t)nBox, hipMemcpyDeviceToHost, c->st));
 		SKG_HIP(hipStreamSynchronize(c->st));
-		dfree(&mask); dfree(&cnt); dfree(&off); dfree(&e4u); dfree(&eb2u); dfree(&esrcu); dfree(&perm2);
+		delete[] c->h_eblk;
+		c->h_eblk = new (std::nothrow) int[nBox + 1];
+		if (!c->h_eblk) return SKG_E_NOMEM;
+		{
+			std::vector<int> cntb(nBox, 0);
+			// boxes without entities have first = -1; counts from the next present box
+			int nextStart = nEu;
+			for (int b = nBox - 1; b >= 0; --b)
+				if (hf[b] >= 0) { cntb[b] = nextStart - hf[b]; nextStart = hf[b]; }
+			int blk = 0;
+			for (int b = 0; b < nBox; ++b) { c->h_eblk[b] = blk; blk += (cntb[b] + 255) / 256; }
+			c->h_eblk[nBox] = blk;
+		}
+		const int nE = 256 * c->h_eblk[nBox];
+		c->nE = nE;
+		float *eb2s = nullptr;
+		if (dalloc(&eb2s, nE) || dalloc(&c->e_pos, nE) || dalloc(&c->e_aux, nE) || dalloc(&c->e_rho, nE) ||
+		    dalloc(&c->e_org, nE) || dalloc(&c->e_src, nE) || dalloc(&c->e_live, nE) || dalloc(&c->e_hit, nE) ||
+		    dalloc(&c->e_box, nE) || dalloc(&c->d_eblk, nBox + 1))
+			return SKG_E_NOMEM;
+		SKG_HIP(hipMemcpyAsync(c->d_eblk, c->h_eblk, 4 * (size_t)(nBox + 1), hipMemcpyHostToDevice, c->st));
+		k_entity_fill<<<nblk(nE, 256), 256, 0, c->st>>>(nE, c->d_eblk, nBox, c->e_pos, c->e_aux, c->e_org, c->e_rho,
+				c->e_src, c->e_live, c->e_hit, eb2s, c->e_box);
+		k_entity_setup<<<nblk(nEu, 256), 256, 0, c->st>>>(e4u, eb2u, esrcu, perm2, nEu, ebu, bfirst, c->d_eblk,
+				nullptr, c->s_pos, per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, c->e_pos, c->e_aux,
+				c->e_org, c->e_rho, c->e_src, c->e_live, c->e_hit, eb2s);
+		if ((rc = skg_build_tree(c, &c->te, c->e_pos, nE, B, eb2s, c->e_box))) return rc;
+		SKG_HIP(hipStreamSynchronize(c->st));
+		dfree(&mask); dfree(&cnt); dfree(&roff); dfree(&e4u); dfree(&eb2u); dfree(&esrcu); dfree(&perm2);
+		dfree(&ebu); dfree(&bfirst);
 		dfree(&c->e_b2);
 		c->e_b2 = eb2s;
 
 		// scatter half of the density: balls of others holding each scatterer
-		if ((rc = cover_launch(c, 0, c->s_pos, nullptr, nS, c->s_rhoS, 0, 0.f, 24.0 * nS))) return rc;
+		if ((rc = cover_launch(c, 0, c->s_pos, nullptr, nS, c->s_box, c->s_rhoS, 0, 0, 24.0 * nS))) return rc;
 		float *rhoS = nullptr;
 		if (dalloc(&rhoS, nS)) return SKG_E_NOMEM;
 		k_combine_rho<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_rhoG, c->s_rhoS, c->s_ball2, c->s_ord, nS, rhoS,
@@ -1038,8 +1268,8 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		dfree(&rhoS);
 		skg_timer_stop(c, SKG_T_DENSITY);
 	}
-	if (fBall2) SKG_HIP(hipMemcpyAsync(fBall2, c->d_ball2, 4 * (size_t)c->n, hipMemcpyDeviceToHost, c->st));
-	if (rho) SKG_HIP(hipMemcpyAsync(rho, c->d_rho, 4 * (size_t)c->n, hipMemcpyDeviceToHost, c->st));
+	if (fBall2) SKG_HIP(hipMemcpyAsync(fBall2, c->d_ball2, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+	if (rho) SKG_HIP(hipMemcpyAsync(rho, c->d_rho, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
 	if (nExtraScat) *nExtraScat = c->nRep;
 	c->stage = 2;
@@ -1069,6 +1299,8 @@ extern "C" int skg_neighbors(skg_ctx *c, int *nbr, float *d2) {
 	return SKG_OK;
 }
 
+static int count_active(skg_ctx *c);
+
 extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	if (!c) return SKG_E_PARAM;
 	if (c->stage < 2) return SKG_E_STATE;
@@ -1076,7 +1308,7 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	int rc, n = c->n;
 	int *flag = nullptr, *off = nullptr;
 	if (dalloc(&flag, n) || dalloc(&off, n)) return SKG_E_NOMEM;
-	k_mover_flags<<<nblk(n, 256), 256, 0, c->st>>>(n, c->nGas, c->nDark, c->inType, c->d_rho, c->d_mass,
+	k_mover_flags<<<nblk(n, 256), 256, 0, c->st>>>(n, c->d_type, c->inType, c->d_rho, c->d_mass,
 			c->d_temp, c->P.fDensMin, c->P.fTempMax, c->P.fMassMax, c->P.bGasAndDark, c->P.bGasOnly, flag);
 	if ((rc = skg_exclusive_scan(c, flag, off, n))) return rc;
 	int last[2];
@@ -1086,12 +1318,14 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	const int nm = last[0] + last[1];
 	c->nMove = nm;
 	if (dalloc(&c->m_ord, nm) || dalloc(&c->m_pos, nm) || dalloc(&c->m_old, nm) || dalloc(&c->m_list, nm) ||
-	    dalloc(&c->m_list2, nm))
+	    dalloc(&c->m_list2, nm) || dalloc(&c->m_box, nm))
 		return SKG_E_NOMEM;
 	k_compact<<<nblk(n, 256), 256, 0, c->st>>>(nullptr, flag, off, n, c->m_ord);
-	k_mover_init<<<nblk(nm, 256), 256, 0, c->st>>>(c->m_ord, nm, c->d_pos, c->m_pos, c->m_old);
+	k_mover_init<<<nblk(nm, 256), 256, 0, c->st>>>(c->m_ord, nm, c->d_pos, c->d_boxOf, c->m_pos, c->m_old, c->m_box);
 	c->nActive = nm;
-	if (nm > 0 && (rc = skg_morton_sort(c, c->m_pos, nullptr, nm, c->m_list))) return rc;
+	if (nm > 0 && (rc = skg_morton_sort(c, c->m_pos, nullptr, nm, c->m_list, c->nBox > 1 ? c->m_box : nullptr)))
+		return rc;
+	if ((rc = count_active(c))) return rc;
 	SKG_HIP(hipStreamSynchronize(c->st));
 	dfree(&flag); dfree(&off);
 	c->step = 0;
@@ -1100,17 +1334,28 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	return SKG_OK;
 }
 
-// One Denmax step: gradient + move of the active movers, then ScatterCut.
-static int denmax_step(skg_ctx *c, float fStep, int bInitial) {
+// Active movers per box (for the per-box Ittr counters).
+static int count_active(skg_ctx *c) {
+	k_fill_i<<<nblk(c->nBox, 256), 256, 0, c->st>>>(c->d_bact, c->nBox, 0);
+	if (c->nActive > 0)
+		k_box_count<<<nblk(c->nActive, 256), 256, 0, c->st>>>(c->m_list, c->nActive, c->m_box, c->d_bact);
+	SKG_HIP(hipGetLastError());
+	return 0;
+}
+
+// One Denmax step: gradient + move of the active movers, then ScatterCut per
+// box (each box's threshold from its own entities).
+static int denmax_step(skg_ctx *c, int micro, int bInitial) {
 	int rc;
 	const int stamp = ++c->step;
-	if ((rc = cover_launch(c, 1, c->m_pos, c->m_list, c->nActive, nullptr, stamp, fStep,
+	if ((rc = cover_launch(c, 1, c->m_pos, c->m_list, c->nActive, c->m_box, nullptr, stamp, micro,
 			24.0 * c->nActive + 24.0 * c->nLive))) return rc;
 	const int nb = nblk(c->nE, 256);
 	if (skg_ensure_part(c, (size_t)nb)) return SKG_E_NOMEM;
 	k_scat_thr<<<nb, 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->nE, stamp, bInitial, c->part);
-	k_min_final<<<1, 256, 0, c->st>>>(c->part, nb, c->d_thr);
-	k_scat_cut<<<nb, 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE, c->d_thr, (int *)c->part);
+	k_box_min<<<c->nBox, 64, 0, c->st>>>(c->part, c->d_eblk, c->d_bthr);
+	k_scat_cut<<<nb, 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE, c->e_box, c->d_bthr, (int *)c->part);
+	k_box_sum<<<c->nBox, 64, 0, c->st>>>((const int *)c->part, c->d_eblk, c->d_bscat);
 	k_sum_final<<<1, 256, 0, c->st>>>((const int *)c->part, nb, c->d_cnt);
 	SKG_HIP(hipGetLastError());
 	return 0;
@@ -1120,7 +1365,7 @@ static int refit_entities(skg_ctx *c) {
 	float *b2 = nullptr;
 	if (dalloc(&b2, c->nE)) return SKG_E_NOMEM;
 	k_live_b2<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_aux, c->nE, b2);
-	int rc = skg_build_tree(c, &c->te, c->e_pos, c->nE, c->te.B, b2);
+	int rc = skg_build_tree(c, &c->te, c->e_pos, c->nE, c->te.B, b2, c->e_box);
 	SKG_HIP(hipStreamSynchronize(c->st));
 	dfree(&b2);
 	return rc;
@@ -1133,8 +1378,7 @@ static int prune(skg_ctx *c) {
 	if (dalloc(&flag, na) || dalloc(&off, na)) return SKG_E_NOMEM;
 	const float *L = c->P.fPeriod;
 	const float hx = 0.5f * L[0], hy = 0.5f * L[1], hz = 0.5f * L[2];
-	const float cv2 = c->P.fCvg * c->P.fCvg;
-	k_prune<<<nblk(na, 256), 256, 0, c->st>>>(c->m_list, na, c->m_pos, c->m_old, hx, hy, hz, cv2, flag);
+	k_prune<<<nblk(na, 256), 256, 0, c->st>>>(c->m_list, na, c->m_pos, c->m_old, hx, hy, hz, c->m_box, c->d_bt, flag);
 	if ((rc = skg_exclusive_scan(c, flag, off, na))) return rc;
 	int last[2];
 	SKG_HIP(hipMemcpyAsync(&last[0], off + na - 1, 4, hipMemcpyDeviceToHost, c->st));
@@ -1142,11 +1386,12 @@ static int prune(skg_ctx *c) {
 	SKG_HIP(hipStreamSynchronize(c->st));
 	const int nn = last[0] + last[1];
 	k_compact<<<nblk(na, 256), 256, 0, c->st>>>(c->m_list, flag, off, na, c->m_list2);
-	if (nn > 0 && (rc = skg_morton_sort(c, c->m_pos, c->m_list2, nn, c->m_list))) return rc;
+	if (nn > 0 && (rc = skg_morton_sort(c, c->m_pos, c->m_list2, nn, c->m_list, c->nBox > 1 ? c->m_box : nullptr)))
+		return rc;
 	SKG_HIP(hipStreamSynchronize(c->st));
 	dfree(&flag); dfree(&off);
 	c->nActive = nn;
-	return 0;
+	return count_active(c);
 }
 
 static int read_count(skg_ctx *c, int *v) {
@@ -1164,7 +1409,7 @@ extern "C" int skg_move_first(skg_ctx *c, int *nActive, int *nScatter) {
 	skg_timer_start(c);
 	const int bInit = (c->inType == SKG_DARK) || c->P.bForceInitialCut;   // kd.c:696-699
 	if (c->nActive > 0 && c->nE > 0) {
-		if ((rc = denmax_step(c, c->P.fStep, bInit))) return rc;
+		if ((rc = denmax_step(c, 0, bInit))) return rc;
 		if ((rc = read_count(c, &ns))) return rc;
 		if ((rc = refit_entities(c))) return rc;
 	}
@@ -1183,7 +1428,7 @@ extern "C" int skg_move_block(skg_ctx *c, int *nActive, int *nScatter) {
 	skg_timer_start(c);
 	if (c->nActive > 0 && c->nE > 0) {
 		for (int i = 0; i < 5; ++i)
-			if ((rc = denmax_step(c, c->P.fStep, 0))) return rc;
+			if ((rc = denmax_step(c, 0, 0))) return rc;
 		if ((rc = read_count(c, &ns))) return rc;
 		if ((rc = prune(c))) return rc;
 		if ((rc = refit_entities(c))) return rc;
@@ -1203,11 +1448,12 @@ extern "C" int skg_microsteps(skg_ctx *c, int *nScatter) {
 	int rc, ns = c->nE;
 	skg_timer_start(c);
 	c->nActive = c->nMove;                                  // kdReactivateMove (kd.c:796-799)
-	if (c->nMove > 0 && (rc = skg_morton_sort(c, c->m_pos, nullptr, c->nMove, c->m_list))) return rc;
+	if (c->nMove > 0 && (rc = skg_morton_sort(c, c->m_pos, nullptr, c->nMove, c->m_list,
+			c->nBox > 1 ? c->m_box : nullptr))) return rc;
+	if ((rc = count_active(c))) return rc;
 	if (c->nMove > 0 && c->nE > 0) {
-		const float micro = (float)(0.1 * (double)c->P.fStep);   // MICRO_STEP*fStep (main.c:435)
-		for (int i = 0; i < 5; ++i)
-			if ((rc = denmax_step(c, micro, 0))) return rc;
+		for (int i = 0; i < 5; ++i)                          // MICRO_STEP*fStep (main.c:435)
+			if ((rc = denmax_step(c, 1, 0))) return rc;
 		if ((rc = read_count(c, &ns))) return rc;
 	}
 	SKG_HIP(hipStreamSynchronize(c->st));
@@ -1234,6 +1480,7 @@ extern "C" int skg_get_moved(skg_ctx *c, float *r) {
 extern "C" int skg_field_gradient(skg_ctx *c, int nq, const float *qpos, float *acc) {
 	if (!c || nq < 0 || (nq && (!qpos || !acc))) return SKG_E_PARAM;
 	if (c->stage < 2) return SKG_E_STATE;
+	if (c->nBox > 1) return SKG_E_PARAM;   // a point does not name its box
 	if (nq == 0) return SKG_OK;
 	(void)hipSetDevice(c->dev);
 	float4 *q4 = nullptr;

@@ -10,6 +10,8 @@
 // is merged without per-pair tests; the bound is computed from the same f32
 // operands as the pair deltas, so it never admits a pair the test rejects.
 // Group ids are numbered in order of each group's smallest input index.
+// Batched boxes: every point links only within its own box, with its box's
+// tau; the tree carries box ranges, so a lane opens only subtrees of its box.
 #include "skg_common.h"
 #include <climits>
 #include <cfloat>
@@ -51,12 +53,15 @@ __device__ __forceinline__ float max_delta(float q, float qm, float qp, float lo
 template <bool PER>
 __global__ __launch_bounds__(64) void k_fof_links(const float4 *__restrict__ fp, int n,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int depth, int B,
-		float L, float tau2, int *par, int *united) {
+		float L, float tau2c, const int *__restrict__ fbox, const DevBox *__restrict__ bt, int *par, int *united) {
 	const int lane = threadIdx.x;
 	const int base = blockIdx.x * 64;
 	const int i = base + lane;
 	const bool valid = i < n;
 	const float4 q = fp[valid ? i : base];
+	const int qb = fbox ? fbox[valid ? i : base] : 0;
+	const float tau2 = fbox ? bt[qb].tau2 : tau2c;
+	const float tau2w = wave_max(tau2);
 	const float hL = 0.5f * L;
 	const float qxm = fsub(q.x, L), qxp = fadd(q.x, L), qym = fsub(q.y, L), qyp = fadd(q.y, L);
 	const float qzm = fsub(q.z, L), qzp = fadd(q.z, L);
@@ -66,17 +71,17 @@ __global__ __launch_bounds__(64) void k_fof_links(const float4 *__restrict__ fp,
 	for (;;) {
 		const float4 lo = nlo[node], hi = nhi[node];
 		bool descend = false;
-		if (box_box_d2<PER>(lo, hi, blo, bhi, L) < tau2) {
+		if (box_box_d2<PER>(lo, hi, blo, bhi, L) < tau2w) {
 			// per-lane: contained / overlapping / disjoint
 			const int first = ((node << (depth - lev)) - nLeaf2) * B;
 			int last = (((node + 1) << (depth - lev)) - nLeaf2) * B;
 			last = min(last, n);
 			bool overlap = false, contained = false;
-			if (valid && first < n) {
+			if (valid && first < n && brange_has(hi.w, qb)) {
 				float gx = gap_axis<PER>(lo.x, hi.x, q.x, q.x, L), gy = gap_axis<PER>(lo.y, hi.y, q.y, q.y, L);
 				float gz = gap_axis<PER>(lo.z, hi.z, q.z, q.z, L);
 				overlap = sq3(gx, gy, gz) < tau2;
-				if (overlap && last - 1 > i) {
+				if (overlap && last - 1 > i && brange_is(hi.w, qb)) {
 					float mx = max_delta<PER>(q.x, qxm, qxp, lo.x, hi.x, hL);
 					float my = max_delta<PER>(q.y, qym, qyp, lo.y, hi.y, hL);
 					float mz = max_delta<PER>(q.z, qzm, qzp, lo.z, hi.z, hL);
@@ -95,6 +100,7 @@ __global__ __launch_bounds__(64) void k_fof_links(const float4 *__restrict__ fp,
 				if (node < nLeaf2) descend = true;
 				else if (want) {
 					for (int j = max(first, i + 1); j < last; ++j) {
+						if (fbox && fbox[j] != qb) continue;
 						const float4 c = fp[j];
 						float dx = img_delta<PER>(q.x, qxm, qxp, c.x, hL);
 						float dy = img_delta<PER>(q.y, qym, qyp, c.y, hL);
@@ -151,26 +157,33 @@ __global__ void k_fof_label(const int *__restrict__ par, int n, const int *__res
 	int o = orig[i];
 	grp[outmap ? outmap[o] : o] = rank[cmin[par[i]]] + 1;
 }
-__global__ void k_gather4(const float4 *__restrict__ in, const int *__restrict__ perm, int n, float4 *out) {
+__global__ void k_gather4(const float4 *__restrict__ in, const int *__restrict__ perm, int n, float4 *out,
+		const int *__restrict__ box, int *obox) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) out[i] = in[perm[i]];
+	if (i < n) {
+		out[i] = in[perm[i]];
+		if (box) obox[i] = box[perm[i]];
+	}
 }
 
 // FOF of pos[0,n) (float4); writes grp[outmap ? outmap[o] : o] = 1-based id for
-// every point o, numbered by smallest o.  Returns the number of groups.
-static int fof_run(skg_ctx *c, const float4 *pos, int n, float tau, const int *outmap, int *grp, int *ngOut) {
+// every point o, numbered by smallest o.  Returns the number of groups.  With
+// box (indexed like pos) points link only within a box, using its tau.
+static int fof_run(skg_ctx *c, const float4 *pos, int n, float tau, const int *outmap, int *grp, int *ngOut,
+		const int *box) {
 	int rc;
 	const bool per = c->P.bPeriodic != 0;
 	const float L = per ? c->P.fPeriod[0] : FLT_MAX;
 	int *perm = nullptr, *par = nullptr, *united = nullptr, *cmin = nullptr, *isFirst = nullptr, *rank = nullptr;
+	int *fbox = nullptr;
 	float4 *fp = nullptr;
 	DevTree t;
 	if (dalloc(&perm, n) || dalloc(&par, n) || dalloc(&cmin, n) || dalloc(&isFirst, n) || dalloc(&rank, n) ||
-	    dalloc(&fp, n))
+	    dalloc(&fp, n) || (box && dalloc(&fbox, n)))
 		return SKG_E_NOMEM;
-	if ((rc = skg_morton_sort(c, pos, nullptr, n, perm))) return rc;
-	k_gather4<<<nblk(n, 256), 256, 0, c->st>>>(pos, perm, n, fp);
-	if ((rc = skg_build_tree(c, &t, fp, n, c->P.nBucket, nullptr))) return rc;
+	if ((rc = skg_morton_sort(c, pos, nullptr, n, perm, box))) return rc;
+	k_gather4<<<nblk(n, 256), 256, 0, c->st>>>(pos, perm, n, fp, box, fbox);
+	if ((rc = skg_build_tree(c, &t, fp, n, c->P.nBucket, nullptr, fbox))) return rc;
 	if (dalloc(&united, 2 * (size_t)t.nLeaf2)) return SKG_E_NOMEM;
 	int depth = 0;
 	while ((1 << depth) < t.nLeaf2) ++depth;
@@ -178,8 +191,10 @@ static int fof_run(skg_ctx *c, const float4 *pos, int n, float tau, const int *o
 	k_fof_init<<<nblk(n, 256), 256, 0, c->st>>>(par, n);
 	const float tau2 = tau * tau;
 	const int pslot = skg_prof_begin(c);
-	if (per) k_fof_links<true><<<nblk(n, 64), 64, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2, par, united);
-	else k_fof_links<false><<<nblk(n, 64), 64, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2, par, united);
+	if (per) k_fof_links<true><<<nblk(n, 64), 64, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2,
+			fbox, c->d_bt, par, united);
+	else k_fof_links<false><<<nblk(n, 64), 64, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2,
+			fbox, c->d_bt, par, united);
 	skg_prof_end(c, pslot, SKG_K_FOF, 16.0 * n);
 	k_fill<<<nblk(n, 256), 256, 0, c->st>>>(cmin, n, INT_MAX);
 	k_zero<<<nblk(n, 256), 256, 0, c->st>>>(isFirst, n);
@@ -193,7 +208,7 @@ static int fof_run(skg_ctx *c, const float4 *pos, int n, float tau, const int *o
 	SKG_HIP(hipMemcpyAsync(&last[1], isFirst + n - 1, 4, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
 	*ngOut = last[0] + last[1];
-	dfree(&perm); dfree(&par); dfree(&united); dfree(&cmin); dfree(&isFirst); dfree(&rank); dfree(&fp);
+	dfree(&perm); dfree(&par); dfree(&united); dfree(&cmin); dfree(&isFirst); dfree(&rank); dfree(&fp); dfree(&fbox);
 	skg_free_tree(&t);
 	return 0;
 }
@@ -206,7 +221,8 @@ extern "C" int skg_fof(skg_ctx *c, int *grp, int *nGroup) {
 	skg_timer_start(c);
 	k_zero<<<nblk(c->n, 256), 256, 0, c->st>>>(c->d_grp, c->n);
 	if (c->nMove > 0) {
-		if ((rc = fof_run(c, c->m_pos, c->nMove, c->P.fTau, c->m_ord, c->d_grp, &ng))) return rc;
+		if ((rc = fof_run(c, c->m_pos, c->nMove, c->P.fTau, c->m_ord, c->d_grp, &ng,
+				c->nBox > 1 ? c->m_box : nullptr))) return rc;
 	}
 	skg_timer_stop(c, SKG_T_FOF);
 	c->nGroup = ng + 1;                                     // kd.c:905
@@ -232,7 +248,7 @@ extern "C" int skg_fof_points(skg_ctx *c, int n, const float *pos, float fTau, i
 	SKG_HIP(hipMemcpyAsync(p4, h, 16 * (size_t)n, hipMemcpyHostToDevice, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
 	free(h);
-	if ((rc = fof_run(c, p4, n, fTau, nullptr, g, &ng))) return rc;
+	if ((rc = fof_run(c, p4, n, fTau, nullptr, g, &ng, nullptr))) return rc;
 	SKG_HIP(hipMemcpyAsync(grp, g, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
 	dfree(&p4); dfree(&g);

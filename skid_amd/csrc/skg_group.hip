@@ -763,15 +763,20 @@ __global__ __launch_bounds__(256) void k_group_cm(const int *__restrict__ bg, co
 // Scoop: group-0 particles within fScoop of the group centre add their
 // potential to every member (kdAddScoopPot, grav.c:63-135).  One wavefront
 // per group walks the group-0 tree; lanes split each bucket, then the members.
+// Batched boxes (zbox given): a group scoops only group-0 particles of its own
+// box, within its box's fScoop.
 __global__ __launch_bounds__(64) void k_unb_scoop(const int *__restrict__ gs, const int *__restrict__ ge,
 		int nG, const float4 *__restrict__ gcen, const float4 *__restrict__ grel, const float4 *__restrict__ zp,
 		const float *__restrict__ zeps, int nz, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
-		float L, int per, float scoop2, const float4 *__restrict__ uq, const float4 *__restrict__ uv,
-		float2 *uphi, float G, int soft, int nMax) {
+		float L, int per, float scoop2c, const float4 *__restrict__ uq, const float4 *__restrict__ uv,
+		float2 *uphi, float G, int soft, int nMax, const int *__restrict__ zbox, const int *__restrict__ mem,
+		const int *__restrict__ boxOf, const DevBox *__restrict__ bt) {
 	const int g = blockIdx.x + 1;
 	if (g >= nG) return;
 	const int s = gs[g], e = ge[g];
 	if (e - s <= 0 || e - s >= nMax) return;
+	const int gb = zbox ? boxOf[mem[s]] : 0;
+	const float scoop2 = zbox ? bt[gb].scoop2 : scoop2c;
 	const int lane = threadIdx.x;
 	const float4 c = gcen[g], rl = grel[g];
 	const float hL = 0.5f * L;
@@ -783,7 +788,7 @@ __global__ __launch_bounds__(64) void k_unb_scoop(const int *__restrict__ gs, co
 		float gx = per ? gap_axis<true>(lo.x, hi.x, c.x, c.x, L) : gap_axis<false>(lo.x, hi.x, c.x, c.x, L);
 		float gy = per ? gap_axis<true>(lo.y, hi.y, c.y, c.y, L) : gap_axis<false>(lo.y, hi.y, c.y, c.y, L);
 		float gz = per ? gap_axis<true>(lo.z, hi.z, c.z, c.z, L) : gap_axis<false>(lo.z, hi.z, c.z, c.z, L);
-		const bool visit = sq3(gx, gy, gz) < scoop2;
+		const bool visit = brange_has(hi.w, gb) && sq3(gx, gy, gz) < scoop2;
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit) {
 			const int st = (node - nLeaf2) * B;
@@ -798,7 +803,7 @@ __global__ __launch_bounds__(64) void k_unb_scoop(const int *__restrict__ gs, co
 					float dx = per ? img_delta<true>(c.x, cxm, cxp, p.x, hL) : fsub(c.x, p.x);
 					float dy = per ? img_delta<true>(c.y, cym, cyp, p.y, hL) : fsub(c.y, p.y);
 					float dz = per ? img_delta<true>(c.z, czm, czp, p.z, hL) : fsub(c.z, p.z);
-					hit = sq3(dx, dy, dz) < scoop2;
+					hit = sq3(dx, dy, dz) < scoop2 && (!zbox || zbox[j] == gb);
 				}
 				unsigned long long b = __ballot(hit);
 				while (b) {
@@ -983,17 +988,19 @@ __global__ void k_radius(const int *__restrict__ grp, int n, const float *__rest
 }
 
 __global__ void k_gather_pos_members(const int *__restrict__ mem, int n, const float *__restrict__ pos,
-		const float *__restrict__ mass, const float *__restrict__ soft, float4 *out, float *eps) {
+		const float *__restrict__ mass, const float *__restrict__ soft, const int *__restrict__ boxOf,
+		float4 *out, float *eps, int *obox) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
 	int o = mem[i];
 	out[i] = make_float4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], mass[o]);
 	eps[i] = soft[o];
+	obox[i] = boxOf[o];
 }
 __global__ void k_gather4g(const float4 *__restrict__ in, const float *__restrict__ ein,
-		const int *__restrict__ perm, int n, float4 *out, float *eout) {
+		const int *__restrict__ bin, const int *__restrict__ perm, int n, float4 *out, float *eout, int *bout) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < n) { out[i] = in[perm[i]]; eout[i] = ein[perm[i]]; }
+	if (i < n) { out[i] = in[perm[i]]; eout[i] = ein[perm[i]]; bout[i] = bin[perm[i]]; }
 }
 
 
@@ -1113,21 +1120,23 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		if (n0 > 0) {
 			float4 *zp = nullptr, *zs = nullptr;
 			float *ze = nullptr, *zes = nullptr;
-			int *perm = nullptr;
+			int *perm = nullptr, *zb = nullptr, *zbs = nullptr;
 			DevTree t;
-			if (dalloc(&zp, n0) || dalloc(&zs, n0) || dalloc(&perm, n0) || dalloc(&ze, n0) || dalloc(&zes, n0))
+			const bool batch = c->nBox > 1;
+			if (dalloc(&zp, n0) || dalloc(&zs, n0) || dalloc(&perm, n0) || dalloc(&ze, n0) || dalloc(&zes, n0) ||
+			    dalloc(&zb, n0) || dalloc(&zbs, n0))
 				return SKG_E_NOMEM;
 			k_gather_pos_members<<<nblk(n0, 256), 256, 0, c->st>>>(G->mem + hs[0], n0, c->d_pos, c->d_mass,
-					c->d_soft, zp, ze);
-			if ((rc = skg_morton_sort(c, zp, nullptr, n0, perm))) return rc;
-			k_gather4g<<<nblk(n0, 256), 256, 0, c->st>>>(zp, ze, perm, n0, zs, zes);
-			if ((rc = skg_build_tree(c, &t, zs, n0, c->P.nBucket, nullptr))) return rc;
+					c->d_soft, c->d_boxOf, zp, ze, zb);
+			if ((rc = skg_morton_sort(c, zp, nullptr, n0, perm, batch ? zb : nullptr))) return rc;
+			k_gather4g<<<nblk(n0, 256), 256, 0, c->st>>>(zp, ze, zb, perm, n0, zs, zes, zbs);
+			if ((rc = skg_build_tree(c, &t, zs, n0, c->P.nBucket, nullptr, batch ? zbs : nullptr))) return rc;
 			const float sc2 = c->P.fScoop * c->P.fScoop;
 			k_unb_scoop<<<nG - 1, 64, 0, c->st>>>(G->gstart, G->gend, nG, G->cen, G->rel, zs, zes, n0, t.lo, t.hi,
 					t.nLeaf2, t.B, c->P.bPeriodic ? L[0] : FLT_MAX, c->P.bPeriodic, sc2, G->uq, G->uv, G->uphi,
-					c->P.G, c->P.iSoftType, nMax);
+					c->P.G, c->P.iSoftType, nMax, batch ? zbs : nullptr, G->mem, c->d_boxOf, c->d_bt);
 			SKG_HIP(hipStreamSynchronize(c->st));
-			dfree(&zp); dfree(&zs); dfree(&perm); dfree(&ze); dfree(&zes);
+			dfree(&zp); dfree(&zs); dfree(&perm); dfree(&ze); dfree(&zes); dfree(&zb); dfree(&zbs);
 			skg_free_tree(&t);
 		}
 		double pairs = 0;

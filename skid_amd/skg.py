@@ -58,6 +58,8 @@ def lib():
         L.skg_strerror.argtypes = [C.c_int]
         L.skg_strerror.restype = C.c_char_p
         L.skg_load.argtypes = [vp, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, fp]
+        L.skg_load_batch.argtypes = [vp, C.c_int, fp, fp, fp, fp, fp, fp, fp, fp, fp]
+        L.skg_box_counts.argtypes = [vp, fp, fp]
         L.skg_density.argtypes = [vp, fp, fp, ip]
         L.skg_neighbors.argtypes = [vp, fp, fp]
         L.skg_init_move.argtypes = [vp, ip]
@@ -267,3 +269,72 @@ class Context:
         self.too_small()
         g, ng = self.groups()
         return tr, g, ng
+
+
+class BatchContext(Context):
+    """Many independent boxes in one context (skg_load_batch): every stage call
+    runs all boxes in the same launches.  taus (per box) sets each box's tau,
+    fCvg, fStep and fScoop as main.c:343-345 does from -tau."""
+
+    def __init__(self, boxes, params: SkgParams, taus=None):
+        self.L = lib()
+        self.boxes = list(boxes)
+        h = C.c_void_p()
+        _check(self.L.skg_create(C.byref(params), C.byref(h)), "skg_create")
+        self.h = h
+        self.k = params.nSmooth
+        self.nbox = len(self.boxes)
+        self.sizes = np.array([b.n for b in self.boxes], np.int64)
+        self.offsets = np.concatenate([[0], np.cumsum(self.sizes)]).astype(np.int64)
+        self.n = int(self.offsets[-1])
+        cat = lambda f: np.ascontiguousarray(np.concatenate([f(b) for b in self.boxes]), np.float32)
+        self._keep = [cat(lambda b: b.pos.reshape(-1)), cat(lambda b: b.vel.reshape(-1)),
+                      cat(lambda b: b.mass), cat(lambda b: b.soft), cat(lambda b: b.temp)]
+        ng = np.array([b.n_gas for b in self.boxes], np.int32)
+        nd = np.array([b.n_dark for b in self.boxes], np.int32)
+        ns = np.array([b.n_star for b in self.boxes], np.int32)
+        tau = None if taus is None else np.ascontiguousarray(taus, np.float32)
+        self._keep += [ng, nd, ns, tau]
+        _check(self.L.skg_load_batch(h, self.nbox, _p(ng), _p(nd), _p(ns), _p(tau),
+                                     *[_p(a) for a in self._keep[:5]]), "skg_load_batch")
+
+    def split(self, a):
+        """Per-box views of an array indexed by the concatenated iOrder."""
+        return [a[self.offsets[i]:self.offsets[i + 1]] for i in range(self.nbox)]
+
+    def box_counts(self):
+        a = np.empty(self.nbox, np.int32)
+        s = np.empty(self.nbox, np.int32)
+        _check(self.L.skg_box_counts(self.h, _p(a), _p(s)), "skg_box_counts")
+        return a, s
+
+    def move_loop(self, max_blocks: int = 100000):
+        """The Denmax loop for every box at once; returns per-box Ittr traces,
+        each ending where its own loop would (the first line with nActive 0)."""
+        self.init_move()
+        self.move_first()
+        rows = [self.box_counts()]
+        while rows[-1][0].sum():
+            if len(rows) > max_blocks:
+                raise SkgError(f"move loop did not converge in {max_blocks} blocks")
+            self.move_block()
+            rows.append(self.box_counts())
+        act = np.stack([r[0] for r in rows], 1)
+        sca = np.stack([r[1] for r in rows], 1)
+        out = []
+        for b in range(self.nbox):
+            z = np.flatnonzero(act[b] == 0)
+            end = (z[0] + 1) if z.size else act.shape[1]
+            out.append(np.stack([act[b, :end], sca[b, :end]], 1).astype(np.int32))
+        return out
+
+    def box_groups(self, g):
+        """Per-box group ids renumbered 1.. within each box (0 stays 0)."""
+        out = []
+        for gb in self.split(g):
+            gb = gb.copy()
+            pos = gb > 0
+            if pos.any():
+                gb[pos] -= gb[pos].min() - 1
+            out.append(gb)
+        return out
