@@ -83,31 +83,32 @@ __global__ __launch_bounds__(256) void k_adj_gather(const unsigned long long *__
 // the radii as built.  MODE 0: dx_j -= c_ij r_hat; MODE 1: a(x).  pbox
 // (batched boxes) restricts each point to its own box's entities.
 template <int MODE>
-__global__ __launch_bounds__(64) void k_adj_cover(const float4 *__restrict__ pts, int np,
+__global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restrict__ pts, int np,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float *__restrict__ eb2,
 		const float4 *__restrict__ eorg, const int *__restrict__ esrc, const int *__restrict__ sord, int ne,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		const float *__restrict__ mass, const float *__restrict__ gbar, const int *__restrict__ pord,
 		int parts, float Lx, float Ly, float Lz, const int *__restrict__ pbox, float *out) {
-	const int lane = threadIdx.x, base = blockIdx.x * 64, ii = base + lane;
+	const int lane = threadIdx.x & 63, base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63),
+			ii = base + lane;
+	if (base >= np) return;                       // whole wavefront past the end
 	const bool valid = ii < np;
 	const float4 p = pts[valid ? ii : base];
 	const int pb = pbox ? pbox[valid ? ii : base] : 0;
-	const float3 blo = make_float3(wave_min(p.x), wave_min(p.y), wave_min(p.z));
-	const float3 bhi = make_float3(wave_max(p.x), wave_max(p.y), wave_max(p.z));
 	float mj = 0.f, gj = 0.f;
 	if (MODE == 0 && valid) { const int oj = pord[ii]; mj = mass[oj]; gj = gbar[oj]; }
 	Acc s0, s1, s2;
 	s0.init(); s1.init(); s2.init();
 	int node = 1;
 	for (;;) {
+		node = uniform_int(node);
 		const float4 lo = nlo[node], hi = nhi[node];
 		const bool need = valid && brange_has(hi.w, pb) && point_box_d2_np(lo, hi, p) < lo.w;
-		const bool visit = box_box_d2<false>(lo, hi, blo, bhi, 0.f) < lo.w && __any(need);
+		const bool visit = __any(need);
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit && need) {
-			const int start = (node - nLeaf2) * B;
-			for (int e = start; e < min(ne, start + B); ++e) {
+			const int start = uniform_int((node - nLeaf2) * B), stop = uniform_int(min(ne, start + B));
+			for (int e = start; e < stop; ++e) {
 				const float4 A = epos[e];
 				float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
 				const float d2 = sq3(dx, dy, dz);
@@ -178,7 +179,7 @@ static void launch_cover(skg_ctx *c, const float4 *pts, int np, const int *pord,
 		const int *pbox, float *out) {
 	const float *L = c->P.fPeriod;
 	const bool per = c->P.bPeriodic != 0;
-	k_adj_cover<MODE><<<nblk(np, 64), 64, 0, c->st>>>(pts, np, c->e_pos, c->e_aux, c->e_b2, c->e_org, c->e_src,
+	k_adj_cover<MODE><<<nblk(np, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(pts, np, c->e_pos, c->e_aux, c->e_b2, c->e_org, c->e_src,
 			c->s_ord, c->nE, c->tfull.lo, c->tfull.hi, c->tfull.nLeaf2, c->tfull.B, c->d_mass, gbar, pord, parts,
 			per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, c->nBox > 1 ? pbox : nullptr, out);
 }

@@ -17,6 +17,9 @@
 #include "../../include/skg.h"
 
 #define SKG_WAVE 64
+// Wave-per-tile kernels run four independent wavefronts per workgroup: a
+// one-wave workgroup would cap residency at the per-CU workgroup limit.
+#define SKG_TILE_T 256
 
 // Per-box constants of a batched context (box b = particles [off, off+n) by
 // iOrder).  A single-box context has one entry holding the parameter block's
@@ -116,6 +119,10 @@ struct skg_ctx {
 	int *v32a = nullptr, *v32b = nullptr;
 	size_t vcap = 0;
 	float *part = nullptr; size_t partCap = 0;   // per-block reduction partials
+
+	// launch shapes (set at create; SKG_COVER_SUB_MAX overrides): point sets of
+	// at most subMax points use the sub-wave cover kernel
+	long subMax = 0;
 };
 
 // ---------------------------------------------------------------- errors
@@ -142,6 +149,17 @@ __device__ __forceinline__ int wave_max_i(int v) {
 #pragma unroll
 	for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
 	return v;
+}
+
+// Workgroups are dealt to the 8 XCDs round-robin (block b to XCD b % 8), and
+// each XCD has its own L2.  Tile-ordered kernels remap the block index so that
+// XCD x walks one contiguous range of tiles: neighbouring tiles (Morton
+// neighbours, sharing most of their tree nodes and entities) then hit the
+// same L2.  A bijection of [0, gridDim.x).
+#define SKG_NXCD 8
+__device__ __forceinline__ int xcd_tile(int b, int G) {
+	const int q = G / SKG_NXCD, r = G % SKG_NXCD, x = b % SKG_NXCD;
+	return x * q + min(x, r) + b / SKG_NXCD;
 }
 
 // ---------------------------------------------------------------- box ranges
@@ -185,6 +203,9 @@ struct Acc {
 		s = t;
 	}
 	__device__ __forceinline__ float val() const { return fadd(s, c); }
+	__device__ __forceinline__ float cc() const { return c; }
+	// fold in another partial (its sum and its carried error)
+	__device__ __forceinline__ void merge(float os, float oc) { add(os); c = fadd(c, oc); }
 };
 template <bool COMP> struct Sum;
 template <> struct Sum<true> : Acc {};
@@ -193,6 +214,8 @@ template <> struct Sum<false> {
 	__device__ __forceinline__ void init() { s = 0.f; }
 	__device__ __forceinline__ void add(float x) { s = fadd(s, x); }
 	__device__ __forceinline__ float val() const { return s; }
+	__device__ __forceinline__ float cc() const { return 0.f; }
+	__device__ __forceinline__ void merge(float os, float) { add(os); }
 };
 
 // ---------------------------------------------------------------- box distances
@@ -265,6 +288,19 @@ __device__ __forceinline__ float grad_factor(float d2, float ih2, float gnorm) {
 	if (r2 < 1.0f) rs = (float)fma(2.25, (double)rs, -3.0);
 	else rs = (float)((-3.0 / (double)rs + 3.0) - 0.75 * (double)rs);
 	return fmul(rs, gnorm);
+}
+
+// The same factor in fp32 for the compensated mode.  For q >= 1 the
+// reference's -3/q + 3 - 0.75 q equals -0.75 (2 - q)^2 / q, which has no
+// cancellation near q = 2 (2 - q is exact there), so a few fp32 roundings
+// give a few ulp of relative error with no fp64 divide.
+__device__ __forceinline__ float grad_factor_f32(float d2, float ih2, float gnorm) {
+	const float r2 = fmul(d2, ih2);
+	const float q = __fsqrt_rn(r2);
+	float g;
+	if (r2 < 1.0f) g = fmaf(2.25f, q, -3.0f);
+	else { const float t = fsub(2.0f, q); g = fmul(-0.75f, fmul(t, t)) / q; }
+	return fmul(g, gnorm);
 }
 
 // ---------------------------------------------------------------- host helpers (skg_core.hip)

@@ -347,7 +347,7 @@ __global__ __launch_bounds__(64) void k_knn(
 	__shared__ int stk[64];
 	__shared__ float stkd[64];
 	const int lane = threadIdx.x;
-	const int base = blockIdx.x * 64;
+	const int base = xcd_tile(blockIdx.x, gridDim.x) * 64;
 	const int qi = base + lane;
 	const bool valid = qi < n;
 	const float4 q = spos[valid ? qi : base];
@@ -464,41 +464,52 @@ __global__ __launch_bounds__(64) void k_knn(
 // leaf, so every entity a lane tests is of its own box.  MODE 1 takes the
 // box's step (micro: its micro-step).
 template <int MODE, bool COMP>
-__global__ __launch_bounds__(64) void k_cover(
+__global__ __launch_bounds__(SKG_TILE_T) void k_cover(
 		float4 *__restrict__ ppos, const int *__restrict__ plist, int np,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
 		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
 		const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
 		float cx, float cy, float cz, float Lx, float Ly, float Lz) {
-	const int lane = threadIdx.x;
-	const int base = blockIdx.x * 64;
+	const int lane = threadIdx.x & 63;
+	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
+	if (base >= np) return;                       // whole wavefront past the end
 	const int ii = base + lane;
 	const bool valid = ii < np;
 	const int pi = plist ? plist[valid ? ii : base] : (valid ? ii : base);
 	const float4 p = ppos[pi];
 	const int pb = pbox ? pbox[pi] : 0;
-	const float3 blo = make_float3(wave_min(p.x), wave_min(p.y), wave_min(p.z));
-	const float3 bhi = make_float3(wave_max(p.x), wave_max(p.y), wave_max(p.z));
 	Sum<COMP> s0, s1, s2;
 	s0.init(); s1.init(); s2.init();
-	// A node is opened when its largest live ball reaches the block's box and
-	// at least one lane's own point (so scattered blocks late in the move loop
-	// walk only the union of their lanes' paths).
+	// A node is opened when its largest live ball reaches at least one lane's
+	// own point (so scattered blocks late in the move loop walk only the union
+	// of their lanes' paths).  The node index is wave-uniform, so node bounds
+	// and entity records are read through the scalar cache.
 	int node = 1;
 	for (;;) {
+		node = uniform_int(node);
 		const float4 lo = nlo[node], hi = nhi[node];
 		const bool need = valid && brange_has(hi.w, pb) && point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
-		const bool visit = box_box_d2<false>(lo, hi, blo, bhi, 0.f) < lo.w && __any(need);
+		const bool visit = __any(need);
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit) {
-			const int start = (node - nLeaf2) * B;
+			const int start = uniform_int((node - nLeaf2) * B);
 			if (start < ne) {
-				const int cnt = min(B, ne - start);
+				const int cnt = uniform_int(min(B, ne - start));
 				if (need) {
-					for (int t = 0; t < cnt; ++t) {
-						const int e = start + t;
-						const float4 A = epos[e], X = eaux[e];
+					// records are read four at a time (one wait per four)
+					for (int t0 = 0; t0 < cnt; t0 += 4) {
+					float4 A4[4], X4[4];
+#pragma unroll
+					for (int u = 0; u < 4; ++u) {
+						const int e = min(start + t0 + u, start + cnt - 1);
+						A4[u] = epos[e]; X4[u] = eaux[e];
+					}
+#pragma unroll
+					for (int u = 0; u < 4; ++u) {
+						if (t0 + u >= cnt) break;
+						const int e = start + t0 + u;
+						const float4 A = A4[u], X = X4[u];
 						float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
 						float d2 = sq3(dx, dy, dz);
 						if (d2 < X.x) {
@@ -515,11 +526,12 @@ __global__ __launch_bounds__(64) void k_cover(
 							if (MODE == 0) {
 								s0.add(fmul(m4_term(COMP ? sq3(dx, dy, dz) : d2, X.y, X.w), A.w));
 							} else {
-								const float f = grad_factor(COMP ? sq3(dx, dy, dz) : d2, X.y, X.z);
+								const float f = COMP ? grad_factor_f32(sq3(dx, dy, dz), X.y, X.z) : grad_factor(d2, X.y, X.z);
 								s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
 								ehit[e] = stamp;
 							}
 						}
+					}
 					}
 				}
 			}
@@ -533,6 +545,95 @@ __global__ __launch_bounds__(64) void k_cover(
 	if (!valid) return;
 	if (MODE == 0) { out_sum[ii] = s0.val(); return; }
 	// kdMoveParticles (kd.c:702-732)
+	const float fStep = micro ? bt[pb].fMicro : bt[pb].fStep;
+	const float ax = s0.val(), ay = s1.val(), az = s2.val();
+	const float ss = sq3(ax, ay, az);
+	float ai = (float)sqrt((double)ss);
+	if (ai > 0.0f) ai = (float)((double)fStep / sqrt((double)ss));
+	else ai = 0.0f;
+	float r[3] = {fsub(p.x, fmul(ai, ax)), fsub(p.y, fmul(ai, ay)), fsub(p.z, fmul(ai, az))};
+	const float C[3] = {cx, cy, cz}, LL[3] = {Lx, Ly, Lz};
+	for (int j = 0; j < 3; ++j) {
+		if ((double)r[j] > (double)C[j] + 0.5 * (double)LL[j]) r[j] = fsub(r[j], LL[j]);
+		if ((double)r[j] <= (double)C[j] - 0.5 * (double)LL[j]) r[j] = fadd(r[j], LL[j]);
+	}
+	ppos[pi] = make_float4(r[0], r[1], r[2], p.w);
+}
+
+// Sub-wave variant: S lanes per point (64/S points per wavefront), each lane
+// testing every S-th entity of a visited leaf, partial sums folded across the
+// S lanes at the end.  A lane-per-point wavefront walks the union of its
+// lanes' paths, which costs 64 paths when the active points are scattered
+// (late in the move loop); here a wavefront walks S-lane-wide paths of 64/S
+// points.  Same arithmetic per pair as k_cover; only the order of the sums
+// differs.
+template <int MODE, bool COMP, int S>
+__global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
+		float4 *__restrict__ ppos, const int *__restrict__ plist, int np,
+		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
+		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
+		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
+		const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
+		float cx, float cy, float cz, float Lx, float Ly, float Lz) {
+	const int blk = xcd_tile(blockIdx.x, gridDim.x);
+	const int gl = blk * SKG_TILE_T + threadIdx.x;
+	const int ii = gl / S, sub = gl % S;
+	const int wbase = (blk * SKG_TILE_T + (threadIdx.x & ~63)) / S;
+	if (wbase >= np) return;                      // whole wavefront past the end
+	const bool valid = ii < np;
+	const int pi = plist ? plist[valid ? ii : wbase] : (valid ? ii : wbase);
+	const float4 p = ppos[pi];
+	const int pb = pbox ? pbox[pi] : 0;
+	Sum<COMP> s0, s1, s2;
+	s0.init(); s1.init(); s2.init();
+	int node = valid ? 1 : 0;
+	while (node) {                                // the S lanes of a point move in lockstep
+		const float4 lo = nlo[node], hi = nhi[node];
+		const bool need = brange_has(hi.w, pb) && point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
+		if (need && node < nLeaf2) { node = 2 * node; continue; }
+		if (need) {
+			const int start = (node - nLeaf2) * B;
+			const int stop = min(ne, start + B);
+			for (int e = start + sub; e < stop; e += S) {
+				const float4 A = epos[e], X = eaux[e];
+				float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
+				float d2 = sq3(dx, dy, dz);
+				if (d2 < X.x) {
+					if (COMP) {
+						const float4 O = eorg[e];
+						const int code = (int)O.w;     // 9(ix+1) + 3(iy+1) + (iz+1)
+						if (code != 13) {
+							dx = exact_delta(O.x, p.x, (float)(code / 9 - 1) * Lx);
+							dy = exact_delta(O.y, p.y, (float)((code / 3) % 3 - 1) * Ly);
+							dz = exact_delta(O.z, p.z, (float)(code % 3 - 1) * Lz);
+						}
+					}
+					if (MODE == 0) {
+						s0.add(fmul(m4_term(COMP ? sq3(dx, dy, dz) : d2, X.y, X.w), A.w));
+					} else {
+						const float f = COMP ? grad_factor_f32(sq3(dx, dy, dz), X.y, X.z) : grad_factor(d2, X.y, X.z);
+						s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
+						ehit[e] = stamp;
+					}
+				}
+			}
+		}
+		int nx = node;
+		while (nx & 1) nx >>= 1;
+		node = nx + 1;
+		if (node == 1) node = 0;
+	}
+	// fold the S partial sums of each point (xor partners stay within the group)
+#pragma unroll
+	for (int o = S / 2; o > 0; o >>= 1) {
+		s0.merge(__shfl_xor(s0.s, o, 64), __shfl_xor(s0.cc(), o, 64));
+		if (MODE == 1) {
+			s1.merge(__shfl_xor(s1.s, o, 64), __shfl_xor(s1.cc(), o, 64));
+			s2.merge(__shfl_xor(s2.s, o, 64), __shfl_xor(s2.cc(), o, 64));
+		}
+	}
+	if (!valid || sub != 0) return;
+	if (MODE == 0) { out_sum[ii] = s0.val(); return; }
 	const float fStep = micro ? bt[pb].fMicro : bt[pb].fStep;
 	const float ax = s0.val(), ay = s1.val(), az = s2.val();
 	const float ss = sq3(ax, ay, az);
@@ -951,6 +1052,8 @@ extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
 	if (!c) return SKG_E_NOMEM;
 	c->P = *P;
 	c->dev = P->device;
+	c->subMax = 1 << 18;
+	if (const char *e = getenv("SKG_COVER_SUB_MAX")) c->subMax = atol(e);
 	if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
 	    hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
 		delete c;
@@ -1109,15 +1212,25 @@ static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, in
 		float *out, int stamp, int micro, double work) {
 	if (np <= 0) return 0;
 	const int pslot = skg_prof_begin(c);
-	const int nb = nblk(np, 64);
+	const int nb = nblk(np, SKG_TILE_T);
 	const bool comp = c->P.accumulate == SKG_ACC_COMPENSATED;
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
 #define CL(M_, C_)                                                                             \
-	k_cover<M_, C_><<<nb, 64, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, \
+	k_cover<M_, C_><<<nb, SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, \
 			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, \
 			C[0], C[1], C[2], L[0], L[1], L[2])
-	if (mode == 0) { if (comp) CL(0, true); else CL(0, false); }
-	else { if (comp) CL(1, true); else CL(1, false); }
+#define CS(M_, C_)                                                                             \
+	k_cover_sub<M_, C_, 16><<<nblk(16L * np, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, \
+			c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp,       \
+			c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2])
+	if (np <= c->subMax) {
+		if (mode == 0) { if (comp) CS(0, true); else CS(0, false); }
+		else { if (comp) CS(1, true); else CS(1, false); }
+	} else {
+		if (mode == 0) { if (comp) CL(0, true); else CL(0, false); }
+		else { if (comp) CL(1, true); else CL(1, false); }
+	}
+#undef CS
 #undef CL
 	skg_prof_end(c, pslot, mode == 0 ? SKG_K_COVER_DENS : SKG_K_COVER_MOVE, work);
 	SKG_HIP(hipGetLastError());

@@ -51,11 +51,12 @@ __device__ __forceinline__ float max_delta(float q, float qm, float qp, float lo
 }
 
 template <bool PER>
-__global__ __launch_bounds__(64) void k_fof_links(const float4 *__restrict__ fp, int n,
+__global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restrict__ fp, int n,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int depth, int B,
 		float L, float tau2c, const int *__restrict__ fbox, const DevBox *__restrict__ bt, int *par, int *united) {
-	const int lane = threadIdx.x;
-	const int base = blockIdx.x * 64;
+	const int lane = threadIdx.x & 63;
+	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
+	if (base >= n) return;                        // whole wavefront past the end
 	const int i = base + lane;
 	const bool valid = i < n;
 	const float4 q = fp[valid ? i : base];
@@ -69,6 +70,8 @@ __global__ __launch_bounds__(64) void k_fof_links(const float4 *__restrict__ fp,
 	const float3 bhi = make_float3(wave_max(q.x), wave_max(q.y), wave_max(q.z));
 	int node = 1, lev = 0;
 	for (;;) {
+		node = uniform_int(node);
+		lev = uniform_int(lev);
 		const float4 lo = nlo[node], hi = nhi[node];
 		bool descend = false;
 		if (box_box_d2<PER>(lo, hi, blo, bhi, L) < tau2w) {
@@ -191,9 +194,9 @@ static int fof_run(skg_ctx *c, const float4 *pos, int n, float tau, const int *o
 	k_fof_init<<<nblk(n, 256), 256, 0, c->st>>>(par, n);
 	const float tau2 = tau * tau;
 	const int pslot = skg_prof_begin(c);
-	if (per) k_fof_links<true><<<nblk(n, 64), 64, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2,
+	if (per) k_fof_links<true><<<nblk(n, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2,
 			fbox, c->d_bt, par, united);
-	else k_fof_links<false><<<nblk(n, 64), 64, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2,
+	else k_fof_links<false><<<nblk(n, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2,
 			fbox, c->d_bt, par, united);
 	skg_prof_end(c, pslot, SKG_K_FOF, 16.0 * n);
 	k_fill<<<nblk(n, 256), 256, 0, c->st>>>(cmin, n, INT_MAX);
