@@ -88,7 +88,8 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restri
 		const float4 *__restrict__ eorg, const int *__restrict__ esrc, const int *__restrict__ sord, int ne,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		const float *__restrict__ mass, const float *__restrict__ gbar, const int *__restrict__ pord,
-		int parts, float Lx, float Ly, float Lz, const int *__restrict__ pbox, float *out) {
+		int parts, float Lx, float Ly, float Lz, const int *__restrict__ pbox, const int *__restrict__ lstart,
+		float *out) {
 	const int lane = threadIdx.x & 63, base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63),
 			ii = base + lane;
 	if (base >= np) return;                       // whole wavefront past the end
@@ -107,7 +108,8 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restri
 		const bool visit = __any(need);
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit && need) {
-			const int start = uniform_int((node - nLeaf2) * B), stop = uniform_int(min(ne, start + B));
+			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
+			const int start = uniform_int(rg.x), stop = uniform_int(rg.y);
 			for (int e = start; e < stop; ++e) {
 				const float4 A = epos[e];
 				float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
@@ -168,10 +170,17 @@ __global__ void k_add3_ord(const float *__restrict__ in, const int *__restrict__
 	for (int c = 0; c < 3; ++c) dx[3 * o + c] = fadd(dx[3 * o + c], in[3 * i + c]);
 }
 
-// The entity tree with every radius as built (independent of later ScatterCuts).
+// The entity tree with every radius as built (independent of later ScatterCuts):
+// the move tree's leaves, refitted with the radii as built.
 static int full_tree(skg_ctx *c) {
 	if (c->tfull.lo && c->tfull.n == c->nE) return 0;
-	return skg_build_tree(c, &c->tfull, c->e_pos, c->nE, c->te.B, c->e_b2, c->e_box);
+	DevTree *t = &c->tfull;
+	skg_free_tree(t);
+	if (dalloc(&t->lo, 2 * (size_t)c->te.nLeaf2) || dalloc(&t->hi, 2 * (size_t)c->te.nLeaf2)) return SKG_E_NOMEM;
+	int rc;
+	if ((rc = skg_copy_tree_leaves(c, t, &c->te))) return rc;
+	t->n = c->te.n; t->B = c->te.B; t->nLeaf = c->te.nLeaf; t->nLeaf2 = c->te.nLeaf2;
+	return skg_refit_tree(c, t, c->e_pos, c->e_b2, c->e_box);
 }
 
 template <int MODE>
@@ -181,7 +190,7 @@ static void launch_cover(skg_ctx *c, const float4 *pts, int np, const int *pord,
 	const bool per = c->P.bPeriodic != 0;
 	k_adj_cover<MODE><<<nblk(np, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(pts, np, c->e_pos, c->e_aux, c->e_b2, c->e_org, c->e_src,
 			c->s_ord, c->nE, c->tfull.lo, c->tfull.hi, c->tfull.nLeaf2, c->tfull.B, c->d_mass, gbar, pord, parts,
-			per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, c->nBox > 1 ? pbox : nullptr, out);
+			per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, c->nBox > 1 ? pbox : nullptr, c->tfull.lstart, out);
 }
 
 int skg_gradient_points(skg_ctx *c, const float4 *q, int nq, float *out) {

@@ -42,6 +42,7 @@ struct DevTree {
 	int nLeaf2 = 0;    // next power of two >= nLeaf (>= 1)
 	float4 *lo = nullptr;   // [2*nLeaf2], index 0 unused; lo.w = max live fBall2 below
 	float4 *hi = nullptr;   // hi.w: bits of the box range [b0, b1] below (b0 | b1 << 16)
+	int *lstart = nullptr;  // [nLeaf2+1] item offset of each leaf (none: leaf l = [lB, lB+B))
 };
 
 struct skg_ctx {
@@ -160,6 +161,13 @@ __device__ __forceinline__ int wave_max_i(int v) {
 __device__ __forceinline__ int xcd_tile(int b, int G) {
 	const int q = G / SKG_NXCD, r = G % SKG_NXCD, x = b % SKG_NXCD;
 	return x * q + min(x, r) + b / SKG_NXCD;
+}
+
+// Item range of leaf l (uniform leaves of B items, or the leaf offsets).
+__device__ __forceinline__ int2 leaf_range(const int *lstart, int l, int B, int n) {
+	if (lstart) return make_int2(lstart[l], lstart[l + 1]);
+	const int s = l * B;
+	return make_int2(s, min(n, s + B));
 }
 
 // ---------------------------------------------------------------- box ranges
@@ -311,6 +319,10 @@ int skg_sort_pairs(skg_ctx *c, unsigned long long *kin, unsigned long long *kout
 		int n, int endBit);
 int skg_build_tree(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, const float *ball2live,
 		const int *ibox);
+int skg_build_tree_prefix(skg_ctx *c, DevTree *t, const float4 *pos, const unsigned long long *keys, int n,
+		int B, int minDepth, const float *ball2live, const int *ibox);
+int skg_refit_tree(skg_ctx *c, DevTree *t, const float4 *pos, const float *ball2live, const int *ibox);
+int skg_copy_tree_leaves(skg_ctx *c, DevTree *dst, const DevTree *src);
 int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box);
 int skg_box_bits(const skg_ctx *c);
 int skg_exclusive_scan(skg_ctx *c, const int *in, int *out, int n);

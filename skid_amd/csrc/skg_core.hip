@@ -237,15 +237,16 @@ int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *p
 // Leaf bounds skip filler items (NaN position); hi.w receives the leaf's box
 // range (ibox indexed like pos; none: box 0).
 __global__ void k_tree_leaves(const float4 *__restrict__ pos, int n, int B, int nLeaf, int nLeaf2,
-		const float *__restrict__ b2, const int *__restrict__ ibox, float4 *lo, float4 *hi) {
+		const float *__restrict__ b2, const int *__restrict__ ibox, const int *__restrict__ lstart,
+		float4 *lo, float4 *hi) {
 	int l = blockIdx.x * blockDim.x + threadIdx.x;
 	if (l >= nLeaf2) return;
 	float4 a = make_float4(INFINITY, INFINITY, INFINITY, -INFINITY);
 	float4 b = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.f);
 	int b0 = 0xFFFF, b1 = 0;
 	if (l < nLeaf) {
-		int s = l * B, e = min(n, s + B);
-		for (int i = s; i < e; ++i) {
+		const int2 r = leaf_range(lstart, l, B, n);
+		for (int i = r.x; i < r.y; ++i) {
 			const int bx = ibox ? ibox[i] : 0;
 			b0 = min(b0, bx); b1 = max(b1, bx);
 			float4 p = pos[i];
@@ -271,11 +272,11 @@ __global__ void k_tree_level(float4 *lo, float4 *hi, int first, int count) {
 	hi[i] = make_float4(fmaxf(b0.x, b1.x), fmaxf(b0.y, b1.y), fmaxf(b0.z, b1.z), brange_pack(x0, x1));
 }
 
-void skg_free_tree(DevTree *t) { dfree(&t->lo); dfree(&t->hi); t->n = t->nLeaf = t->nLeaf2 = 0; }
+void skg_free_tree(DevTree *t) { dfree(&t->lo); dfree(&t->hi); dfree(&t->lstart); t->n = t->nLeaf = t->nLeaf2 = 0; }
 
 static int tree_fill(skg_ctx *c, DevTree *t, const float4 *pos, const float *b2, const int *ibox) {
 	k_tree_leaves<<<nblk(t->nLeaf2, 256), 256, 0, c->st>>>(pos, t->n, t->B, t->nLeaf, t->nLeaf2, b2, ibox,
-			t->lo, t->hi);
+			t->lstart, t->lo, t->hi);
 	for (int m = t->nLeaf2 >> 1; m >= 1; m >>= 1)
 		k_tree_level<<<nblk(m, 256), 256, 0, c->st>>>(t->lo, t->hi, m, m);
 	SKG_HIP(hipGetLastError());
@@ -288,8 +289,89 @@ int skg_build_tree(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, cons
 	if (nLeaf2 != t->nLeaf2 || !t->lo) {
 		if (dalloc(&t->lo, 2 * (size_t)nLeaf2) || dalloc(&t->hi, 2 * (size_t)nLeaf2)) return SKG_E_NOMEM;
 	}
+	dfree(&t->lstart);
 	t->n = n; t->B = B; t->nLeaf = nLeaf; t->nLeaf2 = nLeaf2;
 	return tree_fill(c, t, pos, b2, ibox);
+}
+
+// Same structure, new bounds and radii (positions or live radii changed).
+int skg_refit_tree(skg_ctx *c, DevTree *t, const float4 *pos, const float *b2, const int *ibox) {
+	return tree_fill(c, t, pos, b2, ibox);
+}
+
+// ---------------------------------------------------------------------------
+// Prefix leaves: each leaf is a maximal cell of the key hierarchy (a Morton
+// prefix; the box id leads the key) holding at most B items, so leaves are
+// compact octree-like cells instead of runs of B consecutive keys (which
+// straddle cell boundaries and give long thin bounding boxes).  Equal keys are
+// split further by index bits.  For item i the leaf depth is the smallest d
+// with #{j : lcp(i, j) >= d} <= B; a cell of at most B items lies within
+// [i-B, i+B], so counting over that window is exact.
+__device__ __forceinline__ int key_lcp(const unsigned long long *__restrict__ k, int i, int j) {
+	const unsigned long long a = k[i], b = k[j];
+	return a != b ? __clzll((long long)(a ^ b)) : 64 + __clz(i ^ j);
+}
+__global__ void k_leaf_depth(const unsigned long long *__restrict__ keys, int n, int B, int minDepth, int *depth) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int j0 = max(0, i - B), j1 = min(n - 1, i + B);
+	int lo = 0, hi = 97;                  // count(hi) = 1 <= B
+	while (lo < hi) {
+		const int d = (lo + hi) >> 1;
+		int cnt = 1;
+		for (int j = j0; j <= j1; ++j)
+			if (j != i && key_lcp(keys, i, j) >= d) ++cnt;
+		if (cnt <= B) hi = d; else lo = d + 1;
+	}
+	depth[i] = max(lo, minDepth);
+}
+__global__ void k_leaf_flags(const unsigned long long *__restrict__ keys, const int *__restrict__ depth, int n,
+		int *flag) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	flag[i] = (i == 0 || depth[i] != depth[i - 1] || key_lcp(keys, i, i - 1) < depth[i]) ? 1 : 0;
+}
+__global__ void k_leaf_starts(const int *__restrict__ flag, const int *__restrict__ off, int n, int nLeaf,
+		int nLeaf2, int *lstart) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n && flag[i]) lstart[off[i]] = i;
+	if (i >= nLeaf && i <= nLeaf2) lstart[i] = n;     // end marker and empty leaves
+}
+
+int skg_build_tree_prefix(skg_ctx *c, DevTree *t, const float4 *pos, const unsigned long long *keys, int n,
+		int B, int minDepth, const float *b2, const int *ibox) {
+	int rc;
+	if (n <= 0) return skg_build_tree(c, t, pos, n, B, b2, ibox);
+	int *depth = nullptr, *flag = nullptr, *off = nullptr;
+	if (dalloc(&depth, n) || dalloc(&flag, n) || dalloc(&off, n)) return SKG_E_NOMEM;
+	k_leaf_depth<<<nblk(n, 256), 256, 0, c->st>>>(keys, n, B, minDepth, depth);
+	k_leaf_flags<<<nblk(n, 256), 256, 0, c->st>>>(keys, depth, n, flag);
+	if ((rc = skg_exclusive_scan(c, flag, off, n))) return rc;
+	int last[2];
+	SKG_HIP(hipMemcpyAsync(&last[0], off + n - 1, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(&last[1], flag + n - 1, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	const int nLeaf = last[0] + last[1];
+	int nLeaf2 = 1;
+	while (nLeaf2 < nLeaf) nLeaf2 <<= 1;
+	if (nLeaf2 != t->nLeaf2 || !t->lo) {
+		if (dalloc(&t->lo, 2 * (size_t)nLeaf2) || dalloc(&t->hi, 2 * (size_t)nLeaf2)) return SKG_E_NOMEM;
+	}
+	if (dalloc(&t->lstart, (size_t)nLeaf2 + 1)) return SKG_E_NOMEM;
+	k_leaf_starts<<<nblk(std::max(n, nLeaf2 + 1), 256), 256, 0, c->st>>>(flag, off, n, nLeaf, nLeaf2, t->lstart);
+	t->n = n; t->B = B; t->nLeaf = nLeaf; t->nLeaf2 = nLeaf2;
+	rc = tree_fill(c, t, pos, b2, ibox);
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&depth); dfree(&flag); dfree(&off);
+	return rc;
+}
+
+int skg_copy_tree_leaves(skg_ctx *c, DevTree *dst, const DevTree *src) {
+	dfree(&dst->lstart);
+	if (!src->lstart) return 0;
+	if (dalloc(&dst->lstart, (size_t)src->nLeaf2 + 1)) return SKG_E_NOMEM;
+	SKG_HIP(hipMemcpyAsync(dst->lstart, src->lstart, 4 * ((size_t)src->nLeaf2 + 1), hipMemcpyDeviceToDevice, c->st));
+	return 0;
 }
 
 
@@ -470,7 +552,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover(
 		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
 		const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
-		float cx, float cy, float cz, float Lx, float Ly, float Lz) {
+		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart) {
 	const int lane = threadIdx.x & 63;
 	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
 	if (base >= np) return;                       // whole wavefront past the end
@@ -493,9 +575,10 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover(
 		const bool visit = __any(need);
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit) {
-			const int start = uniform_int((node - nLeaf2) * B);
-			if (start < ne) {
-				const int cnt = uniform_int(min(B, ne - start));
+			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
+			const int start = uniform_int(rg.x);
+			const int cnt = uniform_int(rg.y - rg.x);
+			if (cnt > 0) {
 				if (need) {
 					// records are read four at a time (one wait per four)
 					for (int t0 = 0; t0 < cnt; t0 += 4) {
@@ -574,7 +657,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
 		const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
-		float cx, float cy, float cz, float Lx, float Ly, float Lz) {
+		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart) {
 	const int blk = xcd_tile(blockIdx.x, gridDim.x);
 	const int gl = blk * SKG_TILE_T + threadIdx.x;
 	const int ii = gl / S, sub = gl % S;
@@ -592,9 +675,8 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 		const bool need = brange_has(hi.w, pb) && point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
 		if (need && node < nLeaf2) { node = 2 * node; continue; }
 		if (need) {
-			const int start = (node - nLeaf2) * B;
-			const int stop = min(ne, start + B);
-			for (int e = start + sub; e < stop; e += S) {
+			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
+			for (int e = rg.x + sub; e < rg.y; e += S) {
 				const float4 A = epos[e], X = eaux[e];
 				float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
 				float d2 = sq3(dx, dy, dz);
@@ -767,7 +849,8 @@ __global__ void k_box_first(const int *__restrict__ ebu, const int *__restrict__
 }
 // Filler entities of the padded array: dead, NaN position, never hit.
 __global__ void k_entity_fill(int ne, const int *__restrict__ eblk, int nBox, float4 *e4, float4 *eaux,
-		float4 *eorg, float *erho, int *esrc, int *elive, int *ehit, float *eb2s, int *ebox) {
+		float4 *eorg, float *erho, int *esrc, int *elive, int *ehit, float *eb2s, int *ebox,
+		unsigned long long *ekey, int bbits) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= ne) return;
 	int lo = 0, hi = nBox - 1;
@@ -779,6 +862,8 @@ __global__ void k_entity_fill(int ne, const int *__restrict__ eblk, int nBox, fl
 	eaux[i] = make_float4(-1.f, 0.f, 0.f, 0.f);
 	eorg[i] = make_float4(NAN, NAN, NAN, 13.f);
 	erho[i] = 0.f; esrc[i] = -1; elive[i] = 0; ehit[i] = -1; eb2s[i] = -1.f; ebox[i] = lo;
+	// after every real key of the box
+	ekey[i] = ((unsigned long long)lo << (63 - bbits)) | ((1ull << (63 - bbits)) - 1);
 }
 
 // Entities in Morton order + per-entity constants (smooth1.c:447-448, 250-251),
@@ -787,7 +872,8 @@ __global__ void k_entity_setup(const float4 *__restrict__ e4u, const float *__re
 		const int *__restrict__ esrcu, const int *__restrict__ perm, int ne,
 		const int *__restrict__ ebu, const int *__restrict__ bfirst, const int *__restrict__ eblk,
 		const float *__restrict__ srho, const float4 *__restrict__ spos, float Lx, float Ly, float Lz,
-		float4 *e4, float4 *eaux, float4 *eorg, float *erho, int *esrc, int *elive, int *ehit, float *eb2s) {
+		float4 *e4, float4 *eaux, float4 *eorg, float *erho, int *esrc, int *elive, int *ehit, float *eb2s,
+		const unsigned long long *__restrict__ ksorted, unsigned long long *ekey) {
 	int ii = blockIdx.x * blockDim.x + threadIdx.x;
 	if (ii >= ne) return;
 	int j = perm[ii];
@@ -812,6 +898,7 @@ __global__ void k_entity_setup(const float4 *__restrict__ e4u, const float *__re
 	erho[i] = srho ? srho[esrcu[j]] : 0.f;
 	elive[i] = 1;
 	ehit[i] = -1;
+	ekey[i] = ksorted[ii];
 }
 
 __global__ void k_entity_rho(const int *__restrict__ esrc, int ne, const float *__restrict__ srho, float *erho) {
@@ -1218,11 +1305,11 @@ static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, in
 #define CL(M_, C_)                                                                             \
 	k_cover<M_, C_><<<nb, SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, \
 			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, \
-			C[0], C[1], C[2], L[0], L[1], L[2])
+			C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart)
 #define CS(M_, C_)                                                                             \
 	k_cover_sub<M_, C_, 16><<<nblk(16L * np, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, \
 			c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp,       \
-			c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2])
+			c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart)
 	if (np <= c->subMax) {
 		if (mode == 0) { if (comp) CS(0, true); else CS(0, false); }
 		else { if (comp) CS(1, true); else CS(1, false); }
@@ -1351,20 +1438,23 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		const int nE = 256 * c->h_eblk[nBox];
 		c->nE = nE;
 		float *eb2s = nullptr;
-		if (dalloc(&eb2s, nE) || dalloc(&c->e_pos, nE) || dalloc(&c->e_aux, nE) || dalloc(&c->e_rho, nE) ||
+		unsigned long long *ekey = nullptr;
+		if (dalloc(&ekey, nE) || dalloc(&eb2s, nE) || dalloc(&c->e_pos, nE) || dalloc(&c->e_aux, nE) || dalloc(&c->e_rho, nE) ||
 		    dalloc(&c->e_org, nE) || dalloc(&c->e_src, nE) || dalloc(&c->e_live, nE) || dalloc(&c->e_hit, nE) ||
 		    dalloc(&c->e_box, nE) || dalloc(&c->d_eblk, nBox + 1))
 			return SKG_E_NOMEM;
 		SKG_HIP(hipMemcpyAsync(c->d_eblk, c->h_eblk, 4 * (size_t)(nBox + 1), hipMemcpyHostToDevice, c->st));
+		const int bbits = nBox > 1 ? skg_box_bits(c) : 0;
 		k_entity_fill<<<nblk(nE, 256), 256, 0, c->st>>>(nE, c->d_eblk, nBox, c->e_pos, c->e_aux, c->e_org, c->e_rho,
-				c->e_src, c->e_live, c->e_hit, eb2s, c->e_box);
+				c->e_src, c->e_live, c->e_hit, eb2s, c->e_box, ekey, bbits);
 		k_entity_setup<<<nblk(nEu, 256), 256, 0, c->st>>>(e4u, eb2u, esrcu, perm2, nEu, ebu, bfirst, c->d_eblk,
 				nullptr, c->s_pos, per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, c->e_pos, c->e_aux,
-				c->e_org, c->e_rho, c->e_src, c->e_live, c->e_hit, eb2s);
-		if ((rc = skg_build_tree(c, &c->te, c->e_pos, nE, B, eb2s, c->e_box))) return rc;
+				c->e_org, c->e_rho, c->e_src, c->e_live, c->e_hit, eb2s, c->k64b, ekey);
+		// leaves: maximal key cells of at most B entities, never across boxes
+		if ((rc = skg_build_tree_prefix(c, &c->te, c->e_pos, ekey, nE, B, 1 + bbits, eb2s, c->e_box))) return rc;
 		SKG_HIP(hipStreamSynchronize(c->st));
 		dfree(&mask); dfree(&cnt); dfree(&roff); dfree(&e4u); dfree(&eb2u); dfree(&esrcu); dfree(&perm2);
-		dfree(&ebu); dfree(&bfirst);
+		dfree(&ebu); dfree(&bfirst); dfree(&ekey);
 		dfree(&c->e_b2);
 		c->e_b2 = eb2s;
 
@@ -1478,7 +1568,7 @@ static int refit_entities(skg_ctx *c) {
 	float *b2 = nullptr;
 	if (dalloc(&b2, c->nE)) return SKG_E_NOMEM;
 	k_live_b2<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_aux, c->nE, b2);
-	int rc = skg_build_tree(c, &c->te, c->e_pos, c->nE, c->te.B, b2, c->e_box);
+	int rc = skg_refit_tree(c, &c->te, c->e_pos, b2, c->e_box);
 	SKG_HIP(hipStreamSynchronize(c->st));
 	dfree(&b2);
 	return rc;
