@@ -124,6 +124,19 @@ struct skg_ctx {
 	// launch shapes (set at create; SKG_COVER_SUB_MAX overrides): point sets of
 	// at most subMax points use the sub-wave cover kernel
 	long subMax = 0;
+	// per-tile leaf lists of the current 5-step block (SKG_TILE_LISTS=0: off)
+	int useLists = 1, tlCap = 512;
+	int *tl_list = nullptr, *tl_cnt = nullptr;
+	// device-resident move loop: ml[0] step stamp, ml[1] active, ml[2] next
+	int *d_ml = nullptr;
+	int *m_flag = nullptr, *m_bcnt = nullptr;   // prune flags; block counts then offsets
+	float *e_b2live = nullptr;
+	int *h_pin = nullptr;                        // pinned read-back
+	int useGraph = 1;                            // SKG_GRAPH=0: launch blocks directly
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t gexec = nullptr;
+	long gcap = 0;
+	int gsub = 0;
 };
 
 // ---------------------------------------------------------------- errors

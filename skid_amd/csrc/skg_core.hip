@@ -541,6 +541,99 @@ __global__ __launch_bounds__(64) void k_knn(
 // reference's smx->pp).  MODE 0 accumulates the scatter half of the density,
 // MODE 1 the density gradient and then moves the point (kdMoveParticles).
 // =====================================================================
+// Test the cnt entities from start against the lane's point (lanes with want)
+// and accumulate: MODE 0 the scatter density, MODE 1 the gradient (hits are
+// stamped for ScatterCut).  Records are read four at a time (one wait each).
+template <int MODE, bool COMP>
+__device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const float4 &p,
+		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
+		int *__restrict__ ehit, int stamp, float Lx, float Ly, float Lz, Sum<COMP> &s0, Sum<COMP> &s1,
+		Sum<COMP> &s2) {
+	if (cnt <= 0 || !want) return;
+	for (int t0 = 0; t0 < cnt; t0 += 4) {
+		float4 A4[4], X4[4];
+#pragma unroll
+		for (int u = 0; u < 4; ++u) {
+			const int e = min(start + t0 + u, start + cnt - 1);
+			A4[u] = epos[e]; X4[u] = eaux[e];
+		}
+#pragma unroll
+		for (int u = 0; u < 4; ++u) {
+			if (t0 + u >= cnt) break;
+			const int e = start + t0 + u;
+			const float4 A = A4[u], X = X4[u];
+			float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
+			float d2 = sq3(dx, dy, dz);
+			if (d2 < X.x) {
+				if (COMP) {
+					// replica: separation from the source position plus the exact shift
+					const float4 O = eorg[e];
+					const int code = (int)O.w;     // 9(ix+1) + 3(iy+1) + (iz+1)
+					if (code != 13) {
+						dx = exact_delta(O.x, p.x, (float)(code / 9 - 1) * Lx);
+						dy = exact_delta(O.y, p.y, (float)((code / 3) % 3 - 1) * Ly);
+						dz = exact_delta(O.z, p.z, (float)(code % 3 - 1) * Lz);
+					}
+				}
+				if (MODE == 0) {
+					s0.add(fmul(m4_term(COMP ? sq3(dx, dy, dz) : d2, X.y, X.w), A.w));
+				} else {
+					const float f = COMP ? grad_factor_f32(sq3(dx, dy, dz), X.y, X.z) : grad_factor(d2, X.y, X.z);
+					s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
+					ehit[e] = stamp;
+				}
+			}
+		}
+	}
+}
+
+// Leaf lists for a block of move steps.  Entities do not move and ScatterCut
+// only removes balls, so while the active list (and with it every tile of 64
+// movers) is unchanged, the leaves any step can need are those whose ball
+// reach, widened by the distance a mover can still travel (margin), holds a
+// lane's point now.  One walk per block then replaces one walk per step.  A
+// tile with a point within the margin of a periodic face (it may wrap to the
+// far side) or with more than cap leaves gets count -1 and walks the tree.
+__global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__restrict__ ppos,
+		const int *__restrict__ plist, int np, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi,
+		int nLeaf2, const int *__restrict__ pbox, float margin, int per, float cx, float cy, float cz,
+		float Lx, float Ly, float Lz, int cap, int *__restrict__ tlist, int *__restrict__ tcnt,
+		const int *__restrict__ npDev) {
+	if (npDev) np = *npDev;
+	const int lane = threadIdx.x & 63;
+	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
+	if (base >= np) return;
+	const int tile = base >> 6;
+	const int ii = base + lane;
+	const bool valid = ii < np;
+	const int pi = plist ? plist[valid ? ii : base] : (valid ? ii : base);
+	const float4 p = ppos[pi];
+	const int pb = pbox ? pbox[pi] : 0;
+	bool nearFace = false;
+	if (per && valid)
+		nearFace = fabsf(p.x - cx) > 0.5f * Lx - margin || fabsf(p.y - cy) > 0.5f * Ly - margin ||
+				fabsf(p.z - cz) > 0.5f * Lz - margin;
+	if (__any(nearFace)) { if (lane == 0) tcnt[tile] = -1; return; }
+	int n = 0, node = 1;
+	while (node) {
+		node = uniform_int(node);
+		const float4 lo = nlo[node], hi = nhi[node];
+		bool need = false;
+		if (valid && lo.w >= 0.f && brange_has(hi.w, pb)) {
+			const float r = __fsqrt_rn(lo.w) + margin;
+			need = point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < r * r * 1.000001f;
+		}
+		if (__any(need)) {
+			if (node < nLeaf2) { node = 2 * node; continue; }
+			if (lane == 0 && n < cap) tlist[(size_t)tile * cap + n] = node - nLeaf2;
+			++n;
+		}
+		node = (node >> __builtin_ctz(~node)) + 1;
+		if (node == 1) node = 0;
+	}
+	if (lane == 0) tcnt[tile] = n <= cap ? n : -1;
+}
+
 // pbox (indexed like ppos; none: box 0) gives each point's box: a lane opens
 // only nodes holding its box, and the entity array starts every box on a new
 // leaf, so every entity a lane tests is of its own box.  MODE 1 takes the
@@ -552,7 +645,11 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover(
 		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
 		const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
-		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart) {
+		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart,
+		const int *__restrict__ tlist, const int *__restrict__ tcnt, int tcap, const int *__restrict__ npDev,
+		const int *__restrict__ stampDev) {
+	if (npDev) np = *npDev;                       // device-resident active count and step stamp
+	if (stampDev) stamp = *stampDev;
 	const int lane = threadIdx.x & 63;
 	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
 	if (base >= np) return;                       // whole wavefront past the end
@@ -563,12 +660,24 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover(
 	const int pb = pbox ? pbox[pi] : 0;
 	Sum<COMP> s0, s1, s2;
 	s0.init(); s1.init(); s2.init();
-	// A node is opened when its largest live ball reaches at least one lane's
-	// own point (so scattered blocks late in the move loop walk only the union
-	// of their lanes' paths).  The node index is wave-uniform, so node bounds
-	// and entity records are read through the scalar cache.
-	int node = 1;
-	for (;;) {
+	// With a leaf list for this tile (built at the start of the block, see
+	// k_cover_lists) the wavefront scans those leaves and walks no tree.
+	// Otherwise a node is opened when its largest live ball reaches at least
+	// one lane's own point (so scattered blocks late in the move loop walk
+	// only the union of their lanes' paths).  The node index is wave-uniform,
+	// so node bounds and entity records are read through the scalar cache.
+	const int tile = base >> 6;
+	const int nl = tcnt ? uniform_int(tcnt[tile]) : -1;
+	int node = nl >= 0 ? 0 : 1;
+	for (int k = 0; k < nl; ++k) {
+		const int leaf = uniform_int(tlist[(size_t)tile * tcap + k]);
+		const int2 rg = leaf_range(lstart, leaf, B, ne);
+		// leaves hold one box each; a tile may straddle two boxes
+		const bool want = valid && (!pbox || brange_has(nhi[nLeaf2 + leaf].w, pb));
+		cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), want, p, epos, eaux, eorg, ehit, stamp,
+				Lx, Ly, Lz, s0, s1, s2);
+	}
+	while (node) {
 		node = uniform_int(node);
 		const float4 lo = nlo[node], hi = nhi[node];
 		const bool need = valid && brange_has(hi.w, pb) && point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
@@ -576,54 +685,12 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover(
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit) {
 			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
-			const int start = uniform_int(rg.x);
-			const int cnt = uniform_int(rg.y - rg.x);
-			if (cnt > 0) {
-				if (need) {
-					// records are read four at a time (one wait per four)
-					for (int t0 = 0; t0 < cnt; t0 += 4) {
-					float4 A4[4], X4[4];
-#pragma unroll
-					for (int u = 0; u < 4; ++u) {
-						const int e = min(start + t0 + u, start + cnt - 1);
-						A4[u] = epos[e]; X4[u] = eaux[e];
-					}
-#pragma unroll
-					for (int u = 0; u < 4; ++u) {
-						if (t0 + u >= cnt) break;
-						const int e = start + t0 + u;
-						const float4 A = A4[u], X = X4[u];
-						float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
-						float d2 = sq3(dx, dy, dz);
-						if (d2 < X.x) {
-							if (COMP) {
-								// replica: separation from the source position plus the exact shift
-								const float4 O = eorg[e];
-								const int code = (int)O.w;     // 9(ix+1) + 3(iy+1) + (iz+1)
-								if (code != 13) {
-									dx = exact_delta(O.x, p.x, (float)(code / 9 - 1) * Lx);
-									dy = exact_delta(O.y, p.y, (float)((code / 3) % 3 - 1) * Ly);
-									dz = exact_delta(O.z, p.z, (float)(code % 3 - 1) * Lz);
-								}
-							}
-							if (MODE == 0) {
-								s0.add(fmul(m4_term(COMP ? sq3(dx, dy, dz) : d2, X.y, X.w), A.w));
-							} else {
-								const float f = COMP ? grad_factor_f32(sq3(dx, dy, dz), X.y, X.z) : grad_factor(d2, X.y, X.z);
-								s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
-								ehit[e] = stamp;
-							}
-						}
-					}
-					}
-				}
-			}
+			cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), need, p, epos, eaux, eorg, ehit,
+					stamp, Lx, Ly, Lz, s0, s1, s2);
 		}
-		// next node in pre-order after node's subtree
-		int nx = node;
-		while (nx & 1) nx >>= 1;
-		node = nx + 1;
-		if (node == 1) break;
+		// next node in pre-order after node's subtree (strip trailing ones, step)
+		node = (node >> __builtin_ctz(~node)) + 1;
+		if (node == 1) node = 0;
 	}
 	if (!valid) return;
 	if (MODE == 0) { out_sum[ii] = s0.val(); return; }
@@ -657,7 +724,10 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
 		const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
-		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart) {
+		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart,
+		const int *__restrict__ npDev, const int *__restrict__ stampDev) {
+	if (npDev) np = *npDev;
+	if (stampDev) stamp = *stampDev;
 	const int blk = xcd_tile(blockIdx.x, gridDim.x);
 	const int gl = blk * SKG_TILE_T + threadIdx.x;
 	const int ii = gl / S, sub = gl % S;
@@ -934,7 +1004,8 @@ __global__ void k_iota(int *a, int n) {
 // active mover this step; on the initial step live entities that held none
 // get density 0 (smooth1.c:460-470, 497-507).  Block minima go to partials.
 __global__ __launch_bounds__(256) void k_scat_thr(const int *__restrict__ elive, const int *__restrict__ ehit,
-		float *erho, int ne, int stamp, int bInitial, float *part) {
+		float *erho, int ne, const int *__restrict__ stampDev, int bInitial, float *part) {
+	const int stamp = *stampDev;
 	__shared__ float sm[4];
 	int e = blockIdx.x * blockDim.x + threadIdx.x;
 	float v = INFINITY;
@@ -1005,6 +1076,66 @@ __global__ __launch_bounds__(256) void k_sum_final(const int *__restrict__ part,
 	if (threadIdx.x == 0) out[0] = sm[0] + sm[1] + sm[2] + sm[3];
 }
 
+// Device-side move-loop bookkeeping: ml[0] step stamp, ml[1] active count,
+// ml[2] count after the prune.  Everything a block of steps launches reads
+// these, so the block can be replayed as one graph with no host round trip.
+__global__ void k_ml_set(int *ml, int i, int v) { ml[i] = v; }
+__global__ void k_bump(int *ml) { ml[0] += 1; }
+__global__ void k_ml_copy(int *ml) { ml[1] = ml[2]; }
+
+// Stream compaction of the active list by the prune flags: per-256 counts, a
+// one-workgroup scan of the counts, then ballot prefixes within each block.
+// Order is kept, so a Morton-sorted active list stays sorted.
+__global__ __launch_bounds__(256) void k_flag_counts(const int *__restrict__ flag, const int *__restrict__ nDev,
+		int *bcnt) {
+	__shared__ int sm[4];
+	const int n = *nDev;
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	const int f = i < n ? flag[i] : 0;
+	const int c = __popcll(__ballot(f));
+	if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = c;
+	__syncthreads();
+	if (threadIdx.x == 0) bcnt[blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
+}
+__global__ __launch_bounds__(1024) void k_scan_counts(const int *__restrict__ bcnt, int nb, int *boff, int *total) {
+	__shared__ int sm[1024];
+	const int t = threadIdx.x;
+	const int per = (nb + 1023) / 1024;
+	const int a = min(nb, t * per), b = min(nb, a + per);
+	int sum = 0;
+	for (int i = a; i < b; ++i) sum += bcnt[i];
+	sm[t] = sum;
+	__syncthreads();
+	for (int o = 1; o < 1024; o <<= 1) {
+		const int v = t >= o ? sm[t - o] : 0;
+		__syncthreads();
+		sm[t] += v;
+		__syncthreads();
+	}
+	int run = sm[t] - sum;
+	for (int i = a; i < b; ++i) { boff[i] = run; run += bcnt[i]; }
+	if (t == 1023) *total = sm[1023];
+}
+__global__ __launch_bounds__(256) void k_compact_dev(const int *__restrict__ list, const int *__restrict__ flag,
+		const int *__restrict__ nDev, const int *__restrict__ boff, int *out) {
+	__shared__ int sm[4];
+	const int n = *nDev;
+	const int i = blockIdx.x * 256 + threadIdx.x;
+	const int f = i < n ? flag[i] : 0;
+	const unsigned long long m = __ballot(f);
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	if (lane == 0) sm[w] = __popcll(m);
+	__syncthreads();
+	int off = boff[blockIdx.x];
+	for (int k = 0; k < w; ++k) off += sm[k];
+	if (f) out[off + __popcll(m & ((1ull << lane) - 1ull))] = list[i];
+}
+__global__ void k_copy_list(const int *__restrict__ in, int *ml, int *out) {
+	const int n = ml[2];
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) out[i] = in[i];
+}
+
 __global__ void k_live_b2(const float4 *__restrict__ eaux, int ne, float *b2live) {
 	int e = blockIdx.x * blockDim.x + threadIdx.x;
 	if (e < ne) b2live[e] = eaux[e].x;
@@ -1012,8 +1143,10 @@ __global__ void k_live_b2(const float4 *__restrict__ eaux, int ne, float *b2live
 
 // kdPruneInactive test (kd.c:754-776): a mover stays active while it moved at
 // least fCvg over the last block; survivors restart from here.
-__global__ void k_prune(const int *__restrict__ list, int na, const float4 *__restrict__ mpos, float4 *mold,
-		float hx, float hy, float hz, const int *__restrict__ mbox, const DevBox *__restrict__ bt, int *flag) {
+__global__ void k_prune(const int *__restrict__ list, const int *__restrict__ naDev, const float4 *__restrict__ mpos,
+		float4 *mold, float hx, float hy, float hz, const int *__restrict__ mbox, const DevBox *__restrict__ bt,
+		int *flag) {
+	const int na = *naDev;
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= na) return;
 	int m = list[i];
@@ -1038,7 +1171,9 @@ __global__ void k_compact(const int *__restrict__ list, const int *__restrict__ 
 }
 
 // Active movers per box (lanes of one box add once per wavefront).
-__global__ void k_box_count(const int *__restrict__ list, int n, const int *__restrict__ mbox, int *cnt) {
+__global__ void k_box_count(const int *__restrict__ list, const int *__restrict__ nDev, const int *__restrict__ mbox,
+		int *cnt) {
+	const int n = *nDev;
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	const int b = i < n ? mbox[list[i]] : -1;
 	const int b0 = __shfl(b, 0, 64);
@@ -1141,6 +1276,8 @@ extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
 	c->dev = P->device;
 	c->subMax = 1 << 18;
 	if (const char *e = getenv("SKG_COVER_SUB_MAX")) c->subMax = atol(e);
+	if (const char *e = getenv("SKG_TILE_LISTS")) c->useLists = atoi(e);
+	if (const char *e = getenv("SKG_GRAPH")) c->useGraph = atoi(e);
 	if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
 	    hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
 		delete c;
@@ -1152,6 +1289,7 @@ extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
 }
 
 void skg_group_free(skg_ctx *c);
+void skg_drop_graph(skg_ctx *c);
 
 extern "C" int skg_destroy(skg_ctx *c) {
 	if (!c) return SKG_OK;
@@ -1166,6 +1304,10 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->e_b2); dfree(&c->e_pos); dfree(&c->e_aux); dfree(&c->e_org); dfree(&c->e_rho); dfree(&c->e_src); dfree(&c->e_live); dfree(&c->e_hit);
 	dfree(&c->e_box);
 	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2); dfree(&c->m_box);
+	dfree(&c->tl_list); dfree(&c->tl_cnt);
+	skg_drop_graph(c);
+	dfree(&c->d_ml); dfree(&c->m_flag); dfree(&c->m_bcnt); dfree(&c->e_b2live);
+	if (c->h_pin) { (void)hipHostFree(c->h_pin); c->h_pin = nullptr; }
 	dfree(&c->d_bt); dfree(&c->d_boxOf); dfree(&c->d_type); dfree(&c->d_eblk); dfree(&c->d_bact); dfree(&c->d_bscat);
 	dfree(&c->d_bthr);
 	delete[] c->hbox; delete[] c->h_eblk;
@@ -1305,11 +1447,12 @@ static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, in
 #define CL(M_, C_)                                                                             \
 	k_cover<M_, C_><<<nb, SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, \
 			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, \
-			C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart)
+			C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart, nullptr, nullptr, c->tlCap, nullptr, nullptr)
 #define CS(M_, C_)                                                                             \
 	k_cover_sub<M_, C_, 16><<<nblk(16L * np, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, \
 			c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp,       \
-			c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart)
+			c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart, \
+			nullptr, nullptr)
 	if (np <= c->subMax) {
 		if (mode == 0) { if (comp) CS(0, true); else CS(0, false); }
 		else { if (comp) CS(1, true); else CS(1, false); }
@@ -1502,7 +1645,113 @@ extern "C" int skg_neighbors(skg_ctx *c, int *nbr, float *d2) {
 	return SKG_OK;
 }
 
-static int count_active(skg_ctx *c);
+// ---------------------------------------------------------------------------
+// Move loop.  Every step reads the active count and step stamp from device
+// memory (ml), and a block of steps (lists, 5 x [gradient+move, ScatterCut],
+// prune, compaction, refit) launches with grids sized for a capacity, so the
+// whole block can be captured once as a hipGraph and replayed while the
+// active set shrinks; it is recaptured when the count halves (fewer idle
+// waves) or crosses the sparse-kernel size.  Without graphs (SKG_GRAPH=0, or
+// kernel profiling on) the same launches run directly.
+
+static void ml_set(skg_ctx *c, int i, int v) { k_ml_set<<<1, 1, 0, c->st>>>(c->d_ml, i, v); }
+
+// Active movers per box (for the per-box Ittr counters).
+static int count_active(skg_ctx *c, long cap) {
+	k_fill_i<<<nblk(c->nBox, 256), 256, 0, c->st>>>(c->d_bact, c->nBox, 0);
+	if (cap > 0)
+		k_box_count<<<nblk(cap, 256), 256, 0, c->st>>>(c->m_list, c->d_ml + 1, c->m_box, c->d_bact);
+	SKG_HIP(hipGetLastError());
+	return 0;
+}
+
+// Leaf lists for the next nsteps steps (margin: the distance a mover can
+// travel in nsteps steps of the largest box step).
+static void enqueue_lists(skg_ctx *c, long cap, int nsteps, int micro) {
+	float smax = 0.f;
+	for (int b = 0; b < c->nBox; ++b) smax = std::max(smax, c->hbox[b].step);
+	if (micro) smax = (float)(0.1 * (double)smax);
+	const float margin = 1.02f * nsteps * smax + 1e-6f * c->P.fPeriod[0] * (c->P.bPeriodic ? 1.f : 0.f);
+	const float *C = c->P.fCenter, *L = c->P.fPeriod;
+	k_cover_lists<<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap,
+			c->te.lo, c->te.hi, c->te.nLeaf2, c->nBox > 1 ? c->m_box : nullptr, margin, c->P.bPeriodic,
+			C[0], C[1], C[2], L[0], L[1], L[2], c->tlCap, c->tl_list, c->tl_cnt, c->d_ml + 1);
+}
+
+// nsteps Denmax steps over the active list (at most cap movers).
+static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps, int micro, int bInitial,
+		bool capturing) {
+	const bool comp = c->P.accumulate == SKG_ACC_COMPENSATED;
+	const float *C = c->P.fCenter, *L = c->P.fPeriod;
+	const int *pbox = c->nBox > 1 ? c->m_box : nullptr;
+	const int nb = nblk(c->nE, 256);
+	if (lists) enqueue_lists(c, cap, nsteps, micro);
+	for (int i = 0; i < nsteps; ++i) {
+		k_bump<<<1, 1, 0, c->st>>>(c->d_ml);
+		const int pslot = capturing ? -1 : skg_prof_begin(c);
+#define MV(C_)                                                                                                \
+		if (sub) k_cover_sub<1, C_, 16><<<nblk(16L * cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos,       \
+				c->m_list, (int)cap, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2,      \
+				c->te.B, nullptr, c->e_hit, 0, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2],         \
+				c->te.lstart, c->d_ml + 1, c->d_ml);                                                             \
+		else k_cover<1, C_><<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap,    \
+				c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, nullptr,         \
+				c->e_hit, 0, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart,             \
+				lists ? c->tl_list : nullptr, lists ? c->tl_cnt : nullptr, c->tlCap, c->d_ml + 1, c->d_ml)
+		if (comp) { MV(true); } else { MV(false); }
+#undef MV
+		if (pslot >= 0) skg_prof_end(c, pslot, SKG_K_COVER_MOVE, 24.0 * c->nActive + 24.0 * c->nLive);
+		k_scat_thr<<<nb, 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->nE, c->d_ml, bInitial, c->part);
+		k_box_min<<<c->nBox, 64, 0, c->st>>>(c->part, c->d_eblk, c->d_bthr);
+		k_scat_cut<<<nb, 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE, c->e_box, c->d_bthr, (int *)c->part);
+		k_box_sum<<<c->nBox, 64, 0, c->st>>>((const int *)c->part, c->d_eblk, c->d_bscat);
+		k_sum_final<<<1, 256, 0, c->st>>>((const int *)c->part, nb, c->d_cnt);
+	}
+	SKG_HIP(hipGetLastError());
+	return 0;
+}
+
+// kdPruneInactive (kd.c:735-793) + compaction + the entity tree refitted to
+// the live balls.
+static int enqueue_prune(skg_ctx *c, long cap) {
+	const float *L = c->P.fPeriod;
+	const float hx = 0.5f * L[0], hy = 0.5f * L[1], hz = 0.5f * L[2];
+	const int nb = nblk(cap, 256);
+	k_prune<<<nb, 256, 0, c->st>>>(c->m_list, c->d_ml + 1, c->m_pos, c->m_old, hx, hy, hz, c->m_box, c->d_bt,
+			c->m_flag);
+	k_flag_counts<<<nb, 256, 0, c->st>>>(c->m_flag, c->d_ml + 1, c->m_bcnt);
+	k_scan_counts<<<1, 1024, 0, c->st>>>(c->m_bcnt, nb, c->m_bcnt + nb, c->d_ml + 2);
+	k_compact_dev<<<nb, 256, 0, c->st>>>(c->m_list, c->m_flag, c->d_ml + 1, c->m_bcnt + nb, c->m_list2);
+	k_copy_list<<<nb, 256, 0, c->st>>>(c->m_list2, c->d_ml, c->m_list);
+	k_ml_copy<<<1, 1, 0, c->st>>>(c->d_ml);
+	int rc;
+	if ((rc = count_active(c, cap))) return rc;
+	k_live_b2<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_aux, c->nE, c->e_b2live);
+	return skg_refit_tree(c, &c->te, c->e_pos, c->e_b2live, c->e_box);
+}
+
+static int refit_entities(skg_ctx *c) {
+	k_live_b2<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_aux, c->nE, c->e_b2live);
+	return skg_refit_tree(c, &c->te, c->e_pos, c->e_b2live, c->e_box);
+}
+
+// Host copies of the live-entity total and the active count.
+static int read_counts(skg_ctx *c, int *nLive) {
+	SKG_HIP(hipMemcpyAsync(c->h_pin, c->d_ml, 12, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(c->h_pin + 4, c->d_cnt, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	c->nActive = c->h_pin[1];
+	c->nLive = c->h_pin[4];
+	if (nLive) *nLive = c->nLive;
+	return 0;
+}
+
+static void drop_graph(skg_ctx *c) {
+	if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+	if (c->graph) { (void)hipGraphDestroy(c->graph); c->graph = nullptr; }
+	c->gcap = 0;
+}
+void skg_drop_graph(skg_ctx *c) { drop_graph(c); }
 
 extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	if (!c) return SKG_E_PARAM;
@@ -1520,88 +1769,32 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	SKG_HIP(hipStreamSynchronize(c->st));
 	const int nm = last[0] + last[1];
 	c->nMove = nm;
+	drop_graph(c);
+	const size_t nbm = (size_t)nblk(std::max(nm, 1), 256);
 	if (dalloc(&c->m_ord, nm) || dalloc(&c->m_pos, nm) || dalloc(&c->m_old, nm) || dalloc(&c->m_list, nm) ||
-	    dalloc(&c->m_list2, nm) || dalloc(&c->m_box, nm))
+	    dalloc(&c->m_list2, nm) || dalloc(&c->m_box, nm) || dalloc(&c->m_flag, nm) || dalloc(&c->m_bcnt, 2 * nbm) ||
+	    dalloc(&c->e_b2live, std::max(c->nE, 1)))
 		return SKG_E_NOMEM;
+	if (!c->d_ml && dalloc(&c->d_ml, 4)) return SKG_E_NOMEM;
+	if (!c->h_pin && hipHostMalloc((void **)&c->h_pin, 32, hipHostMallocDefault) != hipSuccess) return SKG_E_NOMEM;
+	if (c->useLists && nm > c->subMax) {
+		const size_t nt = (size_t)nblk(nm, 64);
+		if (dalloc(&c->tl_list, nt * c->tlCap) || dalloc(&c->tl_cnt, nt)) return SKG_E_NOMEM;
+	}
+	if (skg_ensure_part(c, (size_t)nblk(std::max(c->nE, 1), 256))) return SKG_E_NOMEM;
 	k_compact<<<nblk(n, 256), 256, 0, c->st>>>(nullptr, flag, off, n, c->m_ord);
 	k_mover_init<<<nblk(nm, 256), 256, 0, c->st>>>(c->m_ord, nm, c->d_pos, c->d_boxOf, c->m_pos, c->m_old, c->m_box);
 	c->nActive = nm;
 	if (nm > 0 && (rc = skg_morton_sort(c, c->m_pos, nullptr, nm, c->m_list, c->nBox > 1 ? c->m_box : nullptr)))
 		return rc;
-	if ((rc = count_active(c))) return rc;
+	ml_set(c, 0, 0);
+	ml_set(c, 1, nm);
+	if ((rc = count_active(c, nm))) return rc;
 	SKG_HIP(hipStreamSynchronize(c->st));
 	dfree(&flag); dfree(&off);
-	c->step = 0;
 	if (nMove) *nMove = nm;
 	c->stage = 3;
 	return SKG_OK;
-}
-
-// Active movers per box (for the per-box Ittr counters).
-static int count_active(skg_ctx *c) {
-	k_fill_i<<<nblk(c->nBox, 256), 256, 0, c->st>>>(c->d_bact, c->nBox, 0);
-	if (c->nActive > 0)
-		k_box_count<<<nblk(c->nActive, 256), 256, 0, c->st>>>(c->m_list, c->nActive, c->m_box, c->d_bact);
-	SKG_HIP(hipGetLastError());
-	return 0;
-}
-
-// One Denmax step: gradient + move of the active movers, then ScatterCut per
-// box (each box's threshold from its own entities).
-static int denmax_step(skg_ctx *c, int micro, int bInitial) {
-	int rc;
-	const int stamp = ++c->step;
-	if ((rc = cover_launch(c, 1, c->m_pos, c->m_list, c->nActive, c->m_box, nullptr, stamp, micro,
-			24.0 * c->nActive + 24.0 * c->nLive))) return rc;
-	const int nb = nblk(c->nE, 256);
-	if (skg_ensure_part(c, (size_t)nb)) return SKG_E_NOMEM;
-	k_scat_thr<<<nb, 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->nE, stamp, bInitial, c->part);
-	k_box_min<<<c->nBox, 64, 0, c->st>>>(c->part, c->d_eblk, c->d_bthr);
-	k_scat_cut<<<nb, 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE, c->e_box, c->d_bthr, (int *)c->part);
-	k_box_sum<<<c->nBox, 64, 0, c->st>>>((const int *)c->part, c->d_eblk, c->d_bscat);
-	k_sum_final<<<1, 256, 0, c->st>>>((const int *)c->part, nb, c->d_cnt);
-	SKG_HIP(hipGetLastError());
-	return 0;
-}
-
-static int refit_entities(skg_ctx *c) {
-	float *b2 = nullptr;
-	if (dalloc(&b2, c->nE)) return SKG_E_NOMEM;
-	k_live_b2<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_aux, c->nE, b2);
-	int rc = skg_refit_tree(c, &c->te, c->e_pos, b2, c->e_box);
-	SKG_HIP(hipStreamSynchronize(c->st));
-	dfree(&b2);
-	return rc;
-}
-
-static int prune(skg_ctx *c) {
-	int rc, na = c->nActive;
-	if (na == 0) return 0;
-	int *flag = nullptr, *off = nullptr;
-	if (dalloc(&flag, na) || dalloc(&off, na)) return SKG_E_NOMEM;
-	const float *L = c->P.fPeriod;
-	const float hx = 0.5f * L[0], hy = 0.5f * L[1], hz = 0.5f * L[2];
-	k_prune<<<nblk(na, 256), 256, 0, c->st>>>(c->m_list, na, c->m_pos, c->m_old, hx, hy, hz, c->m_box, c->d_bt, flag);
-	if ((rc = skg_exclusive_scan(c, flag, off, na))) return rc;
-	int last[2];
-	SKG_HIP(hipMemcpyAsync(&last[0], off + na - 1, 4, hipMemcpyDeviceToHost, c->st));
-	SKG_HIP(hipMemcpyAsync(&last[1], flag + na - 1, 4, hipMemcpyDeviceToHost, c->st));
-	SKG_HIP(hipStreamSynchronize(c->st));
-	const int nn = last[0] + last[1];
-	k_compact<<<nblk(na, 256), 256, 0, c->st>>>(c->m_list, flag, off, na, c->m_list2);
-	if (nn > 0 && (rc = skg_morton_sort(c, c->m_pos, c->m_list2, nn, c->m_list, c->nBox > 1 ? c->m_box : nullptr)))
-		return rc;
-	SKG_HIP(hipStreamSynchronize(c->st));
-	dfree(&flag); dfree(&off);
-	c->nActive = nn;
-	return count_active(c);
-}
-
-static int read_count(skg_ctx *c, int *v) {
-	SKG_HIP(hipMemcpyAsync(v, c->d_cnt, 4, hipMemcpyDeviceToHost, c->st));
-	SKG_HIP(hipStreamSynchronize(c->st));
-	c->nLive = *v;
-	return 0;
 }
 
 extern "C" int skg_move_first(skg_ctx *c, int *nActive, int *nScatter) {
@@ -1612,9 +1805,9 @@ extern "C" int skg_move_first(skg_ctx *c, int *nActive, int *nScatter) {
 	skg_timer_start(c);
 	const int bInit = (c->inType == SKG_DARK) || c->P.bForceInitialCut;   // kd.c:696-699
 	if (c->nActive > 0 && c->nE > 0) {
-		if ((rc = denmax_step(c, 0, bInit))) return rc;
-		if ((rc = read_count(c, &ns))) return rc;
+		if ((rc = enqueue_steps(c, c->nActive, c->nActive <= c->subMax, false, 1, 0, bInit, false))) return rc;
 		if ((rc = refit_entities(c))) return rc;
+		if ((rc = read_counts(c, &ns))) return rc;
 	}
 	skg_timer_stop(c, SKG_T_MOVE);
 	if (nActive) *nActive = c->nActive;
@@ -1630,14 +1823,35 @@ extern "C" int skg_move_block(skg_ctx *c, int *nActive, int *nScatter) {
 	int rc, ns = c->nE;
 	skg_timer_start(c);
 	if (c->nActive > 0 && c->nE > 0) {
-		for (int i = 0; i < 5; ++i)
-			if ((rc = denmax_step(c, 0, 0))) return rc;
-		if ((rc = read_count(c, &ns))) return rc;
-		if ((rc = prune(c))) return rc;
-		if ((rc = refit_entities(c))) return rc;
+		const long na = c->nActive;
+		const bool sub = na <= c->subMax;
+		const bool lists = c->useLists && !sub && c->tl_list;
+		if (c->useGraph && !c->prof) {
+			if (!c->gexec || na * 2 < c->gcap || sub != (c->gsub != 0)) {
+				drop_graph(c);
+				hipGraph_t g = nullptr;
+				SKG_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+				int e1 = enqueue_steps(c, na, sub, lists, 5, 0, 0, true);
+				int e2 = e1 ? e1 : enqueue_prune(c, na);
+				hipError_t ee = hipStreamEndCapture(c->st, &g);
+				if (e1 || e2) { if (g) (void)hipGraphDestroy(g); return e1 ? e1 : e2; }
+				SKG_HIP(ee);
+				c->graph = g;
+				SKG_HIP(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
+				c->gcap = na; c->gsub = sub;
+			}
+			SKG_HIP(hipGraphLaunch(c->gexec, c->st));
+		} else {
+			if ((rc = enqueue_steps(c, na, sub, lists, 5, 0, 0, false))) return rc;
+			if ((rc = enqueue_prune(c, na))) return rc;
+		}
+		// nScatter of the Ittr line: the live count after the last step (the
+		// prune does not change it)
+		if ((rc = read_counts(c, &ns))) return rc;
 	} else {
 		c->nActive = 0;
 	}
+	if (c->nActive == 0) drop_graph(c);
 	skg_timer_stop(c, SKG_T_MOVE);
 	if (nActive) *nActive = c->nActive;
 	if (nScatter) *nScatter = ns;
@@ -1649,15 +1863,18 @@ extern "C" int skg_microsteps(skg_ctx *c, int *nScatter) {
 	if (c->stage != 5) return SKG_E_STATE;
 	(void)hipSetDevice(c->dev);
 	int rc, ns = c->nE;
+	drop_graph(c);
 	skg_timer_start(c);
 	c->nActive = c->nMove;                                  // kdReactivateMove (kd.c:796-799)
 	if (c->nMove > 0 && (rc = skg_morton_sort(c, c->m_pos, nullptr, c->nMove, c->m_list,
 			c->nBox > 1 ? c->m_box : nullptr))) return rc;
-	if ((rc = count_active(c))) return rc;
+	ml_set(c, 1, c->nMove);
+	if ((rc = count_active(c, c->nMove))) return rc;
 	if (c->nMove > 0 && c->nE > 0) {
-		for (int i = 0; i < 5; ++i)                          // MICRO_STEP*fStep (main.c:435)
-			if ((rc = denmax_step(c, 1, 0))) return rc;
-		if ((rc = read_count(c, &ns))) return rc;
+		const bool sub = c->nMove <= c->subMax;
+		// MICRO_STEP*fStep (main.c:435)
+		if ((rc = enqueue_steps(c, c->nMove, sub, c->useLists && !sub && c->tl_list, 5, 1, 0, false))) return rc;
+		if ((rc = read_counts(c, &ns))) return rc;
 	}
 	SKG_HIP(hipStreamSynchronize(c->st));
 	skg_timer_stop(c, SKG_T_MOVE);

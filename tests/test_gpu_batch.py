@@ -35,7 +35,11 @@ def _run_all(ctx):
     return b2, rho, tr, fg, g
 
 
-def test_batch_matches_each_box_alone():
+@pytest.mark.parametrize("sub_max", [None, "500"])
+def test_batch_matches_each_box_alone(sub_max, monkeypatch):
+    # sub_max 500: dense steps on the lane-per-mover kernel with leaf lists
+    if sub_max:
+        monkeypatch.setenv("SKG_COVER_SUB_MAX", sub_max)
     boxes, taus = _boxes()
     bc = skg.BatchContext(boxes, _params(), taus)
     b2, rho, trs, fg, g = _run_all(bc)

@@ -106,3 +106,34 @@ def test_two_species_catalogue():
     frac = O.match_fraction(g, og)
     print(f"two species: groups gpu {ng - 1} oracle {ong - 1} match {frac:.5f}")
     assert frac >= 0.999
+
+
+def test_move_loop_launch_paths_agree():
+    """The move loop gives the same trace and groups however its blocks are
+    launched: one hipGraph per block (default), direct launches, and with or
+    without per-tile leaf lists.  A low SKG_COVER_SUB_MAX makes the dense steps
+    use the lane-per-mover kernel (and the lists) and the sparse tail the
+    sub-wave kernel."""
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, '.');"
+        "from skid_amd import gen, skg;"
+        "b = gen.config1();"
+        "ctx = skg.Context(b, skg.make_params(0.005, 32, period=1.0));"
+        "tr, g, ng = ctx.run(); np.save(sys.argv[1], np.concatenate([tr.reshape(-1), g]))"
+    )
+    outs = []
+    base = {"SKG_COVER_SUB_MAX": "2000"}
+    for tag, env_add in (("graph", {}), ("direct", {"SKG_GRAPH": "0"}), ("nolists", {"SKG_TILE_LISTS": "0"}),
+                         ("direct_nolists", {"SKG_GRAPH": "0", "SKG_TILE_LISTS": "0"})):
+        env = dict(os.environ)
+        env.update(base)
+        env.update(env_add)
+        path = f"/tmp/skg_paths_{tag}.npy"
+        subprocess.run([sys.executable, "-c", code, path], check=True, env=env, timeout=240,
+                       cwd=os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+        outs.append(np.load(path))
+    for o in outs[1:]:
+        assert np.array_equal(outs[0], o)
