@@ -132,11 +132,11 @@ struct skg_ctx {
 	int *m_flag = nullptr, *m_bcnt = nullptr;   // prune flags; block counts then offsets
 	float *e_b2live = nullptr;
 	int *h_pin = nullptr;                        // pinned read-back
-	int useGraph = 1;                            // SKG_GRAPH=0: launch blocks directly
+	int useGraph = 0;                            // SKG_GRAPH=1: replay each block as a captured graph
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t gexec = nullptr;
 	long gcap = 0;
-	int gsub = 0;
+	int gsub = 0, gShrink = 2, nCaptures = 0;   // recapture when the count falls below gcap/gShrink
 };
 
 // ---------------------------------------------------------------- errors

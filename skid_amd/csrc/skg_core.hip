@@ -12,6 +12,7 @@
 #include <cfloat>
 #include <cstring>
 #include <vector>
+#include <chrono>
 
 static const unsigned long long KEY_SENT = 0x7F800000FFFFFFFFull;   // d2 = +inf, index = max
 
@@ -1278,6 +1279,7 @@ extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
 	if (const char *e = getenv("SKG_COVER_SUB_MAX")) c->subMax = atol(e);
 	if (const char *e = getenv("SKG_TILE_LISTS")) c->useLists = atoi(e);
 	if (const char *e = getenv("SKG_GRAPH")) c->useGraph = atoi(e);
+	if (const char *e = getenv("SKG_GRAPH_SHRINK")) c->gShrink = std::max(1, atoi(e));
 	if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
 	    hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
 		delete c;
@@ -1649,10 +1651,12 @@ extern "C" int skg_neighbors(skg_ctx *c, int *nbr, float *d2) {
 // Move loop.  Every step reads the active count and step stamp from device
 // memory (ml), and a block of steps (lists, 5 x [gradient+move, ScatterCut],
 // prune, compaction, refit) launches with grids sized for a capacity, so the
-// whole block can be captured once as a hipGraph and replayed while the
-// active set shrinks; it is recaptured when the count halves (fewer idle
-// waves) or crosses the sparse-kernel size.  Without graphs (SKG_GRAPH=0, or
-// kernel profiling on) the same launches run directly.
+// whole block can be captured once as a hipGraph (SKG_GRAPH=1) and replayed
+// while the active set shrinks; it is recaptured when the count falls below
+// gcap/gShrink (fewer idle waves) or crosses the sparse-kernel size.  By
+// default, and with kernel profiling on, the same launches run directly: the
+// launches here are large, the host waits once per block either way, and the
+// capacity-sized grids of a replayed graph cost more than they save.
 
 static void ml_set(skg_ctx *c, int i, int v) { k_ml_set<<<1, 1, 0, c->st>>>(c->d_ml, i, v); }
 
@@ -1827,7 +1831,8 @@ extern "C" int skg_move_block(skg_ctx *c, int *nActive, int *nScatter) {
 		const bool sub = na <= c->subMax;
 		const bool lists = c->useLists && !sub && c->tl_list;
 		if (c->useGraph && !c->prof) {
-			if (!c->gexec || na * 2 < c->gcap || sub != (c->gsub != 0)) {
+			if (!c->gexec || na * c->gShrink < c->gcap || sub != (c->gsub != 0)) {
+				const auto t0 = std::chrono::steady_clock::now();
 				drop_graph(c);
 				hipGraph_t g = nullptr;
 				SKG_HIP(hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
@@ -1839,6 +1844,8 @@ extern "C" int skg_move_block(skg_ctx *c, int *nActive, int *nScatter) {
 				c->graph = g;
 				SKG_HIP(hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0));
 				c->gcap = na; c->gsub = sub;
+				c->tms[SKG_T_N - 1] += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+				c->nCaptures++;
 			}
 			SKG_HIP(hipGraphLaunch(c->gexec, c->st));
 		} else {

@@ -110,7 +110,7 @@ def test_two_species_catalogue():
 
 def test_move_loop_launch_paths_agree():
     """The move loop gives the same trace and groups however its blocks are
-    launched: one hipGraph per block (default), direct launches, and with or
+    launched: direct launches (default), one hipGraph per block, and with or
     without per-tile leaf lists.  A low SKG_COVER_SUB_MAX makes the dense steps
     use the lane-per-mover kernel (and the lists) and the sparse tail the
     sub-wave kernel."""
@@ -126,8 +126,8 @@ def test_move_loop_launch_paths_agree():
     )
     outs = []
     base = {"SKG_COVER_SUB_MAX": "2000"}
-    for tag, env_add in (("graph", {}), ("direct", {"SKG_GRAPH": "0"}), ("nolists", {"SKG_TILE_LISTS": "0"}),
-                         ("direct_nolists", {"SKG_GRAPH": "0", "SKG_TILE_LISTS": "0"})):
+    for tag, env_add in (("direct", {}), ("graph", {"SKG_GRAPH": "1"}), ("nolists", {"SKG_TILE_LISTS": "0"}),
+                         ("graph_nolists", {"SKG_GRAPH": "1", "SKG_TILE_LISTS": "0"})):
         env = dict(os.environ)
         env.update(base)
         env.update(env_add)
