@@ -163,6 +163,13 @@ int skg_get_catalogue(skg_ctx *ctx, float *mass, float *center, float *vcm,
    parts selects the gather (1), scatter (2) or both (3) halves of rho. */
 int skg_density_adjoint(skg_ctx *ctx, const float *gbar, int parts, float *dx, float *dh);
 
+/* Sensitivity of each density to nSmooth (no reference counterpart; the
+   derivative of the density of smooth1.c:249-263 along its smoothing lengths):
+   with the neighbour sets frozen and every h_j = sqrt(fBall2_j)/2 scaling as
+   nSmooth^(1/3), drho[i] = d rho_i / d ln nSmooth (by iOrder; 0 for
+   non-scatterers).  Forward product of d rho/dh along dh_j = h_j/3. */
+int skg_density_dlnk(skg_ctx *ctx, float *drho);
+
 /* Frozen-field gradient a(x) = sum_s (r_s - x) m_s g(q)/(pi h_s^5) at nq
    arbitrary points (the quantity smAccDensity accumulates, smooth1.c:447-459),
    over all scatterers and replicas of the last skg_density. */
@@ -172,6 +179,27 @@ int skg_field_gradient(skg_ctx *ctx, int nq, const float *qpos, float *acc);
    positions), labels numbered by smallest index.  Standalone: needs only a
    context for the device and stream. */
 int skg_fof_points(skg_ctx *ctx, int n, const float *pos, float fTau, int *grp, int *nGroup);
+
+/* Autotuner (no reference counterpart): times launch shapes on the loaded
+   particles and keeps the fastest for this context.  what is a mask of
+     SKG_TUNE_BUCKET    leaf size nBucket in {8, 16, 32, 64} (skg_density plus
+                        the first 6 move steps per candidate);
+     SKG_TUNE_MAPPING   the mover-to-lane mapping: for active sets of 4^k * 4096
+                        strided movers, one lane per mover (64 per wavefront)
+                        against 16 lanes per mover; the largest size at which
+                        the latter wins becomes the switch point;
+     SKG_TUNE_POT_TILE  the unbinding potential's workgroup / LDS tile in
+                        {64, 128, 256}, timed on the groups of the next
+                        skg_unbind.
+   Needs loaded particles; leaves the context loaded (stage of skg_load), so
+   the caller runs the path again from skg_density.  Outputs (any may be
+   NULL): the chosen nBucket, switch size and tile (the tile as chosen so
+   far; the next skg_unbind updates it). */
+#define SKG_TUNE_BUCKET   1
+#define SKG_TUNE_MAPPING  2
+#define SKG_TUNE_POT_TILE 4
+#define SKG_TUNE_ALL      7
+int skg_autotune(skg_ctx *ctx, int what, int *nBucket, long long *subMax, int *potTile);
 
 /* Per-stage device time in ms since skg_create, indexed by SKG_T_*. */
 #define SKG_T_TREE 0

@@ -78,9 +78,42 @@ __global__ __launch_bounds__(256) void k_adj_gather(const unsigned long long *__
 	dh[oi] = ahh.val();
 }
 
+// Sensitivity of the density to nSmooth with the neighbour sets frozen: all
+// smoothing lengths scale as nSmooth^(1/3), so h dW/dh = -(3w + q w')/(pi h^3)
+// gives d rho_i / d ln k = -(1/6) [sum_{j in N_i} m_j (3w + q w')(q_ij/h_i)/(pi h_i^3)
+//                                 + sum_{j : i in N_j} m_j (3w + q w')(q_ij/h_j)/(pi h_j^3)].
+// Gather half here (per query over its kNN list, self at q = 0 included);
+// the scatter half is k_adj_cover MODE 2.
+__global__ __launch_bounds__(256) void k_dlnk_gather(const unsigned long long *__restrict__ heap, int k,
+		const int *__restrict__ sord, const float *__restrict__ sb2, int nS, const float *__restrict__ pos,
+		const float *__restrict__ mass, float L, int per, float *out) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= nS) return;
+	const int oi = sord[i];
+	const float xi = pos[3 * oi], yi = pos[3 * oi + 1], zi = pos[3 * oi + 2];
+	const float h = 0.5f * sqrtf(sb2[i]), ih = 1.0f / h;
+	Acc a;
+	a.init();
+	for (int s = 1; s < k; ++s) {
+		const int j = (int)(unsigned)heap[(size_t)i * k + s];
+		float r = 0.f;
+		if (j != oi) {
+			const float dx = per_delta(xi, pos[3 * j], L, per), dy = per_delta(yi, pos[3 * j + 1], L, per);
+			const float dz = per_delta(zi, pos[3 * j + 2], L, per);
+			r = sqrtf(sq3(dx, dy, dz));
+		}
+		const float q = r * ih;
+		float w, wp;
+		m4_derivs(q, w, wp);
+		a.add(mass[j] * (3.0f * w + q * wp));
+	}
+	out[oi] = a.val() * (float)(-M_1_PI / 6.0) * ih * ih * ih;
+}
+
 // Pass 2 (and the field gradient): per point, every entity (scatterer or
 // periodic replica) whose ball strictly holds it, over the entity tree with
-// the radii as built.  MODE 0: dx_j -= c_ij r_hat; MODE 1: a(x).  pbox
+// the radii as built.  MODE 0: dx_j -= c_ij r_hat; MODE 1: a(x); MODE 2 the
+// scatter half of d rho / d ln k (out[ii], one value).  pbox
 // (batched boxes) restricts each point to its own box's entities.
 template <int MODE>
 __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restrict__ pts, int np,
@@ -126,6 +159,11 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restri
 				if (MODE == 1) {
 					const float f = grad_factor(sq3(dx, dy, dz), X.y, X.z);
 					s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
+				} else if (MODE == 2) {
+					const float ih = 2.0f / sqrtf(eb2[e]), q = sqrtf(sq3(dx, dy, dz)) * ih;
+					float w, wp;
+					m4_derivs(q, w, wp);
+					s0.add(A.w * (3.0f * w + q * wp) * ih * ih * ih);
 				} else {
 					const float r = sqrtf(sq3(dx, dy, dz));
 					if (!(r > 0.f)) continue;
@@ -146,6 +184,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restri
 		if (node == 1) break;
 	}
 	if (!valid) return;
+	if (MODE == 2) { out[ii] = s0.val() * (float)(-M_1_PI / 6.0); return; }
 	out[3 * ii] = s0.val(); out[3 * ii + 1] = s1.val(); out[3 * ii + 2] = s2.val();
 }
 
@@ -162,6 +201,10 @@ __global__ void k_unpermute3(const float *__restrict__ in, const int *__restrict
 	if (i >= n) return;
 	const int o = perm[i];
 	out[3 * o] = in[3 * i]; out[3 * o + 1] = in[3 * i + 1]; out[3 * o + 2] = in[3 * i + 2];
+}
+__global__ void k_add_ord(const float *__restrict__ in, const int *__restrict__ ord, int n, float *out) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) out[ord[i]] = fadd(out[ord[i]], in[i]);
 }
 __global__ void k_add3_ord(const float *__restrict__ in, const int *__restrict__ ord, int n, float *dx) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -233,5 +276,27 @@ extern "C" int skg_density_adjoint(skg_ctx *c, const float *gbar, int parts, flo
 	SKG_HIP(hipMemcpyAsync(dh, ddh, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
 	dfree(&dg); dfree(&ddx); dfree(&ddh); dfree(&part2);
+	return SKG_OK;
+}
+
+extern "C" int skg_density_dlnk(skg_ctx *c, float *drho) {
+	if (!c || !drho) return SKG_E_PARAM;
+	if (c->stage < 2 || !c->s_heap || c->nS == 0) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	int rc;
+	const int n = c->n, nS = c->nS;
+	float *d = nullptr, *part = nullptr;
+	if (dalloc(&d, n) || dalloc(&part, nS)) return SKG_E_NOMEM;
+	SKG_HIP(hipMemsetAsync(d, 0, 4 * (size_t)n, c->st));
+	const bool per = c->P.bPeriodic != 0;
+	k_dlnk_gather<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_heap, c->P.nSmooth, c->s_ord, c->s_ball2, nS, c->d_pos,
+			c->d_mass, per ? c->P.fPeriod[0] : 0.f, per, d);
+	if ((rc = full_tree(c))) return rc;
+	launch_cover<2>(c, c->s_pos, nS, c->s_ord, nullptr, 0, c->s_box, part);
+	k_add_ord<<<nblk(nS, 256), 256, 0, c->st>>>(part, c->s_ord, nS, d);
+	SKG_HIP(hipGetLastError());
+	SKG_HIP(hipMemcpyAsync(drho, d, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&d); dfree(&part);
 	return SKG_OK;
 }

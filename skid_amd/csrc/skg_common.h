@@ -137,6 +137,9 @@ struct skg_ctx {
 	hipGraphExec_t gexec = nullptr;
 	long gcap = 0;
 	int gsub = 0, gShrink = 2, nCaptures = 0;   // recapture when the count falls below gcap/gShrink
+	// unbinding potential: workgroup / LDS tile size; tunePot: time 64/128/256
+	// on the next skg_unbind's groups and keep the fastest
+	int potTile = 256, tunePot = 0;
 };
 
 // ---------------------------------------------------------------- errors

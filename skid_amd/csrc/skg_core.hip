@@ -1929,3 +1929,84 @@ extern "C" int skg_stage_times(skg_ctx *c, double *ms) {
 	for (int i = 0; i < SKG_T_N; ++i) ms[i] = c->tms[i];
 	return SKG_OK;
 }
+
+// =====================================================================
+// Autotuner: time launch shapes on the loaded particles and keep the fastest.
+// =====================================================================
+__global__ void k_stride_list(const int *__restrict__ full, int n, int stride, int m, int *out) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < m) out[i] = full[(int)min((long)n - 1, (long)i * stride)];
+}
+
+static float elapsed_ms(skg_ctx *c) {
+	float ms = 0.f;
+	(void)hipEventRecord(c->e1, c->st);
+	(void)hipEventSynchronize(c->e1);
+	(void)hipEventElapsedTime(&ms, c->e0, c->e1);
+	return ms;
+}
+
+extern "C" int skg_autotune(skg_ctx *c, int what, int *nBucket, long long *subMax, int *potTile) {
+	if (!c) return SKG_E_PARAM;
+	if (c->stage < 1) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	int rc, na, ns;
+	const int prof = c->prof;
+	c->prof = 0;
+	// 1. leaf size: density + the first 6 move steps (all movers active)
+	if (what & SKG_TUNE_BUCKET) {
+		float best = FLT_MAX;
+		int bestB = c->P.nBucket;
+		for (int B : {8, 16, 32, 64}) {
+			c->P.nBucket = B;
+			(void)hipEventRecord(c->e0, c->st);
+			if ((rc = skg_density(c, nullptr, nullptr, nullptr))) return rc;
+			const float td = elapsed_ms(c);
+			if ((rc = skg_init_move(c, nullptr))) return rc;
+			(void)hipEventRecord(c->e0, c->st);
+			if ((rc = skg_move_first(c, &na, &ns))) return rc;
+			if (na > 0 && (rc = skg_move_block(c, &na, &ns))) return rc;
+			const float tm = elapsed_ms(c);
+			// the move loop runs tens of blocks: weight its dense steps
+			const float cost = td + 8.f * tm;
+			if (cost < best) { best = cost; bestB = B; }
+		}
+		c->P.nBucket = bestB;
+	}
+	// 2. mover mapping: per active-set size, one lane per mover (64 per
+	// wavefront) against 16 lanes per mover, on strided (scattered) subsets
+	if (what & SKG_TUNE_MAPPING) {
+		if ((rc = skg_density(c, nullptr, nullptr, nullptr))) return rc;
+		if ((rc = skg_init_move(c, nullptr))) return rc;
+		const int nm = c->nMove;
+		long cross = 0;
+		int *full = nullptr;
+		if (nm > 0 && dalloc(&full, nm)) return SKG_E_NOMEM;
+		if (nm > 0) SKG_HIP(hipMemcpyAsync(full, c->m_list, 4 * (size_t)nm, hipMemcpyDeviceToDevice, c->st));
+		for (long m = 1 << 12; m <= nm / 2; m <<= 2) {
+			const int stride = (int)(nm / m);
+			k_stride_list<<<nblk(m, 256), 256, 0, c->st>>>(full, nm, stride, (int)m, c->m_list);
+			ml_set(c, 1, (int)m);
+			c->nActive = (int)m;
+			float t[2];
+			for (int sub = 0; sub < 2; ++sub) {
+				(void)hipEventRecord(c->e0, c->st);
+				if ((rc = enqueue_steps(c, m, sub != 0, false, 1, 0, 0, false))) return rc;
+				t[sub] = elapsed_ms(c);
+			}
+			if (t[1] < t[0]) cross = m;
+		}
+		dfree(&full);
+		c->subMax = cross > 0 ? cross : 0;
+	}
+	// 3. unbinding potential tile: timed on the groups of the next skg_unbind
+	if (what & SKG_TUNE_POT_TILE) c->tunePot = 1;
+	c->prof = prof;
+	// the caller reruns the path from skg_density with the chosen shapes
+	for (int i = 0; i < SKG_T_N; ++i) c->tms[i] = 0;
+	c->stage = 1;
+	if (nBucket) *nBucket = c->P.nBucket;
+	if (subMax) *subMax = c->subMax;
+	if (potTile) *potTile = c->potTile;
+	return SKG_OK;
+}

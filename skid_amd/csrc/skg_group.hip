@@ -191,10 +191,11 @@ __global__ void k_unb_setup(const int *__restrict__ gs, const int *__restrict__ 
 }
 
 // phi_i += sum_{j != i} G m_j K(r_ij; eps_i + eps_j) over the member's group
-// (kdCellPot, grav.c:8-36): a workgroup holds 256 targets in registers and
-// streams the group's members through LDS in tiles of 256 (both orders of each
-// pair are evaluated; no atomics, fixed summation order).
-#define POT_T 256
+// (kdCellPot, grav.c:8-36): a workgroup holds POT_T targets in registers and
+// streams the group's members through LDS in tiles of POT_T (both orders of
+// each pair are evaluated; no atomics, fixed summation order).  POT_T (64,
+// 128 or 256) is chosen by the autotuner.
+template <int POT_T>
 __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tiles, const int *__restrict__ gs,
 		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv,
 		float2 *uphi, float G, int soft) {
@@ -1093,8 +1094,24 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		if (m <= 0 || m >= nMax) continue;
 		if (m <= SMALL) { gsmall.push_back(g); continue; }
 		(m < BIG ? gmed : gbig).push_back(g);
-		for (int t = hs[g]; t < he[g]; t += POT_T) tiles.push_back(make_int2(g, t));
 	}
+	// potential tiles (group, first target) for a workgroup of T targets
+	auto make_tiles = [&](int T, std::vector<int2> &v) {
+		v.clear();
+		for (const std::vector<int> *L : {&gmed, &gbig})
+			for (int g : *L)
+				for (int t = hs[g]; t < he[g]; t += T) v.push_back(make_int2(g, t));
+	};
+	auto launch_pot = [&](int T, const int2 *dtl, int ntl) {
+		if (ntl <= 0) return;
+		if (T == 64)
+			k_unb_pot<64><<<ntl, 64, 0, c->st>>>(dtl, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
+		else if (T == 128)
+			k_unb_pot<128><<<ntl, 128, 0, c->st>>>(dtl, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
+		else
+			k_unb_pot<256><<<ntl, 256, 0, c->st>>>(dtl, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
+	};
+	make_tiles(c->tunePot ? 64 : c->potTile, tiles);   // the most tiles any candidate needs
 	if (gbig.size() > 128) {   // keep the 128 largest cooperative; the rest one workgroup each
 		std::sort(gbig.begin(), gbig.end(), [&](int a, int b) { return he[a] - hs[a] > he[b] - hs[b]; });
 		gmed.insert(gmed.end(), gbig.begin() + 128, gbig.end());
@@ -1114,6 +1131,29 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		if (!tiles.empty()) SKG_HIP(hipMemcpyAsync(dt, tiles.data(), 8 * tiles.size(), hipMemcpyHostToDevice, c->st));
 		if (!gsmall.empty()) SKG_HIP(hipMemcpyAsync(dgs, gsmall.data(), 4 * gsmall.size(), hipMemcpyHostToDevice, c->st));
 		if (!gmed.empty()) SKG_HIP(hipMemcpyAsync(dgm, gmed.data(), 4 * gmed.size(), hipMemcpyHostToDevice, c->st));
+		if (c->tunePot && !tiles.empty()) {
+			// autotuner: time each LDS tile on these groups (setup resets the potentials)
+			float best = FLT_MAX;
+			for (int T : {64, 128, 256}) {
+				std::vector<int2> tt;
+				make_tiles(T, tt);
+				SKG_HIP(hipMemcpyAsync(dt, tt.data(), 8 * tt.size(), hipMemcpyHostToDevice, c->st));
+				k_unb_setup<<<nblk(nTot, 256), 256, 0, c->st>>>(G->gstart, G->gend, nG, G->mem, c->d_pos, c->d_vel,
+						c->d_mass, c->d_soft, G->rel, hx, hy, hz, nMax, G->uq, G->uv, G->uphi, G->uord, nTot);
+				SKG_HIP(hipEventRecord(c->e0, c->st));
+				launch_pot(T, dt, (int)tt.size());
+				SKG_HIP(hipEventRecord(c->e1, c->st));
+				SKG_HIP(hipEventSynchronize(c->e1));
+				float ms = 0.f;
+				SKG_HIP(hipEventElapsedTime(&ms, c->e0, c->e1));
+				if (ms < best) { best = ms; c->potTile = T; }
+			}
+			c->tunePot = 0;
+			make_tiles(c->potTile, tiles);
+			SKG_HIP(hipMemcpyAsync(dt, tiles.data(), 8 * tiles.size(), hipMemcpyHostToDevice, c->st));
+		} else if (c->tunePot) {
+			c->tunePot = 0;
+		}
 		k_unb_setup<<<nblk(nTot, 256), 256, 0, c->st>>>(G->gstart, G->gend, nG, G->mem, c->d_pos, c->d_vel,
 				c->d_mass, c->d_soft, G->rel, hx, hy, hz, nMax, G->uq, G->uv, G->uphi, G->uord, nTot);
 		// scoop potential from group 0 (always applied by the reference: kd.c:1379-1384)
@@ -1143,9 +1183,7 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		for (int gg : gmed) pairs += (double)(he[gg] - hs[gg]) * (he[gg] - hs[gg] - 1);
 		for (int gg : gbig) pairs += (double)(he[gg] - hs[gg]) * (he[gg] - hs[gg] - 1);
 		const int pslot = skg_prof_begin(c);
-		if (!tiles.empty())
-			k_unb_pot<<<(int)tiles.size(), POT_T, 0, c->st>>>(dt, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G,
-					c->P.iSoftType);
+		launch_pot(c->potTile, dt, (int)tiles.size());
 		skg_prof_end(c, pslot, SKG_K_UNB_POT, 20.0 * pairs);
 		const int bSub = (c->inType == SKG_DARK || c->inType == SKG_STAR);   // kd.c:1441
 		const int lslot = skg_prof_begin(c);

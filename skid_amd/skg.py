@@ -19,6 +19,7 @@ SKG_OK = 0
 SKG_DARK, SKG_GAS, SKG_STAR = 1, 2, 4
 SKG_PLUMMER, SKG_SPLINE = 1, 2
 SKG_ACC_PLAIN, SKG_ACC_COMPENSATED = 0, 1
+SKG_TUNE_BUCKET, SKG_TUNE_MAPPING, SKG_TUNE_POT_TILE, SKG_TUNE_ALL = 1, 2, 4, 7
 STAGES = ("tree", "knn", "density", "move", "fof", "group", "unbind", "spare")
 KERNELS = ("knn", "cover_density", "cover_move", "fof", "unbind_potential", "unbind_loop", "unbind_big")
 FLT_MAX = 3.4028234663852886e38
@@ -75,6 +76,8 @@ def lib():
         L.skg_get_catalogue.argtypes = [vp, fp, fp, fp, fp, fp, fp]
         L.skg_density_adjoint.argtypes = [vp, fp, C.c_int, fp, fp]
         L.skg_field_gradient.argtypes = [vp, C.c_int, fp, fp]
+        L.skg_density_dlnk.argtypes = [vp, fp]
+        L.skg_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
         L.skg_fof_points.argtypes = [vp, C.c_int, fp, C.c_float, fp, ip]
         L.skg_stage_times.argtypes = [vp, C.POINTER(C.c_double)]
         L.skg_set_profiling.argtypes = [vp, C.c_int]
@@ -229,6 +232,12 @@ class Context:
         _check(self.L.skg_density_adjoint(self.h, _p(g), parts, _p(dx), _p(dh)), "skg_density_adjoint")
         return dx.reshape(self.n, 3), dh
 
+    def density_dlnk(self):
+        """d rho / d ln nSmooth per particle (frozen neighbour sets)."""
+        d = np.empty(self.n, np.float32)
+        _check(self.L.skg_density_dlnk(self.h, _p(d)), "skg_density_dlnk")
+        return d
+
     def field_gradient(self, q):
         q = np.ascontiguousarray(q, np.float32).reshape(-1)
         a = np.empty_like(q)
@@ -241,6 +250,13 @@ class Context:
         ng = C.c_int()
         _check(self.L.skg_fof_points(self.h, p.size // 3, _p(p), tau, _p(g), C.byref(ng)), "skg_fof_points")
         return g, ng.value
+
+    def autotune(self, what: int = SKG_TUNE_ALL) -> dict:
+        """Time launch shapes on the loaded particles (skg_autotune); the path
+        must then be run again from density()."""
+        b, m, t = C.c_int(), C.c_longlong(), C.c_int()
+        _check(self.L.skg_autotune(self.h, what, C.byref(b), C.byref(m), C.byref(t)), "skg_autotune")
+        return {"nBucket": b.value, "sub_max": m.value, "pot_tile": t.value}
 
     def stage_times(self) -> dict:
         t = (C.c_double * 8)()
@@ -269,6 +285,40 @@ class Context:
         self.too_small()
         g, ng = self.groups()
         return tr, g, ng
+
+
+def nsmooth_report(rho, drho, groups, dens_min=0.0, rel_k=0.25):
+    """Membership sensitivity to nSmooth from d rho / d ln nSmooth.
+
+    A particle's density moves by about drho * ln(1 + rel_k) when nSmooth
+    changes by the factor 1 + rel_k.  Particles whose density would cross the
+    mover threshold dens_min (-d, main.c:114-115) in either direction change
+    the moving set, and with it the groups they can join.  Returns the
+    distribution of d ln rho / d ln nSmooth over scatterers and, per group
+    (id > 0), its size, mean sensitivity and the fraction of members at risk.
+    """
+    rho = np.asarray(rho, np.float64)
+    drho = np.asarray(drho, np.float64)
+    groups = np.asarray(groups)
+    sc = rho > 0
+    s = np.zeros_like(rho)
+    s[sc] = drho[sc] / rho[sc]
+    dl = abs(np.log1p(rel_k)), -abs(np.log1p(-rel_k)) if rel_k < 1 else -np.inf
+    up = rho + drho * dl[0]
+    dn = rho + drho * dl[1] if np.isfinite(dl[1]) else rho
+    lo, hi = np.minimum(up, dn), np.maximum(up, dn)
+    at_risk = sc & (lo < dens_min) & (hi >= dens_min)
+    out = {"scatterers": int(sc.sum()), "dlnrho_dlnk": {
+        q: float(np.percentile(s[sc], p)) for q, p in (("p5", 5), ("median", 50), ("p95", 95))} if sc.any() else {},
+        "at_risk": int(at_risk.sum()), "rel_k": rel_k, "dens_min": dens_min, "groups": {}}
+    ids = groups[groups > 0]
+    if ids.size:
+        n = np.bincount(groups[groups > 0])
+        ms = np.bincount(groups[groups > 0], weights=s[groups > 0])
+        ar = np.bincount(groups[groups > 0], weights=at_risk[groups > 0].astype(np.float64))
+        for g in np.flatnonzero(n):
+            out["groups"][int(g)] = (int(n[g]), float(ms[g] / n[g]), float(ar[g] / n[g]))
+    return out
 
 
 class BatchContext(Context):

@@ -115,3 +115,42 @@ def test_adjoint_2pow20_gradient_parity():
     g = np.ones(b.n, np.float32)
     dx, dh = ctx.adjoint(g, 3)
     assert np.isfinite(dx).all() and np.isfinite(dh).all()
+
+
+def frozen_rho(x, h, mass, nbr):
+    """rho_i with the kNN sets frozen (head excluded), FP64: both halves."""
+    n, k = nbr.shape
+    i = np.repeat(np.arange(n), k)
+    j = nbr.reshape(-1)
+    d = x[i] - x[j]
+    d -= np.round(d)
+    r = np.sqrt((d * d).sum(1))
+    W = 0.5 * m4(r / h[i]) / (math.pi * h[i] ** 3)
+    return np.bincount(i, weights=W * mass[j], minlength=n) + np.bincount(j, weights=W * mass[i], minlength=n)
+
+
+def test_density_dlnk_central_differences():
+    """d rho / d ln nSmooth against central differences of the FP64 density
+    with frozen sets and every h scaled by exp(eps/3)."""
+    b = gen.config1(n=4096, seed=17)
+    k = 32
+    ctx = skg.Context(b, skg.make_params(0.01, k, period=1.0))
+    b2, rho, _ = ctx.density()
+    d = ctx.density_dlnk()
+    _, nb = O.knn_brute(b.pos, k, lists=True)
+    nbr = nb[:, :-1]
+    x = b.pos.astype(np.float64)
+    h = 0.5 * np.sqrt(b2.astype(np.float64))
+    m = b.mass.astype(np.float64)
+    r0 = frozen_rho(x, h, m, nbr)
+    assert np.allclose(r0, rho, rtol=1e-5)
+    eps = 1e-4
+    fd = (frozen_rho(x, h * math.exp(eps / 3), m, nbr) - frozen_rho(x, h * math.exp(-eps / 3), m, nbr)) / (2 * eps)
+    err = np.abs(fd - d) / r0
+    print(f"d rho/d ln k: max |FD - GPU|/rho = {err.max():.2e}; d ln rho/d ln k median {np.median(d / r0):.3f}")
+    assert err.max() <= 1e-5
+    # densities fall as the smoothing volume grows: the sensitivity is
+    # negative for most particles
+    assert np.median(d / r0) < 0
+    rep = skg.nsmooth_report(rho, d, np.zeros(b.n, np.int32), dens_min=float(np.median(rho)), rel_k=0.25)
+    assert rep["scatterers"] == b.n and rep["at_risk"] > 0

@@ -137,3 +137,24 @@ def test_move_loop_launch_paths_agree():
         outs.append(np.load(path))
     for o in outs[1:]:
         assert np.array_equal(outs[0], o)
+
+
+def test_autotune_keeps_results(box1):
+    """The autotuner picks valid shapes, and the path run with them gives the
+    catalogue of the default shapes (kNN radii exactly; groups >= 99.9%)."""
+    P = skg.make_params(0.005, 32, period=1.0)
+    ref = skg.Context(box1, P)
+    rb2, _, _ = ref.density()
+    rtr, rg, rng = ref.run()
+    ctx = skg.Context(box1, P)
+    t = ctx.autotune(skg.SKG_TUNE_ALL)
+    print("tuned", t)
+    assert t["nBucket"] in (8, 16, 32, 64) and t["sub_max"] >= 0
+    b2, _, _ = ctx.density()
+    assert np.array_equal(b2, rb2)
+    tr, g, ng = ctx.run()
+    t2 = ctx.autotune(0)
+    assert t2["pot_tile"] in (64, 128, 256)
+    frac = O.match_fraction(g, rg)
+    print(f"tuned {t2}: groups {ng} vs {rng}, match {frac:.5f}")
+    assert frac >= 0.999
