@@ -5,11 +5,24 @@ resident in HBM: scatterer tree + kNN density, the Denmax move loop to
 convergence, FOF, micro-steps, group centres, unbinding and kdTooSmall,
 ending with the group ids in host memory (main.c:374-471).
 
-Workload: the two-species box (2^24 gas + 2^24 dark particles, 8000 NFW halos
-with dN/dM ~ M^-1.9 on [1e-7, 1e-3], gas in cored isothermal profiles),
-nSmooth 64, nBucket 16, tau 0.001, -gd, periodic unit box.  The CPU baseline
-is the reference SKID (oracle/_ref) timed on this host's cores on a bounded
-sample of the same recipe (see --cpu-sample).
+Default workload: 256 independent periodic unit boxes with particle counts
+log-uniform in [2048, 524288] (seeded draw, 29.1M particles in all), each
+NFW halos (N/4000 of them) plus a 25% uniform background, nSmooth 32,
+nBucket as tuned, tau = 0.16 N_box^(-1/3) per box.  Three forms are timed:
+
+  value        batched + adjoint + compensated: every box in the same launches
+               (skg_load_batch), fp32 two-sum sums, and each step also forms
+               the nSmooth sensitivity of every density (d rho / d ln k, the
+               dh adjoint direction) and copies it to the host; launch shapes
+               from the autotuner.  The membership report built from it (host
+               post-processing) is formed once after the timed steps;
+  plain_gpu    one context per box run in turn, fp32 running sums (the
+               reference's arithmetic), no adjoint, default shapes; timed on
+               every 4th box in size order (a rate, particles/s);
+  cpu_baseline the reference SKID (oracle/_ref) on one core over four boxes
+               spread across the size range.
+
+--workload two-species runs the 2^24 gas + 2^24 dark box instead (one box).
 """
 from __future__ import annotations
 
@@ -30,29 +43,189 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(n_each: int, tau: float):
-    """The reference (or, without it, the oracle's restatement) on one core."""
-    from oracle import oracle as O
-    from skid_amd import gen
-    box = gen.config3(n_each=n_each, seed=3, n_halos=max(1, int(8000 * n_each / (1 << 24))))
-    kw = dict(nSmooth=64, fTau=tau, fPeriod=1.0, bGasAndDark=1)
+def pin_one_core():
     try:
         os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
     except Exception:
         pass
+
+
+def cpu_run(boxes, kws):
+    """The reference (or, without it, the oracle's restatement) on one core."""
+    from oracle import oracle as O
+    pin_one_core()
     t0 = time.perf_counter()
-    if O.have_ref():
-        O.ref_pipeline(box, **kw)
-        kind = "reference"
-    else:
-        o = O.Oracle(box, **kw)
-        o.run()
-        kind = "port"
-    dt = time.perf_counter() - t0
-    return dict(value=box.n / dt, unit="particles/s", cores=1, kind=kind,
-                sample=f"same recipe at {n_each}+{n_each} particles ({box.n}), tau scaled by N^-1/3 "
-                       f"to {tau:.4g}, nSmooth 64, -gd; end-to-end {dt:.2f}s on 1 core "
-                       f"(per-particle cost grows with N, so this overstates the CPU rate at 2^25)")
+    kind = "reference" if O.have_ref() else "port"
+    for box, kw in zip(boxes, kws):
+        if kind == "reference":
+            O.ref_pipeline(box, **kw)
+        else:
+            O.Oracle(box, **kw).run()
+    return time.perf_counter() - t0, kind
+
+
+def roofline(kst):
+    """Roofline of the kernel class with the most device time (HIP events on
+    the library's stream around every launch of the timed steps)."""
+    dom = max(kst, key=lambda k: kst[k]["ms"])
+    d = kst[dom]
+    avg_ms = d["ms"] / max(1, d["launches"])
+    if dom == "unbind_potential":
+        ach = d["work"] / max(1, d["launches"]) / (avg_ms * 1e-3) / 1e12
+        return dict(bound="valu", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=ach / FP32_PEAK_TFLOPS,
+                    traffic=None, kernel=dom, avg_launch_ms=avg_ms, launches=d["launches"])
+    ach = d["work"] / max(1, d["launches"]) / (avg_ms * 1e-3) / 1e9
+    return dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
+                traffic=None, kernel=dom, avg_launch_ms=avg_ms, launches=d["launches"])
+
+
+def run_path(ctx, adjoint=False):
+    """main.c:374-471 on a context; with adjoint, also d rho / d ln nSmooth.
+    Returns (trace, nGroup, (rho, drho, groups))."""
+    b2, rho, _ = ctx.density()
+    drho = ctx.density_dlnk() if adjoint else None
+    tr = ctx.move_loop()
+    ctx.fof()
+    ctx.microsteps()
+    ctx.centers()
+    ctx.unbind()
+    ctx.too_small()
+    g, ng = ctx.groups()
+    return tr, ng, (rho, drho, g)
+
+
+def bench_batched(args):
+    import numpy as np
+    from skid_amd import gen, skg
+    sizes = gen.config4_sizes(args.boxes)
+    t = time.time()
+    boxes = [gen.config4_box(int(n), b) for b, n in enumerate(sizes)]
+    taus = np.array([gen.config4_tau(int(n)) for n in sizes], np.float32)
+    N = int(sum(b.n for b in boxes))
+    log(f"generated {len(boxes)} boxes, {N} particles in {time.time() - t:.1f}s")
+    nb = 16
+    if args.shapes:                                  # fixed shapes (e.g. a profiled rerun of tuned ones)
+        nb, sub_max, tile = (int(v) for v in args.shapes.split(","))
+        os.environ["SKG_COVER_SUB_MAX"] = str(sub_max)
+        os.environ["SKG_POT_TILE"] = str(tile)
+    P = skg.make_params(0.01, 32, nBucket=nb, period=1.0, accumulate=skg.SKG_ACC_COMPENSATED)
+    ctx = skg.BatchContext(boxes, P, taus)           # particles resident in HBM from here on
+    t = time.time()
+    tuned = ctx.autotune(0 if args.shapes else skg.SKG_TUNE_ALL)
+    log(f"autotune {time.time() - t:.1f}s: {tuned}")
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        tr, ng, last = run_path(ctx, adjoint=True)
+        log(f"warmup {w}: {time.perf_counter() - t0:.3f}s groups={ng - 1}")
+    tuned = ctx.autotune(0)                          # the tile chosen on the warmup's groups
+    ctx.set_profiling(True)
+    st0 = ctx.stage_times()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ts = time.perf_counter()
+        tr, ng, last = run_path(ctx, adjoint=True)
+        log(f"step {s}: {time.perf_counter() - ts:.3f}s groups={ng - 1}")
+    wall = time.perf_counter() - t0
+    rho, drho, g = last
+    rp = skg.nsmooth_report(rho, drho, g, dens_min=float(np.quantile(rho[rho > 0][::64], 0.25)), rel_k=0.25)
+    report = {"dlnrho_dlnk": rp["dlnrho_dlnk"], "dens_min_probe": rp["dens_min"], "rel_k": rp["rel_k"],
+              "at_risk": rp["at_risk"], "groups_with_risk": int((rp["group_frac_at_risk"][1:] > 0).sum())}
+    log("nSmooth sensitivity report:", report)
+    kst = ctx.kernel_stats()
+    st = {k: (v - st0[k]) / args.steps for k, v in ctx.stage_times().items()}
+    log("stage ms per step:", {k: round(v, 1) for k, v in st.items()})
+    log("kernels:", json.dumps({k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                                 for k, v in kst.items()}))
+    ctx.close()
+    value = N / (wall / args.steps)
+    knn = kst["knn"]
+    knn_qps = (knn["work"] / 16.0) / (knn["ms"] * 1e-3) if knn["ms"] > 0 else None
+
+    plain = None
+    if args.plain_every > 0:
+        order = np.argsort(sizes)
+        pick = [int(i) for i in order[::args.plain_every]]
+        ctxs = [skg.Context(boxes[i], skg.make_params(float(taus[i]), 32, nBucket=16, period=1.0,
+                                                     accumulate=skg.SKG_ACC_PLAIN)) for i in pick]
+        npl = int(sum(boxes[i].n for i in pick))
+        run_path(ctxs[0])                            # warm the kernels
+        t0 = time.perf_counter()
+        for c in ctxs:
+            run_path(c)
+        dt = time.perf_counter() - t0
+        for c in ctxs:
+            c.close()
+        plain = dict(value=npl / dt, unit="particles/s", boxes=len(pick), particles=npl, seconds=dt)
+        log("plain GPU form:", plain)
+
+    cpu = None
+    if args.cpu_boxes > 0:
+        order = np.argsort(sizes)
+        qs = [0.05 + 0.6 * i / max(1, args.cpu_boxes - 1) for i in range(args.cpu_boxes)]
+        pick = [int(order[int(q * (len(order) - 1))]) for q in qs]
+        sb = [boxes[i] for i in pick]
+        kws = [dict(nSmooth=32, fTau=float(taus[i]), fPeriod=1.0) for i in pick]
+        dt, kind = cpu_run(sb, kws)
+        ncpu = int(sum(b.n for b in sb))
+        cpu = dict(value=ncpu / dt, unit="particles/s", cores=1, kind=kind,
+                   sample=f"{len(sb)} of the 256 boxes ({', '.join(str(b.n) for b in sb)} particles; 5th-65th "
+                          f"size percentiles), each run end-to-end as one skid process would, {dt:.2f}s on 1 core")
+        log("cpu baseline:", cpu)
+
+    forms = {"batched_adjoint_compensated": value,
+             "plain_gpu_per_box": plain["value"] if plain else None,
+             "cpu_reference_1core": cpu["value"] if cpu else None}
+    if plain:
+        forms["batched_over_plain"] = value / plain["value"]
+    if cpu:
+        forms["gpu_over_cpu"] = value / cpu["value"]
+    config = {"workload": f"{len(boxes)} independent periodic unit boxes, N log-uniform in [2048, 524288] "
+                          f"(seeded, {N} particles), NFW halos (N/4000 per box) + 25% background, nSmooth 32, "
+                          f"tau 0.16 N^-1/3 per box, end-to-end group finding of all boxes per step, plus "
+                          f"d rho / d ln nSmooth per particle (the membership report from it is formed once, untimed)",
+              "n_particles": N, "boxes": len(boxes), "form": "batched+adjoint+compensated",
+              "autotune": tuned, "knn_queries_per_s": knn_qps, "forms": forms, "plain_gpu": plain,
+              "nsmooth_report": report}
+    return value, wall, config, roofline(kst), cpu
+
+
+def bench_two_species(args):
+    from skid_amd import gen, skg
+    le = args.log2_each
+    n_each = 1 << le
+    tau = 0.001 * (2.0 ** ((24 - le) / 3.0))
+    t = time.time()
+    box = gen.config3(n_each=n_each)
+    log(f"generated {box.n} particles in {time.time() - t:.1f}s")
+    P = skg.make_params(tau, nSmooth=64, nBucket=16, period=1.0, bGasAndDark=1, accumulate=skg.SKG_ACC_COMPENSATED)
+    ctx = skg.Context(box, P)
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        tr, ng, _ = run_path(ctx)
+        log(f"warmup {w}: {time.perf_counter() - t0:.3f}s ittr={len(tr)} groups={ng - 1}")
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ts = time.perf_counter()
+        tr, ng, _ = run_path(ctx)
+        log(f"step {s}: {time.perf_counter() - ts:.3f}s ittr={len(tr)} groups={ng - 1}")
+    wall = time.perf_counter() - t0
+    kst = ctx.kernel_stats()
+    ctx.close()
+    knn = kst["knn"]
+    cpu = None
+    if args.cpu_boxes > 0:
+        ce = 1 << 16
+        sb = gen.config3(n_each=ce, seed=3, n_halos=max(1, int(8000 * ce / (1 << 24))))
+        tc = 0.001 * (2.0 ** ((24 - 16) / 3.0))
+        dt, kind = cpu_run([sb], [dict(nSmooth=64, fTau=tc, fPeriod=1.0, bGasAndDark=1)])
+        cpu = dict(value=sb.n / dt, unit="particles/s", cores=1, kind=kind,
+                   sample=f"same recipe at 2^16+2^16 particles, tau {tc:.4g}, {dt:.2f}s on 1 core")
+    config = {"workload": f"two-species box 2^{le} gas + 2^{le} dark (8000 NFW halos, dN/dM~M^-1.9), nSmooth 64, "
+                          f"nBucket 16, tau {tau:.4g}, -gd, periodic unit box, end-to-end group finding",
+              "n_particles": box.n, "form": "compensated",
+              "knn_queries_per_s": (knn["work"] / 16.0) / (knn["ms"] * 1e-3) if knn["ms"] > 0 else None}
+    return box.n / (wall / args.steps), wall, config, roofline(kst), cpu
 
 
 def main():
@@ -60,92 +233,29 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--log2-each", type=int, default=24, help="particles per species = 2^this")
-    ap.add_argument("--plain", action="store_true", help="fp32 running sums (reference arithmetic)")
-    ap.add_argument("--cpu-sample", type=int, default=16, help="log2 particles per species of the CPU sample; 0 skips")
+    ap.add_argument("--workload", choices=("batched", "two-species"), default="batched")
+    ap.add_argument("--boxes", type=int, default=256)
+    ap.add_argument("--log2-each", type=int, default=24, help="two-species: particles per species = 2^this")
+    ap.add_argument("--plain-every", type=int, default=4, help="plain GPU form on every n-th box (0 skips)")
+    ap.add_argument("--cpu-boxes", type=int, default=4, help="boxes in the CPU sample (0 skips)")
+    ap.add_argument("--shapes", default="", help="nBucket,subMax,potTile: skip the autotuner and use these")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported", file=sys.stderr)
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(sharding is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
-
-    from skid_amd import gen, skg
-    n_each = 1 << args.log2_each
-    tau = 0.001 * (2.0 ** ((24 - args.log2_each) / 3.0))
-    t = time.time()
-    box = gen.config3(n_each=n_each)
-    log(f"generated {box.n} particles in {time.time() - t:.1f}s")
-    P = skg.make_params(tau, nSmooth=64, nBucket=16, period=1.0, bGasAndDark=1,
-                        accumulate=skg.SKG_ACC_PLAIN if args.plain else skg.SKG_ACC_COMPENSATED)
-    ctx = skg.Context(box, P)                       # particles resident in HBM from here on
-
-    def one_pass():
-        t0 = time.perf_counter()
-        ctx.density()
-        tr = ctx.move_loop()
-        ctx.fof()
-        ctx.microsteps()
-        ctx.centers()
-        ctx.unbind()
-        ctx.too_small()
-        g, ng = ctx.groups()
-        return time.perf_counter() - t0, len(tr), ng
-
-    for w in range(args.warmup):
-        dt, it, ng = one_pass()
-        log(f"warmup {w}: {dt:.3f}s ittr={it} groups={ng - 1}")
-    ctx.set_profiling(True)
-    t_start = ctx.stage_times()
-    times = []
-    t0 = time.perf_counter()
-    for s in range(args.steps):
-        dt, it, ng = one_pass()
-        times.append(dt)
-        log(f"step {s}: {dt:.3f}s ittr={it} groups={ng - 1}")
-    wall = time.perf_counter() - t0
-    kst = ctx.kernel_stats()
-    st = {k: v - t_start[k] for k, v in ctx.stage_times().items()}
-    ms_per_step = 1000.0 * wall / args.steps
-    value = box.n / (wall / args.steps)
-    log("stage ms per step:", {k: round(v / args.steps, 1) for k, v in st.items()})
-    log("kernels:", json.dumps({k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
-                                 for k, v in kst.items()}))
-
-    # roofline of the kernel class with the most device time
-    dom = max(kst, key=lambda k: kst[k]["ms"])
-    d = kst[dom]
-    avg_ms = d["ms"] / max(1, d["launches"])
-    if dom == "unbind_potential":
-        ach = d["work"] / max(1, d["launches"]) / (avg_ms * 1e-3) / 1e12
-        roof = dict(bound="valu", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=ach / FP32_PEAK_TFLOPS,
-                    traffic=None, kernel=dom, avg_launch_ms=avg_ms, launches=d["launches"])
+    if args.workload == "batched":
+        value, wall, config, roof, cpu = bench_batched(args)
     else:
-        ach = d["work"] / max(1, d["launches"]) / (avg_ms * 1e-3) / 1e9
-        roof = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
-                    traffic=None, kernel=dom, avg_launch_ms=avg_ms, launches=d["launches"])
-    knn = kst["knn"]
-    knn_qps = (knn["work"] / 16.0) / (knn["ms"] * 1e-3) if knn["ms"] > 0 else None
-
-    cpu = None
-    if args.cpu_sample > 0:
-        ce = 1 << args.cpu_sample
-        cpu = cpu_baseline(ce, 0.001 * (2.0 ** ((24 - args.cpu_sample) / 3.0)))
-        log("cpu baseline:", cpu)
-
+        value, wall, config, roof, cpu = bench_two_species(args)
     out = {
         "metric": "end-to-end particles/s at 2^24 (adjoint+batched vs plain GPU vs CPU); kNN q/s",
         "value": value, "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f32", "data": "synthetic",
-        "config": {"workload": f"two-species box {n_each}+{n_each} particles (2^{args.log2_each} gas + "
-                               f"2^{args.log2_each} dark, 8000 NFW halos, dN/dM~M^-1.9), nSmooth 64, nBucket 16, "
-                               f"tau {tau:.4g}, -gd, periodic unit box, end-to-end group finding",
-                   "n_particles": box.n, "form": "plain" if args.plain else "compensated",
-                   "knn_queries_per_s": knn_qps},
-        "roofline": roof,
-        "cpu_baseline": cpu,
+        "ms_per_step": 1000.0 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": config,
+        "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
-    ctx.close()
 
 
 if __name__ == "__main__":

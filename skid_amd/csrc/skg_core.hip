@@ -1280,6 +1280,10 @@ extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
 	if (const char *e = getenv("SKG_TILE_LISTS")) c->useLists = atoi(e);
 	if (const char *e = getenv("SKG_GRAPH")) c->useGraph = atoi(e);
 	if (const char *e = getenv("SKG_GRAPH_SHRINK")) c->gShrink = std::max(1, atoi(e));
+	if (const char *e = getenv("SKG_POT_TILE")) {
+		const int t = atoi(e);
+		if (t == 64 || t == 128 || t == 256) c->potTile = t;
+	}
 	if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
 	    hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
 		delete c;

@@ -287,38 +287,38 @@ class Context:
         return tr, g, ng
 
 
-def nsmooth_report(rho, drho, groups, dens_min=0.0, rel_k=0.25):
+def nsmooth_report(rho, drho, groups, dens_min=0.0, rel_k=0.25, sample=1 << 20):
     """Membership sensitivity to nSmooth from d rho / d ln nSmooth.
 
     A particle's density moves by about drho * ln(1 + rel_k) when nSmooth
     changes by the factor 1 + rel_k.  Particles whose density would cross the
     mover threshold dens_min (-d, main.c:114-115) in either direction change
     the moving set, and with it the groups they can join.  Returns the
-    distribution of d ln rho / d ln nSmooth over scatterers and, per group
-    (id > 0), its size, mean sensitivity and the fraction of members at risk.
+    distribution of d ln rho / d ln nSmooth over scatterers (quantiles from a
+    fixed subsample of at most `sample` scatterers), the number at risk, and
+    per group (id > 0) arrays: size, mean sensitivity, fraction at risk.
     """
-    rho = np.asarray(rho, np.float64)
-    drho = np.asarray(drho, np.float64)
+    rho = np.asarray(rho, np.float32)
+    drho = np.asarray(drho, np.float32)
     groups = np.asarray(groups)
     sc = rho > 0
-    s = np.zeros_like(rho)
-    s[sc] = drho[sc] / rho[sc]
-    dl = abs(np.log1p(rel_k)), -abs(np.log1p(-rel_k)) if rel_k < 1 else -np.inf
-    up = rho + drho * dl[0]
-    dn = rho + drho * dl[1] if np.isfinite(dl[1]) else rho
-    lo, hi = np.minimum(up, dn), np.maximum(up, dn)
-    at_risk = sc & (lo < dens_min) & (hi >= dens_min)
-    out = {"scatterers": int(sc.sum()), "dlnrho_dlnk": {
-        q: float(np.percentile(s[sc], p)) for q, p in (("p5", 5), ("median", 50), ("p95", 95))} if sc.any() else {},
-        "at_risk": int(at_risk.sum()), "rel_k": rel_k, "dens_min": dens_min, "groups": {}}
-    ids = groups[groups > 0]
-    if ids.size:
-        n = np.bincount(groups[groups > 0])
-        ms = np.bincount(groups[groups > 0], weights=s[groups > 0])
-        ar = np.bincount(groups[groups > 0], weights=at_risk[groups > 0].astype(np.float64))
-        for g in np.flatnonzero(n):
-            out["groups"][int(g)] = (int(n[g]), float(ms[g] / n[g]), float(ar[g] / n[g]))
-    return out
+    s = np.divide(drho, rho, out=np.zeros_like(rho), where=sc)
+    up, dn = np.float32(np.log1p(rel_k)), np.float32(np.log1p(-rel_k)) if rel_k < 1 else np.float32(-np.inf)
+    a, b = rho + drho * up, rho + drho * dn
+    at_risk = sc & (np.minimum(a, b) < dens_min) & (np.maximum(a, b) >= dens_min)
+    idx = np.flatnonzero(sc)
+    if idx.size > sample:
+        idx = idx[np.linspace(0, idx.size - 1, sample).astype(np.int64)]
+    q = np.quantile(s[idx], [0.05, 0.5, 0.95]) if idx.size else [np.nan] * 3
+    g = groups.astype(np.int64)
+    n = np.bincount(g)
+    n[0] = 0
+    with np.errstate(invalid="ignore", divide="ignore"):
+        mean_s = np.bincount(g, weights=s) / n
+        risk = np.bincount(g, weights=at_risk) / n
+    return {"scatterers": int(sc.sum()), "dlnrho_dlnk": {"p5": float(q[0]), "median": float(q[1]), "p95": float(q[2])},
+            "at_risk": int(at_risk.sum()), "rel_k": rel_k, "dens_min": dens_min,
+            "group_size": n, "group_mean_dlnrho_dlnk": mean_s, "group_frac_at_risk": risk}
 
 
 class BatchContext(Context):
