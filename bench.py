@@ -64,7 +64,21 @@ def cpu_run(boxes, kws):
     return time.perf_counter() - t0, kind
 
 
-def roofline(kst):
+def measured_traffic(name):
+    """HBM bytes per launch of the move-step class from the committed counter
+    summary (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, calibrated on
+    a copy of known size), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def roofline(kst, traffic=None):
     """Roofline of the kernel class with the most device time (HIP events on
     the library's stream around every launch of the timed steps)."""
     dom = max(kst, key=lambda k: kst[k]["ms"])
@@ -75,8 +89,12 @@ def roofline(kst):
         return dict(bound="valu", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=ach / FP32_PEAK_TFLOPS,
                     traffic=None, kernel=dom, avg_launch_ms=avg_ms, launches=d["launches"])
     ach = d["work"] / max(1, d["launches"]) / (avg_ms * 1e-3) / 1e9
-    return dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
-                traffic=None, kernel=dom, avg_launch_ms=avg_ms, launches=d["launches"])
+    out = dict(bound="hbm", achieved=ach, peak=HBM_PEAK_GBS, unit="GB/s", frac=ach / HBM_PEAK_GBS,
+               traffic=traffic if dom == "cover_move" else None, kernel=dom, avg_launch_ms=avg_ms,
+               launches=d["launches"])
+    if out["traffic"]:
+        out["traffic_source"] = "profiles/traffic_batched_256_boxes.md (rocprofv3 FETCH_SIZE/WRITE_SIZE, bytes per launch)"
+    return out
 
 
 def run_path(ctx, adjoint=False):
@@ -186,7 +204,7 @@ def bench_batched(args):
               "n_particles": N, "boxes": len(boxes), "form": "batched+adjoint+compensated",
               "autotune": tuned, "knn_queries_per_s": knn_qps, "forms": forms, "plain_gpu": plain,
               "nsmooth_report": report}
-    return value, wall, config, roofline(kst), cpu
+    return value, wall, config, roofline(kst, measured_traffic("traffic_batched_256_boxes.md")), cpu
 
 
 def bench_two_species(args):

@@ -139,6 +139,23 @@ int skg_get_moved(skg_ctx *ctx, float *r);
 /* kdInitpGroup + kdCalcCenter (kd.c:969-1075). */
 int skg_centers(skg_ctx *ctx);
 
+/* Groups from outside instead of the move loop + FOF: the -unbind re-entry,
+   kdInGroup + kdInitpGroup + kdReadCenter (kd.c:920-1196; main.c:349-373).
+   grp by iOrder (0 = no group; ids 1..max dense or sparse), after skg_load
+   only.  center and vcm (3 per group 1..max, the .gtp star positions and
+   velocities) or both NULL: then each group's centre is its mass-weighted
+   centre relative to its member with the largest iOrder, and vcm its
+   mass-weighted velocity (kd.c:1160-1193).  Batched contexts: a group must lie
+   in one box.  Next calls: skg_group_energies, skg_unbind, skg_too_small. */
+int skg_set_groups(skg_ctx *ctx, const int *grp, const float *center, const float *vcm);
+
+/* Per-group energies before unbinding (no reference counterpart; the first
+   pass of kdUnbind's energy test, kd.c:1376-1399): W[g] = -1/2 sum m_i phi_i
+   from the fp32 compensated member potentials (kdCellPot, members only) and
+   T[g] = 1/2 sum m_i |fShift (v_i - vcm) + fCosmo (r_i - rcm)|^2, both reduced
+   in FP64, for g = 0..nGroup-1 (group 0 gets 0). */
+int skg_group_energies(skg_ctx *ctx, double *W, double *T);
+
 /* Iterative unbinding: kdUnbind (kd.c:1299-1466) with grav.c:8-135. */
 int skg_unbind(skg_ctx *ctx, int *nUnbound);
 

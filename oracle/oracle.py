@@ -382,3 +382,84 @@ def same_partition(a: np.ndarray, b: np.ndarray) -> bool:
     if a.shape != b.shape or not np.array_equal(a == 0, b == 0):
         return False
     return np.array_equal(relabel_min_index(a), relabel_min_index(b))
+
+
+def unbind_f64(pos, vel, mass, soft, grp, centres, period=1.0, G=1.0, soft_type=2, fshift=1.0, fcosmo=0.0,
+               z=0.0, scoop=0.0, sub_pot=True, max_members=None):
+    """FP64 restatement of kdUnbind's removal (kd.c:1330-1447) for checking:
+    per group, coordinates relative to the member with the largest index
+    (wrapped), FP64 centre of mass, potentials from or_potential_f64 plus the
+    scoop (group-0 particles within `scoop` of the group centre add G m K to
+    every member, grav.c:63-135), then E = 1/2|fshift (v - vcm) + fcosmo (r - rcm)|^2
+    - phi (1+z); while max E >= 0 remove the argmax (ties: smallest index), downdate
+    the centre of mass and, if sub_pot, subtract the removed particle's potential.
+    Returns the group ids after unbinding (removed -> 0).  Everything in float64."""
+    from scipy.spatial import cKDTree
+    pos = np.asarray(pos, np.float64)
+    vel = np.asarray(vel, np.float64)
+    mass = np.asarray(mass, np.float64)
+    soft = np.asarray(soft, np.float32)
+    out = np.array(grp, np.int32, copy=True)
+    z0 = np.flatnonzero(out == 0)
+    tree = cKDTree(np.mod(pos[z0] + 0.5 * period, period), boxsize=period) if (scoop > 0 and z0.size) else None
+    order = np.argsort(out, kind="stable")
+    bounds = np.searchsorted(out[order], np.arange(out.max() + 2))
+    for g in range(1, out.max() + 1):
+        mem = order[bounds[g]:bounds[g + 1]]
+        n = mem.size
+        if n == 0 or (max_members is not None and n >= max_members):
+            continue
+        rel = pos[mem.max()]
+        d = pos[mem] - rel
+        d -= period * np.round(d / period)
+        phi = potential_f64(d, mass[mem], soft[mem], G=G, soft_type=soft_type)
+        if tree is not None:
+            c = np.asarray(centres[g], np.float64)
+            hits = tree.query_ball_point(np.mod(c + 0.5 * period, period), scoop)
+            for j in z0[hits]:
+                dj = pos[j] - rel
+                dj -= period * np.round(dj / period)
+                phi += G * mass[j] * _kernel_f64(np.sqrt(((d - dj) ** 2).sum(1)), soft[mem] + soft[j], soft_type)
+        m = mass[mem]
+        v = vel[mem]
+        alive = np.ones(n, bool)
+        M = m.sum()
+        rcm = (m[:, None] * d).sum(0) / M
+        vcm = (m[:, None] * v).sum(0) / M
+        zf = 1.0 + z
+        while alive.any():
+            u = fshift * (v - vcm) + fcosmo * (d - rcm)
+            E = 0.5 * (u * u).sum(1) - phi * zf
+            E[~alive] = -np.inf
+            Emax = E.max()
+            if not (Emax >= 0.0):
+                break
+            cand = np.flatnonzero(E == Emax)
+            b = cand[np.argmin(mem[cand])]
+            alive[b] = False
+            out[mem[b]] = 0
+            M2 = M - m[b]
+            if M2 <= 0:
+                break
+            rcm = (rcm * M - m[b] * d[b]) / M2
+            vcm = (vcm * M - m[b] * v[b]) / M2
+            M = M2
+            if sub_pot:
+                rb = np.sqrt(((d - d[b]) ** 2).sum(1))
+                phi -= G * m[b] * _kernel_f64(rb, soft[mem] + soft[mem[b]], soft_type)
+    return out
+
+
+def _kernel_f64(r, twoh, soft_type):
+    """The softened 1/r of grav.h:11-31 (spline) or grav.c:25-27 (Plummer) in FP64."""
+    r = np.asarray(r, np.float64)
+    twoh = np.asarray(twoh, np.float64)
+    if soft_type == 1:
+        return 1.0 / np.sqrt(r * r + 0.25 * twoh * twoh)
+    dih = 2.0 / twoh
+    u = r * dih
+    with np.errstate(divide="ignore", invalid="ignore"):
+        inner = dih * (1.4 - (2.0 / 3.0) * u ** 2 + 0.3 * u ** 4 - 0.1 * u ** 5)
+        mid = -1.0 / (15.0 * r) + dih * (1.6 - (4.0 / 3.0) * u ** 2 + u ** 3 - 0.3 * u ** 4 + u ** 5 / 30.0)
+        far = 1.0 / r
+    return np.where(r >= twoh, far, np.where(u < 1.0, inner, mid))

@@ -1349,3 +1349,204 @@ extern "C" int skg_get_catalogue(skg_ctx *c, float *mass, float *center, float *
 	return SKG_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// Groups from outside: the -unbind re-entry (kdInGroup + kdInitpGroup +
+// kdReadCenter, kd.c:920-1196; main.c:349-373).
+
+// Per group g >= 1: mass, the reference point rel (member with the largest
+// iOrder: kdInitpGroup's last member in input order), and, when no centres are
+// given, the mass-weighted centre relative to rel and the mass-weighted
+// velocity (kd.c:1160-1193).
+__global__ __launch_bounds__(64) void k_group_com(const int *__restrict__ gs, const int *__restrict__ ge, int nG,
+		const int *__restrict__ mem, const float *__restrict__ pos, const float *__restrict__ vel,
+		const float *__restrict__ mass, float Lx, float Ly, float Lz, float cx, float cy, float cz, int withCentre,
+		float *gmass, float4 *grel, float4 *gcen, float4 *gvcm) {
+	const int g = blockIdx.x + 1;
+	if (g >= nG) return;
+	const int lane = threadIdx.x, s = gs[g], e = ge[g];
+	if (e - s <= 0) return;
+	const int last = mem[e - 1];
+	const float rx = pos[3 * last], ry = pos[3 * last + 1], rz = pos[3 * last + 2];
+	Acc am, ax, ay, az, vx, vy, vz;
+	am.init(); ax.init(); ay.init(); az.init(); vx.init(); vy.init(); vz.init();
+	for (int t = s + lane; t < e; t += 64) {
+		const int o = mem[t];
+		const float m = mass[o];
+		am.add(m);
+		ax.add(fmul(m, wrap_rel(fsub(pos[3 * o], rx), Lx)));
+		ay.add(fmul(m, wrap_rel(fsub(pos[3 * o + 1], ry), Ly)));
+		az.add(fmul(m, wrap_rel(fsub(pos[3 * o + 2], rz), Lz)));
+		vx.add(fmul(m, vel[3 * o])); vy.add(fmul(m, vel[3 * o + 1])); vz.add(fmul(m, vel[3 * o + 2]));
+	}
+	float v[7] = {am.s, ax.s, ay.s, az.s, vx.s, vy.s, vz.s};
+	float w[7] = {am.c, ax.c, ay.c, az.c, vx.c, vy.c, vz.c};
+	for (int k = 0; k < 7; ++k) {
+		double d = (double)v[k] + (double)w[k];
+		for (int o = 32; o > 0; o >>= 1) d += __shfl_xor(d, o, 64);
+		v[k] = (float)d;
+	}
+	if (lane != 0) return;
+	const float M = v[0];
+	gmass[g] = M;
+	grel[g] = make_float4(rx, ry, rz, 0.f);
+	if (!withCentre) return;
+	float c3[3] = {v[1] / M, v[2] / M, v[3] / M};
+	const float r3[3] = {rx, ry, rz}, C[3] = {cx, cy, cz}, L[3] = {Lx, Ly, Lz};
+	for (int j = 0; j < 3; ++j) {
+		float q = fadd(c3[j], r3[j]);
+		if ((double)q > (double)C[j] + 0.5 * (double)L[j]) q = fsub(q, L[j]);
+		if ((double)q <= (double)C[j] - 0.5 * (double)L[j]) q = fadd(q, L[j]);
+		c3[j] = q;
+	}
+	gcen[g] = make_float4(c3[0], c3[1], c3[2], 0.f);
+	gvcm[g] = make_float4(v[4] / M, v[5] / M, v[6] / M, 0.f);
+}
+
+extern "C" int skg_set_groups(skg_ctx *c, const int *grp, const float *center, const float *vcm) {
+	if (!c || !grp || (center && !vcm) || (!center && vcm)) return SKG_E_PARAM;
+	if (c->stage < 1) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	const int n = c->n;
+	int mx = 0;
+	for (int i = 0; i < n; ++i) {
+		if (grp[i] < 0) return SKG_E_PARAM;
+		mx = std::max(mx, grp[i]);
+	}
+	if (c->nBox > 1) {   // a group lives in one box
+		std::vector<int> gbox(mx + 1, -1);
+		for (int b = 0; b < c->nBox; ++b)
+			for (int i = c->hbox[b].off; i < c->hbox[b].off + c->hbox[b].n; ++i) {
+				const int g = grp[i];
+				if (!g) continue;
+				if (gbox[g] >= 0 && gbox[g] != b) return SKG_E_PARAM;
+				gbox[g] = b;
+			}
+	}
+	skg_group_free(c);
+	GroupState *G = c->gs = new (std::nothrow) GroupState();
+	if (!G) return SKG_E_NOMEM;
+	int rc;
+	const int nG = mx + 1;
+	c->nGroup = nG;
+	G->nG = nG;
+	if (dalloc(&G->gstart, nG) || dalloc(&G->gend, nG) || dalloc(&G->mem, n) || dalloc(&G->mass, nG) ||
+	    dalloc(&G->rel, nG) || dalloc(&G->cen, nG) || dalloc(&G->vcm, nG) || dalloc(&G->rbound, nG) ||
+	    dalloc(&G->radius, nG) || dalloc(&G->count, nG) || dalloc(&G->d_nunb, 1))
+		return SKG_E_NOMEM;
+	skg_timer_start(c);
+	SKG_HIP(hipMemcpyAsync(c->d_grp, grp, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
+	SKG_HIP(hipMemsetAsync(G->mass, 0, 4 * (size_t)nG, c->st));
+	SKG_HIP(hipMemsetAsync(G->rel, 0, 16 * (size_t)nG, c->st));
+	SKG_HIP(hipMemsetAsync(G->cen, 0, 16 * (size_t)nG, c->st));
+	SKG_HIP(hipMemsetAsync(G->vcm, 0, 16 * (size_t)nG, c->st));
+	SKG_HIP(hipMemsetAsync(G->rbound, 0, 16 * (size_t)nG, c->st));
+	if ((rc = group_sort(c))) return rc;
+	if (center) {   // from a .gtp: rCenter = star position, vcm = star velocity (kd.c:1147-1150)
+		std::vector<float4> hc(nG), hv(nG);
+		hc[0] = hv[0] = make_float4(0.f, 0.f, 0.f, 0.f);
+		for (int g = 1; g < nG; ++g) {
+			hc[g] = make_float4(center[3 * (g - 1)], center[3 * (g - 1) + 1], center[3 * (g - 1) + 2], 0.f);
+			hv[g] = make_float4(vcm[3 * (g - 1)], vcm[3 * (g - 1) + 1], vcm[3 * (g - 1) + 2], 0.f);
+		}
+		SKG_HIP(hipMemcpyAsync(G->cen, hc.data(), 16 * (size_t)nG, hipMemcpyHostToDevice, c->st));
+		SKG_HIP(hipMemcpyAsync(G->vcm, hv.data(), 16 * (size_t)nG, hipMemcpyHostToDevice, c->st));
+		SKG_HIP(hipStreamSynchronize(c->st));
+	}
+	const float *L = c->P.fPeriod, *C = c->P.fCenter;
+	if (nG > 1)
+		k_group_com<<<nG - 1, 64, 0, c->st>>>(G->gstart, G->gend, nG, G->mem, c->d_pos, c->d_vel, c->d_mass,
+				L[0], L[1], L[2], C[0], C[1], C[2], center ? 0 : 1, G->mass, G->rel, G->cen, G->vcm);
+	SKG_HIP(hipGetLastError());
+	SKG_HIP(hipStreamSynchronize(c->st));
+	skg_timer_stop(c, SKG_T_GROUP);
+	c->stage = 7;
+	return SKG_OK;
+}
+
+// FP64 per-group energies from the fp32 compensated member potentials:
+// W_g = -1/2 sum m_i phi_i (phi from kdCellPot, members only) and
+// T_g = 1/2 sum m_i |fShift (v_i - vcm) + fCosmo (r_i - rcm)|^2 with the
+// FP64 centre of mass (the frame of kd.c:1394-1399).
+__global__ __launch_bounds__(256) void k_group_energy(const int *__restrict__ gs, const int *__restrict__ ge, int nG,
+		const float4 *__restrict__ uq, const float4 *__restrict__ uv, const float2 *__restrict__ uphi,
+		float fShift, float fCosmo, double *W, double *T) {
+	__shared__ double sd[256];
+	const int g = blockIdx.x + 1;
+	if (g >= nG) return;
+	const int s = gs[g], e = ge[g], tid = threadIdx.x;
+	double a[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	for (int i = s + tid; i < e; i += 256) {
+		const float4 q = uq[i], v = uv[i];
+		const double m = q.w;
+		a[0] += m; a[1] += m * q.x; a[2] += m * q.y; a[3] += m * q.z;
+		a[4] += m * v.x; a[5] += m * v.y; a[6] += m * v.z;
+		a[7] += m * ((double)uphi[i].x + (double)uphi[i].y);
+	}
+	double r[8];
+	for (int k = 0; k < 8; ++k) {
+		sd[tid] = a[k];
+		__syncthreads();
+		for (int o = 128; o > 0; o >>= 1) { if (tid < o) sd[tid] += sd[tid + o]; __syncthreads(); }
+		r[k] = sd[0];
+		__syncthreads();
+	}
+	const double M = r[0] > 0 ? r[0] : 1.0;
+	const double cm[7] = {r[0], r[1] / M, r[2] / M, r[3] / M, r[4] / M, r[5] / M, r[6] / M};
+	double t = 0.0;
+	for (int i = s + tid; i < e; i += 256) {
+		const float4 q = uq[i], v = uv[i];
+		const double d0 = (double)fShift * ((double)v.x - cm[4]) + (double)fCosmo * ((double)q.x - cm[1]);
+		const double d1 = (double)fShift * ((double)v.y - cm[5]) + (double)fCosmo * ((double)q.y - cm[2]);
+		const double d2 = (double)fShift * ((double)v.z - cm[6]) + (double)fCosmo * ((double)q.z - cm[3]);
+		t += 0.5 * (double)q.w * (d0 * d0 + d1 * d1 + d2 * d2);
+	}
+	sd[tid] = t;
+	__syncthreads();
+	for (int o = 128; o > 0; o >>= 1) { if (tid < o) sd[tid] += sd[tid + o]; __syncthreads(); }
+	if (tid == 0) { W[g] = -0.5 * r[7]; T[g] = sd[0]; }
+}
+
+extern "C" int skg_group_energies(skg_ctx *c, double *W, double *T) {
+	if (!c || !W || !T) return SKG_E_PARAM;
+	if (c->stage != 7 || !c->gs) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	GroupState *G = c->gs;
+	const int n = c->n, nG = G->nG;
+	std::vector<int> hs(nG), he(nG);
+	SKG_HIP(hipMemcpyAsync(hs.data(), G->gstart, 4 * (size_t)nG, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(he.data(), G->gend, 4 * (size_t)nG, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	if (dalloc(&G->uq, n) || dalloc(&G->uv, n) || dalloc(&G->uphi, n) || dalloc(&G->uord, n)) return SKG_E_NOMEM;
+	const int TT = c->potTile;
+	std::vector<int2> tiles;
+	for (int g = 1; g < nG; ++g)
+		for (int t = hs[g]; t < he[g]; t += TT) tiles.push_back(make_int2(g, t));
+	int2 *dt = nullptr;
+	double *dW = nullptr, *dT = nullptr;
+	if (dalloc(&dt, std::max<size_t>(1, tiles.size())) || dalloc(&dW, nG) || dalloc(&dT, nG)) return SKG_E_NOMEM;
+	if (!tiles.empty()) SKG_HIP(hipMemcpyAsync(dt, tiles.data(), 8 * tiles.size(), hipMemcpyHostToDevice, c->st));
+	SKG_HIP(hipMemsetAsync(dW, 0, 8 * (size_t)nG, c->st));
+	SKG_HIP(hipMemsetAsync(dT, 0, 8 * (size_t)nG, c->st));
+	const float *L = c->P.fPeriod;
+	k_unb_setup<<<nblk(n, 256), 256, 0, c->st>>>(G->gstart, G->gend, nG, G->mem, c->d_pos, c->d_vel, c->d_mass,
+			c->d_soft, G->rel, 0.5f * L[0], 0.5f * L[1], 0.5f * L[2], INT_MAX, G->uq, G->uv, G->uphi, G->uord, n);
+	const int nt = (int)tiles.size();
+	if (nt > 0) {
+		if (TT == 64)
+			k_unb_pot<64><<<nt, 64, 0, c->st>>>(dt, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
+		else if (TT == 128)
+			k_unb_pot<128><<<nt, 128, 0, c->st>>>(dt, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
+		else
+			k_unb_pot<256><<<nt, 256, 0, c->st>>>(dt, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
+	}
+	if (nG > 1)
+		k_group_energy<<<nG - 1, 256, 0, c->st>>>(G->gstart, G->gend, nG, G->uq, G->uv, G->uphi, c->P.fShift,
+				c->P.fCosmo, dW, dT);
+	SKG_HIP(hipGetLastError());
+	SKG_HIP(hipMemcpyAsync(W, dW, 8 * (size_t)nG, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(T, dT, 8 * (size_t)nG, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&dt); dfree(&dW); dfree(&dT);
+	return SKG_OK;
+}

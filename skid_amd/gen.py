@@ -272,3 +272,53 @@ def read_tipsy(path: str, xdr: bool = False) -> Box:
     del t
     return Box(pos=np.ascontiguousarray(pos), vel=np.ascontiguousarray(vel), mass=mass.copy(),
                soft=soft.copy(), temp=temp.astype(np.float32), n_gas=ng, n_dark=nk, n_star=ns)
+
+
+def config5(n: int = 1 << 27, n_groups: int = 60000, seed: int = 5, n_range=(8, 2_000_000),
+            slope: float = -1.9, q_range=(0.5, 2.0), conc_range=(5.0, 15.0), G: float = G_DEFAULT):
+    """Unbinding + precision stress: n_groups NFW spheres with membership drawn
+    from dN/dn ~ n**slope on n_range, velocities isotropic Gaussian with
+    sigma^2 = Q G M / (2 r200) for a virial ratio Q uniform on q_range (so a
+    real fraction is unbound), and the rest of n a uniform background (group 0).
+    Returns (box, grp): grp[i] is particle i's group id (1..n_groups) or 0."""
+    rng = np.random.default_rng(seed)
+    lo, hi = n_range
+    a = slope + 1.0
+    u = rng.uniform(0.0, 1.0, n_groups)
+    sizes = np.floor((lo ** a + u * (hi ** a - lo ** a)) ** (1.0 / a)).astype(np.int64)
+    if sizes.sum() > n:
+        sizes = np.maximum(lo, (sizes * (n / sizes.sum())).astype(np.int64))
+    conc = rng.uniform(conc_range[0], conc_range[1], n_groups)
+    q = rng.uniform(q_range[0], q_range[1], n_groups)
+    centres = rng.uniform(-0.5, 0.5, (n_groups, 3))
+    pos = np.empty((n, 3), np.float32)
+    vel = np.zeros((n, 3), np.float32)
+    grp = np.zeros(n, np.int32)
+    k = 0
+    for g in range(n_groups):
+        m = int(sizes[g])
+        mf = m / n
+        r200 = float(r200_of(mf))
+        r = nfw_radii(rng, m, float(conc[g])) * r200
+        pos[k:k + m] = _wrap(centres[g] + _iso_dirs(rng, m) * r[:, None])
+        sig = np.sqrt(q[g] * G * mf / (2.0 * r200))
+        vel[k:k + m] = rng.normal(0.0, sig, (m, 3))
+        grp[k:k + m] = g + 1
+        k += m
+    pos[k:] = rng.uniform(-0.5, 0.5, (n - k, 3)).astype(np.float32)
+    pos[k:] = np.where(pos[k:] <= np.float32(-0.5), np.float32(0.5), pos[k:])
+    mass = np.full(n, 1.0 / n, np.float32)
+    soft = np.full(n, 0.0288 * n ** (-1.0 / 3.0), np.float32)
+    box = Box(pos=pos, vel=vel, mass=mass, soft=soft, temp=np.zeros(n, np.float32), n_gas=0, n_dark=n,
+              halo_id=grp - 1)
+    return box, grp
+
+
+def config5_cosmology(z: float = 0.5, omega0: float = 0.3, lam: float = 0.7, h0: float = 2.8944):
+    """fShift = 1/(1+z) and fCosmo = fShift * H(fShift) for a flat Lambda model,
+    H(a) = H0 sqrt(Om a + Ok a^2 + OL a^4) / a^2 in FP64 (cosmo.c:46-58, no
+    radiation or quintessence)."""
+    a = 1.0 / (1.0 + z)
+    ok = 1.0 - omega0 - lam
+    H = h0 * np.sqrt(omega0 * a + ok * a * a + lam * a ** 4) / (a * a)
+    return a, a * H

@@ -77,6 +77,8 @@ def lib():
         L.skg_density_adjoint.argtypes = [vp, fp, C.c_int, fp, fp]
         L.skg_field_gradient.argtypes = [vp, C.c_int, fp, fp]
         L.skg_density_dlnk.argtypes = [vp, fp]
+        L.skg_set_groups.argtypes = [vp, fp, fp, fp]
+        L.skg_group_energies.argtypes = [vp, fp, fp]
         L.skg_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
         L.skg_fof_points.argtypes = [vp, C.c_int, fp, C.c_float, fp, ip]
         L.skg_stage_times.argtypes = [vp, C.POINTER(C.c_double)]
@@ -196,6 +198,21 @@ class Context:
         r = np.empty(3 * self.n, np.float32)
         _check(self.L.skg_get_moved(self.h, _p(r)), "skg_get_moved")
         return r.reshape(self.n, 3)
+
+    def set_groups(self, grp, center=None, vcm=None):
+        """Groups from outside (the -unbind re-entry); returns nGroup."""
+        g = np.ascontiguousarray(grp, np.int32)
+        c = None if center is None else np.ascontiguousarray(center, np.float32).reshape(-1)
+        v = None if vcm is None else np.ascontiguousarray(vcm, np.float32).reshape(-1)
+        _check(self.L.skg_set_groups(self.h, _p(g), _p(c), _p(v)), "skg_set_groups")
+        return int(g.max()) + 1 if g.size else 1
+
+    def group_energies(self, ngroup: int):
+        """(W, T) per group before unbinding, FP64."""
+        W = np.zeros(ngroup, np.float64)
+        T = np.zeros(ngroup, np.float64)
+        _check(self.L.skg_group_energies(self.h, _p(W), _p(T)), "skg_group_energies")
+        return W, T
 
     def centers(self):
         _check(self.L.skg_centers(self.h), "skg_centers")

@@ -182,3 +182,59 @@ def test_generator_deterministic():
     assert np.array_equal(a.pos, b.pos) and np.array_equal(a.vel, b.vel)
     assert a.pos.min() > -0.5 and a.pos.max() <= 0.5
     assert a.mass.dtype == np.float32 and math.isclose(a.mass.sum(), 1.0, rel_tol=1e-5)
+
+
+# ---- the FP64 unbinding restatement (oracle.unbind_f64) -------------------------
+def test_unbind_f64_hand_cases():
+    m = np.array([0.5, 0.5, 0.5], np.float32)
+    sf = np.full(3, 1e-4, np.float32)
+    pos = np.array([[0, 0, 0], [0.01, 0, 0], [0.3, 0, 0]], np.float32)
+    g = np.array([1, 1, 1], np.int32)
+    # at rest: everything bound
+    assert np.array_equal(O.unbind_f64(pos, np.zeros((3, 3), np.float32), m, sf, g, {}), [1, 1, 1])
+    # the distant member flies off: removed first and alone, the close pair stays
+    vel = np.array([[0, 0, 0], [0, 0, 0], [0, 20, 0]], np.float32)
+    assert np.array_equal(O.unbind_f64(pos, vel, m, sf, g, {}), [1, 1, 0])
+    # E exactly as kd.c:1394-1399 with fshift / fcosmo: a pure Hubble flow
+    # fcosmo * (r - rcm) unbinds the wide pair but not the close one
+    vel0 = np.zeros((3, 3), np.float32)
+    out = O.unbind_f64(pos, vel0, m, sf, g, {}, fcosmo=30.0)
+    assert out[2] == 0 and out[0] == out[1] == 1
+
+
+def test_unbind_f64_virial_spheres():
+    rng = np.random.default_rng(2)
+    n = 600
+    r = gen.nfw_radii(rng, n, 10.0) * 0.05
+    d = rng.normal(size=(n, 3))
+    pos = (d / np.linalg.norm(d, axis=1)[:, None] * r[:, None]).astype(np.float32)
+    m = np.full(n, 1.0 / n, np.float32)
+    sf = np.full(n, 1e-3, np.float32)
+    g = np.ones(n, np.int32)
+    sig = np.sqrt(1.0 / (2 * 0.05))
+    fr = []
+    for q in (0.5, 1.0, 2.0, 4.0):
+        vel = rng.normal(0, np.sqrt(q) * sig, (n, 3)).astype(np.float32)
+        fr.append((O.unbind_f64(pos, vel, m, sf, g, {}) > 0).mean())
+    # bound fraction falls with the virial ratio: all but a few bound at 0.5,
+    # a real fraction unbound at 2, nearly all at 4
+    assert fr[0] > 0.98 and fr[0] >= fr[1] >= fr[2] >= fr[3], fr
+    assert 0.2 < fr[2] < 0.9 and fr[3] < 0.1, fr
+
+
+def test_unbind_f64_matches_reference_oracle(box1):
+    """On config 1 (no scoop: every particle is in a group) the FP64 removal
+    gives the catalogue of the reference-faithful oracle (bit-exact to the
+    reference) to >= 99.9% of particles."""
+    o = O.Oracle(box1, nSmooth=32, fTau=0.005, fPeriod=1.0)
+    o.density()
+    o.move_loop()
+    fg, _ = o.fof()
+    og, _ = o.finish()
+    u = O.unbind_f64(box1.pos, box1.vel, box1.mass, box1.soft, fg, {})
+    cnt = np.bincount(u)
+    small = np.flatnonzero(cnt < 8)
+    u[np.isin(u, small[small > 0])] = 0          # kdTooSmall with nMembers 8
+    frac = O.match_fraction(u, og)
+    print(f"FP64 unbinding vs reference oracle: match {frac:.5f}")
+    assert frac >= 0.999
