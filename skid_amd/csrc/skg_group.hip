@@ -1084,7 +1084,14 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 	// candidate kernel, one workgroup each.  SKG_UNBIND_COOP_MIN routes groups
 	// from that size to the cooperative all-GPU kernel instead, and
 	// SKG_UNBIND_SMALL_MAX lowers SMALL (tests drive both paths with these).
-	int BIG = INT_MAX, SMALL = 64;
+	// Groups of at least BIG members go to the cooperative all-GPU removal when
+	// kdSubPot runs (DARK or STAR input): each removal then lowers every
+	// potential, the candidate bound loosens with every removal, and one
+	// workgroup took 12.9 s on a 10^6-member group (0.7 s cooperative).
+	// Without kdSubPot the bound stays tight and the candidate kernel is the
+	// faster one at every size (a 2^25 gas+dark box: 0.09 s vs 11 s).
+	const int bSub0 = (c->inType == SKG_DARK || c->inType == SKG_STAR);   // kd.c:1441
+	int BIG = bSub0 ? 100000 : INT_MAX, SMALL = 64;
 	if (const char *e = getenv("SKG_UNBIND_COOP_MIN")) BIG = std::max(65, atoi(e));
 	if (const char *e = getenv("SKG_UNBIND_SMALL_MAX")) SMALL = std::min(64, std::max(1, atoi(e)));
 	std::vector<int2> tiles;
