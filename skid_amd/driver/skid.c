@@ -156,6 +156,74 @@ static double now_s(void)
 	return t.tv_sec + 1e-9*t.tv_nsec;
 }
 
+/* kdInGroup (kd.c:920-962): an ASCII .grp, the particle count then one group
+   id per particle in input order. */
+static int read_grp(const char *path, int n, int *grp, int *nGroup)
+{
+	FILE *fp = fopen(path, "r");
+	int i, m = 0, mx = 0;
+	if (!fp) { fprintf(stderr, "ERROR: Could not open file:%s\n", path); return 1; }
+	if (fscanf(fp, "%d", &m) != 1 || m != n) {
+		fprintf(stderr, "ERROR: Mismatched number of particles\n");
+		fprintf(stderr, "Number in Group file %s: %d\n", path, m);
+		fprintf(stderr, "Number in TIPSY BINARY input file: %d\n", n);
+		fclose(fp);
+		return 1;
+	}
+	for (i = 0; i < n; ++i) {
+		if (fscanf(fp, "%d", &grp[i]) != 1 || grp[i] < 0) {
+			fprintf(stderr, "ERROR: bad group id at line %d of %s\n", i + 2, path);
+			fclose(fp);
+			return 1;
+		}
+		if (grp[i] > mx) mx = grp[i];
+	}
+	fclose(fp);
+	*nGroup = mx + 1;
+	return 0;
+}
+
+/* kdReadCenter (kd.c:1083-1196): the .gtp's star records give each group's
+   centre and velocity.  No file: *cen = *vcm = NULL (centre of mass). */
+static int read_gtp(const char *path, int xdr, int nGroup, const Snap *S, const int *grp, float **cen, float **vcm)
+{
+	FILE *fp = fopen(path, "rb");
+	unsigned char h[32], rec[44];
+	int i, k, nsph, ndark, nstar;
+	double m1 = 0.0;
+	*cen = *vcm = NULL;
+	if (!fp) return 0;
+	if (fread(h, 32, 1, fp) != 1) { fprintf(stderr, "ERROR: cannot read %s\n", path); fclose(fp); return 1; }
+	nsph = rdi(h + 16, xdr); ndark = rdi(h + 20, xdr); nstar = rdi(h + 24, xdr);
+	if (nstar != nGroup - 1) {
+		fprintf(stderr, "ERROR: grp and gtp files don't match: %d vs. %d.\n", nstar, nGroup - 1);
+		fclose(fp);
+		return 1;
+	}
+	/* skip gas (12 floats) and dark (9 floats) records */
+	if (fseek(fp, (long)nsph * 48 + (long)ndark * 36, SEEK_CUR)) { fclose(fp); return 1; }
+	*cen = malloc(12 * (size_t)(nstar ? nstar : 1));
+	*vcm = malloc(12 * (size_t)(nstar ? nstar : 1));
+	if (!*cen || !*vcm) { fprintf(stderr, "skid: out of memory\n"); fclose(fp); return 1; }
+	for (i = 0; i < S->n; ++i) if (grp[i] == 1) m1 += S->mass[i];
+	for (i = 0; i < nstar; ++i) {
+		if (fread(rec, 44, 1, fp) != 1) { fprintf(stderr, "ERROR: short file %s\n", path); fclose(fp); return 1; }
+		/* mass pos[3] vel[3] metals tform eps phi; the first group's mass is
+		   checked against the .grp (kd.c:1141-1149) */
+		if (i == 0 && rdf(rec, xdr) != (float)m1) {
+			fprintf(stderr, "WARNING: Mismatch in group masses between\n");
+			fprintf(stderr, "         the .grp and .gtp files.\n");
+			fprintf(stderr, "         %g vs. %g\n", rdf(rec, xdr), (float)m1);
+		}
+		for (k = 0; k < 3; ++k) {
+			(*cen)[3 * i + k] = rdf(rec + 4 + 4 * k, xdr);
+			(*vcm)[3 * i + k] = rdf(rec + 16 + 4 * k, xdr);
+		}
+	}
+	fclose(fp);
+	return 0;
+}
+
 int main(int argc, char **argv)
 {
 	skg_params P;
@@ -165,7 +233,8 @@ int main(int argc, char **argv)
 	int bUnbindOnly = 0, bPlain = 0, i, rc, nActive, nScat, nIttr, nGroup, nUnb, nExtra;
 	float fTau = 0, z = 0, Omega0 = 1, G = 1, H0 = 0, Lambda = 0, fQuintess = 0, fCvg = 0, fScoop = 0, fEps = 0;
 	float fStep;
-	char achName[256] = "skid", achFile[512];
+	char achName[256] = "skid", achFile[512], achGroup[256] = "";
+	float *gcen = NULL, *gvcm = NULL;
 	double t0, t1, tDen = 0, tMove = 0, tFof = 0, tMicro = 0, tUnb = 0;
 	float *rho = NULL, *r = NULL;
 	int *grp;
@@ -186,7 +255,11 @@ int main(int argc, char **argv)
 		else if (!strcmp(argv[i], "-nu")) { P.bNoUnbind = 1; ++i; }
 		else if (!strcmp(argv[i], "-gd")) { P.bGasAndDark = 1; ++i; }
 		else if (!strcmp(argv[i], "-go")) { P.bGasOnly = 1; ++i; }
-		else if (!strcmp(argv[i], "-unbind")) { if (++i >= argc) usage(); bUnbindOnly = 1; ++i; }
+		else if (!strcmp(argv[i], "-unbind")) {
+			if (++i >= argc) usage();
+			snprintf(achGroup, sizeof(achGroup), "%s", argv[i++]);
+			bUnbindOnly = 1;
+		}
 		else if (!strcmp(argv[i], "-spline")) { P.iSoftType = SKG_SPLINE; ++i; }
 		else if (!strcmp(argv[i], "-plummer")) { P.iSoftType = SKG_PLUMMER; ++i; }
 		else if (!strcmp(argv[i], "-e")) { if (++i >= argc) usage(); fEps = atof(argv[i++]); bEps = 1; }
@@ -212,10 +285,6 @@ int main(int argc, char **argv)
 #undef ARG_I
 	}
 	if (!bTau) usage();                                     /* main.c:339 */
-	if (bUnbindOnly) {
-		fprintf(stderr, "skid: -unbind re-entry is not provided by this build\n");
-		return 1;
-	}
 	if (!bCvg) fCvg = 0.5*fTau;                             /* main.c:343-345 */
 	if (!bScoop) fScoop = 2.0*fTau;
 	fStep = 0.5*fCvg;
@@ -237,6 +306,16 @@ int main(int argc, char **argv)
 	grp = malloc(4*(size_t)S.n);
 	if (!rho || !grp) { fprintf(stderr, "skid: out of memory\n"); return 1; }
 
+	if (bUnbindOnly) {                                      /* main.c:349-373 */
+		size_t l = strlen(achGroup);
+		if (l >= 4 && !strcmp(achGroup + l - 4, ".grp")) achGroup[l - 4] = 0;
+		snprintf(achFile, sizeof(achFile), "%s.grp", achGroup);
+		if (read_grp(achFile, S.n, grp, &nGroup)) return 1;
+		snprintf(achFile, sizeof(achFile), "%s.gtp", achGroup);
+		if (read_gtp(achFile, bStandard, nGroup, &S, grp, &gcen, &gvcm)) return 1;
+		if ((rc = skg_set_groups(ctx, grp, gcen, gvcm))) die_skg("skg_set_groups", rc);
+		goto unbind_only;
+	}
 	t0 = now_s();
 	if ((rc = skg_density(ctx, NULL, rho, &nExtra))) die_skg("skg_density", rc);
 	t1 = now_s(); tDen = t1 - t0;
@@ -296,6 +375,7 @@ int main(int argc, char **argv)
 		fclose(fp);
 	}
 	if ((rc = skg_centers(ctx))) die_skg("skg_centers", rc);
+unbind_only:
 	printf("Groups before Unbind:%d\n", nGroup - 1);
 	t0 = now_s();
 	if ((rc = skg_unbind(ctx, &nUnb))) die_skg("skg_unbind", rc);
@@ -349,14 +429,16 @@ int main(int argc, char **argv)
 	}
 	if (bOutStats) fprintf(stderr, "skid: -stats output is not provided by this build\n");
 	printf("SKID GPU wall time:\n");
-	printf("   Initial Density:    %.6f\n", tDen);
-	printf("   Moving Particles:   %.6f\n", tMove);
-	printf("   Friends of Friends: %.6f\n", tFof);
-	printf("   Microstepping:      %.6f\n", tMicro);
+	if (!bUnbindOnly) {
+		printf("   Initial Density:    %.6f\n", tDen);
+		printf("   Moving Particles:   %.6f\n", tMove);
+		printf("   Friends of Friends: %.6f\n", tFof);
+		printf("   Microstepping:      %.6f\n", tMicro);
+	}
 	if (!P.bNoUnbind) printf("   Unbinding:          %.6f\n", tUnb);
 	fflush(stdout);
 	skg_destroy(ctx);
-	free(rho); free(grp); free(r);
+	free(rho); free(grp); free(r); free(gcen); free(gvcm);
 	free(S.pos); free(S.vel); free(S.mass); free(S.soft); free(S.temp);
 	return 0;
 }
