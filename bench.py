@@ -23,6 +23,12 @@ nBucket as tuned, tau = 0.16 N_box^(-1/3) per box.  Three forms are timed:
                spread across the size range.
 
 --workload two-species runs the 2^24 gas + 2^24 dark box instead (one box).
+--workload unbind runs config 5: 2^27 particles, 60,000 groups with membership
+dN/dn ~ n^-1.9 on [8, 2e6] (NFW spheres, virial ratios in [0.5, 2]) in a uniform
+background, flat Lambda (Om 0.3, OL 0.7) at z = 0.5, entered through the -unbind
+re-entry (skg_set_groups): a step is the FP64 per-group energies plus the
+unbinding and kdTooSmall of every group; CPU baseline: the reference's -unbind
+path on the groups of at most 2e4 members of a smaller draw.
 """
 from __future__ import annotations
 
@@ -246,12 +252,85 @@ def bench_two_species(args):
     return box.n / (wall / args.steps), wall, config, roofline(kst), cpu
 
 
+def bench_unbind(args):
+    import numpy as np
+    from skid_amd import gen, skg
+    n = 1 << args.log2_n
+    t = time.time()
+    box, grp = gen.config5(n=n, n_groups=args.groups)
+    sz = np.bincount(grp)[1:]
+    pairs = float((sz * (sz - 1.0)).sum())
+    log(f"generated {n} particles, {args.groups} groups ({sz.sum()} members, largest {sz.max()}, "
+        f"sum n(n-1) {pairs:.3g}) in {time.time() - t:.1f}s")
+    tau = 0.16 * n ** (-1.0 / 3.0)
+    fs, fc = gen.config5_cosmology()
+    P = skg.make_params(tau, 32, period=1.0, z=0.5, fShift=fs, fCosmo=fc)
+    ctx = skg.Context(box, P)
+
+    def one():
+        ng = ctx.set_groups(grp)
+        W, T = ctx.group_energies(ng)
+        nu = ctx.unbind()
+        ctx.too_small()
+        g, ngf = ctx.groups()
+        return nu, ngf, W, T
+
+    for w in range(args.warmup):
+        t0 = time.perf_counter()
+        nu, ngf, W, T = one()
+        log(f"warmup {w}: {time.perf_counter() - t0:.3f}s unbound={nu} groups={ngf - 1}")
+    ctx.set_profiling(True)
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        ts = time.perf_counter()
+        nu, ngf, W, T = one()
+        log(f"step {s}: {time.perf_counter() - ts:.3f}s unbound={nu} groups={ngf - 1}")
+    wall = time.perf_counter() - t0
+    kst = ctx.kernel_stats()
+    log("kernels:", json.dumps({k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
+                                 for k, v in kst.items()}))
+    ctx.close()
+    members = int(sz.sum())
+    # the potential is the VALU-bound kernel; report its roofline
+    d = kst["unbind_potential"]
+    avg = d["ms"] / max(1, d["launches"])
+    ach = d["work"] / max(1, d["launches"]) / (avg * 1e-3) / 1e12
+    roof = dict(bound="valu", achieved=ach, peak=FP32_PEAK_TFLOPS, unit="TFLOP/s", frac=ach / FP32_PEAK_TFLOPS,
+                traffic=None, kernel="unbind_potential", avg_launch_ms=avg, launches=d["launches"],
+                note="20 flops per ordered member pair; the removal loops (FP64 test) are latency-bound")
+    cpu = None
+    if args.cpu_boxes > 0:
+        # the reference's -unbind path (main.c:349-373) on a 2^20 draw with groups capped at 2e4 members
+        nb = 1 << 20
+        b2, g2 = gen.config5(n=nb, n_groups=500, seed=6, n_range=(8, 20000))
+        kw = dict(nSmooth=32, fTau=0.16 * nb ** (-1.0 / 3.0), fPeriod=1.0, z=0.5, Omega0=0.3, Lambda=0.7,
+                  H0=2.8944)
+        from oracle import oracle as O
+        if O.have_ref():
+            pin_one_core()
+            t1 = time.perf_counter()
+            O.ref_unbind(b2, g2, **kw)
+            dt = time.perf_counter() - t1
+            m2 = int((g2 > 0).sum())
+            cpu = dict(value=m2 / dt, unit="group members/s", cores=1, kind="reference",
+                       sample=f"reference -unbind path on a 2^20 draw of the same recipe with 500 groups of "
+                              f"8..2e4 members ({m2} members), {dt:.2f}s on 1 core")
+    value = members / (wall / args.steps)
+    config = {"workload": f"unbinding stress: {n} particles, {args.groups} power-law groups ({members} members, "
+                          f"largest {int(sz.max())}, sum n(n-1) {pairs:.3g}), z=0.5 flat Lambda, -unbind re-entry, "
+                          f"FP32 compensated potentials, FP64 energies and sign test",
+              "n_particles": n, "members": members, "form": "unbind"}
+    return value, wall, config, roof, cpu
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=("batched", "two-species"), default="batched")
+    ap.add_argument("--workload", choices=("batched", "two-species", "unbind"), default="batched")
+    ap.add_argument("--log2-n", type=int, default=27, help="unbind: particles = 2^this")
+    ap.add_argument("--groups", type=int, default=60000, help="unbind: number of groups")
     ap.add_argument("--boxes", type=int, default=256)
     ap.add_argument("--log2-each", type=int, default=24, help="two-species: particles per species = 2^this")
     ap.add_argument("--plain-every", type=int, default=4, help="plain GPU form on every n-th box (0 skips)")
@@ -264,11 +343,13 @@ def main():
         sys.exit(2)
     if args.workload == "batched":
         value, wall, config, roof, cpu = bench_batched(args)
+    elif args.workload == "unbind":
+        value, wall, config, roof, cpu = bench_unbind(args)
     else:
         value, wall, config, roof, cpu = bench_two_species(args)
     out = {
         "metric": "end-to-end particles/s at 2^24 (adjoint+batched vs plain GPU vs CPU); kNN q/s",
-        "value": value, "unit": "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "value": value, "unit": "group members/s" if args.workload == "unbind" else "particles/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1000.0 * wall / args.steps, "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": config,
         "roofline": roof, "cpu_baseline": cpu,

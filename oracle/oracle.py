@@ -144,6 +144,9 @@ def ref():
                                   _f, _f, C.POINTER(C.c_int)]
         R.ref_pipeline.argtypes = [C.POINTER(RefParams), C.c_int, C.c_int, C.c_int, _f, _f, _f,
                                    _f, _f, C.POINTER(RefOut)]
+        _i = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        R.ref_unbind.argtypes = [C.POINTER(RefParams), C.c_int, C.c_int, C.c_int, _f, _f, _f, _f, _f, _i, _i,
+                                 C.POINTER(C.c_int)]
         _ref = R
     return _ref
 
@@ -272,6 +275,19 @@ def ref_pipeline(box, max_trace=100000, max_group=1 << 20, **kw):
                grpVcm=gv[: 3 * ng].reshape(-1, 3))
     out["rMoved"] = out["rMoved"].reshape(-1, 3)
     out["rMicro"] = out["rMicro"].reshape(-1, 3)
+    return out
+
+
+def ref_unbind(box, grp, **kw):
+    """The reference's -unbind path on caller groups (centres of mass); returns
+    the final group ids after kdUnbind + kdTooSmall."""
+    p = params(**kw)
+    R = ref()
+    out = np.empty(box.n, np.int32)
+    ng = C.c_int()
+    R.ref_unbind(C.byref(_fill(RefParams, p)), box.n_gas, box.n_dark, box.n_star, box.pos.reshape(-1),
+                 box.vel.reshape(-1), box.mass, box.soft, box.temp, np.ascontiguousarray(grp, np.int32), out,
+                 C.byref(ng))
     return out
 
 

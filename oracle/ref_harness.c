@@ -209,3 +209,37 @@ int ref_pipeline(const REF_PARAMS *P, int nGas, int nDark, int nStar,
 	kdFinish(kd);
 	return 0;
 	}
+
+/* The -unbind re-entry: main.c:349-373 then 457-471, with the group ids from
+   the caller instead of a .grp file and no .gtp (so kdReadCenter takes the
+   centres of mass, kd.c:1160-1193).  grpOut receives the final ids by iOrder. */
+int ref_unbind(const REF_PARAMS *P, int nGas, int nDark, int nStar,
+		const float *pos, const float *vel, const float *mass, const float *soft,
+		const float *temp, const int *grpIn, int *grpOut, int *nGroupOut)
+{
+	KD kd;
+	float fPeriod[3], fCenter[3], fScoop;
+	int i, j, mx = 0;
+	for (j = 0; j < 3; ++j) { fPeriod[j] = P->bPeriodic ? P->fPeriod : FLT_MAX; fCenter[j] = P->fCenter[j]; }
+	fScoop = (P->fScoop > 0) ? P->fScoop : 2.0*P->fTau;
+	kdInit(&kd, P->nBucket, fPeriod, fCenter, 0);
+	ref_load(kd, nGas, nDark, nStar, pos, vel, mass, soft, temp);
+	/* kdInGroup (kd.c:920-962) from memory */
+	kd->piGroup = (int *)malloc(kd->nParticles*sizeof(int));
+	for (i = 0; i < kd->nParticles; ++i) {
+		kd->piGroup[i] = grpIn[i];
+		if (grpIn[i] > mx) mx = grpIn[i];
+		}
+	kd->nGroup = mx + 1;
+	kdInitpGroup(kd);
+	kdReadCenter(kd, "", 0);
+	kdSetUniverse(kd, P->G, P->Omega0, P->Lambda, P->fQuintess, P->H0, P->z);
+	if (P->fEps > 0) kdSetSoft(kd, P->fEps);
+	kdUnbind(kd, P->iSoftType, fScoop, P->bGasAndDark, P->bNoUnbind,
+			P->nMaxMembers > 0 ? P->nMaxMembers : INT_MAX);
+	kdTooSmall(kd, P->nMembers);
+	memcpy(grpOut, kd->piGroup, kd->nParticles*sizeof(int));
+	if (nGroupOut) *nGroupOut = kd->nGroup;
+	kdFinish(kd);
+	return 0;
+	}

@@ -78,3 +78,33 @@ def test_fof_bfs_partition_equals_brute(box1):
     pos = r["rMoved"]
     bg, _ = O.fof_brute(pos, 0.005)
     assert O.same_partition(r["iFofGroup"], bg)
+
+
+def test_fp64_unbinding_matches_reference_unbind_path():
+    """The -unbind re-entry of the real reference (centres of mass, scoop of
+    the background, flat Lambda at z = 0.5) against the FP64 restatement that
+    pins the GPU unbinding: the same bound/unbound status for >= 99.9% of the
+    members of a power-law group draw."""
+    n = 1 << 16
+    box, grp = gen.config5(n=n, n_groups=150, seed=7, n_range=(8, 3000))
+    tau = 0.16 * n ** (-1.0 / 3.0)
+    kw = dict(nSmooth=32, fTau=tau, fPeriod=1.0, z=0.5, Omega0=0.3, Lambda=0.7, H0=2.8944, nMembers=1)
+    rg = O.ref_unbind(box, grp, **kw)
+    fs, fc = gen.config5_cosmology()
+    # centres of mass as kdReadCenter forms them: relative to the largest-iOrder member
+    cen = {}
+    for g in range(1, grp.max() + 1):
+        mem = np.flatnonzero(grp == g)
+        rel = box.pos[mem.max()].astype(np.float64)
+        d = box.pos[mem].astype(np.float64) - rel
+        d -= np.round(d)
+        m = box.mass[mem].astype(np.float64)
+        c = rel + (m[:, None] * d).sum(0) / m.sum()
+        cen[g] = c - np.round(c)
+    u = O.unbind_f64(box.pos, box.vel, box.mass, box.soft, grp, cen, fshift=fs, fcosmo=fc, z=0.5,
+                     scoop=2.0 * tau, sub_pot=True)
+    inm = grp > 0
+    same = ((rg > 0) == (u > 0))[inm].mean()
+    print(f"reference -unbind vs FP64: unbound {int(((rg == 0) & inm).sum())} vs {int(((u == 0) & inm).sum())}, "
+          f"same status {same:.5f}")
+    assert same >= 0.999
