@@ -20,6 +20,8 @@
 // Wave-per-tile kernels run four independent wavefronts per workgroup: a
 // one-wave workgroup would cap residency at the per-CU workgroup limit.
 #define SKG_TILE_T 256
+// records allocated past the last entity of e_pos / e_aux (reads in blocks of 4)
+#define SKG_EPAD 4
 
 // Per-box constants of a batched context (box b = particles [off, off+n) by
 // iOrder).  A single-box context has one entry holding the parameter block's

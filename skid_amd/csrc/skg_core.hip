@@ -552,12 +552,13 @@ __device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const 
 		Sum<COMP> &s2) {
 	if (cnt <= 0 || !want) return;
 	for (int t0 = 0; t0 < cnt; t0 += 4) {
+		// four consecutive records in one scalar load each; the entity arrays
+		// carry SKG_EPAD records past the last entity, so the reads past a
+		// leaf's end stay in the allocation (their values are never used)
+		const float4 *ap = epos + start + t0, *xp = eaux + start + t0;
 		float4 A4[4], X4[4];
 #pragma unroll
-		for (int u = 0; u < 4; ++u) {
-			const int e = min(start + t0 + u, start + cnt - 1);
-			A4[u] = epos[e]; X4[u] = eaux[e];
-		}
+		for (int u = 0; u < 4; ++u) { A4[u] = ap[u]; X4[u] = xp[u]; }
 #pragma unroll
 		for (int u = 0; u < 4; ++u) {
 			if (t0 + u >= cnt) break;
@@ -597,9 +598,9 @@ __device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const 
 // far side) or with more than cap leaves gets count -1 and walks the tree.
 __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__restrict__ ppos,
 		const int *__restrict__ plist, int np, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi,
-		int nLeaf2, const int *__restrict__ pbox, float margin, int per, float cx, float cy, float cz,
-		float Lx, float Ly, float Lz, int cap, int *__restrict__ tlist, int *__restrict__ tcnt,
-		const int *__restrict__ npDev) {
+		int nLeaf2, const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro, float msteps,
+		float mabs, int per, float cx, float cy, float cz, float Lx, float Ly, float Lz, int cap,
+		int *__restrict__ tlist, int *__restrict__ tcnt, const int *__restrict__ npDev) {
 	if (npDev) np = *npDev;
 	const int lane = threadIdx.x & 63;
 	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
@@ -610,6 +611,8 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__rest
 	const int pi = plist ? plist[valid ? ii : base] : (valid ? ii : base);
 	const float4 p = ppos[pi];
 	const int pb = pbox ? pbox[pi] : 0;
+	// each lane's margin: msteps (1.02 x the block's steps) of its own box's step
+	const float margin = fmaf(msteps, micro ? bt[pb].fMicro : bt[pb].fStep, mabs);
 	bool nearFace = false;
 	if (per && valid)
 		nearFace = fabsf(p.x - cx) > 0.5f * Lx - margin || fabsf(p.y - cy) > 0.5f * Ly - margin ||
@@ -672,9 +675,14 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover(
 	int node = nl >= 0 ? 0 : 1;
 	for (int k = 0; k < nl; ++k) {
 		const int leaf = uniform_int(tlist[(size_t)tile * tcap + k]);
+		// the list holds what any of the block's steps can reach; this step
+		// scans a leaf only where the walk below would open it (leaves hold
+		// one box each, and a tile may straddle two boxes)
+		const float4 lo = nlo[nLeaf2 + leaf], hi = nhi[nLeaf2 + leaf];
+		const bool want = valid && (!pbox || brange_has(hi.w, pb)) &&
+				point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
+		if (!__any(want)) continue;
 		const int2 rg = leaf_range(lstart, leaf, B, ne);
-		// leaves hold one box each; a tile may straddle two boxes
-		const bool want = valid && (!pbox || brange_has(nhi[nLeaf2 + leaf].w, pb));
 		cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), want, p, epos, eaux, eorg, ehit, stamp,
 				Lx, Ly, Lz, s0, s1, s2);
 	}
@@ -1588,7 +1596,7 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		c->nE = nE;
 		float *eb2s = nullptr;
 		unsigned long long *ekey = nullptr;
-		if (dalloc(&ekey, nE) || dalloc(&eb2s, nE) || dalloc(&c->e_pos, nE) || dalloc(&c->e_aux, nE) || dalloc(&c->e_rho, nE) ||
+		if (dalloc(&ekey, nE) || dalloc(&eb2s, nE) || dalloc(&c->e_pos, nE + SKG_EPAD) || dalloc(&c->e_aux, nE + SKG_EPAD) || dalloc(&c->e_rho, nE) ||
 		    dalloc(&c->e_org, nE) || dalloc(&c->e_src, nE) || dalloc(&c->e_live, nE) || dalloc(&c->e_hit, nE) ||
 		    dalloc(&c->e_box, nE) || dalloc(&c->d_eblk, nBox + 1))
 			return SKG_E_NOMEM;
@@ -1674,16 +1682,14 @@ static int count_active(skg_ctx *c, long cap) {
 }
 
 // Leaf lists for the next nsteps steps (margin: the distance a mover can
-// travel in nsteps steps of the largest box step).
+// travel in nsteps steps of its own box's step, a lane at a time).
 static void enqueue_lists(skg_ctx *c, long cap, int nsteps, int micro) {
-	float smax = 0.f;
-	for (int b = 0; b < c->nBox; ++b) smax = std::max(smax, c->hbox[b].step);
-	if (micro) smax = (float)(0.1 * (double)smax);
-	const float margin = 1.02f * nsteps * smax + 1e-6f * c->P.fPeriod[0] * (c->P.bPeriodic ? 1.f : 0.f);
+	const float mabs = 1e-6f * c->P.fPeriod[0] * (c->P.bPeriodic ? 1.f : 0.f);
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
 	k_cover_lists<<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap,
-			c->te.lo, c->te.hi, c->te.nLeaf2, c->nBox > 1 ? c->m_box : nullptr, margin, c->P.bPeriodic,
-			C[0], C[1], C[2], L[0], L[1], L[2], c->tlCap, c->tl_list, c->tl_cnt, c->d_ml + 1);
+			c->te.lo, c->te.hi, c->te.nLeaf2, c->nBox > 1 ? c->m_box : nullptr, c->d_bt, micro, 1.02f * nsteps,
+			mabs, c->P.bPeriodic, C[0], C[1], C[2], L[0], L[1], L[2], c->tlCap, c->tl_list, c->tl_cnt,
+			c->d_ml + 1);
 }
 
 // nsteps Denmax steps over the active list (at most cap movers).
