@@ -75,6 +75,10 @@ int skg_create(const skg_params *P, skg_ctx **out);
 /* Replaces kdFinish (kd.c:1842-1850) + smFinish (smooth1.c:32-38). */
 int skg_destroy(skg_ctx *ctx);
 const char *skg_strerror(int status);
+/* After SKG_E_HIP: the failing HIP call, its error string and source line, for
+   the calling host thread (empty before any failure).  No reference
+   counterpart: SKID's CPU code has no runtime errors of this kind. */
+const char *skg_last_error(void);
 
 /* Particle input in tipsy species order, n = nGas + nDark + nStar; replaces the
    array fill of kdReadTipsy (kd.c:122-222).  pos/vel are n*3 floats; vel, temp
@@ -161,6 +165,11 @@ int skg_unbind(skg_ctx *ctx, int *nUnbound);
 
 /* kdTooSmall (kd.c:1251-1293). */
 int skg_too_small(skg_ctx *ctx);
+/* Diagnostics of the last skg_unbind(): groups unbound by the cooperative
+   all-GPU kernel, cooperative launches the device refused (those groups then
+   went to the one-workgroup kernel, with the same result), and full passes of
+   the bounded-candidate kernels.  No reference counterpart. */
+int skg_unbind_info(skg_ctx *ctx, int *nBig, int *nCoopRefused, long long *nFullPasses);
 
 /* Group ids by iOrder and the group count (groups + 1, as kd->nGroup). */
 int skg_get_groups(skg_ctx *ctx, int *grp, int *nGroup);

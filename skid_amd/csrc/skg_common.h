@@ -105,7 +105,7 @@ struct skg_ctx {
 	int *d_cnt = nullptr;        // small counters
 
 	// groups
-	int nGroup = 0, nBigGroups = 0;
+	int nGroup = 0, nBigGroups = 0, nCoopFallbacks = 0;
 	long long nFullPasses = 0;
 	struct GroupState *gs = nullptr;
 
@@ -145,7 +145,11 @@ struct skg_ctx {
 };
 
 // ---------------------------------------------------------------- errors
-#define SKG_HIP(x) do { hipError_t _e = (x); if (_e != hipSuccess) return SKG_E_HIP; } while (0)
+// A failing HIP call returns SKG_E_HIP; the call, its error and the place are
+// kept per host thread for skg_last_error().
+void skg_note_hip(hipError_t e, const char *call, const char *file, int line);
+#define SKG_HIP(x) do { hipError_t _e = (x); if (_e != hipSuccess) { skg_note_hip(_e, #x, __FILE__, __LINE__); \
+		return SKG_E_HIP; } } while (0)
 
 // ---------------------------------------------------------------- wave helpers
 __device__ __forceinline__ float wave_min(float v) {

@@ -1257,6 +1257,14 @@ extern "C" void skg_default_params(skg_params *P) {
 	P->accumulate = SKG_ACC_COMPENSATED;
 }
 
+static thread_local char g_last_err[512];
+void skg_note_hip(hipError_t e, const char *call, const char *file, int line) {
+	const char *f = strrchr(file, '/');
+	snprintf(g_last_err, sizeof(g_last_err), "%s (%d) from %.200s at %s:%d", hipGetErrorString(e), (int)e, call,
+			f ? f + 1 : file, line);
+}
+extern "C" const char *skg_last_error(void) { return g_last_err; }
+
 extern "C" const char *skg_strerror(int s) {
 	switch (s) {
 	case SKG_OK: return "ok";

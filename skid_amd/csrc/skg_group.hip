@@ -16,6 +16,7 @@
 #include <cfloat>
 #include <climits>
 #include <cstdlib>
+#include <string>
 #include <vector>
 
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
@@ -737,6 +738,298 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 	}
 }
 
+// Large groups, all at once in one cooperative launch, with the bounded
+// candidates of k_unb_cand spread over many workgroups.  Group gi owns
+// workgroups [wg0, wg0 + nwg), each a fixed slice of its members.  A full
+// pass (every workgroup over its slice) applies the kdSubPot subtractions of
+// the removals since the last pass, keeps the slice's cw most energetic
+// members and bounds the rest (E_cut, R = max |u - w0|, as in k_unb_cand).
+// After a grid barrier every workgroup of the group holds all nwg*cw
+// candidates and runs the same removal sequence on them -- the same inputs,
+// reductions and decisions in each -- so a removal needs no barrier until the
+// bound fails and the next full pass is due.  A round is [removals | full
+// pass] barrier [read the candidates | finish] barrier [leave when no group is
+// unfinished]; every workgroup passes both barriers of every round.
+#define CPC 8            // candidate slots per thread (COOP_T * CPC per group)
+#define CPEND 2048       // removals held between full passes
+struct CoopCand { float4 q, v; double phi; int idx, pad; };
+struct CoopHdr { double ecut, dmax, emin; int imin; float epsmin; };
+
+__device__ __forceinline__ void blk_argmax(double &e, int &i, double *sE, int *sI) {
+	for (int o = 32; o > 0; o >>= 1) {
+		const double oe = __shfl_xor(e, o, 64);
+		const int oi = __shfl_xor(i, o, 64);
+		if (oe > e || (oe == e && oi < i)) { e = oe; i = oi; }
+	}
+	__syncthreads();                       // readers of the previous call are done
+	if ((threadIdx.x & 63) == 0) { sE[threadIdx.x >> 6] = e; sI[threadIdx.x >> 6] = i; }
+	__syncthreads();
+	e = sE[0]; i = sI[0];
+	for (int k = 1; k < COOP_T / 64; ++k)
+		if (sE[k] > e || (sE[k] == e && sI[k] < i)) { e = sE[k]; i = sI[k]; }
+}
+__device__ __forceinline__ void blk_argmin(double &e, int &i, double *sE, int *sI) {
+	e = -e;
+	blk_argmax(e, i, sE, sI);
+	e = -e;
+}
+__device__ __forceinline__ double blk_max(double e, double *sE) {
+	int i = 0;
+	blk_argmax(e, i, sE, (int *)(sE + COOP_T / 64));
+	return e;
+}
+
+__global__ __launch_bounds__(COOP_T) void k_unb_coopc(const int *__restrict__ bg, const int *__restrict__ wg0,
+		const int *__restrict__ nwgG, const int *__restrict__ wgGroup, const int *__restrict__ cwG,
+		const int *__restrict__ gs, const int *__restrict__ ge, const float4 *__restrict__ uq,
+		const float4 *__restrict__ uv, float2 *uphi, int *uord, int *grp, int *gcount, float *gmass, float4 *gvcm,
+		float4 *grbound, const float4 *__restrict__ grel, float fShift, float fCosmo, float z, float G, int soft,
+		int bSub, int bNoUnbind, float hx, int *nunb, CoopCand *pub, CoopHdr *hdr,
+		const double *__restrict__ cmInit, int *active, unsigned long long *npairs, unsigned long long *nfull) {
+	namespace cg = cooperative_groups;
+	cg::grid_group grid = cg::this_grid();
+	__shared__ float4 pq[CPEND];           // pending removals: x, y, z, m
+	__shared__ float pe[CPEND];            // and softening
+	__shared__ double sE[2 * COOP_T / 64];
+	__shared__ int sI[COOP_T / 64];
+	__shared__ float4 bq, bv;
+	const int tid = threadIdx.x, wg = blockIdx.x;
+	const int gi = wgGroup[wg], g = bg[gi], lw = wg - wg0[gi], nwg = nwgG[gi], cw = cwG[gi];
+	const int s = gs[g], n0 = ge[g] - s;
+	const int chunk = (n0 + nwg - 1) / nwg;
+	const int a0 = min(n0, lw * chunk), a1 = min(n0, a0 + chunk);
+	const int K = nwg * cw;                      // <= COOP_T * CPC (host)
+	CoopCand *gp = pub + (size_t)gi * COOP_T * CPC;
+	const double zf = 1.0 + (double)z;
+	double cm[7];
+	for (int k = 0; k < 7; ++k) cm[k] = cmInit[8 * gi + k];
+	int cI[CPC];
+	float4 cQ[CPC], cV[CPC];
+	double cP[CPC];
+#pragma unroll
+	for (int c = 0; c < CPC; ++c) { cI[c] = -1; cQ[c] = cV[c] = make_float4(0.f, 0.f, 0.f, 0.f); cP[c] = 0.0; }
+	int n = n0, removed = 0, npend = 0, imin = 0;
+	unsigned long long pairs = 0, fulls = 0;
+	double w0x = 0, w0y = 0, w0z = 0, Ecut = -INFINITY, Rmax = 0, grow = 0;
+	float epsMin = 0.f;
+	bool done = false, needFull = true, lastWasFull = false, fin = false, finishNow = false;
+	for (;;) {
+		// ---- removals from the candidates (block-uniform control flow)
+		if (!done && !needFull) {
+			for (;;) {
+				double bE = -INFINITY;
+				int bi = INT_MAX;
+#pragma unroll
+				for (int c = 0; c < CPC; ++c) {
+					if (cI[c] < 0) continue;
+					const double d0 = (double)fShift * ((double)cV[c].x - cm[4]) + (double)fCosmo * ((double)cQ[c].x - cm[1]);
+					const double d1 = (double)fShift * ((double)cV[c].y - cm[5]) + (double)fCosmo * ((double)cQ[c].y - cm[2]);
+					const double d2 = (double)fShift * ((double)cV[c].z - cm[6]) + (double)fCosmo * ((double)cQ[c].z - cm[3]);
+					const double E = 0.5 * (d0 * d0 + d1 * d1 + d2 * d2) - cP[c] * zf;
+					if (E > bE || (E == bE && cI[c] < bi)) { bE = E; bi = cI[c]; }
+				}
+				blk_argmax(bE, bi, sE, sI);
+				const double BE = bE;
+				const int BI = bi;
+				const double wx = (double)fShift * cm[4] + (double)fCosmo * cm[1];
+				const double wy = (double)fShift * cm[5] + (double)fCosmo * cm[2];
+				const double wz = (double)fShift * cm[6] + (double)fCosmo * cm[3];
+				const double dl = sqrt((wx - w0x) * (wx - w0x) + (wy - w0y) * (wy - w0y) + (wz - w0z) * (wz - w0z));
+				const double slack = 1e-12 * (fabs(Ecut) + Rmax * Rmax + 1.0);
+				const double bound = lastWasFull ? Ecut : Ecut + Rmax * dl + 0.5 * dl * dl + grow + slack;
+				if (!(lastWasFull ? BE >= bound : BE > bound)) {
+					if (BE < 0.0 && bound < 0.0) { fin = true; if (lastWasFull) finishNow = true; }   // every member is bound
+					needFull = !finishNow;
+					break;
+				}
+				if (BE < 0.0) {                                       // the global maximum is negative
+					fin = true;
+					if (lastWasFull) finishNow = true;
+					needFull = !finishNow;
+					break;
+				}
+				lastWasFull = false;
+				// remove BI: its slot publishes the record; the workgroup whose
+				// slice holds BI marks it (it is the one that scans it next)
+#pragma unroll
+				for (int c = 0; c < CPC; ++c)
+					if (cI[c] == BI) { bq = cQ[c]; bv = cV[c]; cI[c] = -1; }
+				if (tid == 0 && BI >= a0 && BI < a1) {
+					const int ob = uord[s + BI];
+					grp[ob] = 0;
+					uord[s + BI] = ~ob;
+				}
+				__syncthreads();
+				const float4 qb = bq, vb = bv;
+				--n; ++removed;
+				if (n == 0) { cm[0] = 0.0; cm[4] = cm[5] = cm[6] = 0.0; finishNow = true; break; }
+				cm[0] -= (double)qb.w;
+				const double f = (double)qb.w / cm[0];
+				cm[1] += f * (cm[1] - (double)qb.x); cm[2] += f * (cm[2] - (double)qb.y); cm[3] += f * (cm[3] - (double)qb.z);
+				cm[4] += f * (cm[4] - (double)vb.x); cm[5] += f * (cm[5] - (double)vb.y); cm[6] += f * (cm[6] - (double)vb.z);
+				if (bSub) {
+					pairs += n;
+#pragma unroll
+					for (int c = 0; c < CPC; ++c) {
+						if (cI[c] < 0) continue;
+						const float dx = fsub(qb.x, cQ[c].x), dy = fsub(qb.y, cQ[c].y), dz = fsub(qb.z, cQ[c].z);
+						cP[c] -= (double)fmul(fmul(G, qb.w), pot_kernel(soft, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), fadd(vb.w, cV[c].w)));
+					}
+					grow += zf * (double)fmul(fmul(G, qb.w), kernel_max(soft, fadd(vb.w, epsMin)));
+					if (tid == 0) { pq[npend] = qb; pe[npend] = vb.w; }
+					++npend;
+					if (npend >= CPEND) { needFull = true; break; }
+				}
+			}
+		}
+		// ---- full pass over this workgroup's slice
+		if (!done && needFull) {
+			__syncthreads();                                       // pq / pe complete
+			const double wx = (double)fShift * cm[4] + (double)fCosmo * cm[1];
+			const double wy = (double)fShift * cm[5] + (double)fCosmo * cm[2];
+			const double wz = (double)fShift * cm[6] + (double)fCosmo * cm[3];
+			double tE[CPC], tD[CPC];
+			int tI[CPC];
+#pragma unroll
+			for (int c = 0; c < CPC; ++c) { tE[c] = -INFINITY; tD[c] = 0.0; tI[c] = -1; }
+			double dispE = -INFINITY, dispD = 0.0, mE = INFINITY;
+			int mi = INT_MAX;
+			float eMin = INFINITY;
+			for (int i = a0 + tid; i < a1; i += COOP_T) {
+				if (uord[s + i] < 0) continue;
+				const float4 q = uq[s + i], v = uv[s + i];
+				float2 ph = uphi[s + i];
+				if (bSub && npend) {                                   // kdSubPot (grav.c:39-60)
+					Acc ac; ac.s = ph.x; ac.c = ph.y;
+					for (int k = 0; k < npend; ++k) {
+						const float4 b = pq[k];
+						const float dx = fsub(b.x, q.x), dy = fsub(b.y, q.y), dz = fsub(b.z, q.z);
+						ac.add(-fmul(fmul(G, b.w), pot_kernel(soft, fmaf(dx, dx, fmaf(dy, dy, dz * dz)), fadd(pe[k], v.w))));
+					}
+					ph = make_float2(ac.s, ac.c);
+					uphi[s + i] = ph;
+				}
+				const double d0 = (double)fShift * ((double)v.x - cm[4]) + (double)fCosmo * ((double)q.x - cm[1]);
+				const double d1 = (double)fShift * ((double)v.y - cm[5]) + (double)fCosmo * ((double)q.y - cm[2]);
+				const double d2 = (double)fShift * ((double)v.z - cm[6]) + (double)fCosmo * ((double)q.z - cm[3]);
+				const double dd = d0 * d0 + d1 * d1 + d2 * d2;
+				const double E = 0.5 * dd - ((double)ph.x + (double)ph.y) * zf;
+				if (E < mE || (E == mE && i < mi)) { mE = E; mi = i; }
+				eMin = fminf(eMin, v.w);
+				int lo = 0;
+				double loE = tE[0];
+#pragma unroll
+				for (int c = 1; c < CPC; ++c) if (tE[c] < loE) { loE = tE[c]; lo = c; }
+				double outE = E, outD = dd;
+				if (E > loE) {
+#pragma unroll
+					for (int c = 0; c < CPC; ++c)
+						if (c == lo) {
+							outE = tI[c] >= 0 ? tE[c] : -INFINITY;
+							outD = tI[c] >= 0 ? tD[c] : 0.0;
+							tE[c] = E; tD[c] = dd; tI[c] = i;
+						}
+				}
+				dispE = fmax(dispE, outE);
+				dispD = fmax(dispD, outD);
+			}
+			// the slice's cw most energetic members become candidates
+			for (int r = 0; r < cw; ++r) {
+				double e = -INFINITY;
+				int i = INT_MAX;
+#pragma unroll
+				for (int c = 0; c < CPC; ++c)
+					if (tI[c] >= 0 && (tE[c] > e || (tE[c] == e && tI[c] < i))) { e = tE[c]; i = tI[c]; }
+				blk_argmax(e, i, sE, sI);
+				CoopCand *slot = gp + lw * cw + r;
+				if (i == INT_MAX) {
+					if (tid == 0) slot->idx = -1;
+					continue;
+				}
+#pragma unroll
+				for (int c = 0; c < CPC; ++c)
+					if (tI[c] == i) {
+						tI[c] = -1;
+						const float2 ph = uphi[s + i];
+						slot->q = uq[s + i];
+						slot->v = uv[s + i];
+						slot->phi = (double)ph.x + (double)ph.y;
+						slot->idx = i;
+					}
+			}
+			// what stays behind is bounded by E_cut and R
+#pragma unroll
+			for (int c = 0; c < CPC; ++c)
+				if (tI[c] >= 0) { dispE = fmax(dispE, tE[c]); dispD = fmax(dispD, tD[c]); }
+			const double ecut = blk_max(dispE, sE);
+			const double dmax = blk_max(dispD, sE);
+			const double emin = -blk_max(-(double)eMin, sE);
+			blk_argmin(mE, mi, sE, sI);
+			if (tid == 0) {
+				CoopHdr h;
+				h.ecut = ecut; h.dmax = dmax; h.emin = mE; h.imin = mi; h.epsmin = (float)emin;
+				hdr[wg] = h;
+			}
+			w0x = wx; w0y = wy; w0z = wz;
+		}
+		grid.sync();
+		__threadfence();
+		// ---- every workgroup of the group takes all its candidates and bounds
+		if (!done && needFull) {
+#pragma unroll
+			for (int c = 0; c < CPC; ++c) {
+				const int j = tid + c * COOP_T;
+				if (j < K) {
+					const CoopCand r = gp[j];
+					cI[c] = r.idx; cQ[c] = r.q; cV[c] = r.v; cP[c] = r.phi;
+				} else {
+					cI[c] = -1;
+				}
+			}
+			double ec = -INFINITY, dm = 0.0, em = -INFINITY, me = INFINITY;
+			int mi = INT_MAX;
+			for (int w = tid; w < nwg; w += COOP_T) {
+				const CoopHdr h = hdr[wg0[gi] + w];
+				ec = fmax(ec, h.ecut);
+				dm = fmax(dm, h.dmax);
+				em = fmax(em, -(double)h.epsmin);
+				if (h.emin < me || (h.emin == me && h.imin < mi)) { me = h.emin; mi = h.imin; }
+			}
+			Ecut = blk_max(ec, sE);
+			Rmax = sqrt(blk_max(dm, sE));
+			epsMin = (float)(-blk_max(em, sE));
+			blk_argmin(me, mi, sE, sI);
+			imin = mi;
+			grow = 0.0; npend = 0; needFull = false; lastWasFull = true; ++fulls;
+			if (fin || bNoUnbind || n == 0) finishNow = true;
+		}
+		if (!done && finishNow) {
+			done = true;
+			if (lw == 0 && tid == 0) {
+				gcount[g] = n;
+				gmass[g] = (float)cm[0];
+				gvcm[g] = make_float4((float)cm[4], (float)cm[5], (float)cm[6], 0.f);
+				const float4 q = uq[s + (imin >= 0 && imin < n0 ? imin : 0)], r = grel[g];
+				float b[3] = {fadd(q.x, r.x), fadd(q.y, r.y), fadd(q.z, r.z)};
+				for (int j = 0; j < 3; ++j) {                            // hx on all axes (kd.c:1452-1456)
+					if (b[j] > hx) b[j] = fsub(b[j], 2 * hx);
+					if (b[j] <= -hx) b[j] = fadd(b[j], 2 * hx);
+				}
+				grbound[g] = make_float4(b[0], b[1], b[2], 0.f);
+				atomicAdd(nunb, removed);
+				atomicAdd(npairs, pairs);
+				atomicAdd(nfull, fulls);
+				__threadfence();
+				atomicSub(active, 1);
+			}
+		}
+		// groups finish only between the two barriers, so every workgroup
+		// reads the same count here and all leave on the same round
+		grid.sync();
+		if (__hip_atomic_load(active, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0) break;
+	}
+}
+
 // Mass, centre of mass and velocity of one group in fp64 (kd.c:1360-1375).
 __global__ __launch_bounds__(256) void k_group_cm(const int *__restrict__ bg, const int *__restrict__ gs,
 		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv, double *outAll) {
@@ -1209,8 +1502,13 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		skg_prof_end(c, lslot, SKG_K_UNB_LOOP, 20.0 * (double)hp[0]);
 		const int bslot = skg_prof_begin(c);
 		if (!gbig.empty()) {
+			// SKG_UNBIND_COOP=scan: one full scan per removal (k_unb_coop);
+			// default: bounded candidates (k_unb_coopc)
+			const char *ce = getenv("SKG_UNBIND_COOP");
+			const bool scan = ce && std::string(ce) == "scan";
 			int per = 0, ncu = 0;
-			SKG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_unb_coop, COOP_T, 0));
+			SKG_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, scan ? (const void *)k_unb_coop :
+					(const void *)k_unb_coopc, COOP_T, 0));
 			SKG_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, c->dev));
 			const int nb = (int)gbig.size();
 			int nwgAll = std::max(nb, std::max(1, std::min(per, 2)) * ncu);
@@ -1241,11 +1539,45 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 			if (nwgAll > std::max(1, per) * ncu) return SKG_E_HIP;   // must be co-resident
 			float fS = c->P.fShift, fC = c->P.fCosmo, zz = c->P.z, GG = c->P.G, hxx = hx;
 			int sft = c->P.iSoftType, sub = bSub, nou = c->P.bNoUnbind;
-			unsigned long long *npair1 = npair + 1;
-			void *args[] = {&dbg, &dw0, &dwn, &dwm, &G->gstart, &G->gend, &G->uq, &G->uv, &G->uphi, &G->uord,
-					&c->d_grp, &G->count, &G->mass, &G->vcm, &G->rbound, &G->rel, &fS, &fC, &zz, &GG, &sft, &sub,
-					&nou, &hxx, &G->d_nunb, &part, &cmAll, &dact, &npair1};
-			SKG_HIP(hipLaunchCooperativeKernel((const void *)k_unb_coop, dim3(nwgAll), dim3(COOP_T), args, 0, c->st));
+			unsigned long long *npair1 = npair + 1, *npair2 = npair + 2;
+			// A cooperative launch can be refused (another process on the device
+			// may hold its cooperative resources): the same groups then go to the
+			// one-workgroup candidate kernel, which gives the same removals.
+			auto fallback = [&](hipError_t e) -> int {
+				(void)hipGetLastError();
+				c->nCoopFallbacks++;
+				k_unb_cand<<<nb, CAND_T, 0, c->st>>>(dbg, G->gstart, G->gend, G->uq, G->uv, G->uphi, G->uord,
+						c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel, c->P.fShift, c->P.fCosmo, c->P.z,
+						c->P.G, c->P.iSoftType, bSub, c->P.bNoUnbind, hx, G->d_nunb, npair1, npair + 2);
+				(void)e;
+				return 0;
+			};
+			if (scan) {
+				void *args[] = {&dbg, &dw0, &dwn, &dwm, &G->gstart, &G->gend, &G->uq, &G->uv, &G->uphi, &G->uord,
+						&c->d_grp, &G->count, &G->mass, &G->vcm, &G->rbound, &G->rel, &fS, &fC, &zz, &GG, &sft, &sub,
+						&nou, &hxx, &G->d_nunb, &part, &cmAll, &dact, &npair1};
+				const hipError_t le = hipLaunchCooperativeKernel((const void *)k_unb_coop, dim3(nwgAll), dim3(COOP_T),
+						args, 0, c->st);
+				if (le != hipSuccess) fallback(le);
+			} else {
+				// candidates per workgroup: the group's COOP_T * CPC slots shared out, at most 64
+				std::vector<int> cwv(nb);
+				for (int i = 0; i < nb; ++i) cwv[i] = std::max(1, std::min(64, COOP_T * CPC / wn[i]));
+				int *dcw = nullptr;
+				CoopCand *pubAll = nullptr;
+				CoopHdr *hdrAll = nullptr;
+				if (dalloc(&dcw, nb) || dalloc(&pubAll, (size_t)nb * COOP_T * CPC) || dalloc(&hdrAll, nwgAll))
+					return SKG_E_NOMEM;
+				SKG_HIP(hipMemcpyAsync(dcw, cwv.data(), 4 * nb, hipMemcpyHostToDevice, c->st));
+				void *args[] = {&dbg, &dw0, &dwn, &dwm, &dcw, &G->gstart, &G->gend, &G->uq, &G->uv, &G->uphi,
+						&G->uord, &c->d_grp, &G->count, &G->mass, &G->vcm, &G->rbound, &G->rel, &fS, &fC, &zz, &GG,
+						&sft, &sub, &nou, &hxx, &G->d_nunb, &pubAll, &hdrAll, &cmAll, &dact, &npair1, &npair2};
+				const hipError_t le = hipLaunchCooperativeKernel((const void *)k_unb_coopc, dim3(nwgAll), dim3(COOP_T),
+						args, 0, c->st);
+				if (le != hipSuccess) fallback(le);
+				SKG_HIP(hipStreamSynchronize(c->st));
+				dfree(&dcw); dfree(&pubAll); dfree(&hdrAll);
+			}
 			SKG_HIP(hipStreamSynchronize(c->st));
 			dfree(&dbg); dfree(&dw0); dfree(&dwn); dfree(&dwm); dfree(&dact); dfree(&cmAll);
 		}
@@ -1254,6 +1586,7 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		SKG_HIP(hipStreamSynchronize(c->st));
 		skg_prof_end(c, bslot, SKG_K_UNB_BIG, 20.0 * (double)hp[1]);
 		c->nBigGroups = (int)gbig.size();
+		SKG_HIP(hipMemcpy(&hp[2], npair + 2, 8, hipMemcpyDeviceToHost));
 		c->nFullPasses = (long long)hp[2];
 		dfree(&dt); dfree(&dgs); dfree(&dgm); dfree(&part); dfree(&cmv); dfree(&npair);
 	}
@@ -1268,6 +1601,14 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 	skg_timer_stop(c, SKG_T_UNBIND);
 	if (nUnbound) *nUnbound = nu;
 	c->stage = 8;
+	return SKG_OK;
+}
+
+extern "C" int skg_unbind_info(skg_ctx *c, int *nBig, int *nCoopRefused, long long *nFullPasses) {
+	if (!c) return SKG_E_PARAM;
+	if (nBig) *nBig = c->nBigGroups;
+	if (nCoopRefused) *nCoopRefused = c->nCoopFallbacks;
+	if (nFullPasses) *nFullPasses = c->nFullPasses;
 	return SKG_OK;
 }
 

@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libskg.so")
 
 SKG_OK = 0
+SKG_E_NODEVICE, SKG_E_PARAM, SKG_E_NOMEM, SKG_E_STATE, SKG_E_HIP, SKG_E_NSMOOTH = -1, -2, -3, -4, -5, -6
 SKG_DARK, SKG_GAS, SKG_STAR = 1, 2, 4
 SKG_PLUMMER, SKG_SPLINE = 1, 2
 SKG_ACC_PLAIN, SKG_ACC_COMPENSATED = 0, 1
@@ -58,6 +59,8 @@ def lib():
         L.skg_destroy.argtypes = [vp]
         L.skg_strerror.argtypes = [C.c_int]
         L.skg_strerror.restype = C.c_char_p
+        L.skg_last_error.argtypes = []
+        L.skg_last_error.restype = C.c_char_p
         L.skg_load.argtypes = [vp, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, fp]
         L.skg_load_batch.argtypes = [vp, C.c_int, fp, fp, fp, fp, fp, fp, fp, fp, fp]
         L.skg_box_counts.argtypes = [vp, fp, fp]
@@ -95,7 +98,10 @@ def _p(a):
 
 def _check(rc: int, what: str):
     if rc != SKG_OK:
-        raise SkgError(f"{what}: {lib().skg_strerror(rc).decode()} ({rc})")
+        msg = f"{what}: {lib().skg_strerror(rc).decode()} ({rc})"
+        if rc == SKG_E_HIP:
+            msg += ": " + lib().skg_last_error().decode()
+        raise SkgError(msg)
 
 
 def make_params(tau: float, nSmooth: int = 64, nBucket: int = 16, period: float | None = None,
@@ -221,6 +227,12 @@ class Context:
         n = C.c_int()
         _check(self.L.skg_unbind(self.h, C.byref(n)), "skg_unbind")
         return n.value
+
+    def unbind_info(self):
+        """(groups unbound cooperatively, refused cooperative launches, full passes)."""
+        a, b, f = C.c_int(), C.c_int(), C.c_longlong()
+        _check(self.L.skg_unbind_info(self.h, C.byref(a), C.byref(b), C.byref(f)), "skg_unbind_info")
+        return a.value, b.value, f.value
 
     def too_small(self):
         _check(self.L.skg_too_small(self.h), "skg_too_small")

@@ -35,6 +35,7 @@ def test_parameter_validation_without_compute():
     assert L.skg_create(C.byref(P), C.byref(h)) == -2
     assert L.skg_create(None, C.byref(h)) == -2
     assert L.skg_strerror(-6).decode().startswith("nSmooth")
+    assert L.skg_last_error().decode() == ""                    # no HIP failure on this thread
     assert L.skg_destroy(None) == 0
     for fn in ("skg_density", "skg_init_move", "skg_fof", "skg_unbind", "skg_centers"):
         assert getattr(L, fn)(None, *([None] * (len(getattr(L, fn).argtypes) - 1))) == -2

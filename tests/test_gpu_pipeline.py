@@ -68,9 +68,10 @@ def test_move_trace_and_catalogue_config1(box1):
 
 
 def test_unbinding_paths_agree(box1):
-    """The three removal kernels -- one wavefront per small group, the bounded
-    candidate kernel (one workgroup per group) and the cooperative all-GPU
-    kernel -- give the same catalogue (SKG_UNBIND_* route groups between them)."""
+    """The four removal kernels -- one wavefront per small group, the bounded
+    candidate kernel (one workgroup per group), and the cooperative all-GPU
+    kernels with bounded candidates and with one scan per removal -- give the
+    same catalogue (SKG_UNBIND_* route groups between them)."""
     import os
     import subprocess
     import sys
@@ -79,11 +80,14 @@ def test_unbinding_paths_agree(box1):
         "from skid_amd import gen, skg;"
         "b = gen.config1();"
         "ctx = skg.Context(b, skg.make_params(0.005, 32, period=1.0));"
-        "tr, g, ng = ctx.run(); np.save(sys.argv[1], g)"
+        "tr, g, ng = ctx.run(); np.save(sys.argv[1], g);"
+        "big, refused, _ = ctx.unbind_info();"
+        "assert refused == 0 and (big > 0) == ('SKG_UNBIND_COOP_MIN' in __import__('os').environ), (big, refused)"
     )
     outs = []
     for tag, env_add in (("default", {}), ("cand", {"SKG_UNBIND_SMALL_MAX": "1"}),
-                         ("coop", {"SKG_UNBIND_COOP_MIN": "100"})):
+                         ("coop", {"SKG_UNBIND_COOP_MIN": "100"}),
+                         ("coopscan", {"SKG_UNBIND_COOP_MIN": "100", "SKG_UNBIND_COOP": "scan"})):
         env = dict(os.environ)
         env.update(env_add)
         path = f"/tmp/skg_unb_{tag}.npy"
@@ -92,6 +96,7 @@ def test_unbinding_paths_agree(box1):
         outs.append(np.load(path))
     assert np.array_equal(outs[0], outs[1])
     assert np.array_equal(outs[0], outs[2])
+    assert np.array_equal(outs[0], outs[3])
 
 
 def test_two_species_catalogue():

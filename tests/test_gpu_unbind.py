@@ -90,3 +90,56 @@ def test_c5_unbinding_vs_fp64():
           f"same bound/unbound status {same:.6f}")
     assert same >= 0.999
     assert O.match_fraction(g, ref) >= 0.999
+
+
+_COOP_CHILD = r"""
+import os, sys
+import numpy as np
+sys.path.insert(0, os.getcwd())
+from skid_amd import gen, skg
+coop_min, path = sys.argv[1], sys.argv[2]
+n = 1 << 18
+box, grp = gen.config5(n=n, n_groups=300, seed=12, n_range=(8, 12000))
+fs, fc = gen.config5_cosmology()
+P = skg.make_params(0.16 * n ** (-1.0 / 3.0), 32, period=1.0, z=0.5, fShift=fs, fCosmo=fc)
+out = {}
+for tag, env in (("cand", {}), ("coopc", {"SKG_UNBIND_COOP_MIN": coop_min}),
+                 ("coopscan", {"SKG_UNBIND_COOP_MIN": coop_min, "SKG_UNBIND_COOP": "scan"})):
+    for k in ("SKG_UNBIND_COOP_MIN", "SKG_UNBIND_COOP"):
+        os.environ.pop(k, None)
+    os.environ.update(env)
+    ctx = skg.Context(box, P)
+    ng = ctx.set_groups(grp)
+    nunb = ctx.unbind()
+    info = ctx.unbind_info()
+    g, _ = ctx.groups()
+    out[tag + "_g"] = g
+    out[tag + "_n"] = np.array([nunb, *info])
+    ctx.close()
+np.savez(path, grp=grp, **out)
+"""
+
+
+@pytest.mark.parametrize("coop_min", ["2000", "300"])
+def test_c5_cooperative_removal_paths_agree(coop_min, tmp_path):
+    """Power-law groups with kdSubPot: the cooperative kernels (bounded
+    candidates, the default, and one scan per removal) unbind the groups routed
+    to them as the one-workgroup candidate kernel does.  Run in a child process:
+    a process that has made cooperative launches can keep the device's
+    cooperative resources, and this suite's other children need them."""
+    import os
+    import subprocess
+    import sys
+    path = str(tmp_path / "coop.npz")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    subprocess.run([sys.executable, "-c", _COOP_CHILD, coop_min, path], check=True, timeout=240, cwd=root)
+    r = np.load(path)
+    sizes = np.bincount(r["grp"])[1:]
+    print(f"groups >= {coop_min}: {(sizes >= int(coop_min)).sum()}; [unbound, coop groups, refused, full passes] "
+          + ", ".join(f"{k} {r[k + '_n'].tolist()}" for k in ("cand", "coopc", "coopscan")))
+    assert r["cand_n"][1] == 0
+    for tag in ("coopc", "coopscan"):
+        nunb, nbig, refused, _ = r[tag + "_n"]
+        assert nbig == min(128, (sizes >= int(coop_min)).sum()) and refused == 0
+        assert O.match_fraction(r[tag + "_g"], r["cand_g"]) >= 0.9999
+        assert abs(nunb - r["cand_n"][0]) <= max(2, r["cand_n"][0] // 1000)
