@@ -95,6 +95,21 @@ def cpu_run(boxes, kws):
     return time.perf_counter() - t0, kind
 
 
+def copy_calibration():
+    """A plain device copy of known size (256 MiB read, 256 MiB written, three
+    dispatches): what FETCH_SIZE and WRITE_SIZE read for it gives the factor
+    from counter KiB to HBM bytes."""
+    import torch
+    n = 64 << 20
+    a = torch.ones(n, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    log(f"copied {4 * n} bytes x3")
+
+
 def measured_traffic(name):
     """HBM bytes per launch of the move-step class from the committed counter
     summary (two rocprofv3 --pmc runs, FETCH_SIZE and WRITE_SIZE, calibrated on
@@ -369,7 +384,13 @@ def main():
     ap.add_argument("--plain-every", type=int, default=4, help="plain GPU form on every n-th box (0 skips)")
     ap.add_argument("--cpu-boxes", type=int, default=4, help="boxes in the CPU sample (0 skips)")
     ap.add_argument("--shapes", default="", help="nBucket,subMax,potTile: skip the autotuner and use these")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 256 MiB on the device three times (the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE to bytes) and exit")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               f"(sharding across GPUs is not built)", file=sys.stderr)
