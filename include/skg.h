@@ -165,6 +165,17 @@ int skg_unbind(skg_ctx *ctx, int *nUnbound);
 
 /* kdTooSmall (kd.c:1251-1293). */
 int skg_too_small(skg_ctx *ctx);
+/* Per-group statistics of the .stat file (kdOutStats, kd.c:1703-1839), after
+   skg_too_small: SKG_STAT_N floats per group for groups 0..nGroup-1 (row 0 is
+   zero): total mass, gas mass (rho >= fDensMin, T <= fTempMax), star mass,
+   Vmax^2 = max G M(<r)/r over members outside twice their softening (else the
+   outer value), V(r_half)^2, V(r_out)^2, r at Vmax, r_half (first member past
+   half the mass), r_out^2, 1-D dispersion of fExp (v - vcm) + fExpHub r,
+   rCenter (3), vcm (3), rBound (3).  Running masses are FP64 (the reference
+   sums in fp32).  The driver prints square roots where kdOutStats does. */
+#define SKG_STAT_N 19
+int skg_group_stats(skg_ctx *ctx, float *stats);
+
 /* Diagnostics of the last skg_unbind(): groups unbound by the cooperative
    all-GPU kernel, cooperative launches the device refused (those groups then
    went to the one-workgroup kernel, with the same result), and full passes of

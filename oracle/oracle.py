@@ -479,3 +479,96 @@ def _kernel_f64(r, twoh, soft_type):
         mid = -1.0 / (15.0 * r) + dih * (1.6 - (4.0 / 3.0) * u ** 2 + u ** 3 - 0.3 * u ** 4 + u ** 5 / 30.0)
         far = 1.0 / r
     return np.where(r >= twoh, far, np.where(u < 1.0, inner, mid))
+
+
+def group_stats_f32(pos, vel, mass, soft, grp, centres, vcms, ptype=None, rho=None, temp=None, G=1.0,
+                    fexp=1.0, fexphub=0.0, period=1.0, dens_min=0.0, temp_max=3.4028235e38):
+    """kdOutStats (kd.c:1703-1839) restated in the reference's own fp32/fp64 mix,
+    for checking: per group g >= 1, members relative to centres[g] (wrapped at
+    half a period), ordered by r^2 = ((x^2) + y^2) + z^2 in fp32 (ties by
+    index, where the reference's qsort leaves them unordered), fp32 running
+    masses, then the ten computed columns of the .stat line in the layout of
+    skg_group_stats: total mass, gas mass, star mass, Vmax^2, V(r_half)^2,
+    V(r_out)^2, r at Vmax, r_half, r_out^2, 1-D dispersion.  ptype by index
+    (2 gas, 4 star, else dark).  Returns an (nGroup, 10) float64 array, row 0
+    zero."""
+    f = np.float32
+    pos = np.asarray(pos, f).reshape(-1, 3)
+    vel = np.asarray(vel, f).reshape(-1, 3)
+    mass = np.asarray(mass, f)
+    soft = np.asarray(soft, f)
+    grp = np.asarray(grp)
+    ng = int(grp.max()) + 1 if grp.size else 1
+    hx = f(0.5) * f(period)
+    Gf, fe, fh = f(G), f(fexp), f(fexphub)
+    order = np.argsort(grp, kind="stable")
+    bounds = np.searchsorted(grp[order], np.arange(ng + 1))
+    out = np.zeros((ng, 10), np.float64)
+    for g in range(1, ng):
+        mem = order[bounds[g]:bounds[g + 1]]
+        n = mem.size
+        if n == 0:
+            continue
+        d = pos[mem] - np.asarray(centres[g], f)
+        d = np.where(d > hx, d - f(2) * hx, d)
+        d = np.where(d <= -hx, d + f(2) * hx, d)
+        r2 = ((f(0) + d[:, 0] * d[:, 0]) + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        srt = np.argsort(r2, kind="stable")
+        mem, d, r2 = mem[srt], d[srt], r2[srt]
+        m = mass[mem]
+        half = np.cumsum(f(0.5) * m, dtype=f)[-1]
+        M = np.cumsum(m, dtype=f)
+        r = np.sqrt(r2.astype(np.float64))
+        sf = soft[mem].astype(np.float64)
+        v = (Gf * M).astype(np.float64) / np.where(r > 0, r, np.inf)
+        ok = (r2.astype(np.float64) > 4.0 * sf * sf) & (v > 0)
+        vcirc = f(0)
+        rvmax = f(0)
+        for j in np.flatnonzero(ok):                              # the running test against the stored fp32 value
+            if v[j] > np.float64(vcirc):
+                rvmax = f(r[j])
+                vcirc = f(Gf * M[j]) / rvmax
+        jh = np.flatnonzero(M > half)
+        rhalf = vhalf = f(0)
+        if jh.size:
+            j = int(jh[0])
+            rhalf = f(r[j])
+            vhalf = f(Gf * M[j]) / rhalf
+        vout = f(np.float64(Gf * M[-1]) / r[-1]) if r[-1] > 0 else f(0)
+        if vcirc == 0:
+            vcirc = vout
+            rvmax = f(r[-1])
+        pt = np.asarray(ptype)[mem] if ptype is not None else np.ones(n, np.int32)
+        gas_sel = pt == 2
+        if rho is not None:
+            gas_sel &= np.asarray(rho, f)[mem] >= f(dens_min)
+        if temp is not None:
+            gas_sel &= np.asarray(temp, f)[mem] <= f(temp_max)
+        gas = np.cumsum(m[gas_sel], dtype=f)[-1] if gas_sel.any() else f(0)
+        star = np.cumsum(m[pt == 4], dtype=f)[-1] if (pt == 4).any() else f(0)
+        dv = fe * (vel[mem] - np.asarray(vcms[g], f)) + fh * d
+        vd = np.cumsum((dv * dv).reshape(-1), dtype=f)[-1]
+        out[g] = [M[-1], gas, star, vcirc, vhalf, vout, rvmax, rhalf, r2[-1], f(np.sqrt(np.float64(vd) / (3.0 * n)))]
+    return out
+
+
+def read_stat(path):
+    """A .stat file (kd.c:1827-1836) as an (nGroup - 1, 21) float64 array."""
+    rows = [list(map(float, line.split())) for line in open(path) if line.strip()]
+    return np.array(rows, np.float64).reshape(-1, 21)
+
+
+def read_gtp(path, xdr=True):
+    """A .gtp file (kdWriteGroup, kd.c:1612-1687): per group (mass, rCenter,
+    vcm, radius) from the tipsy star records after the 32-byte header."""
+    raw = np.fromfile(path, np.uint8)
+    rec = np.frombuffer(raw[32:].tobytes(), ">f4" if xdr else "<f4").reshape(-1, 11).astype(np.float32)
+    return dict(mass=rec[:, 0], pos=rec[:, 1:4], vel=rec[:, 4:7], radius=rec[:, 9])   # metals, tform, eps, phi
+
+
+def read_grp(path):
+    """A .grp file (kdOutGroup, kd.c:1502-1524): group id by iOrder."""
+    with open(path) as f:
+        n = int(f.readline())
+        g = np.loadtxt(f, dtype=np.int64, max_rows=n)
+    return g.reshape(-1)

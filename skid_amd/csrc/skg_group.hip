@@ -1699,6 +1699,154 @@ extern "C" int skg_get_catalogue(skg_ctx *c, float *mass, float *center, float *
 
 
 // ---------------------------------------------------------------------------
+// Per-group statistics of the .stat file (kdOutStats, kd.c:1703-1839): members
+// sorted by squared distance from rCenter (one radix sort of (group, r^2)
+// keys), then one wavefront per group walks its members in that order with a
+// running (FP64) mass: the largest G M(<r)/r outside twice the softening, the
+// half-mass radius, the outer values, gas (rho >= fDensMin, T <= fTempMax) and
+// star masses, and the 1-D dispersion of fExp (v - vcm) + fExpHub r.
+
+__global__ void k_stat_keys(const int *__restrict__ grp, int n, const float *__restrict__ pos,
+		const float4 *__restrict__ cen, float hx, float hy, float hz, unsigned long long *key, int *val) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int g = grp[i];
+	float r2 = 0.f;
+	if (g > 0) {
+		const float4 c = cen[g];
+		float d[3] = {fsub(pos[3 * i], c.x), fsub(pos[3 * i + 1], c.y), fsub(pos[3 * i + 2], c.z)};
+		const float h[3] = {hx, hy, hz};
+		for (int k = 0; k < 3; ++k) {
+			if (d[k] > h[k]) d[k] = fsub(d[k], 2 * h[k]);
+			if (d[k] <= -h[k]) d[k] = fadd(d[k], 2 * h[k]);
+			r2 = fadd(r2, fmul(d[k], d[k]));                   // kd.c:1771-1775, in this order
+		}
+	}
+	key[i] = ((unsigned long long)(unsigned)g << 32) | (unsigned long long)__float_as_uint(r2);
+	val[i] = i;
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+	return v;
+}
+
+__global__ __launch_bounds__(64) void k_group_stats(const int *__restrict__ gs, const int *__restrict__ ge, int nG,
+		const unsigned long long *__restrict__ key, const int *__restrict__ ord, const float *__restrict__ pos,
+		const float *__restrict__ vel, const float *__restrict__ mass, const float *__restrict__ soft,
+		const float *__restrict__ rho, const float *__restrict__ temp, const unsigned char *__restrict__ type,
+		const float4 *__restrict__ cen, const float4 *__restrict__ vcm, const float4 *__restrict__ rbound, float G,
+		float fExp, float fExpHub, float hx, float hy, float hz, float fDensMin, float fTempMax, float *out) {
+	const int g = blockIdx.x + 1, lane = threadIdx.x;
+	if (g >= nG) return;
+	const int s = gs[g], e = ge[g], n = e - s;
+	float *row = out + (size_t)g * SKG_STAT_N;
+	if (n <= 0) { if (lane < SKG_STAT_N) row[lane] = 0.f; return; }
+	const float4 c = cen[g], vc = vcm[g];
+	double half = 0.0;
+	for (int i = s + lane; i < e; i += 64) half += 0.5 * (double)mass[ord[i]];
+	half = wave_sum_d(half);
+	double carry = 0.0, vBest = 0.0, mBest = 0.0, mHalf = 0.0, gas = 0.0, star = 0.0, vd = 0.0;
+	int iBest = INT_MAX, iHalf = INT_MAX;
+	float r2Best = 0.f, r2Half = 0.f;
+	for (int b = s; b < e; b += 64) {
+		const int i = b + lane;
+		const bool in = i < e;
+		const int o = in ? ord[i] : ord[s];
+		const float m = in ? mass[o] : 0.f;
+		const float r2 = __uint_as_float((unsigned)key[in ? i : s]);
+		double M = (double)m;
+#pragma unroll
+		for (int d = 1; d < 64; d <<= 1) {
+			const double t = __shfl_up(M, d, 64);
+			if (lane >= d) M += t;
+		}
+		M += carry;
+		carry = __shfl(M, 63, 64);
+		if (!in) continue;
+		const double r = sqrt((double)r2);
+		const double sf = (double)soft[o];
+		if ((double)r2 > 4.0 * sf * sf) {                      // kd.c:1789-1793
+			const double v = (double)G * M / r;
+			if (v > vBest) { vBest = v; mBest = M; iBest = i; r2Best = r2; }
+		}
+		if (M > half && iHalf == INT_MAX) { iHalf = i; mHalf = M; r2Half = r2; }
+		const int t = type[o];
+		if (t == SKG_GAS && rho[o] >= fDensMin && temp[o] <= fTempMax) gas += (double)m;
+		if (t == SKG_STAR) star += (double)m;
+		float d[3] = {fsub(pos[3 * o], c.x), fsub(pos[3 * o + 1], c.y), fsub(pos[3 * o + 2], c.z)};
+		const float h[3] = {hx, hy, hz}, w[3] = {vc.x, vc.y, vc.z};
+		for (int k = 0; k < 3; ++k) {
+			if (d[k] > h[k]) d[k] = fsub(d[k], 2 * h[k]);
+			if (d[k] <= -h[k]) d[k] = fadd(d[k], 2 * h[k]);
+			const double dv = (double)fExp * ((double)vel[3 * o + k] - (double)w[k]) + (double)fExpHub * (double)d[k];
+			vd += dv * dv;
+		}
+	}
+	// the first member reaching the largest circular velocity, the first past half the mass
+	for (int o = 32; o > 0; o >>= 1) {
+		const double ov = __shfl_xor(vBest, o, 64), om = __shfl_xor(mBest, o, 64);
+		const int oi = __shfl_xor(iBest, o, 64);
+		const float orr = __shfl_xor(r2Best, o, 64);
+		if (ov > vBest || (ov == vBest && oi < iBest)) { vBest = ov; mBest = om; iBest = oi; r2Best = orr; }
+		const int hi = __shfl_xor(iHalf, o, 64);
+		const double hm = __shfl_xor(mHalf, o, 64);
+		const float hr = __shfl_xor(r2Half, o, 64);
+		if (hi < iHalf) { iHalf = hi; mHalf = hm; r2Half = hr; }
+	}
+	gas = wave_sum_d(gas);
+	star = wave_sum_d(star);
+	vd = wave_sum_d(vd);
+	if (lane != 0) return;
+	const float mTot = (float)carry;
+	const float r2Out = __uint_as_float((unsigned)key[e - 1]);
+	float vcirc = 0.f, rVmax = 0.f;
+	if (iBest != INT_MAX) {
+		rVmax = (float)sqrt((double)r2Best);
+		vcirc = fmul(G, (float)mBest) / rVmax;
+	}
+	float rHalf = 0.f, vHalf = 0.f;
+	if (iHalf != INT_MAX) {
+		rHalf = (float)sqrt((double)r2Half);
+		vHalf = fmul(G, (float)mHalf) / rHalf;
+	}
+	const float vOut = (float)((double)fmul(G, mTot) / sqrt((double)r2Out));
+	if (vcirc == 0.f) { vcirc = vOut; rVmax = (float)sqrt((double)r2Out); }   // kd.c:1816-1819
+	const float4 rb = rbound[g];
+	const float v[SKG_STAT_N] = {mTot, (float)gas, (float)star, vcirc, vHalf, vOut, rVmax, rHalf, r2Out,
+			(float)sqrt(vd / (3.0 * n)), c.x, c.y, c.z, vc.x, vc.y, vc.z, rb.x, rb.y, rb.z};
+	for (int k = 0; k < SKG_STAT_N; ++k) row[k] = v[k];
+}
+
+extern "C" int skg_group_stats(skg_ctx *c, float *stats) {
+	if (!c || !stats) return SKG_E_PARAM;
+	if (c->stage < 9 || !c->gs) return SKG_E_STATE;
+	(void)hipSetDevice(c->dev);
+	GroupState *G = c->gs;
+	const int n = c->n, nG = G->nG;
+	int rc;
+	const float *L = c->P.fPeriod;
+	const float hx = 0.5f * L[0], hy = 0.5f * L[1], hz = 0.5f * L[2];
+	if (skg_ensure_keys(c, n)) return SKG_E_NOMEM;
+	float *dst = nullptr;
+	if (dalloc(&dst, (size_t)nG * SKG_STAT_N)) return SKG_E_NOMEM;
+	SKG_HIP(hipMemsetAsync(dst, 0, 4 * (size_t)SKG_STAT_N, c->st));        // group 0's row
+	k_stat_keys<<<nblk(n, 256), 256, 0, c->st>>>(c->d_grp, n, c->d_pos, G->cen, hx, hy, hz, c->k64a, c->v32a);
+	// (group, r^2) order: the group segments are those of group_sort's (group, iOrder) order
+	if ((rc = skg_sort_pairs(c, c->k64a, c->k64b, c->v32a, c->v32b, n, 64))) return rc;
+	if (nG > 1)
+		k_group_stats<<<nG - 1, 64, 0, c->st>>>(G->gstart, G->gend, nG, c->k64b, c->v32b, c->d_pos, c->d_vel,
+				c->d_mass, c->d_soft, c->d_rho, c->d_temp, c->d_type, G->cen, G->vcm, G->rbound, c->P.G,
+				c->P.fShift, c->P.fCosmo, hx, hy, hz, c->P.fDensMin, c->P.fTempMax, dst);
+	SKG_HIP(hipGetLastError());
+	SKG_HIP(hipMemcpyAsync(stats, dst, 4 * (size_t)nG * SKG_STAT_N, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&dst);
+	return SKG_OK;
+}
+
+// ---------------------------------------------------------------------------
 // Groups from outside: the -unbind re-entry (kdInGroup + kdInitpGroup +
 // kdReadCenter, kd.c:920-1196; main.c:349-373).
 

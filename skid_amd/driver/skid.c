@@ -427,7 +427,28 @@ unbind_only:
 		fclose(fp);
 		free(m); free(cen); free(vcm); free(rad);
 	}
-	if (bOutStats) fprintf(stderr, "skid: -stats output is not provided by this build\n");
+	if (bOutStats) {                                        /* kdOutStats (kd.c:1703-1839) */
+		FILE *fp;
+		int ng = nGroup, k;
+		float *st = malloc(4*(size_t)ng*SKG_STAT_N);
+		int *cnt = malloc(4*(size_t)ng);
+		if (!st || !cnt) { fprintf(stderr, "skid: out of memory\n"); return 1; }
+		if ((rc = skg_group_stats(ctx, st))) die_skg("skg_group_stats", rc);
+		if ((rc = skg_get_catalogue(ctx, NULL, NULL, NULL, NULL, NULL, cnt))) die_skg("skg_get_catalogue", rc);
+		snprintf(achFile, sizeof(achFile), "%s.stat", achName);
+		fp = fopen(achFile, "w");
+		if (!fp) { perror(achFile); return 1; }
+		for (i = 1; i < ng; ++i) {
+			const float *r = st + (size_t)i*SKG_STAT_N;
+			/* id, n, masses, sqrt of the three V^2, r_Vmax, r_half, r_out, dispersion, centre, vcm, rBound */
+			fprintf(fp, "%d %d %g %g %g %g %g %g %g %g %g %g", i, cnt[i], r[0], r[1], r[2], sqrt(r[3]),
+					sqrt(r[4]), sqrt(r[5]), r[6], r[7], sqrt(r[8]), r[9]);
+			for (k = 10; k < SKG_STAT_N; ++k) fprintf(fp, " %g", r[k]);
+			fprintf(fp, "\n");
+		}
+		fclose(fp);
+		free(st); free(cnt);
+	}
 	printf("SKID GPU wall time:\n");
 	if (!bUnbindOnly) {
 		printf("   Initial Density:    %.6f\n", tDen);

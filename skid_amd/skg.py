@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libskg.so")
 
 SKG_OK = 0
+SKG_STAT_N = 19
 SKG_E_NODEVICE, SKG_E_PARAM, SKG_E_NOMEM, SKG_E_STATE, SKG_E_HIP, SKG_E_NSMOOTH = -1, -2, -3, -4, -5, -6
 SKG_DARK, SKG_GAS, SKG_STAR = 1, 2, 4
 SKG_PLUMMER, SKG_SPLINE = 1, 2
@@ -227,6 +228,12 @@ class Context:
         n = C.c_int()
         _check(self.L.skg_unbind(self.h, C.byref(n)), "skg_unbind")
         return n.value
+
+    def group_stats(self, ngroup: int):
+        """skg_group_stats: (ngroup, SKG_STAT_N) per-group .stat quantities."""
+        out = np.zeros((ngroup, SKG_STAT_N), np.float32)
+        _check(self.L.skg_group_stats(self.h, _p(out)), "skg_group_stats")
+        return out
 
     def unbind_info(self):
         """(groups unbound cooperatively, refused cooperative launches, full passes)."""

@@ -36,17 +36,44 @@ def test_driver_matches_reference_and_unbind_reentry(tmp_path):
     box = gen.config1()
     inp = tmp_path / "c1.std"
     gen.write_tipsy(str(inp), box, xdr=True)
-    args = ["-tau", "0.005", "-s", "32", "-m", "8", "-p", "1", "-std"]
+    args = ["-tau", "0.005", "-s", "32", "-m", "8", "-p", "1", "-std", "-stats"]
     out = run(SKID, args + ["-o", "gpu"], inp, tmp_path)
     assert "Number of Groups:" in out
     g = read_grp(tmp_path / "gpu.grp")
     assert os.path.getsize(tmp_path / "gpu.gtp") == 32 + 44 * (g.max())
+    st = O.read_stat(str(tmp_path / "gpu.stat"))
+    assert st.shape == (g.max(), 21)
+    assert np.array_equal(st[:, 1], np.bincount(g)[1:])
     if os.path.exists(REF):
         run(REF, args + ["-o", "ref"], inp, tmp_path)
         rg = read_grp(tmp_path / "ref.grp")
         frac = O.match_fraction(g, rg)
         print(f"driver vs reference binary: groups {g.max()} vs {rg.max()}, match {frac:.5f}")
         assert frac >= 0.999
+        # .stat rows of groups with the same members in both runs
+        rst = O.read_stat(str(tmp_path / "ref.stat"))
+        pair = O.overlap_pairing(g, rg)
+        rows = [(a - 1, b - 1) for a, b in pair.items()
+                if a and st[a - 1, 1] == rst[b - 1, 1] and np.array_equal(g == a, rg == b)]
+        x = st[[r[0] for r in rows]]
+        y = rst[[r[1] for r in rows]]
+        rel = np.abs(x - y) / np.maximum(np.abs(y), 1e-30)
+        print(f".stat: {len(rows)} of {g.max()} groups with identical members")
+        print("  columns 2-12 rel p50/p99/max:", [(np.percentile(rel[:, c], 50).round(8), np.percentile(rel[:, c], 99)
+              .round(7), rel[:, c].max().round(6)) for c in range(1, 12)])
+        print("  centre |d| max:", np.abs(x[:, 12:15] - y[:, 12:15]).max(), " rBound |d| max:",
+              np.abs(x[:, 18:21] - y[:, 18:21]).max())
+        assert len(rows) >= 0.99 * g.max()
+        # counts, masses and vcm (the members' own values): %g's 6 digits
+        assert rel[:, [1, 2, 3, 4, 15, 16, 17]].max() <= 6e-6
+        # rCenter averages the members' moved positions (kd.c:1032-1042), whose
+        # last steps differ at the rounding level between the two builds: within
+        # 5% of tau (7.6e-5 = 1.5% of tau measured)
+        assert np.abs(x[:, 12:15] - y[:, 12:15]).max() <= 0.05 * 0.005
+        # rBound is one member's own position: the same member in 99% of the groups
+        assert np.percentile(np.abs(x[:, 18:21] - y[:, 18:21]).max(axis=1), 99) <= 2e-6
+        # what depends on the centre (radii, circular velocities, dispersion) in 99% of the groups
+        assert np.percentile(rel[:, 5:12], 99, axis=0).max() <= 1e-3
     # -unbind re-entry on our own output: the groups are already bound, so
     # unbinding them again changes nothing
     out = run(SKID, ["-unbind", "gpu.grp", "-tau", "0.005", "-m", "8", "-p", "1", "-std", "-o", "gpu_u"], inp, tmp_path)

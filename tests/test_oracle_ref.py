@@ -108,3 +108,46 @@ def test_fp64_unbinding_matches_reference_unbind_path():
     print(f"reference -unbind vs FP64: unbound {int(((rg == 0) & inm).sum())} vs {int(((u == 0) & inm).sum())}, "
           f"same status {same:.5f}")
     assert same >= 0.999
+
+
+def test_group_stats_restatement_matches_reference_stat(tmp_path):
+    """The reference binary's -stats output (kdOutStats) against the fp32
+    restatement given the reference's own groups and centres (.grp/.gtp):
+    every column of every group, z = 0.5 flat Lambda so the Hubble term of the
+    dispersion is on, and gas with a density floor so the gas mass is tested."""
+    import os
+    import subprocess
+    ref = os.path.join(O.HERE, "_ref", "skid")
+    if not os.path.exists(ref):
+        pytest.skip("reference binary not built")
+    box = gen.config3(n_each=4096, seed=5, n_halos=20)
+    inp = tmp_path / "b.std"
+    gen.write_tipsy(str(inp), box, xdr=True)
+    dmin = 30.0
+    args = [ref, "-tau", "0.01", "-s", "32", "-m", "8", "-p", "1", "-std", "-z", "0.5", "-H", "2.8944",
+            "-O", "0.3", "-Lambda", "0.7", "-d", str(dmin), "-den", "-stats", "-o", "ref"]
+    with open(inp, "rb") as f:
+        r = subprocess.run(args, stdin=f, capture_output=True, text=True, timeout=300, cwd=tmp_path)
+    assert r.returncode == 0, r.stderr[-2000:]
+    st = O.read_stat(str(tmp_path / "ref.stat"))
+    grp = O.read_grp(str(tmp_path / "ref.grp"))
+    gtp = O.read_gtp(str(tmp_path / "ref.gtp"))
+    rho = np.loadtxt(str(tmp_path / "ref.den"), skiprows=1).astype(np.float32)
+    ng = int(grp.max()) + 1
+    assert st.shape[0] == ng - 1 and gtp["pos"].shape[0] == ng - 1
+    cen = np.vstack([np.zeros((1, 3), np.float32), gtp["pos"]])
+    vcm = np.vstack([np.zeros((1, 3), np.float32), gtp["vel"]])
+    ptype = np.where(np.arange(box.n) < box.n_gas, 2, 1)
+    fs, fc = gen.config5_cosmology()
+    ours = O.group_stats_f32(box.pos, box.vel, box.mass, box.soft, grp, cen, vcm, ptype=ptype, rho=rho,
+                             temp=box.temp, fexp=fs, fexphub=fc, dens_min=dmin)
+    counts = np.bincount(grp, minlength=ng)
+    want = np.column_stack([np.arange(1, ng), counts[1:], ours[1:, 0:3], np.sqrt(ours[1:, 3:6]),
+                            ours[1:, 6:8], np.sqrt(ours[1:, 8]), ours[1:, 9], cen[1:], vcm[1:]])
+    got = st[:, :18]
+    # %g keeps 6 significant digits
+    rel = np.abs(got - want) / np.maximum(np.abs(want), 1e-30)
+    bad = (rel > 6e-6) & (np.abs(got - want) > 1e-9)
+    print(f"{ng - 1} groups; gas mass in {int((ours[1:, 1] > 0).sum())}; worst column rel "
+          f"{np.max(np.where(np.abs(want) > 0, rel, 0), axis=0).round(8).tolist()}")
+    assert not bad.any(), np.argwhere(bad)[:10]
