@@ -16,6 +16,7 @@
 #include <string.h>
 #include <time.h>
 #include "skg.h"
+#include "textio.h"
 
 #define PRUNE_STEPS 5       /* main.c:13 */
 
@@ -226,6 +227,7 @@ static int read_gtp(const char *path, int xdr, int nGroup, const Snap *S, const 
 
 int main(int argc, char **argv)
 {
+	const int nIo = tio_threads();                         /* threads formatting the text outputs */
 	skg_params P;
 	skg_ctx *ctx;
 	Snap S;
@@ -321,13 +323,8 @@ int main(int argc, char **argv)
 	t1 = now_s(); tDen = t1 - t0;
 	if (P.bPeriodic) printf("nExtraScat:%d\n", nExtra);
 	if (bOutDens) {                                         /* kdOutDensity (kd.c:1526-1547) */
-		FILE *fp;
 		snprintf(achFile, sizeof(achFile), "%s.den", achName);
-		fp = fopen(achFile, "w");
-		if (!fp) { perror(achFile); return 1; }
-		fprintf(fp, "%d\n", S.n);
-		for (i = 0; i < S.n; ++i) fprintf(fp, "%.10g\n", rho[i]);
-		fclose(fp);
+		if (tio_write_floats(achFile, S.n, S.n, rho, "%.10g", 0, nIo)) { perror(achFile); return 1; }
 	}
 	t0 = now_s();
 	if ((rc = skg_init_move(ctx, &nActive))) die_skg("skg_init_move", rc);
@@ -350,29 +347,27 @@ int main(int argc, char **argv)
 	t1 = now_s(); tMicro = t1 - t0;
 	for (i = 0; i < PRUNE_STEPS; ++i) printf("Microstep:%d nScatter:%d\n", i + 1, nScat);
 	if (bOutRay) {                                          /* kdOutVector (kd.c:1550-1606) */
-		FILE *fp;
 		int j;
-		float h = 0.5f*P.fPeriod[0];
+		float *d3;
 		r = malloc(12*(size_t)S.n);
-		if (!r) { fprintf(stderr, "skid: out of memory\n"); return 1; }
+		d3 = malloc(12*(size_t)S.n);
+		if (!r || !d3) { fprintf(stderr, "skid: out of memory\n"); return 1; }
 		if ((rc = skg_get_moved(ctx, r))) die_skg("skg_get_moved", rc);
-		snprintf(achFile, sizeof(achFile), "%s.ray", achName);
-		fp = fopen(achFile, "w");
-		if (!fp) { perror(achFile); return 1; }
-		fprintf(fp, "%d\n", S.n);
-		for (j = 0; j < 3; ++j) {
+		for (j = 0; j < 3; ++j) {                       /* all x, then all y, then all z */
 			float hj = 0.5f*P.fPeriod[j];
 			for (i = 0; i < S.n; ++i) {
-				float d;
-				if (isnan(r[3*i+j])) { fprintf(fp, "0\n"); continue; }
-				d = r[3*i+j] - S.pos[3*i+j];
-				if (d > hj) d -= 2*hj;
-				if (d <= -hj) d += 2*hj;
-				fprintf(fp, "%g\n", d);
+				float d = r[3*i+j];
+				if (!isnan(d)) {                        /* NaN: did not move, printed as 0 */
+					d = d - S.pos[3*i+j];
+					if (d > hj) d -= 2*hj;
+					if (d <= -hj) d += 2*hj;
+				}
+				d3[(size_t)j*S.n + i] = d;
 			}
 		}
-		(void)h;
-		fclose(fp);
+		snprintf(achFile, sizeof(achFile), "%s.ray", achName);
+		if (tio_write_floats(achFile, S.n, 3L*S.n, d3, "%g", 1, nIo)) { perror(achFile); return 1; }
+		free(d3);
 	}
 	if ((rc = skg_centers(ctx))) die_skg("skg_centers", rc);
 unbind_only:
@@ -385,13 +380,8 @@ unbind_only:
 	if ((rc = skg_get_groups(ctx, grp, &nGroup))) die_skg("skg_get_groups", rc);
 	printf("Number of Groups:%d\n", nGroup - 1);
 	{                                                       /* kdOutGroup (kd.c:1502-1523) */
-		FILE *fp;
 		snprintf(achFile, sizeof(achFile), "%s.grp", achName);
-		fp = fopen(achFile, "w");
-		if (!fp) { perror(achFile); return 1; }
-		fprintf(fp, "%d\n", S.n);
-		for (i = 0; i < S.n; ++i) fprintf(fp, "%d\n", grp[i]);
-		fclose(fp);
+		if (tio_write_ints(achFile, S.n, grp, nIo)) { perror(achFile); return 1; }
 	}
 	{                                                       /* kdWriteGroup (kd.c:1609-1687) */
 		FILE *fp;
