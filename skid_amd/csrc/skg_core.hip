@@ -187,6 +187,30 @@ __device__ __forceinline__ unsigned long long spread21(unsigned v) {
 	return x;
 }
 
+// Position on the 3-D Hilbert curve of 21-bit coordinates (Skilling's
+// transpose method, AIP Conf. Proc. 707, 381 (2004)): undo the excess work of
+// the reflected Gray code level by level, Gray-encode, and interleave.  Unlike
+// Morton order the curve has no jumps, so runs of 64 consecutive points are
+// more compact.
+__device__ __forceinline__ unsigned long long hilbert21(unsigned x, unsigned y, unsigned z) {
+	unsigned X[3] = {x, y, z};
+	for (unsigned Q = 1u << 20; Q > 1; Q >>= 1) {
+		const unsigned P = Q - 1;
+		for (int i = 0; i < 3; ++i) {
+			if (X[i] & Q) X[0] ^= P;
+			else { const unsigned t = (X[0] ^ X[i]) & P; X[0] ^= t; X[i] ^= t; }
+		}
+	}
+	X[1] ^= X[0];
+	X[2] ^= X[1];
+	unsigned t = 0;
+	for (unsigned Q = 1u << 20; Q > 1; Q >>= 1)
+		if (X[2] & Q) t ^= Q - 1;
+	X[0] ^= t; X[1] ^= t; X[2] ^= t;
+	return (spread21(X[0]) << 2) | (spread21(X[1]) << 1) | spread21(X[2]);
+}
+
+template <bool HILBERT>
 __global__ void k_morton(const float4 *__restrict__ pos, const int *__restrict__ idx, int n,
 		const unsigned *__restrict__ bb, unsigned long long *keys, int *vals, const int *__restrict__ box,
 		int bbits) {
@@ -204,7 +228,8 @@ __global__ void k_morton(const float4 *__restrict__ pos, const int *__restrict__
 		t = fminf(fmaxf(t, 0.f), 1.f);
 		q[a] = (unsigned)fminf(t * 2097152.0f, 2097151.0f);
 	}
-	unsigned long long key = (spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
+	unsigned long long key = HILBERT ? hilbert21(q[0], q[1], q[2]) :
+			(spread21(q[0]) << 2) | (spread21(q[1]) << 1) | spread21(q[2]);
 	// boxes first: the box id takes the top bits, the coarse Morton digits the rest
 	if (box && bbits > 0) key = ((unsigned long long)box[j] << (63 - bbits)) | (key >> bbits);
 	keys[i] = key;
@@ -219,17 +244,31 @@ int skg_box_bits(const skg_ctx *c) {
 
 // perm_out[i] = index (into pos, or idx[] entry) of the i-th point in Morton
 // order; with box (indexed like pos) the order is by box first.
-int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box) {
+static int curve_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box,
+		bool hilbert) {
 	if (n <= 0) return 0;
 	if (skg_ensure_keys(c, n)) return SKG_E_NOMEM;
 	const int nb = nblk(n, 256);
 	if (skg_ensure_part(c, 6 * (size_t)nb)) return SKG_E_NOMEM;
 	k_bbox<<<nb, 256, 0, c->st>>>(pos, idx, n, c->part);
 	k_bbox_final<<<1, 256, 0, c->st>>>(c->part, nb, (unsigned *)(c->d_cnt + 8));
-	k_morton<<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, (const unsigned *)(c->d_cnt + 8), c->k64a, c->v32a,
-			box, box ? skg_box_bits(c) : 0);
+	if (hilbert)
+		k_morton<true><<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, (const unsigned *)(c->d_cnt + 8), c->k64a,
+				c->v32a, box, box ? skg_box_bits(c) : 0);
+	else
+		k_morton<false><<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, (const unsigned *)(c->d_cnt + 8), c->k64a,
+				c->v32a, box, box ? skg_box_bits(c) : 0);
 	SKG_HIP(hipGetLastError());
 	return skg_sort_pairs(c, c->k64a, c->k64b, c->v32a, perm_out, n, 63);
+}
+int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box) {
+	return curve_sort(c, pos, idx, n, perm_out, box, false);
+}
+// Movers: tiles of 64 consecutive movers scan the union of their leaves, so
+// the more compact Hilbert runs (SKG_MOVER_ORDER=morton: Morton).
+static int mover_sort(skg_ctx *c, int n) {
+	static const bool morton = [] { const char *e = getenv("SKG_MOVER_ORDER"); return e && !strcmp(e, "morton"); }();
+	return curve_sort(c, c->m_pos, nullptr, n, c->m_list, c->nBox > 1 ? c->m_box : nullptr, !morton);
 }
 
 // =====================================================================
@@ -1807,7 +1846,7 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	k_compact<<<nblk(n, 256), 256, 0, c->st>>>(nullptr, flag, off, n, c->m_ord);
 	k_mover_init<<<nblk(nm, 256), 256, 0, c->st>>>(c->m_ord, nm, c->d_pos, c->d_boxOf, c->m_pos, c->m_old, c->m_box);
 	c->nActive = nm;
-	if (nm > 0 && (rc = skg_morton_sort(c, c->m_pos, nullptr, nm, c->m_list, c->nBox > 1 ? c->m_box : nullptr)))
+	if (nm > 0 && (rc = mover_sort(c, nm)))
 		return rc;
 	ml_set(c, 0, 0);
 	ml_set(c, 1, nm);
@@ -1891,8 +1930,7 @@ extern "C" int skg_microsteps(skg_ctx *c, int *nScatter) {
 	drop_graph(c);
 	skg_timer_start(c);
 	c->nActive = c->nMove;                                  // kdReactivateMove (kd.c:796-799)
-	if (c->nMove > 0 && (rc = skg_morton_sort(c, c->m_pos, nullptr, c->nMove, c->m_list,
-			c->nBox > 1 ? c->m_box : nullptr))) return rc;
+	if (c->nMove > 0 && (rc = mover_sort(c, c->nMove))) return rc;
 	ml_set(c, 1, c->nMove);
 	if ((rc = count_active(c, c->nMove))) return rc;
 	if (c->nMove > 0 && c->nE > 0) {
