@@ -132,6 +132,7 @@ struct skg_ctx {
 	// device-resident move loop: ml[0] step stamp, ml[1] active, ml[2] next
 	int *d_ml = nullptr;
 	int *m_flag = nullptr, *m_bcnt = nullptr;   // prune flags; block counts then offsets
+	long lastSort = 0;                          // active count at the last mover sort
 	float *e_b2live = nullptr;
 	int *h_pin = nullptr;                        // pinned read-back
 	int useGraph = 0;                            // SKG_GRAPH=1: replay each block as a captured graph

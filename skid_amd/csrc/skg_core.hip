@@ -266,9 +266,28 @@ int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *p
 }
 // Movers: tiles of 64 consecutive movers scan the union of their leaves, so
 // the more compact Hilbert runs (SKG_MOVER_ORDER=morton: Morton).
-static int mover_sort(skg_ctx *c, int n) {
+static bool mover_morton() {
 	static const bool morton = [] { const char *e = getenv("SKG_MOVER_ORDER"); return e && !strcmp(e, "morton"); }();
-	return curve_sort(c, c->m_pos, nullptr, n, c->m_list, c->nBox > 1 ? c->m_box : nullptr, !morton);
+	return morton;
+}
+static int mover_sort(skg_ctx *c, int n) {
+	c->lastSort = n;
+	return curve_sort(c, c->m_pos, nullptr, n, c->m_list, c->nBox > 1 ? c->m_box : nullptr, !mover_morton());
+}
+// The active list (host count c->nActive) is re-sorted by current position
+// once it has fallen below this fraction of its size at the last sort
+// (SKG_MOVER_RESORT; 0: never).
+static double mover_resort_frac() {
+	static const double f = [] { const char *e = getenv("SKG_MOVER_RESORT"); return e ? atof(e) : 0.9; }();
+	return f;
+}
+static int mover_resort(skg_ctx *c) {
+	const int n = c->nActive;
+	c->lastSort = n;
+	int rc = curve_sort(c, c->m_pos, c->m_list, n, c->m_list2, c->nBox > 1 ? c->m_box : nullptr, !mover_morton());
+	if (rc) return rc;
+	SKG_HIP(hipMemcpyAsync(c->m_list, c->m_list2, 4 * (size_t)n, hipMemcpyDeviceToDevice, c->st));
+	return 0;
 }
 
 // =====================================================================
@@ -1912,6 +1931,12 @@ extern "C" int skg_move_block(skg_ctx *c, int *nActive, int *nScatter) {
 		// nScatter of the Ittr line: the live count after the last step (the
 		// prune does not change it)
 		if ((rc = read_counts(c, &ns))) return rc;
+		// Pruning thins the active list in its old order, so runs of 64 active
+		// movers spread out; once half are gone, re-sort the rest by where they
+		// are now (per-mover sums do not depend on the order).
+		if (c->nActive > 0 && (double)c->nActive < mover_resort_frac() * (double)c->lastSort) {
+			if ((rc = mover_resort(c))) return rc;
+		}
 	} else {
 		c->nActive = 0;
 	}
