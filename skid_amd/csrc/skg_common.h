@@ -133,7 +133,6 @@ struct skg_ctx {
 	int *d_ml = nullptr;
 	int *m_flag = nullptr, *m_bcnt = nullptr;   // prune flags; block counts then offsets
 	long lastSort = 0;                          // active count at the last mover sort
-	float *e_b2live = nullptr;
 	int *h_pin = nullptr;                        // pinned read-back
 	int useGraph = 0;                            // SKG_GRAPH=1: replay each block as a captured graph
 	hipGraph_t graph = nullptr;

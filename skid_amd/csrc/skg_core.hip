@@ -331,6 +331,28 @@ __global__ void k_tree_level(float4 *lo, float4 *hi, int first, int count) {
 	hi[i] = make_float4(fmaxf(b0.x, b1.x), fmaxf(b0.y, b1.y), fmaxf(b0.z, b1.z), brange_pack(x0, x1));
 }
 
+// Refit of the entity tree's largest live ball alone: entities do not move and
+// ScatterCut only removes balls, so the position bounds stay as built and only
+// lo.w (the largest live fBall2 below; dead and filler entities carry -1)
+// changes.
+__global__ void k_leaf_ball(const float4 *__restrict__ eaux, int n, int B, int nLeaf, int nLeaf2,
+		const int *__restrict__ lstart, float4 *lo) {
+	const int l = blockIdx.x * blockDim.x + threadIdx.x;
+	if (l >= nLeaf2) return;
+	float w = -INFINITY;
+	if (l < nLeaf) {
+		const int2 r = leaf_range(lstart, l, B, n);
+		for (int i = r.x; i < r.y; ++i) w = fmaxf(w, eaux[i].x);
+	}
+	lo[nLeaf2 + l].w = w;
+}
+__global__ void k_level_ball(float4 *lo, int first, int count) {
+	const int t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= count) return;
+	const int i = first + t;
+	lo[i].w = fmaxf(lo[2 * i].w, lo[2 * i + 1].w);
+}
+
 void skg_free_tree(DevTree *t) { dfree(&t->lo); dfree(&t->hi); dfree(&t->lstart); t->n = t->nLeaf = t->nLeaf2 = 0; }
 
 static int tree_fill(skg_ctx *c, DevTree *t, const float4 *pos, const float *b2, const int *ibox) {
@@ -1098,20 +1120,21 @@ __global__ __launch_bounds__(256) void k_min_final(const float *__restrict__ par
 // One wavefront per box folds that box's block partials (a box's entities
 // fill whole 256-entity blocks).
 __global__ __launch_bounds__(64) void k_box_min(const float *__restrict__ part, const int *__restrict__ eblk,
-		float *out) {
+		float *out, int *liveTotal) {
 	const int b = blockIdx.x;
+	if (b == 0 && threadIdx.x == 0) liveTotal[0] = 0;       // summed by k_box_sum after the cut
 	float v = INFINITY;
 	for (int i = eblk[b] + (int)threadIdx.x; i < eblk[b + 1]; i += 64) v = fminf(v, part[i]);
 	v = wave_min(v);
 	if (threadIdx.x == 0) out[b] = v;
 }
 __global__ __launch_bounds__(64) void k_box_sum(const int *__restrict__ part, const int *__restrict__ eblk,
-		int *out) {
+		int *out, int *liveTotal) {
 	const int b = blockIdx.x;
 	int v = 0;
 	for (int i = eblk[b] + (int)threadIdx.x; i < eblk[b + 1]; i += 64) v += part[i];
 	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-	if (threadIdx.x == 0) out[b] = v;
+	if (threadIdx.x == 0) { out[b] = v; atomicAdd(liveTotal, v); }
 }
 
 // ScatterCut (smooth1.c:387-405): keep entities with density >= their box's
@@ -1132,15 +1155,6 @@ __global__ __launch_bounds__(256) void k_scat_cut(int *elive, float4 *eaux, cons
 	if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = c;
 	__syncthreads();
 	if (threadIdx.x == 0) part[blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
-}
-__global__ __launch_bounds__(256) void k_sum_final(const int *__restrict__ part, int nb, int *out) {
-	__shared__ int sm[4];
-	int v = 0;
-	for (int b = threadIdx.x; b < nb; b += 256) v += part[b];
-	for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-	if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
-	__syncthreads();
-	if (threadIdx.x == 0) out[0] = sm[0] + sm[1] + sm[2] + sm[3];
 }
 
 // Device-side move-loop bookkeeping: ml[0] step stamp, ml[1] active count,
@@ -1203,10 +1217,6 @@ __global__ void k_copy_list(const int *__restrict__ in, int *ml, int *out) {
 	if (i < n) out[i] = in[i];
 }
 
-__global__ void k_live_b2(const float4 *__restrict__ eaux, int ne, float *b2live) {
-	int e = blockIdx.x * blockDim.x + threadIdx.x;
-	if (e < ne) b2live[e] = eaux[e].x;
-}
 
 // kdPruneInactive test (kd.c:754-776): a mover stays active while it moved at
 // least fCvg over the last block; survivors restart from here.
@@ -1386,7 +1396,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2); dfree(&c->m_box);
 	dfree(&c->tl_list); dfree(&c->tl_cnt);
 	skg_drop_graph(c);
-	dfree(&c->d_ml); dfree(&c->m_flag); dfree(&c->m_bcnt); dfree(&c->e_b2live);
+	dfree(&c->d_ml); dfree(&c->m_flag); dfree(&c->m_bcnt);
 	if (c->h_pin) { (void)hipHostFree(c->h_pin); c->h_pin = nullptr; }
 	dfree(&c->d_bt); dfree(&c->d_boxOf); dfree(&c->d_type); dfree(&c->d_eblk); dfree(&c->d_bact); dfree(&c->d_bscat);
 	dfree(&c->d_bthr);
@@ -1782,14 +1792,15 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 #undef MV
 		if (pslot >= 0) skg_prof_end(c, pslot, SKG_K_COVER_MOVE, 24.0 * c->nActive + 24.0 * c->nLive);
 		k_scat_thr<<<nb, 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->nE, c->d_ml, bInitial, c->part);
-		k_box_min<<<c->nBox, 64, 0, c->st>>>(c->part, c->d_eblk, c->d_bthr);
+		k_box_min<<<c->nBox, 64, 0, c->st>>>(c->part, c->d_eblk, c->d_bthr, c->d_cnt);
 		k_scat_cut<<<nb, 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE, c->e_box, c->d_bthr, (int *)c->part);
-		k_box_sum<<<c->nBox, 64, 0, c->st>>>((const int *)c->part, c->d_eblk, c->d_bscat);
-		k_sum_final<<<1, 256, 0, c->st>>>((const int *)c->part, nb, c->d_cnt);
+		k_box_sum<<<c->nBox, 64, 0, c->st>>>((const int *)c->part, c->d_eblk, c->d_bscat, c->d_cnt);
 	}
 	SKG_HIP(hipGetLastError());
 	return 0;
 }
+
+static int refit_entities(skg_ctx *c);
 
 // kdPruneInactive (kd.c:735-793) + compaction + the entity tree refitted to
 // the live balls.
@@ -1806,13 +1817,16 @@ static int enqueue_prune(skg_ctx *c, long cap) {
 	k_ml_copy<<<1, 1, 0, c->st>>>(c->d_ml);
 	int rc;
 	if ((rc = count_active(c, cap))) return rc;
-	k_live_b2<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_aux, c->nE, c->e_b2live);
-	return skg_refit_tree(c, &c->te, c->e_pos, c->e_b2live, c->e_box);
+	return refit_entities(c);
 }
 
 static int refit_entities(skg_ctx *c) {
-	k_live_b2<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_aux, c->nE, c->e_b2live);
-	return skg_refit_tree(c, &c->te, c->e_pos, c->e_b2live, c->e_box);
+	DevTree *t = &c->te;
+	k_leaf_ball<<<nblk(t->nLeaf2, 256), 256, 0, c->st>>>(c->e_aux, t->n, t->B, t->nLeaf, t->nLeaf2, t->lstart, t->lo);
+	for (int m = t->nLeaf2 >> 1; m >= 1; m >>= 1)
+		k_level_ball<<<nblk(m, 256), 256, 0, c->st>>>(t->lo, m, m);
+	SKG_HIP(hipGetLastError());
+	return 0;
 }
 
 // Host copies of the live-entity total and the active count.
@@ -1852,8 +1866,7 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	drop_graph(c);
 	const size_t nbm = (size_t)nblk(std::max(nm, 1), 256);
 	if (dalloc(&c->m_ord, nm) || dalloc(&c->m_pos, nm) || dalloc(&c->m_old, nm) || dalloc(&c->m_list, nm) ||
-	    dalloc(&c->m_list2, nm) || dalloc(&c->m_box, nm) || dalloc(&c->m_flag, nm) || dalloc(&c->m_bcnt, 2 * nbm) ||
-	    dalloc(&c->e_b2live, std::max(c->nE, 1)))
+	    dalloc(&c->m_list2, nm) || dalloc(&c->m_box, nm) || dalloc(&c->m_flag, nm) || dalloc(&c->m_bcnt, 2 * nbm))
 		return SKG_E_NOMEM;
 	if (!c->d_ml && dalloc(&c->d_ml, 4)) return SKG_E_NOMEM;
 	if (!c->h_pin && hipHostMalloc((void **)&c->h_pin, 32, hipHostMallocDefault) != hipSuccess) return SKG_E_NOMEM;
