@@ -346,6 +346,7 @@ int skg_build_tree_prefix(skg_ctx *c, DevTree *t, const float4 *pos, const unsig
 int skg_refit_tree(skg_ctx *c, DevTree *t, const float4 *pos, const float *ball2live, const int *ibox);
 int skg_copy_tree_leaves(skg_ctx *c, DevTree *dst, const DevTree *src);
 int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box);
+int skg_hilbert_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box);
 int skg_box_bits(const skg_ctx *c);
 int skg_exclusive_scan(skg_ctx *c, const int *in, int *out, int n);
 void skg_free_tree(DevTree *t);

@@ -264,6 +264,9 @@ static int curve_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int 
 int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box) {
 	return curve_sort(c, pos, idx, n, perm_out, box, false);
 }
+int skg_hilbert_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box) {
+	return curve_sort(c, pos, idx, n, perm_out, box, true);
+}
 // Movers: tiles of 64 consecutive movers scan the union of their leaves, so
 // the more compact Hilbert runs (SKG_MOVER_ORDER=morton: Morton).
 static bool mover_morton() {
@@ -1592,7 +1595,9 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		if ((rc = skg_exclusive_scan(c, flag, off, n))) return rc;
 		k_compact<<<nblk(n, 256), 256, 0, c->st>>>(nullptr, flag, off, n, tmpo);
 		k_gather_scat<<<nblk(nS, 256), 256, 0, c->st>>>(c->d_pos, c->d_mass, tmpo, c->d_boxOf, nS, tmp4, tmpb);
-		if ((rc = skg_morton_sort(c, tmp4, nullptr, nS, perm, nBox > 1 ? tmpb : nullptr))) return rc;
+		// scatterers along the Hilbert curve: the kNN tree's leaves and the
+		// wavefronts' runs of 64 queries are compact (no Morton jumps)
+		if ((rc = skg_hilbert_sort(c, tmp4, nullptr, nS, perm, nBox > 1 ? tmpb : nullptr))) return rc;
 		k_permute_s<<<nblk(nS, 256), 256, 0, c->st>>>(tmp4, tmpo, tmpb, perm, nS, c->s_pos, c->s_ord, c->s_box);
 		if ((rc = skg_build_tree(c, &c->ts, c->s_pos, nS, B, nullptr, nBox > 1 ? c->s_box : nullptr))) return rc;
 		SKG_HIP(hipStreamSynchronize(c->st));
