@@ -184,7 +184,7 @@ static int fof_run(skg_ctx *c, const float4 *pos, int n, float tau, const int *o
 	if (dalloc(&perm, n) || dalloc(&par, n) || dalloc(&cmin, n) || dalloc(&isFirst, n) || dalloc(&rank, n) ||
 	    dalloc(&fp, n) || (box && dalloc(&fbox, n)))
 		return SKG_E_NOMEM;
-	if ((rc = skg_morton_sort(c, pos, nullptr, n, perm, box))) return rc;
+	if ((rc = skg_hilbert_sort(c, pos, nullptr, n, perm, box))) return rc;   // compact runs of 64 points
 	k_gather4<<<nblk(n, 256), 256, 0, c->st>>>(pos, perm, n, fp, box, fbox);
 	if ((rc = skg_build_tree(c, &t, fp, n, c->P.nBucket, nullptr, fbox))) return rc;
 	if (dalloc(&united, 2 * (size_t)t.nLeaf2)) return SKG_E_NOMEM;
