@@ -1653,7 +1653,9 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		k_rep_write<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_pos, c->s_ball2, nS, mask, roff,
 				per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, e4u, eb2u, esrcu);
 		k_entity_box<<<nblk(nEu, 256), 256, 0, c->st>>>(esrcu, c->s_box, nEu, ebu);
-		if ((rc = skg_morton_sort(c, e4u, nullptr, nEu, perm2, nBox > 1 ? ebu : nullptr))) return rc;
+		// Hilbert order: its key prefixes are boxes too (octree cells and their
+		// halves and quarters), and consecutive leaves are face neighbours
+		if ((rc = skg_hilbert_sort(c, e4u, nullptr, nEu, perm2, nBox > 1 ? ebu : nullptr))) return rc;
 		// each box's entities start on a 256-entity block (and so on a leaf)
 		k_fill_i<<<nblk(nBox, 256), 256, 0, c->st>>>(bfirst, nBox, -1);
 		k_box_first<<<nblk(nEu, 256), 256, 0, c->st>>>(ebu, perm2, nEu, bfirst);
