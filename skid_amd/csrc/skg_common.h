@@ -84,7 +84,8 @@ struct skg_ctx {
 	// entities S' = S + replicas
 	int nE = 0, nRep = 0, nLive = 0;
 	float4 *e_pos = nullptr, *e_aux = nullptr;
-	float4 *e_org = nullptr;     // source scatterer position; w = 1 for a periodic replica
+	float4 *e_org = nullptr;     // source scatterer position; w = its image code (as e_code)
+	int *e_code = nullptr;       // image code 9(ix+1) + 3(iy+1) + (iz+1) (13: the source itself)
 	float *e_b2 = nullptr;       // fBall2 as built (all entities live)
 	float *e_rho = nullptr;      // effective density (0 once zeroed by the initial cut)
 	int *e_src = nullptr;        // scatterer (sorted S) index

@@ -631,8 +631,8 @@ __global__ __launch_bounds__(64) void k_knn(
 template <int MODE, bool COMP>
 __device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const float4 &p,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
-		int *__restrict__ ehit, int stamp, float Lx, float Ly, float Lz, Sum<COMP> &s0, Sum<COMP> &s1,
-		Sum<COMP> &s2) {
+		const int *__restrict__ ecode, int *__restrict__ ehit, int stamp, float Lx, float Ly, float Lz,
+		Sum<COMP> &s0, Sum<COMP> &s1, Sum<COMP> &s2) {
 	if (cnt <= 0 || !want) return;
 	for (int t0 = 0; t0 < cnt; t0 += 4) {
 		// four consecutive records in one scalar load each; the entity arrays
@@ -640,8 +640,9 @@ __device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const 
 		// leaf's end stay in the allocation (their values are never used)
 		const float4 *ap = epos + start + t0, *xp = eaux + start + t0;
 		float4 A4[4], X4[4];
+		int K4[4];                                         // image codes: a hit waits on no load
 #pragma unroll
-		for (int u = 0; u < 4; ++u) { A4[u] = ap[u]; X4[u] = xp[u]; }
+		for (int u = 0; u < 4; ++u) { A4[u] = ap[u]; X4[u] = xp[u]; if (COMP) K4[u] = ecode[start + t0 + u]; }
 #pragma unroll
 		for (int u = 0; u < 4; ++u) {
 			if (t0 + u >= cnt) break;
@@ -652,9 +653,9 @@ __device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const 
 			if (d2 < X.x) {
 				if (COMP) {
 					// replica: separation from the source position plus the exact shift
-					const float4 O = eorg[e];
-					const int code = (int)O.w;     // 9(ix+1) + 3(iy+1) + (iz+1)
+					const int code = K4[u];        // 9(ix+1) + 3(iy+1) + (iz+1)
 					if (code != 13) {
+						const float4 O = eorg[e];
 						dx = exact_delta(O.x, p.x, (float)(code / 9 - 1) * Lx);
 						dy = exact_delta(O.y, p.y, (float)((code / 3) % 3 - 1) * Ly);
 						dz = exact_delta(O.z, p.z, (float)(code % 3 - 1) * Lz);
@@ -729,8 +730,8 @@ template <int MODE, bool COMP>
 __global__ __launch_bounds__(SKG_TILE_T) void k_cover(
 		float4 *__restrict__ ppos, const int *__restrict__ plist, int np,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
-		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
-		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
+		const int *__restrict__ ecode, int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi,
+		int nLeaf2, int B, float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
 		const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
 		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart,
 		const int *__restrict__ tlist, const int *__restrict__ tcnt, int tcap, const int *__restrict__ npDev,
@@ -766,8 +767,8 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover(
 				point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
 		if (!__any(want)) continue;
 		const int2 rg = leaf_range(lstart, leaf, B, ne);
-		cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), want, p, epos, eaux, eorg, ehit, stamp,
-				Lx, Ly, Lz, s0, s1, s2);
+		cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), want, p, epos, eaux, eorg, ecode, ehit,
+				stamp, Lx, Ly, Lz, s0, s1, s2);
 	}
 	while (node) {
 		node = uniform_int(node);
@@ -777,8 +778,8 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover(
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit) {
 			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
-			cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), need, p, epos, eaux, eorg, ehit,
-					stamp, Lx, Ly, Lz, s0, s1, s2);
+			cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), need, p, epos, eaux, eorg, ecode,
+					ehit, stamp, Lx, Ly, Lz, s0, s1, s2);
 		}
 		// next node in pre-order after node's subtree (strip trailing ones, step)
 		node = (node >> __builtin_ctz(~node)) + 1;
@@ -1011,7 +1012,7 @@ __global__ void k_box_first(const int *__restrict__ ebu, const int *__restrict__
 }
 // Filler entities of the padded array: dead, NaN position, never hit.
 __global__ void k_entity_fill(int ne, const int *__restrict__ eblk, int nBox, float4 *e4, float4 *eaux,
-		float4 *eorg, float *erho, int *esrc, int *elive, int *ehit, float *eb2s, int *ebox,
+		float4 *eorg, int *ecode, float *erho, int *esrc, int *elive, int *ehit, float *eb2s, int *ebox,
 		unsigned long long *ekey, int bbits) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= ne) return;
@@ -1023,6 +1024,7 @@ __global__ void k_entity_fill(int ne, const int *__restrict__ eblk, int nBox, fl
 	e4[i] = make_float4(NAN, NAN, NAN, 0.f);
 	eaux[i] = make_float4(-1.f, 0.f, 0.f, 0.f);
 	eorg[i] = make_float4(NAN, NAN, NAN, 13.f);
+	ecode[i] = 13;
 	erho[i] = 0.f; esrc[i] = -1; elive[i] = 0; ehit[i] = -1; eb2s[i] = -1.f; ebox[i] = lo;
 	// after every real key of the box
 	ekey[i] = ((unsigned long long)lo << (63 - bbits)) | ((1ull << (63 - bbits)) - 1);
@@ -1034,8 +1036,8 @@ __global__ void k_entity_setup(const float4 *__restrict__ e4u, const float *__re
 		const int *__restrict__ esrcu, const int *__restrict__ perm, int ne,
 		const int *__restrict__ ebu, const int *__restrict__ bfirst, const int *__restrict__ eblk,
 		const float *__restrict__ srho, const float4 *__restrict__ spos, float Lx, float Ly, float Lz,
-		float4 *e4, float4 *eaux, float4 *eorg, float *erho, int *esrc, int *elive, int *ehit, float *eb2s,
-		const unsigned long long *__restrict__ ksorted, unsigned long long *ekey) {
+		float4 *e4, float4 *eaux, float4 *eorg, int *ecode, float *erho, int *esrc, int *elive, int *ehit,
+		float *eb2s, const unsigned long long *__restrict__ ksorted, unsigned long long *ekey) {
 	int ii = blockIdx.x * blockDim.x + threadIdx.x;
 	if (ii >= ne) return;
 	int j = perm[ii];
@@ -1054,6 +1056,7 @@ __global__ void k_entity_setup(const float4 *__restrict__ e4u, const float *__re
 		ix = (int)rintf((p.x - o.x) / Lx); iy = (int)rintf((p.y - o.y) / Ly); iz = (int)rintf((p.z - o.z) / Lz);
 	}
 	eorg[i] = make_float4(o.x, o.y, o.z, (float)(9 * (ix + 1) + 3 * (iy + 1) + (iz + 1)));
+	ecode[i] = 9 * (ix + 1) + 3 * (iy + 1) + (iz + 1);
 	eaux[i] = make_float4(b2, ih2, gnorm, dnorm);
 	eb2s[i] = b2;
 	esrc[i] = esrcu[j];
@@ -1394,7 +1397,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->s_pos); dfree(&c->s_ord); dfree(&c->s_ball2); dfree(&c->s_rhoG); dfree(&c->s_rhoS);
 	dfree(&c->s_heap); dfree(&c->s_box);
 	skg_free_tree(&c->ts); skg_free_tree(&c->te); skg_free_tree(&c->tfull);
-	dfree(&c->e_b2); dfree(&c->e_pos); dfree(&c->e_aux); dfree(&c->e_org); dfree(&c->e_rho); dfree(&c->e_src); dfree(&c->e_live); dfree(&c->e_hit);
+	dfree(&c->e_b2); dfree(&c->e_pos); dfree(&c->e_aux); dfree(&c->e_org); dfree(&c->e_code); dfree(&c->e_rho); dfree(&c->e_src); dfree(&c->e_live); dfree(&c->e_hit);
 	dfree(&c->e_box);
 	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2); dfree(&c->m_box);
 	dfree(&c->tl_list); dfree(&c->tl_cnt);
@@ -1538,7 +1541,7 @@ static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, in
 	const bool comp = c->P.accumulate == SKG_ACC_COMPENSATED;
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
 #define CL(M_, C_)                                                                             \
-	k_cover<M_, C_><<<nb, SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, \
+	k_cover<M_, C_><<<nb, SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->e_org, c->e_code, c->nE, c->te.lo, \
 			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, \
 			C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart, nullptr, nullptr, c->tlCap, nullptr, nullptr)
 #define CS(M_, C_)                                                                             \
@@ -1680,16 +1683,16 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		float *eb2s = nullptr;
 		unsigned long long *ekey = nullptr;
 		if (dalloc(&ekey, nE) || dalloc(&eb2s, nE) || dalloc(&c->e_pos, nE + SKG_EPAD) || dalloc(&c->e_aux, nE + SKG_EPAD) || dalloc(&c->e_rho, nE) ||
-		    dalloc(&c->e_org, nE) || dalloc(&c->e_src, nE) || dalloc(&c->e_live, nE) || dalloc(&c->e_hit, nE) ||
+		    dalloc(&c->e_org, nE) || dalloc(&c->e_code, nE + SKG_EPAD) || dalloc(&c->e_src, nE) || dalloc(&c->e_live, nE) || dalloc(&c->e_hit, nE) ||
 		    dalloc(&c->e_box, nE) || dalloc(&c->d_eblk, nBox + 1))
 			return SKG_E_NOMEM;
 		SKG_HIP(hipMemcpyAsync(c->d_eblk, c->h_eblk, 4 * (size_t)(nBox + 1), hipMemcpyHostToDevice, c->st));
 		const int bbits = nBox > 1 ? skg_box_bits(c) : 0;
-		k_entity_fill<<<nblk(nE, 256), 256, 0, c->st>>>(nE, c->d_eblk, nBox, c->e_pos, c->e_aux, c->e_org, c->e_rho,
+		k_entity_fill<<<nblk(nE, 256), 256, 0, c->st>>>(nE, c->d_eblk, nBox, c->e_pos, c->e_aux, c->e_org, c->e_code, c->e_rho,
 				c->e_src, c->e_live, c->e_hit, eb2s, c->e_box, ekey, bbits);
 		k_entity_setup<<<nblk(nEu, 256), 256, 0, c->st>>>(e4u, eb2u, esrcu, perm2, nEu, ebu, bfirst, c->d_eblk,
 				nullptr, c->s_pos, per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, c->e_pos, c->e_aux,
-				c->e_org, c->e_rho, c->e_src, c->e_live, c->e_hit, eb2s, c->k64b, ekey);
+				c->e_org, c->e_code, c->e_rho, c->e_src, c->e_live, c->e_hit, eb2s, c->k64b, ekey);
 		// leaves: maximal key cells of at most B entities, never across boxes
 		if ((rc = skg_build_tree_prefix(c, &c->te, c->e_pos, ekey, nE, B, 1 + bbits, eb2s, c->e_box))) return rc;
 		SKG_HIP(hipStreamSynchronize(c->st));
@@ -1792,7 +1795,7 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 				c->te.B, nullptr, c->e_hit, 0, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2],         \
 				c->te.lstart, c->d_ml + 1, c->d_ml);                                                             \
 		else k_cover<1, C_><<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap,    \
-				c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, nullptr,         \
+				c->e_pos, c->e_aux, c->e_org, c->e_code, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, nullptr, \
 				c->e_hit, 0, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart,             \
 				lists ? c->tl_list : nullptr, lists ? c->tl_cnt : nullptr, c->tlCap, c->d_ml + 1, c->d_ml)
 		if (comp) { MV(true); } else { MV(false); }
