@@ -727,7 +727,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__rest
 // leaf, so every entity a lane tests is of its own box.  MODE 1 takes the
 // box's step (micro: its micro-step).
 template <int MODE, bool COMP>
-__global__ __launch_bounds__(SKG_TILE_T) void k_cover(
+__global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cover(
 		float4 *__restrict__ ppos, const int *__restrict__ plist, int np,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
 		const int *__restrict__ ecode, int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi,
