@@ -173,6 +173,7 @@ def bench_batched(args):
         os.environ["SKG_COVER_SUB_MAX"] = str(sub_max)
         os.environ["SKG_POT_TILE"] = str(tile)
     P = skg.make_params(0.01, 32, nBucket=nb, period=1.0, accumulate=skg.SKG_ACC_COMPENSATED)
+    skg.device_memory(reset_peak=True)
     ctx = skg.BatchContext(boxes, P, taus)           # particles resident in HBM from here on
     t = time.time()
     tuned = ctx.autotune(0 if args.shapes else skg.SKG_TUNE_ALL)
@@ -196,6 +197,7 @@ def bench_batched(args):
               "at_risk": rp["at_risk"], "groups_with_risk": int((rp["group_frac_at_risk"][1:] > 0).sum())}
     log("nSmooth sensitivity report:", report)
     kst = ctx.kernel_stats()
+    dev_peak = skg.device_memory()[1]
     st = {k: (v - st0[k]) / args.steps for k, v in ctx.stage_times().items()}
     log("stage ms per step:", {k: round(v, 1) for k, v in st.items()})
     log("kernels:", json.dumps({k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
@@ -248,7 +250,8 @@ def bench_batched(args):
                           f"tau 0.16 N^-1/3 per box, end-to-end group finding of all boxes per step, plus "
                           f"d rho / d ln nSmooth per particle (the membership report from it is formed once, untimed)",
               "n_particles": N, "boxes": len(boxes), "form": "batched+adjoint+compensated",
-              "autotune": tuned, "knn_queries_per_s": knn_qps, "forms": forms, "plain_gpu": plain,
+              "autotune": tuned, "knn_queries_per_s": knn_qps, "device_peak_bytes": dev_peak,
+              "forms": forms, "plain_gpu": plain,
               "nsmooth_report": report}
     return value, wall, config, roofline(kst, measured_traffic("traffic_batched_256_boxes.md")), cpu
 
@@ -262,6 +265,7 @@ def bench_two_species(args):
     box = gen.config3(n_each=n_each)
     log(f"generated {box.n} particles in {time.time() - t:.1f}s")
     P = skg.make_params(tau, nSmooth=64, nBucket=16, period=1.0, bGasAndDark=1, accumulate=skg.SKG_ACC_COMPENSATED)
+    skg.device_memory(reset_peak=True)
     ctx = skg.Context(box, P)
     for w in range(args.warmup):
         t0 = time.perf_counter()
@@ -275,6 +279,8 @@ def bench_two_species(args):
         log(f"step {s}: {time.perf_counter() - ts:.3f}s ittr={len(tr)} groups={ng - 1}")
     wall = time.perf_counter() - t0
     kst = ctx.kernel_stats()
+    dev_peak = skg.device_memory()[1]
+    log(f"device memory high-water mark {dev_peak / 2**30:.2f} GiB")
     ctx.close()
     knn = kst["knn"]
     cpu = None
@@ -287,7 +293,7 @@ def bench_two_species(args):
                    sample=f"same recipe at 2^16+2^16 particles, tau {tc:.4g}, {dt:.2f}s on 1 core")
     config = {"workload": f"two-species box 2^{le} gas + 2^{le} dark (8000 NFW halos, dN/dM~M^-1.9), nSmooth 64, "
                           f"nBucket 16, tau {tau:.4g}, -gd, periodic unit box, end-to-end group finding",
-              "n_particles": box.n, "form": "compensated",
+              "n_particles": box.n, "form": "compensated", "device_peak_bytes": dev_peak,
               "knn_queries_per_s": (knn["work"] / 16.0) / (knn["ms"] * 1e-3) if knn["ms"] > 0 else None}
     return box.n / (wall / args.steps), wall, config, roofline(kst), cpu
 
@@ -305,6 +311,7 @@ def bench_unbind(args):
     tau = 0.16 * n ** (-1.0 / 3.0)
     fs, fc = gen.config5_cosmology()
     P = skg.make_params(tau, 32, period=1.0, z=0.5, fShift=fs, fCosmo=fc)
+    skg.device_memory(reset_peak=True)
     ctx = skg.Context(box, P)
 
     # the timed step is the reference's -unbind path: kdInGroup/kdInitpGroup/
@@ -328,6 +335,7 @@ def bench_unbind(args):
         log(f"step {s}: {time.perf_counter() - ts:.3f}s unbound={nu} groups={ngf - 1}")
     wall = time.perf_counter() - t0
     kst = ctx.kernel_stats()
+    dev_peak = skg.device_memory()[1]
     # per-group FP64 binding energies of the input groups, outside the timed steps
     ts = time.perf_counter()
     ng = ctx.set_groups(grp)
@@ -367,7 +375,7 @@ def bench_unbind(args):
     config = {"workload": f"unbinding stress: {n} particles, {args.groups} power-law groups ({members} members, "
                           f"largest {int(sz.max())}, sum n(n-1) {pairs:.3g}), z=0.5 flat Lambda, -unbind re-entry, "
                           f"FP32 compensated potentials, FP64 energies and sign test",
-              "n_particles": n, "members": members, "form": "unbind"}
+              "n_particles": n, "members": members, "form": "unbind", "device_peak_bytes": dev_peak}
     return value, wall, config, roof, cpu
 
 

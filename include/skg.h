@@ -80,6 +80,13 @@ const char *skg_strerror(int status);
    counterpart: SKID's CPU code has no runtime errors of this kind. */
 const char *skg_last_error(void);
 
+/* Device memory held by the library in this process, over all contexts: bytes
+   allocated now and the high-water mark since load (or since the last call with
+   reset_peak nonzero, which sets the mark to the current bytes).  No reference
+   counterpart: SKID keeps its arrays in host memory (kd.c malloc); this is for
+   sizing runs against one GPU's HBM. */
+int skg_device_memory(long long *current, long long *peak, int reset_peak);
+
 /* Particle input in tipsy species order, n = nGas + nDark + nStar; replaces the
    array fill of kdReadTipsy (kd.c:122-222).  pos/vel are n*3 floats; vel, temp
    may be NULL. */

@@ -357,11 +357,12 @@ int skg_prof_begin(skg_ctx *c);                        // returns a record slot 
 void skg_prof_end(skg_ctx *c, int slot, int which, double work);
 void skg_prof_flush(skg_ctx *c);
 void skg_timer_stop(skg_ctx *c, int slot);
+// Every device allocation of the library goes through these two, which keep the
+// process-wide byte count and its high-water mark (skg_device_memory).
+int skg_dmalloc(void **p, size_t bytes);
+void skg_dfree(void **p);
 template <typename T> static inline int dalloc(T **p, size_t count) {
-	if (*p) { (void)hipFree(*p); *p = nullptr; }
 	if (count == 0) count = 1;
-	return hipMalloc((void **)p, count * sizeof(T)) == hipSuccess ? 0 : SKG_E_NOMEM;
+	return skg_dmalloc((void **)p, count * sizeof(T));
 }
-template <typename T> static inline void dfree(T **p) {
-	if (*p) { (void)hipFree(*p); *p = nullptr; }
-}
+template <typename T> static inline void dfree(T **p) { skg_dfree((void **)p); }

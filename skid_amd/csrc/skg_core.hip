@@ -13,17 +13,52 @@
 #include <cstring>
 #include <vector>
 #include <chrono>
+#include <mutex>
+#include <unordered_map>
 
 static const unsigned long long KEY_SENT = 0x7F800000FFFFFFFFull;   // d2 = +inf, index = max
 
 // =====================================================================
 // scratch, timers, sorting
 // =====================================================================
+static std::mutex g_mem_mu;
+static std::unordered_map<void *, size_t> g_mem_live;
+static long long g_mem_cur = 0, g_mem_peak = 0;
+
+int skg_dmalloc(void **p, size_t bytes) {
+	skg_dfree(p);
+	if (hipMalloc(p, bytes) != hipSuccess) { *p = nullptr; return SKG_E_NOMEM; }
+	std::lock_guard<std::mutex> lk(g_mem_mu);
+	g_mem_live[*p] = bytes;
+	g_mem_cur += (long long)bytes;
+	g_mem_peak = std::max(g_mem_peak, g_mem_cur);
+	return 0;
+}
+
+void skg_dfree(void **p) {
+	if (!*p) return;
+	{
+		std::lock_guard<std::mutex> lk(g_mem_mu);
+		auto it = g_mem_live.find(*p);
+		if (it != g_mem_live.end()) { g_mem_cur -= (long long)it->second; g_mem_live.erase(it); }
+	}
+	(void)hipFree(*p);
+	*p = nullptr;
+}
+
+extern "C" int skg_device_memory(long long *current, long long *peak, int reset_peak) {
+	std::lock_guard<std::mutex> lk(g_mem_mu);
+	if (current) *current = g_mem_cur;
+	if (peak) *peak = g_mem_peak;
+	if (reset_peak) g_mem_peak = g_mem_cur;
+	return SKG_OK;
+}
+
 int skg_ensure_tmp(skg_ctx *c, size_t bytes) {
 	if (bytes <= c->tmpBytes && c->tmp) return 0;
-	if (c->tmp) { (void)hipFree(c->tmp); c->tmp = nullptr; c->tmpBytes = 0; }
+	c->tmpBytes = 0;
 	bytes = std::max<size_t>(bytes, 1 << 20);
-	if (hipMalloc(&c->tmp, bytes) != hipSuccess) return SKG_E_NOMEM;
+	if (skg_dmalloc(&c->tmp, bytes)) return SKG_E_NOMEM;
 	c->tmpBytes = bytes;
 	return 0;
 }
@@ -1408,7 +1443,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->d_bthr);
 	delete[] c->hbox; delete[] c->h_eblk;
 	dfree(&c->d_thr); dfree(&c->d_cnt);
-	if (c->tmp) (void)hipFree(c->tmp);
+	skg_dfree(&c->tmp);
 	dfree(&c->k64a); dfree(&c->k64b); dfree(&c->v32a); dfree(&c->v32b); dfree(&c->part);
 	for (int i = 0; i < c->caprec; ++i) { (void)hipEventDestroy(c->prec[i].a); (void)hipEventDestroy(c->prec[i].b); }
 	free(c->prec);

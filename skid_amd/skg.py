@@ -62,6 +62,7 @@ def lib():
         L.skg_strerror.restype = C.c_char_p
         L.skg_last_error.argtypes = []
         L.skg_last_error.restype = C.c_char_p
+        L.skg_device_memory.argtypes = [C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), C.c_int]
         L.skg_load.argtypes = [vp, C.c_int, C.c_int, C.c_int, fp, fp, fp, fp, fp]
         L.skg_load_batch.argtypes = [vp, C.c_int, fp, fp, fp, fp, fp, fp, fp, fp, fp]
         L.skg_box_counts.argtypes = [vp, fp, fp]
@@ -91,6 +92,13 @@ def lib():
                                        C.POINTER(C.c_double)]
         _lib = L
     return _lib
+
+
+def device_memory(reset_peak: bool = False) -> tuple[int, int]:
+    """(bytes the library holds on the device now, high-water mark), process-wide."""
+    cur, peak = C.c_longlong(), C.c_longlong()
+    _check(lib().skg_device_memory(C.byref(cur), C.byref(peak), int(reset_peak)), "skg_device_memory")
+    return cur.value, peak.value
 
 
 def _p(a):

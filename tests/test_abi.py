@@ -36,6 +36,8 @@ def test_parameter_validation_without_compute():
     assert L.skg_create(None, C.byref(h)) == -2
     assert L.skg_strerror(-6).decode().startswith("nSmooth")
     assert L.skg_last_error().decode() == ""                    # no HIP failure on this thread
+    cur, peak = skg.device_memory()
+    assert 0 <= cur <= peak                                      # process-wide byte count and high-water mark
     assert L.skg_destroy(None) == 0
     for fn in ("skg_density", "skg_init_move", "skg_fof", "skg_unbind", "skg_centers"):
         assert getattr(L, fn)(None, *([None] * (len(getattr(L, fn).argtypes) - 1))) == -2

@@ -83,3 +83,16 @@ def test_knn_radii_at_2pow22_against_fp64_tree():
     print(f"2^22: kNN radius^2 vs FP64 tree on {idx.size} queries: max rel {(err / ref).max():.2e}, "
           f"worst err/bound {(err / bound)[w]:.2f} at {np.sqrt(ref[w]):.2e} from a face {face:.2e} away")
     assert np.all(err <= bound)
+
+
+def test_device_memory_is_accounted_and_released():
+    cur0, _ = skg.device_memory(reset_peak=True)
+    n = 1 << 16
+    box = gen.config4_box(n, 12)
+    ctx = skg.Context(box, skg.make_params(gen.config4_tau(n), 32, period=1.0))
+    ctx.run()
+    cur, peak = skg.device_memory()
+    # at least the particle arrays (pos, vel, mass, soft, rho, ball2, group: 40 B each)
+    assert cur - cur0 >= 40 * n and peak >= cur
+    ctx.close()
+    assert skg.device_memory()[0] == cur0
