@@ -147,8 +147,8 @@ __global__ __launch_bounds__(64) void k_centers(const int *__restrict__ gs, cons
 __device__ __forceinline__ float pot_kernel(int soft, float d2, float twoh) {
 	if (soft == SKG_PLUMMER) return __builtin_amdgcn_rsqf(fmaf(0.25f * twoh, twoh, d2));
 	const float ir = __builtin_amdgcn_rsqf(d2);
+	if (d2 >= twoh * twoh) return ir;            // outside the kernel (r >= 2h): 1/r
 	const float r = d2 > 0.f ? d2 * ir : 0.f;
-	if (r >= twoh) return ir;
 	const float dih = 2.0f * __builtin_amdgcn_rcpf(twoh), u = r * dih, u2 = u * u;
 	if (u < 1.0f) {
 		float p = fmaf(-0.1f, u, 0.3f);
@@ -195,8 +195,51 @@ __global__ void k_unb_setup(const int *__restrict__ gs, const int *__restrict__ 
 // (kdCellPot, grav.c:8-36): a workgroup holds POT_T targets in registers and
 // streams the group's members through LDS in tiles of POT_T (both orders of
 // each pair are evaluated; no atomics, fixed summation order).  POT_T (64,
-// 128 or 256) is chosen by the autotuner.
-template <int POT_T>
+// 128 or 256) is chosen by the autotuner.  Summation: the terms are added in
+// fp32 partials of BLK consecutive sources and each partial enters a two-sum
+// accumulator, so the rounding of the whole sum stays that of one BLK-term
+// partial (at most (BLK-1) ulp relative, the terms being positive) whatever the
+// group size; BLK = 1 is a two-sum per pair.  The index test that drops the
+// self pair runs only in the tile holding the workgroup's own targets (tiles of
+// targets and of sources both start at the group's first member).
+template <bool PLUM>
+__device__ __forceinline__ float pot_kernel_t(float d2, float twoh) {
+	return pot_kernel(PLUM ? SKG_PLUMMER : SKG_SPLINE, d2, twoh);
+}
+// A zero the compiler cannot prove uniform: tile addresses offset by it stay in
+// one VGPR per block of BLK sources (with immediate offsets per source) instead
+// of being rematerialised from scalar registers for every source.
+__device__ __forceinline__ int vgpr_zero() {
+	int z;
+	asm volatile("v_mov_b32 %0, 0" : "=v"(z));
+	return z;
+}
+template <int BLK, bool SELF, bool PLUM, int POT_T>
+__device__ __forceinline__ void pot_tile(Acc &acc, const float4 (&sq)[POT_T], const float (&se)[POT_T], int nt,
+		float4 qi, float ei, int self, int vz) {
+	auto term = [&](const float4 qj, float ej, int t) {
+		const float dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
+		const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+		const float v = qj.w * pot_kernel_t<PLUM>(d2, ei + ej);
+		return SELF && t == self ? 0.f : v;
+	};
+	int t = 0;
+	for (; t + BLK <= nt; t += BLK) {
+		const float4 *pq = sq + (t + vz);
+		const float *pe = se + (t + vz);
+		float p = term(pq[0], pe[0], t);
+#pragma unroll
+		for (int u = 1; u < BLK; ++u) p = fadd(p, term(pq[u], pe[u], t + u));
+		acc.add(p);
+	}
+	if (t < nt) {
+		float p = term(sq[t], se[t], t);
+		for (++t; t < nt; ++t) p = fadd(p, term(sq[t], se[t], t));
+		acc.add(p);
+	}
+}
+
+template <int POT_T, int BLK>
 __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tiles, const int *__restrict__ gs,
 		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv,
 		float2 *uphi, float G, int soft) {
@@ -208,6 +251,7 @@ __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tile
 	const bool valid = i < e;
 	const float4 qi = uq[valid ? i : s];
 	const float ei = uv[valid ? i : s].w;
+	const int vz = vgpr_zero();
 	Acc acc;
 	acc.init();
 	for (int j0 = s; j0 < e; j0 += POT_T) {
@@ -219,13 +263,12 @@ __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tile
 			se[threadIdx.x] = uv[j0 + threadIdx.x].w;
 		}
 		__syncthreads();
-		const int self = i - j0;
-		for (int t = 0; t < nt; ++t) {
-			const float4 qj = sq[t];
-			const float dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
-			const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-			const float term = qj.w * pot_kernel(soft, d2, ei + se[t]);
-			acc.add(t == self ? 0.f : term);
+		if (soft == SKG_PLUMMER) {
+			if (j0 == tl.y) pot_tile<BLK, true, true>(acc, sq, se, nt, qi, ei, (int)threadIdx.x, vz);
+			else pot_tile<BLK, false, true>(acc, sq, se, nt, qi, ei, -1, vz);
+		} else {
+			if (j0 == tl.y) pot_tile<BLK, true, false>(acc, sq, se, nt, qi, ei, (int)threadIdx.x, vz);
+			else pot_tile<BLK, false, false>(acc, sq, se, nt, qi, ei, -1, vz);
 		}
 	}
 	if (valid) {
@@ -234,6 +277,24 @@ __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tile
 		a.add(acc.s); a.add(acc.c);
 		uphi[i] = make_float2(a.s, a.c);
 	}
+}
+
+// Potential launch over (group, first target) tiles of T targets.  SKG_POT_BLOCK=1
+// selects a two-sum per pair instead of per 8-term partial (for error studies).
+template <int T>
+static void launch_unb_pot_t(skg_ctx *c, const GroupState *G, const int2 *tiles, int ntl, int blk) {
+	if (blk == 1)
+		k_unb_pot<T, 1><<<ntl, T, 0, c->st>>>(tiles, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
+	else
+		k_unb_pot<T, 8><<<ntl, T, 0, c->st>>>(tiles, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
+}
+static void launch_unb_pot(skg_ctx *c, const GroupState *G, int T, const int2 *tiles, int ntl) {
+	if (ntl <= 0) return;
+	const char *e = getenv("SKG_POT_BLOCK");
+	const int blk = (e && atoi(e) == 1) ? 1 : 8;
+	if (T == 64) launch_unb_pot_t<64>(c, G, tiles, ntl, blk);
+	else if (T == 128) launch_unb_pot_t<128>(c, G, tiles, ntl, blk);
+	else launch_unb_pot_t<256>(c, G, tiles, ntl, blk);
 }
 
 // Groups of at most 64 members: one wavefront per group, member i in lane i.
@@ -1404,12 +1465,7 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 	};
 	auto launch_pot = [&](int T, const int2 *dtl, int ntl) {
 		if (ntl <= 0) return;
-		if (T == 64)
-			k_unb_pot<64><<<ntl, 64, 0, c->st>>>(dtl, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
-		else if (T == 128)
-			k_unb_pot<128><<<ntl, 128, 0, c->st>>>(dtl, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
-		else
-			k_unb_pot<256><<<ntl, 256, 0, c->st>>>(dtl, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
+		launch_unb_pot(c, G, T, dtl, ntl);
 	};
 	make_tiles(c->tunePot ? 64 : c->potTile, tiles);   // the most tiles any candidate needs
 	if (gbig.size() > 128) {   // keep the 128 largest cooperative; the rest one workgroup each
@@ -2029,12 +2085,7 @@ extern "C" int skg_group_energies(skg_ctx *c, double *W, double *T) {
 			c->d_soft, G->rel, 0.5f * L[0], 0.5f * L[1], 0.5f * L[2], INT_MAX, G->uq, G->uv, G->uphi, G->uord, n);
 	const int nt = (int)tiles.size();
 	if (nt > 0) {
-		if (TT == 64)
-			k_unb_pot<64><<<nt, 64, 0, c->st>>>(dt, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
-		else if (TT == 128)
-			k_unb_pot<128><<<nt, 128, 0, c->st>>>(dt, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
-		else
-			k_unb_pot<256><<<nt, 256, 0, c->st>>>(dt, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
+		launch_unb_pot(c, G, TT, dt, nt);
 	}
 	if (nG > 1)
 		k_group_energy<<<nG - 1, 256, 0, c->st>>>(G->gstart, G->gend, nG, G->uq, G->uv, G->uphi, c->P.fShift,
