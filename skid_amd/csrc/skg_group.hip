@@ -35,6 +35,10 @@ struct GroupState {
 	float4 *uv = nullptr;             // velocity, softening
 	float2 *uphi = nullptr;           // compensated potential
 	int *uord = nullptr;
+	float4 *psrc = nullptr;           // potential sources: x, y, z, G m (aligned with mem[])
+	float *peps = nullptr;            // and softening
+	long nsrc = 0;
+	int *d_epsMax = nullptr;          // largest softening over the workspace (float bits)
 	int *d_nunb = nullptr;
 	int unbound = 0;
 };
@@ -45,6 +49,7 @@ void skg_group_free(skg_ctx *c) {
 	dfree(&g->gstart); dfree(&g->gend); dfree(&g->mem); dfree(&g->mass); dfree(&g->rel); dfree(&g->cen);
 	dfree(&g->vcm); dfree(&g->rbound); dfree(&g->radius); dfree(&g->count); dfree(&g->mpos_ord);
 	dfree(&g->uq); dfree(&g->uv); dfree(&g->uphi); dfree(&g->uord); dfree(&g->d_nunb);
+	dfree(&g->psrc); dfree(&g->peps); dfree(&g->d_epsMax);
 	delete g;
 	c->gs = nullptr;
 }
@@ -144,10 +149,8 @@ __global__ __launch_bounds__(64) void k_centers(const int *__restrict__ gs, cons
 // checked against the FP64 oracle, not for bit parity), so the far field
 // r >= eps_i + eps_j -- nearly every pair of a large group -- costs one
 // transcendental.  Coincident points (d2 = 0) take the r = 0 branch.
-__device__ __forceinline__ float pot_kernel(int soft, float d2, float twoh) {
-	if (soft == SKG_PLUMMER) return __builtin_amdgcn_rsqf(fmaf(0.25f * twoh, twoh, d2));
-	const float ir = __builtin_amdgcn_rsqf(d2);
-	if (d2 >= twoh * twoh) return ir;            // outside the kernel (r >= 2h): 1/r
+// Spline softening inside the kernel (r < 2h = twoh), given ir = rsq(d2).
+__device__ __forceinline__ float pot_spline_in(float d2, float ir, float twoh) {
 	const float r = d2 > 0.f ? d2 * ir : 0.f;
 	const float dih = 2.0f * __builtin_amdgcn_rcpf(twoh), u = r * dih, u2 = u * u;
 	if (u < 1.0f) {
@@ -161,6 +164,12 @@ __device__ __forceinline__ float pot_kernel(int soft, float d2, float twoh) {
 	p = fmaf(p, u, -4.0f / 3.0f);
 	p = fmaf(p, u2, 1.6f);
 	return fmaf(dih, p, (-1.0f / 15.0f) * ir);
+}
+__device__ __forceinline__ float pot_kernel(int soft, float d2, float twoh) {
+	if (soft == SKG_PLUMMER) return __builtin_amdgcn_rsqf(fmaf(0.25f * twoh, twoh, d2));
+	const float ir = __builtin_amdgcn_rsqf(d2);
+	if (d2 >= twoh * twoh) return ir;            // outside the kernel (r >= 2h): 1/r
+	return pot_spline_in(d2, ir, twoh);
 }
 
 // Relative coordinates and unbinding workspace for groups to unbind (kd.c:1341-1355).
@@ -192,58 +201,75 @@ __global__ void k_unb_setup(const int *__restrict__ gs, const int *__restrict__ 
 }
 
 // phi_i += sum_{j != i} G m_j K(r_ij; eps_i + eps_j) over the member's group
-// (kdCellPot, grav.c:8-36): a workgroup holds POT_T targets in registers and
-// streams the group's members through LDS in tiles of POT_T (both orders of
-// each pair are evaluated; no atomics, fixed summation order).  POT_T (64,
-// 128 or 256) is chosen by the autotuner.  Summation: the terms are added in
-// fp32 partials of BLK consecutive sources and each partial enters a two-sum
-// accumulator, so the rounding of the whole sum stays that of one BLK-term
-// partial (at most (BLK-1) ulp relative, the terms being positive) whatever the
-// group size; BLK = 1 is a two-sum per pair.  The index test that drops the
-// self pair runs only in the tile holding the workgroup's own targets (tiles of
-// targets and of sources both start at the group's first member).
+// (kdCellPot, grav.c:8-36).  Both orders of each pair are evaluated; no
+// atomics, fixed summation order.  Summation: the terms are added in fp32
+// partials of BLK consecutive sources (by fma) and each partial enters a
+// two-sum accumulator, so the rounding of the whole sum stays that of one
+// BLK-term partial (at most BLK ulp relative, the terms being positive)
+// whatever the group size; BLK = 1 is a two-sum per pair.
+//
+// Far test: with cfar = fl(fl(eps_i + eps_max)^2), d2 >= cfar implies
+// d2 >= fl(fl(eps_i + eps_j)^2) (rounding is monotone), so such a pair is 1/r
+// exactly as in pot_kernel, at one compare; the others take pot_kernel's own
+// test and spline, sharing the rsq.
 template <bool PLUM>
-__device__ __forceinline__ float pot_kernel_t(float d2, float twoh) {
-	return pot_kernel(PLUM ? SKG_PLUMMER : SKG_SPLINE, d2, twoh);
+__device__ __forceinline__ float pot_k(float d2, float ei, float ej, float cfar) {
+	if (PLUM) return pot_kernel(SKG_PLUMMER, d2, ei + ej);
+	const float ir = __builtin_amdgcn_rsqf(d2);
+	if (d2 >= cfar) return ir;
+	const float twoh = ei + ej;
+	if (d2 >= twoh * twoh) return ir;
+	return pot_spline_in(d2, ir, twoh);
 }
-// A zero the compiler cannot prove uniform: tile addresses offset by it stay in
-// one VGPR per block of BLK sources (with immediate offsets per source) instead
-// of being rematerialised from scalar registers for every source.
+
+// BLK-blocked sum over sources src(0..n-1) for the lane's target (qi, ei);
+// self: the source index of the lane's own target, tested only when SELF.
+template <int BLK, bool SELF, bool PLUM, typename Src>
+__device__ __forceinline__ void pot_sum(Acc &acc, Src src, int n, float4 qi, float ei, float cfar, int self) {
+	auto one = [&](int t, float p, bool first) {
+		float4 qj; float ej;
+		src(t, qj, ej);
+		const float dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
+		const float k = pot_k<PLUM>(fmaf(dx, dx, fmaf(dy, dy, dz * dz)), ei, ej, cfar);
+		const float gm = SELF && t == self ? 0.f : qj.w;
+		return first ? fmul(gm, k) : fmaf(gm, k, p);
+	};
+	int t = 0;
+	for (; t + BLK <= n; t += BLK) {
+		float p = one(t, 0.f, true);
+#pragma unroll
+		for (int u = 1; u < BLK; ++u) p = one(t + u, p, false);
+		acc.add(p);
+	}
+	if (t < n) {
+		float p = one(t, 0.f, true);
+		for (++t; t < n; ++t) p = one(t, p, false);
+		acc.add(p);
+	}
+}
+
+// A zero the compiler cannot prove uniform: LDS tile addresses offset by it
+// stay in one VGPR per block of sources (with immediate offsets per source)
+// instead of being rematerialised from scalar registers for every source.
 __device__ __forceinline__ int vgpr_zero() {
 	int z;
 	asm volatile("v_mov_b32 %0, 0" : "=v"(z));
 	return z;
 }
-template <int BLK, bool SELF, bool PLUM, int POT_T>
-__device__ __forceinline__ void pot_tile(Acc &acc, const float4 (&sq)[POT_T], const float (&se)[POT_T], int nt,
-		float4 qi, float ei, int self, int vz) {
-	auto term = [&](const float4 qj, float ej, int t) {
-		const float dx = qi.x - qj.x, dy = qi.y - qj.y, dz = qi.z - qj.z;
-		const float d2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-		const float v = qj.w * pot_kernel_t<PLUM>(d2, ei + ej);
-		return SELF && t == self ? 0.f : v;
-	};
-	int t = 0;
-	for (; t + BLK <= nt; t += BLK) {
-		const float4 *pq = sq + (t + vz);
-		const float *pe = se + (t + vz);
-		float p = term(pq[0], pe[0], t);
-#pragma unroll
-		for (int u = 1; u < BLK; ++u) p = fadd(p, term(pq[u], pe[u], t + u));
-		acc.add(p);
-	}
-	if (t < nt) {
-		float p = term(sq[t], se[t], t);
-		for (++t; t < nt; ++t) p = fadd(p, term(sq[t], se[t], t));
-		acc.add(p);
-	}
-}
 
+// A workgroup holds POT_T targets in registers and streams the group's sources
+// (x, y, z, G m; eps) through LDS in tiles of POT_T.  Target and source tiles
+// both start at the group's first member, so the self pair lies in the tile
+// with j0 == first target.  POT_T (64, 128 or 256) is chosen by the autotuner.
+// Far pairs cost 9 VALU operations (3 sub, mul, 2 fma for d2; rsq; compare;
+// fma into the partial) and one 16-byte LDS broadcast; eps_j is read only for
+// pairs the far test does not settle.
 template <int POT_T, int BLK>
 __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tiles, const int *__restrict__ gs,
 		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv,
-		float2 *uphi, float G, int soft) {
-	__shared__ float4 sq[POT_T];      // x, y, z, G*m
+		const float4 *__restrict__ ps, const float *__restrict__ pe, const float *__restrict__ epsMax,
+		float2 *__restrict__ uphi, int soft) {
+	__shared__ float4 sq[POT_T];
 	__shared__ float se[POT_T];
 	const int2 tl = tiles[blockIdx.x];   // (group, first target)
 	const int g = tl.x, s = gs[g], e = ge[g];
@@ -251,6 +277,7 @@ __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tile
 	const bool valid = i < e;
 	const float4 qi = uq[valid ? i : s];
 	const float ei = uv[valid ? i : s].w;
+	const float cf = fadd(ei, *epsMax), cfar = fmul(cf, cf);
 	const int vz = vgpr_zero();
 	Acc acc;
 	acc.init();
@@ -258,17 +285,18 @@ __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tile
 		const int nt = min(POT_T, e - j0);
 		__syncthreads();
 		if (threadIdx.x < nt) {
-			const float4 q = uq[j0 + threadIdx.x];
-			sq[threadIdx.x] = make_float4(q.x, q.y, q.z, fmul(G, q.w));
-			se[threadIdx.x] = uv[j0 + threadIdx.x].w;
+			sq[threadIdx.x] = ps[j0 + threadIdx.x];
+			se[threadIdx.x] = pe[j0 + threadIdx.x];
 		}
 		__syncthreads();
+		auto src = [&](int t, float4 &q, float &ee) { q = sq[t + vz]; ee = se[t + vz]; };
+		const bool diag = j0 == tl.y;
 		if (soft == SKG_PLUMMER) {
-			if (j0 == tl.y) pot_tile<BLK, true, true>(acc, sq, se, nt, qi, ei, (int)threadIdx.x, vz);
-			else pot_tile<BLK, false, true>(acc, sq, se, nt, qi, ei, -1, vz);
+			if (diag) pot_sum<BLK, true, true>(acc, src, nt, qi, ei, cfar, (int)threadIdx.x);
+			else pot_sum<BLK, false, true>(acc, src, nt, qi, ei, cfar, -1);
 		} else {
-			if (j0 == tl.y) pot_tile<BLK, true, false>(acc, sq, se, nt, qi, ei, (int)threadIdx.x, vz);
-			else pot_tile<BLK, false, false>(acc, sq, se, nt, qi, ei, -1, vz);
+			if (diag) pot_sum<BLK, true, false>(acc, src, nt, qi, ei, cfar, (int)threadIdx.x);
+			else pot_sum<BLK, false, false>(acc, src, nt, qi, ei, cfar, -1);
 		}
 	}
 	if (valid) {
@@ -279,22 +307,47 @@ __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tile
 	}
 }
 
-// Potential launch over (group, first target) tiles of T targets.  SKG_POT_BLOCK=1
-// selects a two-sum per pair instead of per 8-term partial (for error studies).
+// Source records of the potential: (x, y, z, G m) and eps of every member, and
+// the largest eps (eps >= 0, so its bits order as integers; *epsMax starts at 0).
+__global__ void k_pot_src(const float4 *__restrict__ uq, const float4 *__restrict__ uv, int n, float G,
+		float4 *ps, float *pe, int *epsMax) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	float ee = 0.f;
+	if (i < n) {
+		const float4 q = uq[i];
+		ee = uv[i].w;
+		ps[i] = make_float4(q.x, q.y, q.z, fmul(G, q.w));
+		pe[i] = ee;
+	}
+	for (int o = 32; o > 0; o >>= 1) ee = fmaxf(ee, __shfl_xor(ee, o, 64));
+	if ((threadIdx.x & 63) == 0 && ee > 0.f) atomicMax(epsMax, __float_as_int(ee));
+}
+
+// Potential launch over (group, first target) tiles of T targets.
+// SKG_POT_BLOCK=1: a two-sum per pair instead of per 8-term partial (error studies).
 template <int T>
 static void launch_unb_pot_t(skg_ctx *c, const GroupState *G, const int2 *tiles, int ntl, int blk) {
-	if (blk == 1)
-		k_unb_pot<T, 1><<<ntl, T, 0, c->st>>>(tiles, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
-	else
-		k_unb_pot<T, 8><<<ntl, T, 0, c->st>>>(tiles, G->gstart, G->gend, G->uq, G->uv, G->uphi, c->P.G, c->P.iSoftType);
+	const float *em = (const float *)G->d_epsMax;
+#define SKG_POT_ARGS tiles, G->gstart, G->gend, G->uq, G->uv, G->psrc, G->peps, em, G->uphi, c->P.iSoftType
+	if (blk == 1) k_unb_pot<T, 1><<<ntl, T, 0, c->st>>>(SKG_POT_ARGS);
+	else k_unb_pot<T, 8><<<ntl, T, 0, c->st>>>(SKG_POT_ARGS);
+#undef SKG_POT_ARGS
 }
-static void launch_unb_pot(skg_ctx *c, const GroupState *G, int T, const int2 *tiles, int ntl) {
-	if (ntl <= 0) return;
+// n: length of the member workspace (uq, uv) the tiles index into.
+static int launch_unb_pot(skg_ctx *c, GroupState *G, int T, const int2 *tiles, int ntl, int n) {
+	if (ntl <= 0) return 0;
 	const char *e = getenv("SKG_POT_BLOCK");
 	const int blk = (e && atoi(e) == 1) ? 1 : 8;
+	if (G->nsrc < n || !G->d_epsMax) {
+		if (dalloc(&G->psrc, n) || dalloc(&G->peps, n) || dalloc(&G->d_epsMax, 1)) return SKG_E_NOMEM;
+		G->nsrc = n;
+	}
+	k_izero<<<1, 64, 0, c->st>>>(G->d_epsMax, 1);
+	k_pot_src<<<nblk(n, 256), 256, 0, c->st>>>(G->uq, G->uv, n, c->P.G, G->psrc, G->peps, G->d_epsMax);
 	if (T == 64) launch_unb_pot_t<64>(c, G, tiles, ntl, blk);
 	else if (T == 128) launch_unb_pot_t<128>(c, G, tiles, ntl, blk);
 	else launch_unb_pot_t<256>(c, G, tiles, ntl, blk);
+	return 0;
 }
 
 // Groups of at most 64 members: one wavefront per group, member i in lane i.
@@ -1463,10 +1516,7 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 			for (int g : *L)
 				for (int t = hs[g]; t < he[g]; t += T) v.push_back(make_int2(g, t));
 	};
-	auto launch_pot = [&](int T, const int2 *dtl, int ntl) {
-		if (ntl <= 0) return;
-		launch_unb_pot(c, G, T, dtl, ntl);
-	};
+	auto launch_pot = [&](int T, const int2 *dtl, int ntl) { return launch_unb_pot(c, G, T, dtl, ntl, nTot); };
 	make_tiles(c->tunePot ? 64 : c->potTile, tiles);   // the most tiles any candidate needs
 	if (gbig.size() > 128) {   // keep the 128 largest cooperative; the rest one workgroup each
 		std::sort(gbig.begin(), gbig.end(), [&](int a, int b) { return he[a] - hs[a] > he[b] - hs[b]; });
@@ -1497,7 +1547,7 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 				k_unb_setup<<<nblk(nTot, 256), 256, 0, c->st>>>(G->gstart, G->gend, nG, G->mem, c->d_pos, c->d_vel,
 						c->d_mass, c->d_soft, G->rel, hx, hy, hz, nMax, G->uq, G->uv, G->uphi, G->uord, nTot);
 				SKG_HIP(hipEventRecord(c->e0, c->st));
-				launch_pot(T, dt, (int)tt.size());
+				if (launch_pot(T, dt, (int)tt.size())) return SKG_E_NOMEM;
 				SKG_HIP(hipEventRecord(c->e1, c->st));
 				SKG_HIP(hipEventSynchronize(c->e1));
 				float ms = 0.f;
@@ -1539,7 +1589,7 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		for (int gg : gmed) pairs += (double)(he[gg] - hs[gg]) * (he[gg] - hs[gg] - 1);
 		for (int gg : gbig) pairs += (double)(he[gg] - hs[gg]) * (he[gg] - hs[gg] - 1);
 		const int pslot = skg_prof_begin(c);
-		launch_pot(c->potTile, dt, (int)tiles.size());
+		if (launch_pot(c->potTile, dt, (int)tiles.size())) return SKG_E_NOMEM;
 		skg_prof_end(c, pslot, SKG_K_UNB_POT, 20.0 * pairs);
 		const int bSub = (c->inType == SKG_DARK || c->inType == SKG_STAR);   // kd.c:1441
 		const int lslot = skg_prof_begin(c);
@@ -2085,7 +2135,7 @@ extern "C" int skg_group_energies(skg_ctx *c, double *W, double *T) {
 			c->d_soft, G->rel, 0.5f * L[0], 0.5f * L[1], 0.5f * L[2], INT_MAX, G->uq, G->uv, G->uphi, G->uord, n);
 	const int nt = (int)tiles.size();
 	if (nt > 0) {
-		launch_unb_pot(c, G, TT, dt, nt);
+		if (launch_unb_pot(c, G, TT, dt, nt, n)) return SKG_E_NOMEM;
 	}
 	if (nG > 1)
 		k_group_energy<<<nG - 1, 256, 0, c->st>>>(G->gstart, G->gend, nG, G->uq, G->uv, G->uphi, c->P.fShift,
