@@ -323,15 +323,19 @@ __device__ __forceinline__ float grad_factor(float d2, float ih2, float gnorm) {
 
 // The same factor in fp32 for the compensated mode.  For q >= 1 the
 // reference's -3/q + 3 - 0.75 q equals -0.75 (2 - q)^2 / q, which has no
-// cancellation near q = 2 (2 - q is exact there), so a few fp32 roundings
-// give a few ulp of relative error with no fp64 divide.
+// cancellation near q = 2 (2 - q is exact there).  q and 1/q come from the
+// hardware square root and reciprocal (about 1 ulp each), so the factor is
+// good to a few ulp; both branches are evaluated and one is selected, so the
+// move step's hit path carries no divide and no branch here (q = 0, the
+// point on its own entity, selects the inner branch; the outer's inf is
+// discarded).
 __device__ __forceinline__ float grad_factor_f32(float d2, float ih2, float gnorm) {
 	const float r2 = fmul(d2, ih2);
-	const float q = __fsqrt_rn(r2);
-	float g;
-	if (r2 < 1.0f) g = fmaf(2.25f, q, -3.0f);
-	else { const float t = fsub(2.0f, q); g = fmul(-0.75f, fmul(t, t)) / q; }
-	return fmul(g, gnorm);
+	const float q = __builtin_amdgcn_sqrtf(r2);
+	const float t = fsub(2.0f, q);
+	const float gin = fmaf(2.25f, q, -3.0f);
+	const float gout = fmul(fmul(-0.75f, fmul(t, t)), __builtin_amdgcn_rcpf(q));
+	return fmul(r2 < 1.0f ? gin : gout, gnorm);
 }
 
 // ---------------------------------------------------------------- host helpers (skg_core.hip)
