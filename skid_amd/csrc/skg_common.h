@@ -66,6 +66,7 @@ struct skg_ctx {
 	int *d_eblk = nullptr;       // same, on the device
 	int *d_bact = nullptr, *d_bscat = nullptr;   // per-box active movers / live entities
 	float *d_bthr = nullptr;     // per-box ScatterCut threshold
+	float *d_bslot = nullptr;    // per-box threshold slots the move step's atomics spread over
 
 	// by iOrder
 	float *d_pos = nullptr, *d_vel = nullptr, *d_mass = nullptr, *d_soft = nullptr, *d_temp = nullptr;

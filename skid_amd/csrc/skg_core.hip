@@ -661,13 +661,16 @@ __global__ __launch_bounds__(64) void k_knn(
 // MODE 1 the density gradient and then moves the point (kdMoveParticles).
 // =====================================================================
 // Test the cnt entities from start against the lane's point (lanes with want)
-// and accumulate: MODE 0 the scatter density, MODE 1 the gradient (hits are
-// stamped for ScatterCut).  Records are read four at a time (one wait each).
+// and accumulate: MODE 0 the scatter density, MODE 1 the gradient.  For
+// ScatterCut, MODE 1 either stamps each hit entity (ehit) or keeps the lane's
+// smallest positive density among its hits (hmin; during the move loop the
+// record's w holds the entity's density).  Records are read four at a time
+// (one wait each).
 template <int MODE, bool COMP>
 __device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const float4 &p,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
-		const int *__restrict__ ecode, int *__restrict__ ehit, int stamp, float Lx, float Ly, float Lz,
-		Sum<COMP> &s0, Sum<COMP> &s1, Sum<COMP> &s2) {
+		const int *__restrict__ ecode, int *__restrict__ ehit, int stamp, float &hmin, float Lx, float Ly,
+		float Lz, Sum<COMP> &s0, Sum<COMP> &s1, Sum<COMP> &s2) {
 	if (cnt <= 0 || !want) return;
 	for (int t0 = 0; t0 < cnt; t0 += 4) {
 		// four consecutive records in one scalar load each; the entity arrays
@@ -701,10 +704,35 @@ __device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const 
 				} else {
 					const float f = COMP ? grad_factor_f32(sq3(dx, dy, dz), X.y, X.z) : grad_factor(d2, X.y, X.z);
 					s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
-					ehit[e] = stamp;
+					if (ehit) ehit[e] = stamp;
+					else hmin = fminf(hmin, X.w > 0.f ? X.w : INFINITY);
 				}
 			}
 		}
+	}
+}
+
+// ScatterCut threshold from the move step (smooth1.c:460-461): each box's
+// smallest positive density over the entities its movers hit this step, as
+// the minimum of the lanes' hmin, one atomic per box present in the
+// wavefront (densities are >= 0, so their bits order as integers; the slots
+// start at +inf).  A box has SKG_THR_SLOTS slots, picked by the wavefront's
+// tile, so that the thousands of wavefronts of a large box do not all meet at
+// one address (device-scope atomics from every XCD serialise there);
+// k_thr_fold takes the minimum over the slots.  All 64 lanes must be active.
+#define SKG_THR_SLOTS 64
+__device__ __forceinline__ void box_min_atomic(float hmin, bool valid, int pb, int *bthr) {
+	const int lane = threadIdx.x & 63;
+	const bool has = valid && hmin < INFINITY;
+	unsigned long long todo = __ballot(has);
+	while (todo) {
+		const int l = __ffsll((long long)todo) - 1;
+		const int b = __shfl(pb, l, 64);
+		const bool mine = has && pb == b;
+		const float m = wave_min(mine ? hmin : INFINITY);
+		const int slot = (int)((blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (SKG_THR_SLOTS - 1));
+		if (lane == l) atomicMin(bthr + (size_t)b * SKG_THR_SLOTS + slot, __float_as_int(m));
+		todo &= ~__ballot(mine);
 	}
 }
 
@@ -767,7 +795,7 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
 		const int *__restrict__ ecode, int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi,
 		int nLeaf2, int B, float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
-		const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
+		int *__restrict__ bthr, const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
 		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart,
 		const int *__restrict__ tlist, const int *__restrict__ tcnt, int tcap, const int *__restrict__ npDev,
 		const int *__restrict__ stampDev) {
@@ -783,6 +811,7 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 	const int pb = pbox ? pbox[pi] : 0;
 	Sum<COMP> s0, s1, s2;
 	s0.init(); s1.init(); s2.init();
+	float hmin = INFINITY;
 	// With a leaf list for this tile (built at the start of the block, see
 	// k_cover_lists) the wavefront scans those leaves and walks no tree.
 	// Otherwise a node is opened when its largest live ball reaches at least
@@ -803,7 +832,7 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		if (!__any(want)) continue;
 		const int2 rg = leaf_range(lstart, leaf, B, ne);
 		cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), want, p, epos, eaux, eorg, ecode, ehit,
-				stamp, Lx, Ly, Lz, s0, s1, s2);
+				stamp, hmin, Lx, Ly, Lz, s0, s1, s2);
 	}
 	while (node) {
 		node = uniform_int(node);
@@ -814,12 +843,13 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		if (visit) {
 			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
 			cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), need, p, epos, eaux, eorg, ecode,
-					ehit, stamp, Lx, Ly, Lz, s0, s1, s2);
+					ehit, stamp, hmin, Lx, Ly, Lz, s0, s1, s2);
 		}
 		// next node in pre-order after node's subtree (strip trailing ones, step)
 		node = (node >> __builtin_ctz(~node)) + 1;
 		if (node == 1) node = 0;
 	}
+	if (MODE == 1 && bthr) box_min_atomic(hmin, valid, pb, bthr);
 	if (!valid) return;
 	if (MODE == 0) { out_sum[ii] = s0.val(); return; }
 	// kdMoveParticles (kd.c:702-732)
@@ -850,7 +880,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 		float4 *__restrict__ ppos, const int *__restrict__ plist, int np,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
 		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
-		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp,
+		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp, int *__restrict__ bthr,
 		const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
 		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart,
 		const int *__restrict__ npDev, const int *__restrict__ stampDev) {
@@ -867,6 +897,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 	const int pb = pbox ? pbox[pi] : 0;
 	Sum<COMP> s0, s1, s2;
 	s0.init(); s1.init(); s2.init();
+	float hmin = INFINITY;
 	int node = valid ? 1 : 0;
 	while (node) {                                // the S lanes of a point move in lockstep
 		const float4 lo = nlo[node], hi = nhi[node];
@@ -893,7 +924,8 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 					} else {
 						const float f = COMP ? grad_factor_f32(sq3(dx, dy, dz), X.y, X.z) : grad_factor(d2, X.y, X.z);
 						s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
-						ehit[e] = stamp;
+						if (ehit) ehit[e] = stamp;
+						else hmin = fminf(hmin, X.w > 0.f ? X.w : INFINITY);
 					}
 				}
 			}
@@ -912,6 +944,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 			s2.merge(__shfl_xor(s2.s, o, 64), __shfl_xor(s2.cc(), o, 64));
 		}
 	}
+	if (MODE == 1 && bthr) box_min_atomic(hmin, valid, pb, bthr);
 	if (!valid || sub != 0) return;
 	if (MODE == 0) { out_sum[ii] = s0.val(); return; }
 	const float fStep = micro ? bt[pb].fMicro : bt[pb].fStep;
@@ -1134,14 +1167,14 @@ __global__ void k_iota(int *a, int n) {
 // active mover this step; on the initial step live entities that held none
 // get density 0 (smooth1.c:460-470, 497-507).  Block minima go to partials.
 __global__ __launch_bounds__(256) void k_scat_thr(const int *__restrict__ elive, const int *__restrict__ ehit,
-		float *erho, int ne, const int *__restrict__ stampDev, int bInitial, float *part) {
+		float *erho, float4 *eaux, int ne, const int *__restrict__ stampDev, int bInitial, float *part) {
 	const int stamp = *stampDev;
 	__shared__ float sm[4];
 	int e = blockIdx.x * blockDim.x + threadIdx.x;
 	float v = INFINITY;
 	if (e < ne && elive[e]) {
 		if (ehit[e] == stamp) { float r = erho[e]; if (r > 0.f) v = r; }
-		else if (bInitial) erho[e] = 0.f;
+		else if (bInitial) { erho[e] = 0.f; eaux[e].w = 0.f; }
 	}
 	v = wave_min(v);
 	if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
@@ -1168,6 +1201,27 @@ __global__ __launch_bounds__(64) void k_box_min(const float *__restrict__ part, 
 	for (int i = eblk[b] + (int)threadIdx.x; i < eblk[b + 1]; i += 64) v = fminf(v, part[i]);
 	v = wave_min(v);
 	if (threadIdx.x == 0) out[b] = v;
+}
+// From the move loop on, an entity record's w (the scatter density's kernel
+// normalisation, used only by skg_density, which rebuilds the records) holds
+// the entity's density, which the move step reads with the record for the
+// ScatterCut threshold.
+__global__ void k_aux_rho(float4 *eaux, const float *__restrict__ erho, int ne) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < ne) eaux[i].w = erho[i];
+}
+
+// Per-step reset for the move kernel's threshold atomics: every box's slots
+// to +inf, and the live total k_box_sum adds into to 0.
+__global__ void k_thr_reset(float *slots, int n, int *liveTotal) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) slots[i] = INFINITY;
+	if (i == 0) liveTotal[0] = 0;
+}
+// thr[b] = the minimum over box b's slots (one wavefront per box).
+__global__ __launch_bounds__(64) void k_thr_fold(const float *__restrict__ slots, float *thr) {
+	const float v = wave_min(slots[(size_t)blockIdx.x * SKG_THR_SLOTS + threadIdx.x]);
+	if (threadIdx.x == 0) thr[blockIdx.x] = v;
 }
 __global__ __launch_bounds__(64) void k_box_sum(const int *__restrict__ part, const int *__restrict__ eblk,
 		int *out, int *liveTotal) {
@@ -1440,7 +1494,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->d_ml); dfree(&c->m_flag); dfree(&c->m_bcnt);
 	if (c->h_pin) { (void)hipHostFree(c->h_pin); c->h_pin = nullptr; }
 	dfree(&c->d_bt); dfree(&c->d_boxOf); dfree(&c->d_type); dfree(&c->d_eblk); dfree(&c->d_bact); dfree(&c->d_bscat);
-	dfree(&c->d_bthr);
+	dfree(&c->d_bthr); dfree(&c->d_bslot);
 	delete[] c->hbox; delete[] c->h_eblk;
 	dfree(&c->d_thr); dfree(&c->d_cnt);
 	skg_dfree(&c->tmp);
@@ -1499,7 +1553,7 @@ extern "C" int skg_load_batch(skg_ctx *c, int nBox, const int *nGas, const int *
 	if (dalloc(&c->d_pos, 3 * n) || dalloc(&c->d_vel, 3 * n) || dalloc(&c->d_mass, n) ||
 	    dalloc(&c->d_soft, n) || dalloc(&c->d_temp, n) || dalloc(&c->d_rho, n) || dalloc(&c->d_ball2, n) ||
 	    dalloc(&c->d_grp, n) || dalloc(&c->d_boxOf, n) || dalloc(&c->d_type, n) || dalloc(&c->d_bt, nBox) ||
-	    dalloc(&c->d_bact, nBox) || dalloc(&c->d_bscat, nBox) || dalloc(&c->d_bthr, nBox) ||
+	    dalloc(&c->d_bact, nBox) || dalloc(&c->d_bscat, nBox) || dalloc(&c->d_bthr, nBox) || dalloc(&c->d_bslot, (size_t)nBox * SKG_THR_SLOTS) ||
 	    dalloc(&dbi, 3 * (size_t)nBox))
 		return SKG_E_NOMEM;
 	SKG_HIP(hipMemcpyAsync(c->d_bt, db.data(), sizeof(DevBox) * nBox, hipMemcpyHostToDevice, c->st));
@@ -1577,12 +1631,12 @@ static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, in
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
 #define CL(M_, C_)                                                                             \
 	k_cover<M_, C_><<<nb, SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->e_org, c->e_code, c->nE, c->te.lo, \
-			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, \
+			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, nullptr, c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, \
 			C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart, nullptr, nullptr, c->tlCap, nullptr, nullptr)
 #define CS(M_, C_)                                                                             \
 	k_cover_sub<M_, C_, 16><<<nblk(16L * np, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, \
 			c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp,       \
-			c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart, \
+			nullptr, c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart, \
 			nullptr, nullptr)
 	if (np <= c->subMax) {
 		if (mode == 0) { if (comp) CS(0, true); else CS(0, false); }
@@ -1820,24 +1874,40 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
 	const int *pbox = c->nBox > 1 ? c->m_box : nullptr;
 	const int nb = nblk(c->nE, 256);
+	const char *ft = getenv("SKG_FUSED_THR");
+	const bool fusedThr = !(ft && atoi(ft) == 0);
 	if (lists) enqueue_lists(c, cap, nsteps, micro);
 	for (int i = 0; i < nsteps; ++i) {
 		k_bump<<<1, 1, 0, c->st>>>(c->d_ml);
+		// the initial step stamps its hits (unhit entities get density 0);
+		// later steps take the threshold from the move kernel itself
+		// (SKG_FUSED_THR=0: stamps and a pass over the entities every step)
+		const bool stamps = bInitial || !fusedThr;
+		int *ehit = stamps ? c->e_hit : nullptr;
+		int *bthr = stamps ? nullptr : (int *)c->d_bslot;
+		if (!stamps)
+			k_thr_reset<<<nblk((long)c->nBox * SKG_THR_SLOTS, 256), 256, 0, c->st>>>(c->d_bslot,
+					c->nBox * SKG_THR_SLOTS, c->d_cnt);
 		const int pslot = capturing ? -1 : skg_prof_begin(c);
 #define MV(C_)                                                                                                \
 		if (sub) k_cover_sub<1, C_, 16><<<nblk(16L * cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos,       \
 				c->m_list, (int)cap, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2,      \
-				c->te.B, nullptr, c->e_hit, 0, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2],         \
-				c->te.lstart, c->d_ml + 1, c->d_ml);                                                             \
+				c->te.B, nullptr, ehit, 0, bthr, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1],             \
+				L[2], c->te.lstart, c->d_ml + 1, c->d_ml);                                                       \
 		else k_cover<1, C_><<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap,    \
 				c->e_pos, c->e_aux, c->e_org, c->e_code, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, nullptr, \
-				c->e_hit, 0, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart,             \
+				ehit, 0, bthr, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart,           \
 				lists ? c->tl_list : nullptr, lists ? c->tl_cnt : nullptr, c->tlCap, c->d_ml + 1, c->d_ml)
 		if (comp) { MV(true); } else { MV(false); }
 #undef MV
 		if (pslot >= 0) skg_prof_end(c, pslot, SKG_K_COVER_MOVE, 24.0 * c->nActive + 24.0 * c->nLive);
-		k_scat_thr<<<nb, 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->nE, c->d_ml, bInitial, c->part);
-		k_box_min<<<c->nBox, 64, 0, c->st>>>(c->part, c->d_eblk, c->d_bthr, c->d_cnt);
+		if (stamps) {
+			k_scat_thr<<<nb, 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->e_aux, c->nE, c->d_ml, bInitial,
+					c->part);
+			k_box_min<<<c->nBox, 64, 0, c->st>>>(c->part, c->d_eblk, c->d_bthr, c->d_cnt);
+		} else {
+			k_thr_fold<<<c->nBox, SKG_THR_SLOTS, 0, c->st>>>(c->d_bslot, c->d_bthr);
+		}
 		k_scat_cut<<<nb, 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE, c->e_box, c->d_bthr, (int *)c->part);
 		k_box_sum<<<c->nBox, 64, 0, c->st>>>((const int *)c->part, c->d_eblk, c->d_bscat, c->d_cnt);
 	}
@@ -1921,6 +1991,7 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	}
 	if (skg_ensure_part(c, (size_t)nblk(std::max(c->nE, 1), 256))) return SKG_E_NOMEM;
 	k_compact<<<nblk(n, 256), 256, 0, c->st>>>(nullptr, flag, off, n, c->m_ord);
+	if (c->nE > 0) k_aux_rho<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_aux, c->e_rho, c->nE);
 	k_mover_init<<<nblk(nm, 256), 256, 0, c->st>>>(c->m_ord, nm, c->d_pos, c->d_boxOf, c->m_pos, c->m_old, c->m_box);
 	c->nActive = nm;
 	if (nm > 0 && (rc = mover_sort(c, nm)))
