@@ -2163,22 +2163,28 @@ extern "C" int skg_autotune(skg_ctx *c, int what, int *nBucket, long long *subMa
 	int rc, na, ns;
 	const int prof = c->prof;
 	c->prof = 0;
-	// 1. leaf size: density + the first 6 move steps (all movers active)
+	// 1. leaf size: density + the whole move loop (its dense early steps and
+	// its sparse tail favour different leaves; a prefix of the loop has
+	// picked a leaf 2% slower over the whole of it)
 	if (what & SKG_TUNE_BUCKET) {
 		float best = FLT_MAX;
 		int bestB = c->P.nBucket;
+		// host clock after a stream sync: the entry points below restart the
+		// context's own stage events, so those cannot bracket them
+		auto now_ms = [&]() {
+			(void)hipStreamSynchronize(c->st);
+			return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+					.count();
+		};
 		for (int B : {8, 16, 32, 64}) {
 			c->P.nBucket = B;
-			(void)hipEventRecord(c->e0, c->st);
+			const double t0 = now_ms();
 			if ((rc = skg_density(c, nullptr, nullptr, nullptr))) return rc;
-			const float td = elapsed_ms(c);
 			if ((rc = skg_init_move(c, nullptr))) return rc;
-			(void)hipEventRecord(c->e0, c->st);
 			if ((rc = skg_move_first(c, &na, &ns))) return rc;
-			if (na > 0 && (rc = skg_move_block(c, &na, &ns))) return rc;
-			const float tm = elapsed_ms(c);
-			// the move loop runs tens of blocks: weight its dense steps
-			const float cost = td + 8.f * tm;
+			for (int blk = 0; na > 0 && blk < 1000; ++blk)
+				if ((rc = skg_move_block(c, &na, &ns))) return rc;
+			const float cost = (float)(now_ms() - t0);
 			if (cost < best) { best = cost; bestB = B; }
 		}
 		c->P.nBucket = bestB;
