@@ -308,19 +308,27 @@ __global__ __launch_bounds__(POT_T) void k_unb_pot(const int2 *__restrict__ tile
 }
 
 // Source records of the potential: (x, y, z, G m) and eps of every member, and
-// the largest eps (eps >= 0, so its bits order as integers; *epsMax starts at 0).
-__global__ void k_pot_src(const float4 *__restrict__ uq, const float4 *__restrict__ uv, int n, float G,
-		float4 *ps, float *pe, int *epsMax) {
-	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+// the largest eps (eps >= 0, so its bits order as integers; *epsMax starts at
+// 0).  Grid-stride with one atomic per workgroup: an atomic per wavefront to
+// the one address serialises across the chip.
+__global__ __launch_bounds__(256) void k_pot_src(const float4 *__restrict__ uq, const float4 *__restrict__ uv,
+		int n, float G, float4 *ps, float *pe, int *epsMax) {
+	__shared__ float sm[4];
 	float ee = 0.f;
-	if (i < n) {
+	for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
 		const float4 q = uq[i];
-		ee = uv[i].w;
+		const float e = uv[i].w;
 		ps[i] = make_float4(q.x, q.y, q.z, fmul(G, q.w));
-		pe[i] = ee;
+		pe[i] = e;
+		ee = fmaxf(ee, e);
 	}
 	for (int o = 32; o > 0; o >>= 1) ee = fmaxf(ee, __shfl_xor(ee, o, 64));
-	if ((threadIdx.x & 63) == 0 && ee > 0.f) atomicMax(epsMax, __float_as_int(ee));
+	if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = ee;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		ee = fmaxf(fmaxf(sm[0], sm[1]), fmaxf(sm[2], sm[3]));
+		if (ee > 0.f) atomicMax(epsMax, __float_as_int(ee));
+	}
 }
 
 // Potential launch over (group, first target) tiles of T targets.
@@ -343,7 +351,8 @@ static int launch_unb_pot(skg_ctx *c, GroupState *G, int T, const int2 *tiles, i
 		G->nsrc = n;
 	}
 	k_izero<<<1, 64, 0, c->st>>>(G->d_epsMax, 1);
-	k_pot_src<<<nblk(n, 256), 256, 0, c->st>>>(G->uq, G->uv, n, c->P.G, G->psrc, G->peps, G->d_epsMax);
+	k_pot_src<<<std::max(1, std::min(nblk(n, 256), 2048)), 256, 0, c->st>>>(G->uq, G->uv, n, c->P.G, G->psrc,
+			G->peps, G->d_epsMax);
 	if (T == 64) launch_unb_pot_t<64>(c, G, tiles, ntl, blk);
 	else if (T == 128) launch_unb_pot_t<128>(c, G, tiles, ntl, blk);
 	else launch_unb_pot_t<256>(c, G, tiles, ntl, blk);

@@ -132,7 +132,8 @@ def test_move_loop_launch_paths_agree():
     outs = []
     base = {"SKG_COVER_SUB_MAX": "2000"}
     for tag, env_add in (("direct", {}), ("graph", {"SKG_GRAPH": "1"}), ("nolists", {"SKG_TILE_LISTS": "0"}),
-                         ("graph_nolists", {"SKG_GRAPH": "1", "SKG_TILE_LISTS": "0"})):
+                         ("graph_nolists", {"SKG_GRAPH": "1", "SKG_TILE_LISTS": "0"}),
+                         ("stamped", {"SKG_FUSED_THR": "0"})):
         env = dict(os.environ)
         env.update(base)
         env.update(env_add)
@@ -142,6 +143,37 @@ def test_move_loop_launch_paths_agree():
         outs.append(np.load(path))
     for o in outs[1:]:
         assert np.array_equal(outs[0], o)
+
+
+def test_scattercut_threshold_paths_agree_batched():
+    """ScatterCut's per-box threshold taken inside the move kernels (per-lane
+    minima, atomics over per-box slots) equals the stamp-and-scan form
+    (SKG_FUSED_THR=0) on a batch of uneven boxes, for both move kernels: the
+    per-box Ittr traces and the group ids are identical."""
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, '.');"
+        "from skid_amd import gen, skg;"
+        "sizes = [2500, 9000, 40000, 120000];"
+        "boxes = [gen.config4_box(n, 30 + i) for i, n in enumerate(sizes)];"
+        "taus = np.array([gen.config4_tau(n) for n in sizes], np.float32);"
+        "bc = skg.BatchContext(boxes, skg.make_params(0.01, 32, period=1.0), taus);"
+        "bc.density(); tr = bc.move_loop(); bc.fof(); bc.microsteps(); bc.centers(); bc.unbind(); bc.too_small();"
+        "g, ng = bc.groups();"
+        "np.save(sys.argv[1], np.concatenate([np.concatenate([np.asarray(t).reshape(-1) for t in tr]), g]))"
+    )
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for sub_max in ("0", "100000000"):
+        outs = []
+        for tag, fused in (("fused", "1"), ("stamped", "0")):
+            env = dict(os.environ)
+            env.update({"SKG_FUSED_THR": fused, "SKG_COVER_SUB_MAX": sub_max})
+            path = f"/tmp/skg_thr_{tag}_{sub_max}.npy"
+            subprocess.run([sys.executable, "-c", code, path], check=True, env=env, timeout=240, cwd=root)
+            outs.append(np.load(path))
+        assert np.array_equal(outs[0], outs[1]), sub_max
 
 
 def test_autotune_keeps_results(box1):
