@@ -282,12 +282,11 @@ __device__ __forceinline__ float point_box_d2_np(float4 lo, float4 hi, float4 p)
 // first when |d| exceeds half a period (the reference's INTERSECT shift).
 template <bool PER>
 __device__ __forceinline__ float img_delta(float q, float qm, float qp, float p, float hL) {
-	float d = fsub(q, p);
-	if (PER) {
-		if (d > hL) d = fsub(qm, p);
-		else if (d < -hL) d = fsub(qp, p);
-	}
-	return d;
+	const float d = fsub(q, p);
+	if (!PER) return d;
+	// all three images, then a select (no divergent branches per candidate)
+	const float dm = fsub(qm, p), dp = fsub(qp, p);
+	return d > hL ? dm : (d < -hL ? dp : d);
 }
 
 // Periodic delta (s + shift) - p accurate to one rounding of the result: the

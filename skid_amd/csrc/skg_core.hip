@@ -565,23 +565,39 @@ __global__ __launch_bounds__(64) void k_knn(
 	for (int s = 0; s < k; ++s) heap[s * 64 + lane] = KEY_SENT;
 	unsigned long long top = KEY_SENT;
 
-#define KNN_TRY(J)                                                                        \
-	{                                                                                     \
-		const float4 cc = spos[J];                                                        \
-		const unsigned oj = (unsigned)sord[J];                                            \
-		if (!BATCH || oj - boff < bn) {                                                   \
-			float dx = img_delta<PER>(q.x, qxm, qxp, cc.x, hL);                          \
-			float dy = img_delta<PER>(q.y, qym, qyp, cc.y, hL);                          \
-			float dz = img_delta<PER>(q.z, qzm, qzp, cc.z, hL);                          \
-			unsigned long long key = ((unsigned long long)__float_as_uint(sq3(dx, dy, dz)) << 32) | oj; \
-			if (key < top) top = heap_replace<KMAX>(heap, lane, k, key);                 \
-		}                                                                                 \
-	}
+	// One candidate (position cc, iOrder oj) against the lane's query.
+	auto try_cand = [&](const float4 cc, unsigned oj) {
+		if (!BATCH || oj - boff < bn) {
+			const float dx = img_delta<PER>(q.x, qxm, qxp, cc.x, hL);
+			const float dy = img_delta<PER>(q.y, qym, qyp, cc.y, hL);
+			const float dz = img_delta<PER>(q.z, qzm, qzp, cc.z, hL);
+			const unsigned long long key = ((unsigned long long)__float_as_uint(sq3(dx, dy, dz)) << 32) | oj;
+			if (key < top) top = heap_replace<KMAX>(heap, lane, k, key);
+		}
+	};
+	// The cnt candidates from start (a uniform range): records are read four
+	// at a time through the scalar cache, one wait per four (the arrays carry
+	// SKG_EPAD entries past the end, so the reads past a range's end stay in
+	// the allocation; their values are never used).  skip: the seeded tile.
+	auto scan = [&](int start, int cnt, bool skip) {
+		for (int t0 = 0; t0 < cnt; t0 += 4) {
+			float4 C4[4];
+			unsigned O4[4];
+#pragma unroll
+			for (int u = 0; u < 4; ++u) { C4[u] = spos[start + t0 + u]; O4[u] = (unsigned)sord[start + t0 + u]; }
+#pragma unroll
+			for (int u = 0; u < 4; ++u) {
+				if (t0 + u >= cnt) break;
+				const int j = start + t0 + u;
+				if (skip && j >= base && j < base + 64) continue;
+				try_cand(C4[u], O4[u]);
+			}
+		}
+	};
 
 	// Seed with the block's own 64 points (the nearest tile of candidates).
 	const int nseed = min(64, n - base);
-	if (valid)
-		for (int t = 0; t < nseed; ++t) KNN_TRY(base + t);
+	if (valid) scan(base, nseed, false);
 	float R2 = wave_max(valid ? key_d2(top) : 0.f);
 
 	int sp = 1;
@@ -612,16 +628,9 @@ __global__ __launch_bounds__(64) void k_knn(
 		const bool need = valid && (!BATCH || brange_has(hi.w, qb)) &&
 				point_box_d2<PER>(lo, hi, q.x, q.y, q.z, L) <= key_d2(top);
 		if (!__any(need)) continue;
-		if (need) {
-			for (int t = 0; t < cnt; ++t) {
-				const int j = start + t;
-				if (j >= base && j < base + 64) continue;
-				KNN_TRY(j);
-			}
-		}
+		if (need) scan(start, cnt, true);
 		R2 = wave_max(valid ? key_d2(top) : 0.f);
 	}
-#undef KNN_TRY
 	if (!valid) return;
 	// fBall2 = the k-th smallest d2; gather half of the symmetric density over
 	// all queue members but the head (smooth1.c:243-263).
@@ -1679,7 +1688,7 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		// scatterers (kdScatterActive) in (box, Morton) order
 		float4 *tmp4 = nullptr;
 		int *tmpo = nullptr, *tmpb = nullptr, *perm = nullptr, *flag = nullptr, *off = nullptr;
-		if (dalloc(&c->s_pos, nS) || dalloc(&c->s_ord, nS) || dalloc(&c->s_box, nS) || dalloc(&c->s_ball2, nS) ||
+		if (dalloc(&c->s_pos, nS + SKG_EPAD) || dalloc(&c->s_ord, nS + SKG_EPAD) || dalloc(&c->s_box, nS) || dalloc(&c->s_ball2, nS) ||
 		    dalloc(&c->s_rhoG, nS) || dalloc(&c->s_rhoS, nS) || dalloc(&tmp4, nS) || dalloc(&tmpo, nS) ||
 		    dalloc(&tmpb, nS) || dalloc(&perm, nS) || dalloc(&flag, n) || dalloc(&off, n))
 			return SKG_E_NOMEM;
