@@ -502,19 +502,19 @@ int skg_copy_tree_leaves(skg_ctx *c, DevTree *dst, const DevTree *src) {
 // with the block's largest current radius, and streams each visited bucket
 // through the lanes as a uniform (scalar-load) candidate tile.
 // =====================================================================
+// Slot k holds 0, below every key, so a node's right child is always
+// readable: both children come in one LDS read (adjacent rows).
 template <int KMAX>
 __device__ __forceinline__ unsigned long long heap_replace(unsigned long long *heap, int lane, int k,
 		unsigned long long key) {
 	int i = 0;
 	for (;;) {
-		int l = 2 * i + 1;
+		const int l = 2 * i + 1;
 		if (l >= k) break;
 		unsigned long long cv = heap[l * 64 + lane];
+		const unsigned long long cr = heap[(l + 1) * 64 + lane];
 		int ci = l;
-		if (l + 1 < k) {
-			unsigned long long cr = heap[(l + 1) * 64 + lane];
-			if (cr > cv) { cv = cr; ci = l + 1; }
-		}
+		if (cr > cv) { cv = cr; ci = l + 1; }
 		if (cv <= key) break;
 		heap[i * 64 + lane] = cv;
 		i = ci;
@@ -544,7 +544,7 @@ __global__ __launch_bounds__(64) void k_knn(
 		float L, int k, const float *__restrict__ mass_ord, const float *__restrict__ pos_ord,
 		float *__restrict__ ball2_out, float *__restrict__ rhoG, unsigned long long *__restrict__ heap_out,
 		const int *__restrict__ sbox, const DevBox *__restrict__ bt) {
-	__shared__ unsigned long long heap[KMAX * 64];
+	__shared__ unsigned long long heap[(KMAX + 1) * 64];   // k slots + the zero sentinel row
 	__shared__ int stk[64];
 	__shared__ float stkd[64];
 	const int lane = threadIdx.x;
@@ -563,6 +563,7 @@ __global__ __launch_bounds__(64) void k_knn(
 	const float3 blo = make_float3(wave_min(q.x), wave_min(q.y), wave_min(q.z));
 	const float3 bhi = make_float3(wave_max(q.x), wave_max(q.y), wave_max(q.z));
 	for (int s = 0; s < k; ++s) heap[s * 64 + lane] = KEY_SENT;
+	heap[k * 64 + lane] = 0ull;
 	unsigned long long top = KEY_SENT;
 
 	// One candidate (position cc, iOrder oj) against the lane's query.
