@@ -135,6 +135,11 @@ struct skg_ctx {
 	int *d_ml = nullptr;
 	int *m_flag = nullptr, *m_bcnt = nullptr;   // prune flags; block counts then offsets
 	long lastSort = 0;                          // active count at the last mover sort
+	// ScatterCut over density-sorted entities (after the initial step): each
+	// box's live entities in ascending density, its first still-live position
+	int *cs_perm = nullptr, *cs_off = nullptr, *cs_pos = nullptr;
+	float *cs_rho = nullptr;
+	bool cutReady = false;
 	int *h_pin = nullptr;                        // pinned read-back
 	int useGraph = 0;                            // SKG_GRAPH=1: replay each block as a captured graph
 	hipGraph_t graph = nullptr;

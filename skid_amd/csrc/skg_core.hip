@@ -1262,6 +1262,61 @@ __global__ __launch_bounds__(256) void k_scat_cut(int *elive, float4 *eaux, cons
 	if (threadIdx.x == 0) part[blockIdx.x] = sm[0] + sm[1] + sm[2] + sm[3];
 }
 
+// ScatterCut after the initial step, from each box's live entities sorted by
+// density.  A box's threshold never falls: every live entity has density >=
+// the last threshold, and the next one is a minimum over hit (live) entities;
+// a box with no hit cuts nothing.  So the entities a cut removes are the next
+// ones in the box's ascending order: one binary search per box, then marking
+// only the newly removed (the same set as k_scat_cut's pass over all
+// entities; densities are finite).  Keys: box in the high word, density bits
+// (>= 0, so ordered as integers) in the low; dead entities sort last.
+__global__ void k_cut_keys(const int *__restrict__ elive, const int *__restrict__ ebox,
+		const float *__restrict__ erho, int ne, unsigned long long *key, int *val) {
+	const int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e >= ne) return;
+	key[e] = elive[e] ? ((unsigned long long)(unsigned)ebox[e] << 32) | __float_as_uint(erho[e]) : ~0ull;
+	val[e] = e;
+}
+// per box b <= nBox: first sorted position of box b (nBox: the live total)
+__global__ void k_cut_bounds(const unsigned long long *__restrict__ key, int ne, int nBox, int *off, int *pos,
+		float *srho, const int *__restrict__ sval, int *perm) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < ne) { srho[i] = __uint_as_float((unsigned)key[i]); perm[i] = sval[i]; }
+	if (i <= nBox) {
+		const unsigned long long k0 = (unsigned long long)(unsigned)i << 32;
+		int a = 0, z = ne;
+		while (a < z) { const int m = (a + z) >> 1; if (key[m] >= k0) z = m; else a = m + 1; }
+		off[i] = a;
+		if (i < nBox) pos[i] = a;
+	}
+}
+__global__ __launch_bounds__(256) void k_cut_sorted(const float *__restrict__ thr, const int *__restrict__ off,
+		int *pos, const float *__restrict__ srho, const int *__restrict__ perm, int *elive, float4 *eaux,
+		int *bscat, int *liveTotal) {
+	__shared__ int snew;
+	const int b = blockIdx.x;
+	const int lo = pos[b], hi = off[b + 1];
+	if (threadIdx.x == 0) {
+		const float t0 = thr[b];
+		const float t = (t0 == INFINITY) ? 0.f : t0;
+		int a = lo, z = hi;
+		while (a < z) { const int m = (a + z) >> 1; if (srho[m] >= t) z = m; else a = m + 1; }
+		snew = a;
+	}
+	__syncthreads();
+	const int nw = snew;
+	for (int i = lo + (int)threadIdx.x; i < nw; i += blockDim.x) {
+		const int e = perm[i];
+		elive[e] = 0;
+		eaux[e].x = -1.f;
+	}
+	if (threadIdx.x == 0) {
+		pos[b] = nw;
+		bscat[b] = hi - nw;
+		atomicAdd(liveTotal, hi - nw);
+	}
+}
+
 // Device-side move-loop bookkeeping: ml[0] step stamp, ml[1] active count,
 // ml[2] count after the prune.  Everything a block of steps launches reads
 // these, so the block can be replayed as one graph with no host round trip.
@@ -1505,6 +1560,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	if (c->h_pin) { (void)hipHostFree(c->h_pin); c->h_pin = nullptr; }
 	dfree(&c->d_bt); dfree(&c->d_boxOf); dfree(&c->d_type); dfree(&c->d_eblk); dfree(&c->d_bact); dfree(&c->d_bscat);
 	dfree(&c->d_bthr); dfree(&c->d_bslot);
+	dfree(&c->cs_perm); dfree(&c->cs_off); dfree(&c->cs_pos); dfree(&c->cs_rho);
 	delete[] c->hbox; delete[] c->h_eblk;
 	dfree(&c->d_thr); dfree(&c->d_cnt);
 	skg_dfree(&c->tmp);
@@ -1918,8 +1974,14 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 		} else {
 			k_thr_fold<<<c->nBox, SKG_THR_SLOTS, 0, c->st>>>(c->d_bslot, c->d_bthr);
 		}
-		k_scat_cut<<<nb, 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE, c->e_box, c->d_bthr, (int *)c->part);
-		k_box_sum<<<c->nBox, 64, 0, c->st>>>((const int *)c->part, c->d_eblk, c->d_bscat, c->d_cnt);
+		if (c->cutReady && !bInitial) {
+			k_cut_sorted<<<c->nBox, 256, 0, c->st>>>(c->d_bthr, c->cs_off, c->cs_pos, c->cs_rho, c->cs_perm,
+					c->e_live, c->e_aux, c->d_bscat, c->d_cnt);
+		} else {
+			k_scat_cut<<<nb, 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE, c->e_box, c->d_bthr,
+					(int *)c->part);
+			k_box_sum<<<c->nBox, 64, 0, c->st>>>((const int *)c->part, c->d_eblk, c->d_bscat, c->d_cnt);
+		}
 	}
 	SKG_HIP(hipGetLastError());
 	return 0;
@@ -1988,6 +2050,7 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	SKG_HIP(hipStreamSynchronize(c->st));
 	const int nm = last[0] + last[1];
 	c->nMove = nm;
+	c->cutReady = false;
 	drop_graph(c);
 	const size_t nbm = (size_t)nblk(std::max(nm, 1), 256);
 	if (dalloc(&c->m_ord, nm) || dalloc(&c->m_pos, nm) || dalloc(&c->m_old, nm) || dalloc(&c->m_list, nm) ||
@@ -2016,6 +2079,29 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	return SKG_OK;
 }
 
+// The density-sorted entity order for the remaining cuts (SKG_SORTED_CUT=0: a
+// pass over every entity at every step instead).
+static int build_sorted_cut(skg_ctx *c) {
+	c->cutReady = false;
+	const char *e = getenv("SKG_SORTED_CUT");
+	if (e && atoi(e) == 0) return 0;
+	const int ne = c->nE, nBox = c->nBox;
+	unsigned long long *k0 = nullptr, *k1 = nullptr;
+	int *v0 = nullptr, *v1 = nullptr, rc;
+	if (dalloc(&k0, ne) || dalloc(&k1, ne) || dalloc(&v0, ne) || dalloc(&v1, ne) || dalloc(&c->cs_perm, ne) ||
+	    dalloc(&c->cs_rho, ne) || dalloc(&c->cs_off, nBox + 1) || dalloc(&c->cs_pos, nBox))
+		return SKG_E_NOMEM;
+	k_cut_keys<<<nblk(ne, 256), 256, 0, c->st>>>(c->e_live, c->e_box, c->e_rho, ne, k0, v0);
+	if ((rc = skg_sort_pairs(c, k0, k1, v0, v1, ne, 64))) return rc;
+	k_cut_bounds<<<nblk(std::max(ne, nBox + 1), 256), 256, 0, c->st>>>(k1, ne, nBox, c->cs_off, c->cs_pos,
+			c->cs_rho, v1, c->cs_perm);
+	SKG_HIP(hipGetLastError());
+	SKG_HIP(hipStreamSynchronize(c->st));
+	dfree(&k0); dfree(&k1); dfree(&v0); dfree(&v1);
+	c->cutReady = true;
+	return 0;
+}
+
 extern "C" int skg_move_first(skg_ctx *c, int *nActive, int *nScatter) {
 	if (!c) return SKG_E_PARAM;
 	if (c->stage != 3) return SKG_E_STATE;
@@ -2027,6 +2113,7 @@ extern "C" int skg_move_first(skg_ctx *c, int *nActive, int *nScatter) {
 		if ((rc = enqueue_steps(c, c->nActive, c->nActive <= c->subMax, false, 1, 0, bInit, false))) return rc;
 		if ((rc = refit_entities(c))) return rc;
 		if ((rc = read_counts(c, &ns))) return rc;
+		if ((rc = build_sorted_cut(c))) return rc;
 	}
 	skg_timer_stop(c, SKG_T_MOVE);
 	if (nActive) *nActive = c->nActive;

@@ -133,7 +133,7 @@ def test_move_loop_launch_paths_agree():
     base = {"SKG_COVER_SUB_MAX": "2000"}
     for tag, env_add in (("direct", {}), ("graph", {"SKG_GRAPH": "1"}), ("nolists", {"SKG_TILE_LISTS": "0"}),
                          ("graph_nolists", {"SKG_GRAPH": "1", "SKG_TILE_LISTS": "0"}),
-                         ("stamped", {"SKG_FUSED_THR": "0"})):
+                         ("stamped", {"SKG_FUSED_THR": "0"}), ("scan_cut", {"SKG_SORTED_CUT": "0"})):
         env = dict(os.environ)
         env.update(base)
         env.update(env_add)
@@ -146,9 +146,10 @@ def test_move_loop_launch_paths_agree():
 
 
 def test_scattercut_threshold_paths_agree_batched():
-    """ScatterCut's per-box threshold taken inside the move kernels (per-lane
-    minima, atomics over per-box slots) equals the stamp-and-scan form
-    (SKG_FUSED_THR=0) on a batch of uneven boxes, for both move kernels: the
+    """ScatterCut with the per-box threshold taken inside the move kernels
+    (per-lane minima, atomics over per-box slots) and the cut over
+    density-sorted entities equals the stamp-and-scan form (SKG_FUSED_THR=0,
+    SKG_SORTED_CUT=0) on a batch of uneven boxes, for both move kernels: the
     per-box Ittr traces and the group ids are identical."""
     import os
     import subprocess
@@ -167,9 +168,9 @@ def test_scattercut_threshold_paths_agree_batched():
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     for sub_max in ("0", "100000000"):
         outs = []
-        for tag, fused in (("fused", "1"), ("stamped", "0")):
+        for tag, fused, cut in (("fused", "1", "1"), ("stamped", "0", "0")):
             env = dict(os.environ)
-            env.update({"SKG_FUSED_THR": fused, "SKG_COVER_SUB_MAX": sub_max})
+            env.update({"SKG_FUSED_THR": fused, "SKG_SORTED_CUT": cut, "SKG_COVER_SUB_MAX": sub_max})
             path = f"/tmp/skg_thr_{tag}_{sub_max}.npy"
             subprocess.run([sys.executable, "-c", code, path], check=True, env=env, timeout=240, cwd=root)
             outs.append(np.load(path))
