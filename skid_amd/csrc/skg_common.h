@@ -140,6 +140,8 @@ struct skg_ctx {
 	int *cs_perm = nullptr, *cs_off = nullptr, *cs_pos = nullptr;
 	float *cs_rho = nullptr;
 	bool cutReady = false;
+	int *e_leaf = nullptr;      // entity -> leaf of the entity tree (with the sorted cut)
+	int *leafDirty = nullptr;   // leaves holding an entity cut since the last refit
 	int *h_pin = nullptr;                        // pinned read-back
 	int useGraph = 0;                            // SKG_GRAPH=1: replay each block as a captured graph
 	hipGraph_t graph = nullptr;

@@ -384,6 +384,28 @@ __global__ void k_leaf_ball(const float4 *__restrict__ eaux, int n, int B, int n
 	}
 	lo[nLeaf2 + l].w = w;
 }
+// The same for the leaves flagged by the sorted cut only (the others' entities
+// are unchanged since the last refit); clears the flags.
+__global__ void k_leaf_ball_dirty(const float4 *__restrict__ eaux, int n, int B, int nLeaf, int nLeaf2,
+		const int *__restrict__ lstart, float4 *lo, int *dirty) {
+	const int l = blockIdx.x * blockDim.x + threadIdx.x;
+	if (l >= nLeaf || !dirty[l]) return;
+	dirty[l] = 0;
+	float w = -INFINITY;
+	const int2 r = leaf_range(lstart, l, B, n);
+	for (int i = r.x; i < r.y; ++i) w = fmaxf(w, eaux[i].x);
+	lo[nLeaf2 + l].w = w;
+}
+// entity -> leaf, and no leaf flagged
+__global__ void k_entity_leaf(int n, int B, int nLeaf, int nLeaf2, const int *__restrict__ lstart, int *eleaf,
+		int *dirty) {
+	const int l = blockIdx.x * blockDim.x + threadIdx.x;
+	if (l >= nLeaf2) return;
+	dirty[l] = 0;
+	if (l >= nLeaf) return;
+	const int2 r = leaf_range(lstart, l, B, n);
+	for (int i = r.x; i < r.y; ++i) eleaf[i] = l;
+}
 __global__ void k_level_ball(float4 *lo, int first, int count) {
 	const int t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= count) return;
@@ -1292,7 +1314,7 @@ __global__ void k_cut_bounds(const unsigned long long *__restrict__ key, int ne,
 }
 __global__ __launch_bounds__(256) void k_cut_sorted(const float *__restrict__ thr, const int *__restrict__ off,
 		int *pos, const float *__restrict__ srho, const int *__restrict__ perm, int *elive, float4 *eaux,
-		int *bscat, int *liveTotal) {
+		int *bscat, int *liveTotal, const int *__restrict__ eleaf, int *dirty) {
 	__shared__ int snew;
 	const int b = blockIdx.x;
 	const int lo = pos[b], hi = off[b + 1];
@@ -1309,6 +1331,7 @@ __global__ __launch_bounds__(256) void k_cut_sorted(const float *__restrict__ th
 		const int e = perm[i];
 		elive[e] = 0;
 		eaux[e].x = -1.f;
+		dirty[eleaf[e]] = 1;
 	}
 	if (threadIdx.x == 0) {
 		pos[b] = nw;
@@ -1561,6 +1584,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->d_bt); dfree(&c->d_boxOf); dfree(&c->d_type); dfree(&c->d_eblk); dfree(&c->d_bact); dfree(&c->d_bscat);
 	dfree(&c->d_bthr); dfree(&c->d_bslot);
 	dfree(&c->cs_perm); dfree(&c->cs_off); dfree(&c->cs_pos); dfree(&c->cs_rho);
+	dfree(&c->e_leaf); dfree(&c->leafDirty);
 	delete[] c->hbox; delete[] c->h_eblk;
 	dfree(&c->d_thr); dfree(&c->d_cnt);
 	skg_dfree(&c->tmp);
@@ -1976,7 +2000,7 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 		}
 		if (c->cutReady && !bInitial) {
 			k_cut_sorted<<<c->nBox, 256, 0, c->st>>>(c->d_bthr, c->cs_off, c->cs_pos, c->cs_rho, c->cs_perm,
-					c->e_live, c->e_aux, c->d_bscat, c->d_cnt);
+					c->e_live, c->e_aux, c->d_bscat, c->d_cnt, c->e_leaf, c->leafDirty);
 		} else {
 			k_scat_cut<<<nb, 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE, c->e_box, c->d_bthr,
 					(int *)c->part);
@@ -2009,7 +2033,13 @@ static int enqueue_prune(skg_ctx *c, long cap) {
 
 static int refit_entities(skg_ctx *c) {
 	DevTree *t = &c->te;
-	k_leaf_ball<<<nblk(t->nLeaf2, 256), 256, 0, c->st>>>(c->e_aux, t->n, t->B, t->nLeaf, t->nLeaf2, t->lstart, t->lo);
+	// with the sorted cut every cut entity's leaf is flagged; before it, all
+	if (c->cutReady)
+		k_leaf_ball_dirty<<<nblk(t->nLeaf, 256), 256, 0, c->st>>>(c->e_aux, t->n, t->B, t->nLeaf, t->nLeaf2,
+				t->lstart, t->lo, c->leafDirty);
+	else
+		k_leaf_ball<<<nblk(t->nLeaf2, 256), 256, 0, c->st>>>(c->e_aux, t->n, t->B, t->nLeaf, t->nLeaf2, t->lstart,
+				t->lo);
 	for (int m = t->nLeaf2 >> 1; m >= 1; m >>= 1)
 		k_level_ball<<<nblk(m, 256), 256, 0, c->st>>>(t->lo, m, m);
 	SKG_HIP(hipGetLastError());
@@ -2088,9 +2118,13 @@ static int build_sorted_cut(skg_ctx *c) {
 	const int ne = c->nE, nBox = c->nBox;
 	unsigned long long *k0 = nullptr, *k1 = nullptr;
 	int *v0 = nullptr, *v1 = nullptr, rc;
+	const DevTree *t = &c->te;
 	if (dalloc(&k0, ne) || dalloc(&k1, ne) || dalloc(&v0, ne) || dalloc(&v1, ne) || dalloc(&c->cs_perm, ne) ||
-	    dalloc(&c->cs_rho, ne) || dalloc(&c->cs_off, nBox + 1) || dalloc(&c->cs_pos, nBox))
+	    dalloc(&c->cs_rho, ne) || dalloc(&c->cs_off, nBox + 1) || dalloc(&c->cs_pos, nBox) ||
+	    dalloc(&c->e_leaf, ne) || dalloc(&c->leafDirty, t->nLeaf2))
 		return SKG_E_NOMEM;
+	k_entity_leaf<<<nblk(t->nLeaf2, 256), 256, 0, c->st>>>(t->n, t->B, t->nLeaf, t->nLeaf2, t->lstart, c->e_leaf,
+			c->leafDirty);
 	k_cut_keys<<<nblk(ne, 256), 256, 0, c->st>>>(c->e_live, c->e_box, c->e_rho, ne, k0, v0);
 	if ((rc = skg_sort_pairs(c, k0, k1, v0, v1, ne, 64))) return rc;
 	k_cut_bounds<<<nblk(std::max(ne, nBox + 1), 256), 256, 0, c->st>>>(k1, ne, nBox, c->cs_off, c->cs_pos,
