@@ -1547,7 +1547,12 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		if (!gsmall.empty()) SKG_HIP(hipMemcpyAsync(dgs, gsmall.data(), 4 * gsmall.size(), hipMemcpyHostToDevice, c->st));
 		if (!gmed.empty()) SKG_HIP(hipMemcpyAsync(dgm, gmed.data(), 4 * gmed.size(), hipMemcpyHostToDevice, c->st));
 		if (c->tunePot && !tiles.empty()) {
-			// autotuner: time each LDS tile on these groups (setup resets the potentials)
+			// autotuner: time each LDS tile on these groups (setup resets the
+			// potentials), with events of its own (the context's bracket this
+			// call's stage time)
+			hipEvent_t t0, t1;
+			SKG_HIP(hipEventCreate(&t0));
+			SKG_HIP(hipEventCreate(&t1));
 			float best = FLT_MAX;
 			for (int T : {64, 128, 256}) {
 				std::vector<int2> tt;
@@ -1555,14 +1560,16 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 				SKG_HIP(hipMemcpyAsync(dt, tt.data(), 8 * tt.size(), hipMemcpyHostToDevice, c->st));
 				k_unb_setup<<<nblk(nTot, 256), 256, 0, c->st>>>(G->gstart, G->gend, nG, G->mem, c->d_pos, c->d_vel,
 						c->d_mass, c->d_soft, G->rel, hx, hy, hz, nMax, G->uq, G->uv, G->uphi, G->uord, nTot);
-				SKG_HIP(hipEventRecord(c->e0, c->st));
+				SKG_HIP(hipEventRecord(t0, c->st));
 				if (launch_pot(T, dt, (int)tt.size())) return SKG_E_NOMEM;
-				SKG_HIP(hipEventRecord(c->e1, c->st));
-				SKG_HIP(hipEventSynchronize(c->e1));
+				SKG_HIP(hipEventRecord(t1, c->st));
+				SKG_HIP(hipEventSynchronize(t1));
 				float ms = 0.f;
-				SKG_HIP(hipEventElapsedTime(&ms, c->e0, c->e1));
+				SKG_HIP(hipEventElapsedTime(&ms, t0, t1));
 				if (ms < best) { best = ms; c->potTile = T; }
 			}
+			(void)hipEventDestroy(t0);
+			(void)hipEventDestroy(t1);
 			c->tunePot = 0;
 			make_tiles(c->potTile, tiles);
 			SKG_HIP(hipMemcpyAsync(dt, tiles.data(), 8 * tiles.size(), hipMemcpyHostToDevice, c->st));
