@@ -400,8 +400,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              f"(sharding across GPUs is not built)", file=sys.stderr)
+        print(f"bench.py: the group-finding path runs on one GPU; --gpus {args.gpus} is not supported "
+              f"(sharding across GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "batched":
         value, wall, config, roof, cpu = bench_batched(args)
