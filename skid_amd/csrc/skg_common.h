@@ -128,6 +128,7 @@ struct skg_ctx {
 	// launch shapes (set at create; SKG_COVER_SUB_MAX overrides): point sets of
 	// at most subMax points use the sub-wave cover kernel
 	long subMax = 0;
+	long s64Max = 0;            // at most this many active movers: 64 lanes (a wavefront) per mover
 	// per-tile leaf lists of the current 5-step block (SKG_TILE_LISTS=0: off)
 	int useLists = 1, tlCap = 512;
 	int *tl_list = nullptr, *tl_cnt = nullptr;

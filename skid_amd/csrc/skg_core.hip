@@ -932,6 +932,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 	float hmin = INFINITY;
 	int node = valid ? 1 : 0;
 	while (node) {                                // the S lanes of a point move in lockstep
+		if (S == 64) node = uniform_int(node);    // one point per wavefront: scalar node reads
 		const float4 lo = nlo[node], hi = nhi[node];
 		const bool need = brange_has(hi.w, pb) && point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
 		if (need && node < nLeaf2) { node = 2 * node; continue; }
@@ -1544,6 +1545,8 @@ extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
 	c->dev = P->device;
 	c->subMax = 1 << 18;
 	if (const char *e = getenv("SKG_COVER_SUB_MAX")) c->subMax = atol(e);
+	c->s64Max = 1 << 14;
+	if (const char *e = getenv("SKG_COVER_S64_MAX")) c->s64Max = atol(e);
 	if (const char *e = getenv("SKG_TILE_LISTS")) c->useLists = atoi(e);
 	if (const char *e = getenv("SKG_GRAPH")) c->useGraph = atoi(e);
 	if (const char *e = getenv("SKG_GRAPH_SHRINK")) c->gShrink = std::max(1, atoi(e));
@@ -1980,7 +1983,11 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 					c->nBox * SKG_THR_SLOTS, c->d_cnt);
 		const int pslot = capturing ? -1 : skg_prof_begin(c);
 #define MV(C_)                                                                                                \
-		if (sub) k_cover_sub<1, C_, 16><<<nblk(16L * cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos,       \
+		if (sub && cap <= c->s64Max) k_cover_sub<1, C_, 64><<<nblk(64L * cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>( \
+				c->m_pos, c->m_list, (int)cap, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi,          \
+				c->te.nLeaf2, c->te.B, nullptr, ehit, 0, bthr, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0],     \
+				L[1], L[2], c->te.lstart, c->d_ml + 1, c->d_ml);                                                 \
+		else if (sub) k_cover_sub<1, C_, 16><<<nblk(16L * cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos,  \
 				c->m_list, (int)cap, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2,      \
 				c->te.B, nullptr, ehit, 0, bthr, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1],             \
 				L[2], c->te.lstart, c->d_ml + 1, c->d_ml);                                                       \
