@@ -2126,21 +2126,25 @@ static int build_sorted_cut(skg_ctx *c) {
 	unsigned long long *k0 = nullptr, *k1 = nullptr;
 	int *v0 = nullptr, *v1 = nullptr, rc;
 	const DevTree *t = &c->te;
+	auto done = [&](int status) {
+		dfree(&k0); dfree(&k1); dfree(&v0); dfree(&v1);
+		return status;
+	};
 	if (dalloc(&k0, ne) || dalloc(&k1, ne) || dalloc(&v0, ne) || dalloc(&v1, ne) || dalloc(&c->cs_perm, ne) ||
 	    dalloc(&c->cs_rho, ne) || dalloc(&c->cs_off, nBox + 1) || dalloc(&c->cs_pos, nBox) ||
 	    dalloc(&c->e_leaf, ne) || dalloc(&c->leafDirty, t->nLeaf2))
-		return SKG_E_NOMEM;
+		return done(SKG_E_NOMEM);
 	k_entity_leaf<<<nblk(t->nLeaf2, 256), 256, 0, c->st>>>(t->n, t->B, t->nLeaf, t->nLeaf2, t->lstart, c->e_leaf,
 			c->leafDirty);
 	k_cut_keys<<<nblk(ne, 256), 256, 0, c->st>>>(c->e_live, c->e_box, c->e_rho, ne, k0, v0);
-	if ((rc = skg_sort_pairs(c, k0, k1, v0, v1, ne, 64))) return rc;
+	if ((rc = skg_sort_pairs(c, k0, k1, v0, v1, ne, 64))) return done(rc);
 	k_cut_bounds<<<nblk(std::max(ne, nBox + 1), 256), 256, 0, c->st>>>(k1, ne, nBox, c->cs_off, c->cs_pos,
 			c->cs_rho, v1, c->cs_perm);
-	SKG_HIP(hipGetLastError());
-	SKG_HIP(hipStreamSynchronize(c->st));
-	dfree(&k0); dfree(&k1); dfree(&v0); dfree(&v1);
+	hipError_t he = hipGetLastError();
+	if (he == hipSuccess) he = hipStreamSynchronize(c->st);
+	if (he != hipSuccess) { skg_note_hip(he, "build_sorted_cut", __FILE__, __LINE__); return done(SKG_E_HIP); }
 	c->cutReady = true;
-	return 0;
+	return done(0);
 }
 
 extern "C" int skg_move_first(skg_ctx *c, int *nActive, int *nScatter) {

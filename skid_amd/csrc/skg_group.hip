@@ -1561,7 +1561,11 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 				k_unb_setup<<<nblk(nTot, 256), 256, 0, c->st>>>(G->gstart, G->gend, nG, G->mem, c->d_pos, c->d_vel,
 						c->d_mass, c->d_soft, G->rel, hx, hy, hz, nMax, G->uq, G->uv, G->uphi, G->uord, nTot);
 				SKG_HIP(hipEventRecord(t0, c->st));
-				if (launch_pot(T, dt, (int)tt.size())) return SKG_E_NOMEM;
+				if (launch_pot(T, dt, (int)tt.size())) {
+					(void)hipEventDestroy(t0);
+					(void)hipEventDestroy(t1);
+					return SKG_E_NOMEM;
+				}
 				SKG_HIP(hipEventRecord(t1, c->st));
 				SKG_HIP(hipEventSynchronize(t1));
 				float ms = 0.f;
