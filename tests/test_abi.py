@@ -63,3 +63,10 @@ def test_driver_cli_usage():
     assert r.returncode == 1
     r = subprocess.run([exe, "-tau", "0.1"], input=b"\x00" * 10, capture_output=True)
     assert r.returncode == 1 and b"tipsy" in r.stderr
+
+
+def test_bench_refuses_several_gpus():
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       timeout=120)
+    assert r.returncode == 2 and b"one GPU" in r.stderr and r.stdout == b""
