@@ -224,6 +224,23 @@ int skg_field_gradient(skg_ctx *ctx, int nq, const float *qpos, float *acc);
    context for the device and stream. */
 int skg_fof_points(skg_ctx *ctx, int n, const float *pos, float fTau, int *grp, int *nGroup);
 
+/* FOF of n points already in device memory, allocating nothing.  No
+   reference counterpart: it is for callers that cannot allocate device memory
+   (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  It gives skg_fof_points's result on the same points: d_pos
+   holds 3n floats (x, y, z) and d_grp receives n group ids (1-based, numbered
+   by smallest index), both in device memory of the context's device; *nGroup
+   (host, optional) = groups + 1.  Its temporaries live in one fixed device
+   array of SKG_NOALLOC_SCRATCH_BYTES per GPU, which the library holds from
+   the load of its code on that GPU to the end of the process.  n above
+   SKG_NOALLOC_MAX_POINTS returns SKG_E_PARAM (so does a context whose
+   nBucket is below 4, when the tree no longer fits the array).  Every call on
+   a GPU shares that GPU's array: calls that overlap on one GPU (contexts used
+   from several host threads at once) race.  Synchronous at return. */
+#define SKG_NOALLOC_MAX_POINTS 524288
+#define SKG_NOALLOC_SCRATCH_BYTES (64 << 20)
+int skg_fof_points_noalloc(skg_ctx *ctx, int n, const float *d_pos, float fTau, int *d_grp, int *nGroup);
+
 /* Autotuner (no reference counterpart): times launch shapes on the loaded
    particles and keeps the fastest for this context.  what is a mask of
      SKG_TUNE_BUCKET    leaf size nBucket in {8, 16, 32, 64} (skg_density plus

@@ -362,6 +362,16 @@ int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *p
 int skg_hilbert_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box);
 int skg_box_bits(const skg_ctx *c);
 int skg_exclusive_scan(skg_ctx *c, const int *in, int *out, int n);
+// The same on caller storage (tmp null: report the bytes needed in *need only).
+int skg_sort_pairs_at(hipStream_t st, void *tmp, size_t tmpBytes, size_t *need, unsigned long long *kin,
+		unsigned long long *kout, int *vin, int *vout, int n, int endBit);
+int skg_exclusive_scan_at(hipStream_t st, void *tmp, size_t tmpBytes, size_t *need, const int *in, int *out,
+		int n);
+int skg_curve_keys(skg_ctx *c, const float4 *pos, const int *idx, int n, const int *box, bool hilbert,
+		unsigned long long *keys, int *vals, float *part, unsigned *bb);
+int skg_tree_leaves2(int n, int B, int *nLeaf);
+int skg_build_tree_at(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, float4 *lo, float4 *hi);
+int skg_noalloc_setup(int dev);                         // skg_fof.hip: the scratch array's address
 void skg_free_tree(DevTree *t);
 int skg_gradient_points(skg_ctx *c, const float4 *q, int nq, float *out);   // skg_adjoint.hip
 void skg_timer_start(skg_ctx *c);

@@ -102,6 +102,31 @@ int skg_exclusive_scan(skg_ctx *c, const int *in, int *out, int n) {
 	return 0;
 }
 
+// The same two primitives on caller storage (tmp of tmpBytes bytes); with tmp
+// null they only report the bytes they need in *need.  rocprim allocates
+// nothing when handed its temporary storage.
+int skg_sort_pairs_at(hipStream_t st, void *tmp, size_t tmpBytes, size_t *need, unsigned long long *kin,
+		unsigned long long *kout, int *vin, int *vout, int n, int endBit) {
+	size_t bytes = 0;
+	SKG_HIP(rocprim::radix_sort_pairs(nullptr, bytes, kin, kout, vin, vout, (size_t)std::max(n, 1), 0u,
+			(unsigned)endBit, st));
+	if (need) *need = bytes;
+	if (!tmp || n <= 0) return 0;
+	if (bytes > tmpBytes) return SKG_E_PARAM;
+	SKG_HIP(rocprim::radix_sort_pairs(tmp, tmpBytes, kin, kout, vin, vout, (size_t)n, 0u, (unsigned)endBit, st));
+	return 0;
+}
+int skg_exclusive_scan_at(hipStream_t st, void *tmp, size_t tmpBytes, size_t *need, const int *in, int *out,
+		int n) {
+	size_t bytes = 0;
+	SKG_HIP(rocprim::exclusive_scan(nullptr, bytes, in, out, 0, (size_t)std::max(n, 1), rocprim::plus<int>(), st));
+	if (need) *need = bytes;
+	if (!tmp || n <= 0) return 0;
+	if (bytes > tmpBytes) return SKG_E_PARAM;
+	SKG_HIP(rocprim::exclusive_scan(tmp, tmpBytes, in, out, 0, (size_t)n, rocprim::plus<int>(), st));
+	return 0;
+}
+
 void skg_timer_start(skg_ctx *c) { (void)hipEventRecord(c->e0, c->st); }
 void skg_timer_stop(skg_ctx *c, int slot) {
 	float ms = 0.f;
@@ -277,23 +302,32 @@ int skg_box_bits(const skg_ctx *c) {
 	return b;
 }
 
+// Curve keys of n points (bounding box, then Hilbert or Morton keys) into
+// keys/vals, on caller storage: part holds 6 floats per 256 points, bb 6
+// words.  vals[i] = the point's index into pos (or its idx[] entry).
+int skg_curve_keys(skg_ctx *c, const float4 *pos, const int *idx, int n, const int *box, bool hilbert,
+		unsigned long long *keys, int *vals, float *part, unsigned *bb) {
+	if (n <= 0) return 0;
+	const int nb = nblk(n, 256);
+	k_bbox<<<nb, 256, 0, c->st>>>(pos, idx, n, part);
+	k_bbox_final<<<1, 256, 0, c->st>>>(part, nb, bb);
+	if (hilbert)
+		k_morton<true><<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, bb, keys, vals, box, box ? skg_box_bits(c) : 0);
+	else
+		k_morton<false><<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, bb, keys, vals, box, box ? skg_box_bits(c) : 0);
+	SKG_HIP(hipGetLastError());
+	return 0;
+}
+
 // perm_out[i] = index (into pos, or idx[] entry) of the i-th point in Morton
 // order; with box (indexed like pos) the order is by box first.
 static int curve_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box,
 		bool hilbert) {
 	if (n <= 0) return 0;
 	if (skg_ensure_keys(c, n)) return SKG_E_NOMEM;
-	const int nb = nblk(n, 256);
-	if (skg_ensure_part(c, 6 * (size_t)nb)) return SKG_E_NOMEM;
-	k_bbox<<<nb, 256, 0, c->st>>>(pos, idx, n, c->part);
-	k_bbox_final<<<1, 256, 0, c->st>>>(c->part, nb, (unsigned *)(c->d_cnt + 8));
-	if (hilbert)
-		k_morton<true><<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, (const unsigned *)(c->d_cnt + 8), c->k64a,
-				c->v32a, box, box ? skg_box_bits(c) : 0);
-	else
-		k_morton<false><<<nblk(n, 256), 256, 0, c->st>>>(pos, idx, n, (const unsigned *)(c->d_cnt + 8), c->k64a,
-				c->v32a, box, box ? skg_box_bits(c) : 0);
-	SKG_HIP(hipGetLastError());
+	if (skg_ensure_part(c, 6 * (size_t)nblk(n, 256))) return SKG_E_NOMEM;
+	int rc = skg_curve_keys(c, pos, idx, n, box, hilbert, c->k64a, c->v32a, c->part, (unsigned *)(c->d_cnt + 8));
+	if (rc) return rc;
 	return skg_sort_pairs(c, c->k64a, c->k64b, c->v32a, perm_out, n, 63);
 }
 int skg_morton_sort(skg_ctx *c, const float4 *pos, const int *idx, int n, int *perm_out, const int *box) {
@@ -433,6 +467,25 @@ int skg_build_tree(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, cons
 	dfree(&t->lstart);
 	t->n = n; t->B = B; t->nLeaf = nLeaf; t->nLeaf2 = nLeaf2;
 	return tree_fill(c, t, pos, b2, ibox);
+}
+
+// Leaves and their power-of-two count for n items in leaves of B.
+int skg_tree_leaves2(int n, int B, int *nLeaf) {
+	const int nl = std::max(1, (n + B - 1) / B);
+	int nl2 = 1;
+	while (nl2 < nl) nl2 <<= 1;
+	if (nLeaf) *nLeaf = nl;
+	return nl2;
+}
+
+// The same tree on caller storage: lo and hi hold 2 * skg_tree_leaves2(n, B)
+// records each and stay the caller's (the tree must not be freed).
+int skg_build_tree_at(skg_ctx *c, DevTree *t, const float4 *pos, int n, int B, float4 *lo, float4 *hi) {
+	int nLeaf = 0;
+	const int nLeaf2 = skg_tree_leaves2(n, B, &nLeaf);
+	t->lo = lo; t->hi = hi; t->lstart = nullptr;
+	t->n = n; t->B = B; t->nLeaf = nLeaf; t->nLeaf2 = nLeaf2;
+	return tree_fill(c, t, pos, nullptr, nullptr);
 }
 
 // Same structure, new bounds and radii (positions or live radii changed).
@@ -1560,6 +1613,9 @@ extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
 		return SKG_E_NODEVICE;
 	}
 	if (dalloc(&c->d_cnt, 64) || dalloc(&c->d_thr, 8)) { skg_destroy(c); return SKG_E_NOMEM; }
+	// the device address of the no-allocation entry point's scratch array, once
+	// per device (a failed lookup is kept and reported by that entry point)
+	(void)skg_noalloc_setup(c->dev);
 	*out = c;
 	return SKG_OK;
 }

@@ -15,6 +15,7 @@
 #include "skg_common.h"
 #include <climits>
 #include <cfloat>
+#include <mutex>
 
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
@@ -169,51 +170,80 @@ __global__ void k_gather4(const float4 *__restrict__ in, const int *__restrict__
 	}
 }
 
-// FOF of pos[0,n) (float4); writes grp[outmap ? outmap[o] : o] = 1-based id for
-// every point o, numbered by smallest o.  Returns the number of groups.  With
-// box (indexed like pos) points link only within a box, using its tau.
-static int fof_run(skg_ctx *c, const float4 *pos, int n, float tau, const int *outmap, int *grp, int *ngOut,
-		const int *box) {
+// Storage of one FOF run: the points in curve order (fp, perm, fbox), the
+// union-find arrays, the tree (tlo/thi: 2 nLeaf2 records each; null: the tree
+// allocates its own) and the scan's temporary storage (null: the context's).
+struct FofBufs {
+	float4 *fp = nullptr;
+	int *perm = nullptr, *fbox = nullptr;
+	int *par = nullptr, *cmin = nullptr, *isFirst = nullptr, *rank = nullptr, *united = nullptr;
+	float4 *tlo = nullptr, *thi = nullptr;
+	void *scanTmp = nullptr;
+	size_t scanBytes = 0;
+};
+
+// Links and labels of n points already in curve order (b.fp, with b.perm[i]
+// the input index of sorted point i): writes grp[outmap ? outmap[o] : o] =
+// 1-based id for every input point o, numbered by smallest o, and the number
+// of groups to *ngOut.  With b.fbox (sorted like fp) points link only within
+// a box, using its tau.
+static int fof_label(skg_ctx *c, const FofBufs &b, int n, float tau, const int *outmap, int *grp, int *ngOut) {
 	int rc;
 	const bool per = c->P.bPeriodic != 0;
 	const float L = per ? c->P.fPeriod[0] : FLT_MAX;
-	int *perm = nullptr, *par = nullptr, *united = nullptr, *cmin = nullptr, *isFirst = nullptr, *rank = nullptr;
-	int *fbox = nullptr;
-	float4 *fp = nullptr;
 	DevTree t;
-	if (dalloc(&perm, n) || dalloc(&par, n) || dalloc(&cmin, n) || dalloc(&isFirst, n) || dalloc(&rank, n) ||
-	    dalloc(&fp, n) || (box && dalloc(&fbox, n)))
-		return SKG_E_NOMEM;
-	if ((rc = skg_hilbert_sort(c, pos, nullptr, n, perm, box))) return rc;   // compact runs of 64 points
-	k_gather4<<<nblk(n, 256), 256, 0, c->st>>>(pos, perm, n, fp, box, fbox);
-	if ((rc = skg_build_tree(c, &t, fp, n, c->P.nBucket, nullptr, fbox))) return rc;
-	if (dalloc(&united, 2 * (size_t)t.nLeaf2)) return SKG_E_NOMEM;
+	if (b.tlo) rc = skg_build_tree_at(c, &t, b.fp, n, c->P.nBucket, b.tlo, b.thi);
+	else rc = skg_build_tree(c, &t, b.fp, n, c->P.nBucket, nullptr, b.fbox);
+	if (rc) { if (!b.tlo) skg_free_tree(&t); return rc; }
 	int depth = 0;
 	while ((1 << depth) < t.nLeaf2) ++depth;
-	k_zero<<<nblk(2 * t.nLeaf2, 256), 256, 0, c->st>>>(united, 2 * t.nLeaf2);
-	k_fof_init<<<nblk(n, 256), 256, 0, c->st>>>(par, n);
+	k_zero<<<nblk(2 * t.nLeaf2, 256), 256, 0, c->st>>>(b.united, 2 * t.nLeaf2);
+	k_fof_init<<<nblk(n, 256), 256, 0, c->st>>>(b.par, n);
 	const float tau2 = tau * tau;
 	const int pslot = skg_prof_begin(c);
-	if (per) k_fof_links<true><<<nblk(n, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2,
-			fbox, c->d_bt, par, united);
-	else k_fof_links<false><<<nblk(n, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2,
-			fbox, c->d_bt, par, united);
+	if (per) k_fof_links<true><<<nblk(n, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(b.fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2,
+			b.fbox, c->d_bt, b.par, b.united);
+	else k_fof_links<false><<<nblk(n, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(b.fp, n, t.lo, t.hi, t.nLeaf2, depth, t.B, L, tau2,
+			b.fbox, c->d_bt, b.par, b.united);
 	skg_prof_end(c, pslot, SKG_K_FOF, 16.0 * n);
-	k_fill<<<nblk(n, 256), 256, 0, c->st>>>(cmin, n, INT_MAX);
-	k_zero<<<nblk(n, 256), 256, 0, c->st>>>(isFirst, n);
-	k_fof_root_min<<<nblk(n, 256), 256, 0, c->st>>>(par, n, perm, cmin);
-	k_fof_first<<<nblk(n, 256), 256, 0, c->st>>>(par, n, cmin, isFirst);
-	if ((rc = skg_exclusive_scan(c, isFirst, rank, n))) return rc;
-	k_fof_label<<<nblk(n, 256), 256, 0, c->st>>>(par, n, perm, cmin, rank, outmap, grp);
+	k_fill<<<nblk(n, 256), 256, 0, c->st>>>(b.cmin, n, INT_MAX);
+	k_zero<<<nblk(n, 256), 256, 0, c->st>>>(b.isFirst, n);
+	k_fof_root_min<<<nblk(n, 256), 256, 0, c->st>>>(b.par, n, b.perm, b.cmin);
+	k_fof_first<<<nblk(n, 256), 256, 0, c->st>>>(b.par, n, b.cmin, b.isFirst);
+	rc = b.scanTmp ? skg_exclusive_scan_at(c->st, b.scanTmp, b.scanBytes, nullptr, b.isFirst, b.rank, n)
+	               : skg_exclusive_scan(c, b.isFirst, b.rank, n);
+	if (!b.tlo) skg_free_tree(&t);
+	if (rc) return rc;
+	k_fof_label<<<nblk(n, 256), 256, 0, c->st>>>(b.par, n, b.perm, b.cmin, b.rank, outmap, grp);
 	SKG_HIP(hipGetLastError());
 	int last[2];
-	SKG_HIP(hipMemcpyAsync(&last[0], rank + n - 1, 4, hipMemcpyDeviceToHost, c->st));
-	SKG_HIP(hipMemcpyAsync(&last[1], isFirst + n - 1, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(&last[0], b.rank + n - 1, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(&last[1], b.isFirst + n - 1, 4, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
 	*ngOut = last[0] + last[1];
-	dfree(&perm); dfree(&par); dfree(&united); dfree(&cmin); dfree(&isFirst); dfree(&rank); dfree(&fp); dfree(&fbox);
-	skg_free_tree(&t);
 	return 0;
+}
+
+// FOF of pos[0,n) (float4) on storage the library allocates for the call;
+// see fof_label for the outputs.  With box (indexed like pos) points link
+// only within a box, using its tau.
+static int fof_run(skg_ctx *c, const float4 *pos, int n, float tau, const int *outmap, int *grp, int *ngOut,
+		const int *box) {
+	FofBufs b;
+	auto done = [&](int status) {
+		dfree(&b.perm); dfree(&b.par); dfree(&b.united); dfree(&b.cmin); dfree(&b.isFirst); dfree(&b.rank);
+		dfree(&b.fp); dfree(&b.fbox);
+		return status;
+	};
+	int rc;
+	const int nLeaf2 = skg_tree_leaves2(n, c->P.nBucket, nullptr);
+	if (dalloc(&b.perm, n) || dalloc(&b.par, n) || dalloc(&b.cmin, n) || dalloc(&b.isFirst, n) ||
+	    dalloc(&b.rank, n) || dalloc(&b.fp, n) || dalloc(&b.united, 2 * (size_t)nLeaf2) ||
+	    (box && dalloc(&b.fbox, n)))
+		return done(SKG_E_NOMEM);
+	if ((rc = skg_hilbert_sort(c, pos, nullptr, n, b.perm, box))) return done(rc);   // compact runs of 64 points
+	k_gather4<<<nblk(n, 256), 256, 0, c->st>>>(pos, b.perm, n, b.fp, box, b.fbox);
+	return done(fof_label(c, b, n, tau, outmap, grp, ngOut));
 }
 
 extern "C" int skg_fof(skg_ctx *c, int *grp, int *nGroup) {
@@ -255,6 +285,109 @@ extern "C" int skg_fof_points(skg_ctx *c, int n, const float *pos, float fTau, i
 	SKG_HIP(hipMemcpyAsync(grp, g, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
 	dfree(&p4); dfree(&g);
+	if (nGroup) *nGroup = ng + 1;
+	return SKG_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FOF of device-resident points without allocating (skg_fof_points_noalloc).
+// Every temporary of the call lives in `scratch`, one file-scope device array
+// per GPU.  Host code never passes the array's name to a kernel: the name is a
+// host-side stand-in, and the device address of each GPU's copy comes from
+// hipGetSymbolAddress, looked up once per device by skg_noalloc_setup (run by
+// skg_create with the context's device current) and kept in g_scr_ptr.
+static __device__ __align__(256) unsigned char scratch[SKG_NOALLOC_SCRATCH_BYTES];
+
+#define SKG_SCR_DEVICES 64
+static std::mutex g_scr_mu;
+static void *g_scr_ptr[SKG_SCR_DEVICES];
+static int g_scr_err[SKG_SCR_DEVICES];       // hipError_t of a failed lookup (0: none)
+
+int skg_noalloc_setup(int dev) {
+	if (dev < 0 || dev >= SKG_SCR_DEVICES) return SKG_E_PARAM;
+	std::lock_guard<std::mutex> lk(g_scr_mu);
+	if (g_scr_ptr[dev]) return SKG_OK;
+	void *p = nullptr;
+	const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+	if (e != hipSuccess || !p) {
+		g_scr_err[dev] = e != hipSuccess ? (int)e : (int)hipErrorInvalidSymbol;
+		skg_note_hip((hipError_t)g_scr_err[dev], "hipGetSymbolAddress(scratch)", __FILE__, __LINE__);
+		return SKG_E_HIP;
+	}
+	g_scr_ptr[dev] = p;
+	g_scr_err[dev] = 0;
+	return SKG_OK;
+}
+
+static unsigned char *scratch_of(int dev) {
+	if (dev < 0 || dev >= SKG_SCR_DEVICES) return nullptr;
+	std::lock_guard<std::mutex> lk(g_scr_mu);
+	return (unsigned char *)g_scr_ptr[dev];
+}
+
+__global__ void k_pack4(const float *__restrict__ p3, int n, float4 *p4) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) p4[i] = make_float4(p3[3 * i], p3[3 * i + 1], p3[3 * i + 2], 0.f);
+}
+
+// Byte offsets into the scratch array of one call's arrays (each 256-aligned).
+struct ScrLayout {
+	size_t p4, fp, perm, ka, kb, va, part, bb, tlo, thi, united, par, cmin, isFirst, rank, sortTmp, scanTmp, end;
+	size_t sortBytes, scanBytes;
+};
+
+static int scr_layout(hipStream_t st, int n, int B, ScrLayout *L) {
+	size_t off = 0;
+	auto take = [&](size_t bytes) { const size_t o = off; off += (bytes + 255) & ~(size_t)255; return o; };
+	const size_t N = (size_t)n, nLeaf2 = (size_t)skg_tree_leaves2(n, B, nullptr);
+	int rc;
+	if ((rc = skg_sort_pairs_at(st, nullptr, 0, &L->sortBytes, nullptr, nullptr, nullptr, nullptr, n, 63)) ||
+	    (rc = skg_exclusive_scan_at(st, nullptr, 0, &L->scanBytes, nullptr, nullptr, n)))
+		return rc;
+	L->p4 = take(16 * N); L->fp = take(16 * N); L->perm = take(4 * N);
+	L->ka = take(8 * N); L->kb = take(8 * N); L->va = take(4 * N);
+	L->part = take(24 * (size_t)nblk(n, 256)); L->bb = take(24);
+	L->tlo = take(32 * nLeaf2); L->thi = take(32 * nLeaf2); L->united = take(8 * nLeaf2);
+	L->par = take(4 * N); L->cmin = take(4 * N); L->isFirst = take(4 * N); L->rank = take(4 * N);
+	L->sortTmp = take(L->sortBytes); L->scanTmp = take(L->scanBytes);
+	L->end = off;
+	return SKG_OK;
+}
+
+extern "C" int skg_fof_points_noalloc(skg_ctx *c, int n, const float *d_pos, float fTau, int *d_grp,
+		int *nGroup) {
+	if (!c || n < 0 || (n && (!d_pos || !d_grp)) || !(fTau > 0.f)) return SKG_E_PARAM;
+	if (n > SKG_NOALLOC_MAX_POINTS) return SKG_E_PARAM;
+	if (n == 0) { if (nGroup) *nGroup = 1; return SKG_OK; }
+	(void)hipSetDevice(c->dev);
+	unsigned char *s = scratch_of(c->dev);
+	if (!s) {
+		const int e = (c->dev >= 0 && c->dev < SKG_SCR_DEVICES) ? g_scr_err[c->dev] : 0;
+		skg_note_hip((hipError_t)(e ? e : (int)hipErrorInvalidSymbol), "scratch address (skg_noalloc_setup)",
+				__FILE__, __LINE__);
+		return SKG_E_HIP;
+	}
+	ScrLayout L;
+	int rc;
+	if ((rc = scr_layout(c->st, n, c->P.nBucket, &L))) return rc;
+	if (L.end > (size_t)SKG_NOALLOC_SCRATCH_BYTES) return SKG_E_PARAM;   // only with nBucket < 4
+	float4 *p4 = (float4 *)(s + L.p4);
+	unsigned long long *ka = (unsigned long long *)(s + L.ka), *kb = (unsigned long long *)(s + L.kb);
+	int *va = (int *)(s + L.va);
+	FofBufs b;
+	b.fp = (float4 *)(s + L.fp); b.perm = (int *)(s + L.perm);
+	b.par = (int *)(s + L.par); b.cmin = (int *)(s + L.cmin); b.isFirst = (int *)(s + L.isFirst);
+	b.rank = (int *)(s + L.rank); b.united = (int *)(s + L.united);
+	b.tlo = (float4 *)(s + L.tlo); b.thi = (float4 *)(s + L.thi);
+	b.scanTmp = s + L.scanTmp; b.scanBytes = L.scanBytes;
+	// the same kernels as skg_fof_points: pack, Hilbert order, gather, links, labels
+	k_pack4<<<nblk(n, 256), 256, 0, c->st>>>(d_pos, n, p4);
+	if ((rc = skg_curve_keys(c, p4, nullptr, n, nullptr, true, ka, va, (float *)(s + L.part), (unsigned *)(s + L.bb))))
+		return rc;
+	if ((rc = skg_sort_pairs_at(c->st, s + L.sortTmp, L.sortBytes, nullptr, ka, kb, va, b.perm, n, 63))) return rc;
+	k_gather4<<<nblk(n, 256), 256, 0, c->st>>>(p4, b.perm, n, b.fp, nullptr, nullptr);
+	int ng = 0;
+	if ((rc = fof_label(c, b, n, fTau, nullptr, d_grp, &ng))) return rc;
 	if (nGroup) *nGroup = ng + 1;
 	return SKG_OK;
 }

@@ -22,6 +22,7 @@ SKG_DARK, SKG_GAS, SKG_STAR = 1, 2, 4
 SKG_PLUMMER, SKG_SPLINE = 1, 2
 SKG_ACC_PLAIN, SKG_ACC_COMPENSATED = 0, 1
 SKG_TUNE_BUCKET, SKG_TUNE_MAPPING, SKG_TUNE_POT_TILE, SKG_TUNE_ALL = 1, 2, 4, 7
+SKG_NOALLOC_MAX_POINTS, SKG_NOALLOC_SCRATCH_BYTES = 524288, 64 << 20
 STAGES = ("tree", "knn", "density", "move", "fof", "group", "unbind", "spare")
 KERNELS = ("knn", "cover_density", "cover_move", "fof", "unbind_potential", "unbind_loop", "unbind_big")
 FLT_MAX = 3.4028234663852886e38
@@ -86,6 +87,7 @@ def lib():
         L.skg_group_energies.argtypes = [vp, fp, fp]
         L.skg_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
         L.skg_fof_points.argtypes = [vp, C.c_int, fp, C.c_float, fp, ip]
+        L.skg_fof_points_noalloc.argtypes = [vp, C.c_int, vp, C.c_float, vp, ip]
         L.skg_stage_times.argtypes = [vp, C.POINTER(C.c_double)]
         L.skg_set_profiling.argtypes = [vp, C.c_int]
         L.skg_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
