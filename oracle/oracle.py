@@ -57,7 +57,7 @@ class RefOut(C.Structure):
         ("nExtraScat", C.c_int), ("nMove", C.c_int), ("nFofGroup", C.c_int),
         ("nGroup", C.c_int), ("nIttr", C.c_int), ("trace", C.c_void_p), ("maxTrace", C.c_int),
         ("grpMass", C.c_void_p), ("grpCenter", C.c_void_p), ("grpVcm", C.c_void_p),
-        ("maxGroup", C.c_int),
+        ("maxGroup", C.c_int), ("micro", C.c_int * 5), ("gtpPath", C.c_char_p),
     ]
 
 
@@ -66,7 +66,7 @@ def params(**kw) -> dict:
     p = dict(nSmooth=64, nBucket=16, fPeriod=3.4028234663852886e38, fCenter=(0.0, 0.0, 0.0),
              bPeriodic=0, fTau=0.0, fCvg=0.0, fScoop=0.0, fDensMin=0.0,
              fTempMax=3.4028234663852886e38, fMassMax=3.4028234663852886e38, nMembers=8,
-             nMaxMembers=0, bGasAndDark=0, bGasOnly=0, bForceInitialCut=0, bNoUnbind=0,
+             nMaxMembers=2147483647, bGasAndDark=0, bGasOnly=0, bForceInitialCut=0, bNoUnbind=0,
              iSoftType=2, G=1.0, Omega0=1.0, Lambda=0.0, fQuintess=0.0, H0=0.0, z=0.0,
              fEps=0.0, bOutDens=0, bQuiet=1)
     for k, v in kw.items():
@@ -110,7 +110,7 @@ def lib():
         L.or_move_block.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.or_first_gradient.argtypes = [C.c_void_p, _f]
         L.or_fof.argtypes = [C.c_void_p, C.c_void_p, C.POINTER(C.c_int)]
-        L.or_microsteps.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
+        L.or_microsteps.argtypes = [C.c_void_p, C.c_void_p]
         L.or_get_moved.argtypes = [C.c_void_p, _f]
         L.or_centers.argtypes = [C.c_void_p]
         L.or_unbind.argtypes = [C.c_void_p, C.POINTER(C.c_int)]
@@ -227,14 +227,35 @@ class Oracle:
         self.L.or_fof(self.ctx, _ptr(g), C.byref(ng))
         return g, ng.value
 
+    def microsteps(self):
+        """The 5 micro-steps (main.c:431-438) after fof(); returns the nScatter
+        of each Microstep line."""
+        m = np.zeros(5, np.int32)
+        if self.L.or_microsteps(self.ctx, _ptr(m)):
+            raise RuntimeError("or_microsteps failed")
+        return m
+
     def finish(self):
         """micro-steps, centres, unbinding and kdTooSmall after fof()."""
         s = C.c_int()
-        self.L.or_microsteps(self.ctx, C.byref(s))
+        self.micro = self.microsteps()
         self.L.or_centers(self.ctx)
         self.L.or_unbind(self.ctx, C.byref(s))
         self.L.or_too_small(self.ctx)
         return self.groups()
+
+    def catalogue(self):
+        """Per final group 0..nGroup-1 (kdWriteGroup's .gtp records, kd.c:1612-1687):
+        mass, rCenter, vcm, rBound, radius, nMembers."""
+        ng = self.groups()[1]
+        out = dict(mass=np.zeros(ng, np.float32), center=np.zeros(3 * ng, np.float32),
+                   vcm=np.zeros(3 * ng, np.float32), rbound=np.zeros(3 * ng, np.float32),
+                   radius=np.zeros(ng, np.float32), nmembers=np.zeros(ng, np.int32))
+        self.L.or_get_catalogue(self.ctx, *[_ptr(out[k]) for k in
+                                            ("mass", "center", "vcm", "rbound", "radius", "nmembers")])
+        for k in ("center", "vcm", "rbound"):
+            out[k] = out[k].reshape(-1, 3)
+        return out
 
 
 def ref_density(box, **kw):
@@ -249,7 +270,9 @@ def ref_density(box, **kw):
     return b2, rho, ne.value
 
 
-def ref_pipeline(box, max_trace=100000, max_group=1 << 20, **kw):
+def ref_pipeline(box, max_trace=100000, max_group=1 << 20, gtp_path=None, **kw):
+    """The reference's main.c:374-471 on box; with gtp_path, its kdWriteGroup
+    writes the .gtp there (native byte order)."""
     p = params(**kw)
     R = ref()
     n = box.n
@@ -267,12 +290,13 @@ def ref_pipeline(box, max_trace=100000, max_group=1 << 20, **kw):
     o.maxTrace = max_trace
     o.grpMass, o.grpCenter, o.grpVcm = _ptr(gm), _ptr(gc), _ptr(gv)
     o.maxGroup = max_group
+    o.gtpPath = gtp_path.encode() if gtp_path else None
     R.ref_pipeline(C.byref(_fill(RefParams, p)), box.n_gas, box.n_dark, box.n_star,
                    box.pos.reshape(-1), box.vel.reshape(-1), box.mass, box.soft, box.temp, C.byref(o))
     ng = o.nGroup
     out.update(nExtraScat=o.nExtraScat, nMove=o.nMove, nFofGroup=o.nFofGroup, nGroup=ng,
                trace=tr[: 2 * o.nIttr].reshape(-1, 2), grpMass=gm[:ng], grpCenter=gc[: 3 * ng].reshape(-1, 3),
-               grpVcm=gv[: 3 * ng].reshape(-1, 3))
+               grpVcm=gv[: 3 * ng].reshape(-1, 3), micro=np.array(o.micro[:], np.int32))
     out["rMoved"] = out["rMoved"].reshape(-1, 3)
     out["rMicro"] = out["rMicro"].reshape(-1, 3)
     return out

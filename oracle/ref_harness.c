@@ -31,7 +31,7 @@ typedef struct ref_params {
 	int bPeriodic;
 	float fTau, fCvg, fScoop;   /* fCvg/fScoop <= 0 -> main.c:343-344 defaults */
 	float fDensMin, fTempMax, fMassMax;
-	int nMembers, nMaxMembers;
+	int nMembers, nMaxMembers;  /* nMaxMembers as -maxgroup (main.c:110 default INT_MAX) */
 	int bGasAndDark, bGasOnly, bForceInitialCut, bNoUnbind;
 	int iSoftType;
 	float G, Omega0, Lambda, fQuintess, H0, z;
@@ -58,6 +58,8 @@ typedef struct ref_out {
 	float *grpCenter;   /* 3*nGroup: rCenter */
 	float *grpVcm;      /* 3*nGroup */
 	int maxGroup;
+	int micro[5];       /* nScatter of each Microstep line (main.c:436) */
+	const char *gtpPath;   /* if set: kdWriteGroup writes the .gtp there (native byte order) */
 	} REF_OUT;
 
 static void ref_load(KD kd, int nGas, int nDark, int nStar, const float *pos,
@@ -187,6 +189,7 @@ int ref_pipeline(const REF_PARAMS *P, int nGas, int nDark, int nStar,
 		kdBuildMoveTree(kd);
 		nScat = smAccDensity(smx, 0);
 		kdMoveParticles(kd, REF_MICRO_STEP*fStep);
+		o->micro[i] = nScat;
 		}
 	ref_copy_move(kd, o->rMicro);
 	smFinish(smx);
@@ -194,10 +197,10 @@ int ref_pipeline(const REF_PARAMS *P, int nGas, int nDark, int nStar,
 	kdCalcCenter(kd);
 	kdSetUniverse(kd, P->G, P->Omega0, P->Lambda, P->fQuintess, P->H0, P->z);
 	if (P->fEps > 0) kdSetSoft(kd, P->fEps);
-	kdUnbind(kd, P->iSoftType, fScoop, P->bGasAndDark, P->bNoUnbind,
-			P->nMaxMembers > 0 ? P->nMaxMembers : INT_MAX);
+	kdUnbind(kd, P->iSoftType, fScoop, P->bGasAndDark, P->bNoUnbind, P->nMaxMembers);
 	kdTooSmall(kd, P->nMembers);
 	o->nGroup = kd->nGroup;
+	if (o->gtpPath) kdWriteGroup(kd, (char *)o->gtpPath, 0);
 	if (o->iGroup) memcpy(o->iGroup, kd->piGroup, kd->nParticles*sizeof(int));
 	for (i = 0; i < kd->nGroup && i < o->maxGroup; ++i) {
 		if (o->grpMass) o->grpMass[i] = kd->pGroup[i].fMass;
@@ -235,8 +238,7 @@ int ref_unbind(const REF_PARAMS *P, int nGas, int nDark, int nStar,
 	kdReadCenter(kd, "", 0);
 	kdSetUniverse(kd, P->G, P->Omega0, P->Lambda, P->fQuintess, P->H0, P->z);
 	if (P->fEps > 0) kdSetSoft(kd, P->fEps);
-	kdUnbind(kd, P->iSoftType, fScoop, P->bGasAndDark, P->bNoUnbind,
-			P->nMaxMembers > 0 ? P->nMaxMembers : INT_MAX);
+	kdUnbind(kd, P->iSoftType, fScoop, P->bGasAndDark, P->bNoUnbind, P->nMaxMembers);
 	kdTooSmall(kd, P->nMembers);
 	memcpy(grpOut, kd->piGroup, kd->nParticles*sizeof(int));
 	if (nGroupOut) *nGroupOut = kd->nGroup;

@@ -855,7 +855,7 @@ out:
 }
 
 int or_microsteps(or_ctx *c, int *nScatter)                 /* main.c:431-438 */
-{
+{                                                          /* nScatter: 5 counts, one per micro-step */
 	int i, ns = 0;
 	c->nActive = c->nMove;
 	for (i = 0; i < 5; ++i) {
@@ -864,8 +864,8 @@ int or_microsteps(or_ctx *c, int *nScatter)                 /* main.c:431-438 */
 		if (ns < 0) return -1;
 		move_step(c, 0.1*c->fStep);
 		SAY(c, "Microstep:%d nScatter:%d\n", i + 1, ns);
+		if (nScatter) nScatter[i] = ns;
 	}
-	if (nScatter) *nScatter = ns;
 	return 0;
 }
 
@@ -1012,7 +1012,7 @@ int or_unbind(or_ctx *c, int *nUnbound)                      /* kd.c:1299-1466 *
 {
 	float hx = 0.5*c->L[0], hy = 0.5*c->L[1], hz = 0.5*c->L[2];
 	float fShift, fCosmo;
-	int gi, i, j, nUnb = 0, nMax = c->P.nMaxMembers > 0 ? c->P.nMaxMembers : INT_MAX;
+	int gi, i, j, nUnb = 0, nMax = c->P.nMaxMembers;   /* kd.c:1330: groups of n >= nMax stay */
 	group_order(c);
 	fShift = 1.0/(1.0 + c->P.z);
 	fCosmo = fShift*exp2hub(c, fShift);

@@ -35,7 +35,7 @@ typedef struct or_params {
 	int bPeriodic;
 	float fTau, fCvg, fScoop;   /* fCvg/fScoop <= 0: defaults tau/2 and 2*tau (main.c:343-344) */
 	float fDensMin, fTempMax, fMassMax;
-	int nMembers, nMaxMembers;  /* nMaxMembers <= 0: INT_MAX */
+	int nMembers, nMaxMembers;  /* nMaxMembers: -maxgroup, groups of n >= it are not unbound */
 	int bGasAndDark, bGasOnly, bForceInitialCut, bNoUnbind;
 	int iSoftType;
 	float G, Omega0, Lambda, fQuintess, H0, z;
@@ -62,7 +62,7 @@ int or_move_block(or_ctx *c, int *nActive, int *nScatter);      /* main.c:410-41
    called at main.c:396-401), by iOrder; replaces or_init_move + or_move_first. */
 int or_first_gradient(or_ctx *c, float *acc);
 int or_fof(or_ctx *c, int *iGroup, int *nGroup);                /* main.c:425 */
-int or_microsteps(or_ctx *c, int *nScatter);                     /* main.c:431-438 */
+int or_microsteps(or_ctx *c, int *nScatter);   /* main.c:431-438; nScatter: 5 counts or NULL */
 int or_get_moved(or_ctx *c, float *r);                           /* 3N by iOrder, NaN if not moving */
 int or_centers(or_ctx *c);                                       /* main.c:452-453 */
 int or_unbind(or_ctx *c, int *nUnbound);                         /* main.c:459-469 */

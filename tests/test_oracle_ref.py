@@ -39,10 +39,13 @@ def test_brute_knn_matches_reference_radii(box1):
     assert np.array_equal(O.knn_brute(box1.pos, 32), rb2)
 
 
-def test_pipeline_bitexact(box1):
-    """Whole path main.c:374-471: Ittr trace, FOF labels and final groups identical."""
+def test_pipeline_bitexact(box1, tmp_path):
+    """Whole path main.c:374-471: Ittr trace, FOF labels, the Microstep counts,
+    the final groups and the .gtp catalogue (mass, rCenter, vcm and the radius
+    of kdWriteGroup, kd.c:1612-1687) identical."""
     kw = dict(nSmooth=32, fTau=0.005, fPeriod=1.0)
-    r = O.ref_pipeline(box1, **kw)
+    gtp = str(tmp_path / "ref.gtp")
+    r = O.ref_pipeline(box1, gtp_path=gtp, **kw)
     o = O.Oracle(box1, **kw)
     o.density()
     tr = o.move_loop()
@@ -53,8 +56,84 @@ def test_pipeline_bitexact(box1):
     assert nfg == r["nFofGroup"]
     assert np.array_equal(fg, r["iFofGroup"])
     g, ng = o.finish()
+    assert np.array_equal(o.micro, r["micro"])
     assert ng == r["nGroup"]
     assert np.array_equal(g, r["iGroup"])
+    _check_catalogue(o, gtp, ng)
+
+
+def _check_catalogue(o, gtp_path, ng):
+    """The oracle's per-group catalogue equals the reference's .gtp records."""
+    cat = o.catalogue()
+    ref = O.read_gtp(gtp_path, xdr=False)
+    assert ref["mass"].shape[0] == ng - 1
+    for mine, theirs in (("mass", "mass"), ("center", "pos"), ("vcm", "vel"), ("radius", "radius")):
+        assert np.array_equal(cat[mine][1:], ref[theirs]), mine
+    assert (cat["radius"][1:] > 0).all()
+
+
+def _mass_box():
+    """config-1 recipe at 8192 with every fifth particle four times heavier, so
+    that -M (fMassMax, kd.c:555-597) keeps some particles out of the moving set."""
+    b = gen.config1(n=8192, seed=3)
+    b.mass = b.mass.copy()
+    b.mass[::5] *= 4.0
+    return b
+
+
+# (box, options): the cuts -d -t -M -fic (kd.c:555-597, 630-666, 696-699), a
+# density floor that makes ScatterCut remove scatterers after iteration 0
+# (smooth1.c:387-405, 460-470), Plummer softening (grav.c:25-27), -nu and
+# -maxgroup (kd.c:1330, 1409), on small boxes.
+TAU8K = 0.16 * 8192 ** (-1.0 / 3.0)
+OPTION_CASES = {
+    "densmin_dark": ("c1", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, fDensMin=60.0)),
+    "densmin_fic_gasdark": ("c3", dict(nSmooth=32, fTau=0.01, fPeriod=1.0, fDensMin=30.0, bForceInitialCut=1)),
+    "tempmax_gas": ("c3", dict(nSmooth=32, fTau=0.01, fPeriod=1.0, fTempMax=1.5e4)),
+    "massmax": ("mass", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, fMassMax=2.0 / 8192)),
+    "plummer": ("c1", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, iSoftType=1)),
+    "nounbind": ("c1", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, bNoUnbind=1)),
+    "maxgroup40": ("c1", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, nMaxMembers=40)),
+    "maxgroup0": ("c1", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, nMaxMembers=0)),
+}
+
+
+def option_box(name):
+    if name == "c1":
+        return gen.config1(n=8192, seed=3)
+    if name == "c3":
+        return gen.config3(n_each=4096, seed=5, n_halos=20)
+    return _mass_box()
+
+
+@pytest.mark.parametrize("case", sorted(OPTION_CASES))
+def test_pipeline_options_bitexact(case, tmp_path):
+    """The oracle against the reference under each option: Ittr trace,
+    Microstep counts, moved positions, FOF and final labels, .gtp catalogue."""
+    bname, kw = OPTION_CASES[case]
+    b = option_box(bname)
+    gtp = str(tmp_path / "ref.gtp")
+    r = O.ref_pipeline(b, gtp_path=gtp, **kw)
+    o = O.Oracle(b, **kw)
+    o.density()
+    tr = o.move_loop()
+    assert np.array_equal(tr, r["trace"])
+    assert np.array_equal(o.moved(), r["rMoved"], equal_nan=True)
+    fg, nfg = o.fof()
+    assert nfg == r["nFofGroup"] and np.array_equal(fg, r["iFofGroup"])
+    g, ng = o.finish()
+    assert np.array_equal(o.micro, r["micro"])
+    assert ng == r["nGroup"] and np.array_equal(g, r["iGroup"])
+    _check_catalogue(o, gtp, ng)
+    # each case exercises what it names
+    if "densmin" in case:
+        assert len(set(tr[1:, 1].tolist())) > 1          # ScatterCut after iteration 0
+    if case == "massmax":
+        assert r["nMove"] < b.n
+    if case in ("nounbind", "maxgroup0"):
+        # nothing is unbound: the final groups are the FOF groups of at least nMembers
+        sizes = np.bincount(fg)[1:]
+        assert ng - 1 == int((sizes >= 8).sum())
 
 
 def test_pipeline_scoop_and_den_order():
