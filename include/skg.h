@@ -54,7 +54,8 @@ typedef struct skg_params {
 	float fScoop;           /* -scoop, default 2*tau (main.c:344) */
 	float fDensMin, fTempMax, fMassMax;   /* -d -t -M */
 	int nMembers;           /* -m */
-	int nMaxMembers;        /* -maxgroup (<= 0: unlimited) */
+	int nMaxMembers;        /* -maxgroup: groups of nMaxMembers or more are not unbound
+	                           (kd.c:1330); default INT_MAX (main.c:110) */
 	int bGasAndDark, bGasOnly, bForceInitialCut, bNoUnbind;   /* -gd -go -fic -nu */
 	int iSoftType;          /* SKG_SPLINE or SKG_PLUMMER */
 	float G;                /* -G */
@@ -141,7 +142,9 @@ int skg_move_block(skg_ctx *ctx, int *nActive, int *nScatter);
    of each group's smallest iOrder; particles that do not move get 0. */
 int skg_fof(skg_ctx *ctx, int *grp, int *nGroup);
 
-/* The 5 micro-steps at 0.1*fStep (main.c:431-438). */
+/* The 5 micro-steps at 0.1*fStep (main.c:431-438).  nScatter (optional)
+   receives 5 counts, the nScatter of each Microstep line (main.c:436): live
+   scatterers + replicas after that step, summed over the boxes of a batch. */
 int skg_microsteps(skg_ctx *ctx, int *nScatter);
 
 /* Positions of the moving particles (3n by iOrder; NaN for non-moving). */
@@ -288,6 +291,11 @@ int skg_stage_times(skg_ctx *ctx, double *ms);
 #define SKG_K_UNB_BIG 6      /* removal loops of larger groups (cooperative): 20 flops per kdSubPot pair */
 #define SKG_K_N 7
 int skg_set_profiling(skg_ctx *ctx, int on);
+/* Calibration copy (no reference counterpart): allocates two device buffers of
+   bytes bytes (rounded down to 16) and copies one into the other reps times
+   with a plain load/store kernel (k_copy_calib), so that rocprofv3's
+   FETCH_SIZE / WRITE_SIZE counters can be scaled to bytes.  Synchronous. */
+int skg_device_copy(skg_ctx *ctx, long long bytes, int reps);
 int skg_kernel_stats(skg_ctx *ctx, int which, double *ms, long long *launches, double *work);
 
 #ifdef __cplusplus

@@ -10,6 +10,7 @@
 #include <rocprim/device/device_scan.hpp>
 #include <algorithm>
 #include <cfloat>
+#include <climits>
 #include <cstring>
 #include <vector>
 #include <chrono>
@@ -1556,6 +1557,7 @@ extern "C" void skg_default_params(skg_params *P) {
 	P->fTempMax = FLT_MAX;
 	P->fMassMax = FLT_MAX;
 	P->nMembers = 8;
+	P->nMaxMembers = INT_MAX;                                // main.c:110
 	P->iSoftType = SKG_SPLINE;
 	P->G = 1.f;
 	P->fShift = 1.f;
@@ -2017,8 +2019,12 @@ static void enqueue_lists(skg_ctx *c, long cap, int nsteps, int micro) {
 }
 
 // nsteps Denmax steps over the active list (at most cap movers).
+// cntLog (device, optional): the live-entity total after each step, one int
+// per step (the nScatter of each Microstep line).
+__global__ void k_log_count(const int *__restrict__ cnt, int *dst) { *dst = cnt[0]; }
+
 static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps, int micro, int bInitial,
-		bool capturing) {
+		bool capturing, int *cntLog = nullptr) {
 	const bool comp = c->P.accumulate == SKG_ACC_COMPENSATED;
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
 	const int *pbox = c->nBox > 1 ? c->m_box : nullptr;
@@ -2069,6 +2075,7 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 					(int *)c->part);
 			k_box_sum<<<c->nBox, 64, 0, c->st>>>((const int *)c->part, c->d_eblk, c->d_bscat, c->d_cnt);
 		}
+		if (cntLog) k_log_count<<<1, 1, 0, c->st>>>(c->d_cnt, cntLog + i);
 	}
 	SKG_HIP(hipGetLastError());
 	return 0;
@@ -2278,7 +2285,8 @@ extern "C" int skg_microsteps(skg_ctx *c, int *nScatter) {
 	if (!c) return SKG_E_PARAM;
 	if (c->stage != 5) return SKG_E_STATE;
 	(void)hipSetDevice(c->dev);
-	int rc, ns = c->nE;
+	int rc, ns[5] = {c->nE, c->nE, c->nE, c->nE, c->nE};
+	int *log = c->d_cnt + 32;                               // 5 ints: the count after each micro-step
 	drop_graph(c);
 	skg_timer_start(c);
 	c->nActive = c->nMove;                                  // kdReactivateMove (kd.c:796-799)
@@ -2288,12 +2296,14 @@ extern "C" int skg_microsteps(skg_ctx *c, int *nScatter) {
 	if (c->nMove > 0 && c->nE > 0) {
 		const bool sub = c->nMove <= c->subMax;
 		// MICRO_STEP*fStep (main.c:435)
-		if ((rc = enqueue_steps(c, c->nMove, sub, c->useLists && !sub && c->tl_list, 5, 1, 0, false))) return rc;
-		if ((rc = read_counts(c, &ns))) return rc;
+		if ((rc = enqueue_steps(c, c->nMove, sub, c->useLists && !sub && c->tl_list, 5, 1, 0, false, log)))
+			return rc;
+		SKG_HIP(hipMemcpyAsync(ns, log, sizeof(ns), hipMemcpyDeviceToHost, c->st));
+		if ((rc = read_counts(c, nullptr))) return rc;
 	}
 	SKG_HIP(hipStreamSynchronize(c->st));
 	skg_timer_stop(c, SKG_T_MOVE);
-	if (nScatter) *nScatter = ns;
+	if (nScatter) memcpy(nScatter, ns, sizeof(ns));
 	c->stage = 6;
 	return SKG_OK;
 }
@@ -2330,6 +2340,26 @@ extern "C" int skg_field_gradient(skg_ctx *c, int nq, const float *qpos, float *
 	SKG_HIP(hipStreamSynchronize(c->st));
 	dfree(&q4); dfree(&out);
 	return SKG_OK;
+}
+
+// Calibration copy: a plain 16-byte-per-lane copy, grid-stride.
+__global__ __launch_bounds__(256) void k_copy_calib(const float4 *__restrict__ a, float4 *__restrict__ b, long n) {
+	for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) b[i] = a[i];
+}
+
+extern "C" int skg_device_copy(skg_ctx *c, long long bytes, int reps) {
+	if (!c || bytes < 16 || reps < 1) return SKG_E_PARAM;
+	(void)hipSetDevice(c->dev);
+	const long n = (long)(bytes / 16);
+	float4 *a = nullptr, *b = nullptr;
+	auto done = [&](int status) { dfree(&a); dfree(&b); return status; };
+	if (dalloc(&a, n) || dalloc(&b, n)) return done(SKG_E_NOMEM);
+	k_fill_f<<<nblk(4 * n, 256), 256, 0, c->st>>>((float *)a, (int)std::min<long>(4 * n, INT_MAX), 1.0f);
+	for (int r = 0; r < reps; ++r) k_copy_calib<<<4096, 256, 0, c->st>>>(a, b, n);
+	hipError_t e = hipGetLastError();
+	if (e == hipSuccess) e = hipStreamSynchronize(c->st);
+	if (e != hipSuccess) { skg_note_hip(e, "k_copy_calib", __FILE__, __LINE__); return done(SKG_E_HIP); }
+	return done(SKG_OK);
 }
 
 extern "C" int skg_stage_times(skg_ctx *c, double *ms) {

@@ -1478,7 +1478,7 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 	(void)hipSetDevice(c->dev);
 	GroupState *G = c->gs;
 	const int n = c->n, nG = G->nG;
-	const int nMax = c->P.nMaxMembers > 0 ? c->P.nMaxMembers : INT_MAX;
+	const int nMax = c->P.nMaxMembers;                      // kd.c:1330: groups of n >= nMax stay
 	int rc;
 	skg_timer_start(c);
 	std::vector<int> hs(nG), he(nG);

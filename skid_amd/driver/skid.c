@@ -233,6 +233,7 @@ int main(int argc, char **argv)
 	Snap S;
 	int bTau = 0, bCvg = 0, bScoop = 0, bEps = 0, bOutRay = 0, bOutDens = 0, bOutStats = 0, bStandard = 0;
 	int bUnbindOnly = 0, bPlain = 0, i, rc, nActive, nScat, nIttr, nGroup, nUnb, nExtra;
+	int nMicro[PRUNE_STEPS];
 	float fTau = 0, z = 0, Omega0 = 1, G = 1, H0 = 0, Lambda = 0, fQuintess = 0, fCvg = 0, fScoop = 0, fEps = 0;
 	float fStep;
 	char achName[256] = "skid", achFile[512], achGroup[256] = "";
@@ -343,9 +344,9 @@ int main(int argc, char **argv)
 	if ((rc = skg_fof(ctx, NULL, &nGroup))) die_skg("skg_fof", rc);
 	t1 = now_s(); tFof = t1 - t0;
 	t0 = now_s();
-	if ((rc = skg_microsteps(ctx, &nScat))) die_skg("skg_microsteps", rc);
+	if ((rc = skg_microsteps(ctx, nMicro))) die_skg("skg_microsteps", rc);
 	t1 = now_s(); tMicro = t1 - t0;
-	for (i = 0; i < PRUNE_STEPS; ++i) printf("Microstep:%d nScatter:%d\n", i + 1, nScat);
+	for (i = 0; i < PRUNE_STEPS; ++i) printf("Microstep:%d nScatter:%d\n", i + 1, nMicro[i]);
 	if (bOutRay) {                                          /* kdOutVector (kd.c:1550-1606) */
 		int j;
 		float *d3;

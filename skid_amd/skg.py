@@ -73,7 +73,7 @@ def lib():
         L.skg_move_first.argtypes = [vp, ip, ip]
         L.skg_move_block.argtypes = [vp, ip, ip]
         L.skg_fof.argtypes = [vp, fp, ip]
-        L.skg_microsteps.argtypes = [vp, ip]
+        L.skg_microsteps.argtypes = [vp, fp]
         L.skg_get_moved.argtypes = [vp, fp]
         L.skg_centers.argtypes = [vp]
         L.skg_unbind.argtypes = [vp, ip]
@@ -90,6 +90,7 @@ def lib():
         L.skg_fof_points_noalloc.argtypes = [vp, C.c_int, vp, C.c_float, vp, ip]
         L.skg_stage_times.argtypes = [vp, C.POINTER(C.c_double)]
         L.skg_set_profiling.argtypes = [vp, C.c_int]
+        L.skg_device_copy.argtypes = [vp, C.c_longlong, C.c_int]
         L.skg_kernel_stats.argtypes = [vp, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                                        C.POINTER(C.c_double)]
         _lib = L
@@ -206,10 +207,12 @@ class Context:
         _check(self.L.skg_fof(self.h, _p(g), C.byref(ng)), "skg_fof")
         return g, ng.value
 
-    def microsteps(self) -> int:
-        s = C.c_int()
-        _check(self.L.skg_microsteps(self.h, C.byref(s)), "skg_microsteps")
-        return s.value
+    def microsteps(self):
+        """The 5 micro-steps (main.c:431-438); returns the nScatter of each
+        Microstep line."""
+        s = np.zeros(5, np.int32)
+        _check(self.L.skg_microsteps(self.h, _p(s)), "skg_microsteps")
+        return s
 
     def moved(self):
         r = np.empty(3 * self.n, np.float32)
@@ -309,6 +312,10 @@ class Context:
         _check(self.L.skg_stage_times(self.h, t), "skg_stage_times")
         return {STAGES[i]: t[i] for i in range(8)}
 
+    def device_copy(self, nbytes: int, reps: int = 3):
+        """skg_device_copy: a plain device copy of known size (counter calibration)."""
+        _check(self.L.skg_device_copy(self.h, int(nbytes), int(reps)), "skg_device_copy")
+
     def set_profiling(self, on: bool = True):
         _check(self.L.skg_set_profiling(self.h, int(on)), "skg_set_profiling")
 
@@ -392,6 +399,12 @@ class BatchContext(Context):
         tau = None if taus is None else np.ascontiguousarray(taus, np.float32)
         self._keep += [ng, nd, ns, tau]
         _check(self.L.skg_load_batch(h, self.nbox, _p(ng), _p(nd), _p(ns), _p(tau),
+                                     *[_p(a) for a in self._keep[:5]]), "skg_load_batch")
+
+    def reload(self):
+        """skg_load_batch again from the host arrays kept at construction (the
+        upload from host memory); the path then runs again from density()."""
+        _check(self.L.skg_load_batch(self.h, self.nbox, *[_p(a) for a in self._keep[5:9]],
                                      *[_p(a) for a in self._keep[:5]]), "skg_load_batch")
 
     def split(self, a):

@@ -117,6 +117,68 @@ def test_adjoint_2pow20_gradient_parity():
     assert np.isfinite(dx).all() and np.isfinite(dh).all()
 
 
+def test_adjoint_2pow20_central_differences():
+    """Config 2 (2^20 particles, 500 NFW halos, nSmooth 64): the adjoint dx and
+    dh of L = sum_j gbar_j rho_j against FP64 central differences (step 1e-3 h)
+    per sampled particle, with the GPU's own frozen neighbour sets.  Moving x_p
+    changes only the pairs (i, j in N_i) with i = p or j = p, and h_p only those
+    with i = p, so each difference sums those pairs alone."""
+    b = gen.config2()
+    k = 64
+    ctx = skg.Context(b, skg.make_params(0.002, k, period=1.0))
+    b2, _, _ = ctx.density()
+    rng = np.random.default_rng(20)
+    gbar = rng.normal(size=b.n).astype(np.float32)
+    dx, dh = ctx.adjoint(gbar, 3)
+    nb, _ = ctx.neighbors()
+    ctx.close()
+    nbr = nb[:, :-1].astype(np.int64)                       # the head (largest d2) has weight 0
+    n = b.n
+    x0 = b.pos.astype(np.float64)
+    h = 0.5 * np.sqrt(b2.astype(np.float64))
+    m = b.mass.astype(np.float64)
+    g = gbar.astype(np.float64)
+    ii = np.repeat(np.arange(n), k - 1)
+    jj = nbr.reshape(-1)
+    by_j = np.argsort(jj, kind="stable")
+    jstart = np.searchsorted(jj[by_j], np.arange(n + 1))
+
+    def pairs_of(p):
+        rows = np.arange(p * (k - 1), (p + 1) * (k - 1))
+        return np.concatenate([rows, by_j[jstart[p]:jstart[p + 1]]])
+
+    def L_pairs(x, hh, rows):
+        i, j = ii[rows], jj[rows]
+        d = x[i] - x[j]
+        d -= np.round(d)
+        r = np.sqrt((d * d).sum(1))
+        W = m4(r / hh[i]) / (math.pi * hh[i] ** 3)
+        return float(np.sum(0.5 * W * (g[i] * m[j] + g[j] * m[i])))
+
+    scale_x, scale_h = np.abs(dx).max(), np.abs(dh).max()
+    ex, eh = [], []
+    for p in rng.choice(n, 24, replace=False):
+        rows = pairs_of(int(p))
+        for c in range(3):
+            st = 1e-3 * h[p]
+            xp, xm = x0.copy(), x0.copy()
+            xp[p, c] += st
+            xm[p, c] -= st
+            fd = (L_pairs(xp, h, rows) - L_pairs(xm, h, rows)) / (2 * st)
+            ex.append(abs(fd - dx[p, c]) / scale_x)
+            assert abs(fd - dx[p, c]) <= 2e-5 * scale_x + 1e-4 * abs(fd), (p, c, fd, dx[p, c])
+        rows_h = np.arange(p * (k - 1), (p + 1) * (k - 1))
+        st = 1e-3 * h[p]
+        hp, hm = h.copy(), h.copy()
+        hp[p] += st
+        hm[p] -= st
+        fd = (L_pairs(x0, hp, rows_h) - L_pairs(x0, hm, rows_h)) / (2 * st)
+        eh.append(abs(fd - dh[p]) / scale_h)
+        assert abs(fd - dh[p]) <= 2e-5 * scale_h + 1e-4 * abs(fd), (p, fd, dh[p])
+    print(f"2^20 k=64 adjoint vs central differences over 24 particles: dx max {max(ex):.2e} of max|dx|, "
+          f"dh max {max(eh):.2e} of max|dh|")
+
+
 def frozen_rho(x, h, mass, nbr):
     """rho_i with the kNN sets frozen (head excluded), FP64: both halves."""
     n, k = nbr.shape

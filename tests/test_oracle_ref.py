@@ -6,6 +6,7 @@ import pytest
 
 from oracle import oracle as O
 from skid_amd import gen
+from option_cases import OPTION_CASES, option_box
 
 pytestmark = pytest.mark.skipif(not O.have_ref(), reason="reference harness not built")
 
@@ -70,40 +71,6 @@ def _check_catalogue(o, gtp_path, ng):
     for mine, theirs in (("mass", "mass"), ("center", "pos"), ("vcm", "vel"), ("radius", "radius")):
         assert np.array_equal(cat[mine][1:], ref[theirs]), mine
     assert (cat["radius"][1:] > 0).all()
-
-
-def _mass_box():
-    """config-1 recipe at 8192 with every fifth particle four times heavier, so
-    that -M (fMassMax, kd.c:555-597) keeps some particles out of the moving set."""
-    b = gen.config1(n=8192, seed=3)
-    b.mass = b.mass.copy()
-    b.mass[::5] *= 4.0
-    return b
-
-
-# (box, options): the cuts -d -t -M -fic (kd.c:555-597, 630-666, 696-699), a
-# density floor that makes ScatterCut remove scatterers after iteration 0
-# (smooth1.c:387-405, 460-470), Plummer softening (grav.c:25-27), -nu and
-# -maxgroup (kd.c:1330, 1409), on small boxes.
-TAU8K = 0.16 * 8192 ** (-1.0 / 3.0)
-OPTION_CASES = {
-    "densmin_dark": ("c1", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, fDensMin=60.0)),
-    "densmin_fic_gasdark": ("c3", dict(nSmooth=32, fTau=0.01, fPeriod=1.0, fDensMin=30.0, bForceInitialCut=1)),
-    "tempmax_gas": ("c3", dict(nSmooth=32, fTau=0.01, fPeriod=1.0, fTempMax=1.5e4)),
-    "massmax": ("mass", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, fMassMax=2.0 / 8192)),
-    "plummer": ("c1", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, iSoftType=1)),
-    "nounbind": ("c1", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, bNoUnbind=1)),
-    "maxgroup40": ("c1", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, nMaxMembers=40)),
-    "maxgroup0": ("c1", dict(nSmooth=32, fTau=TAU8K, fPeriod=1.0, nMaxMembers=0)),
-}
-
-
-def option_box(name):
-    if name == "c1":
-        return gen.config1(n=8192, seed=3)
-    if name == "c3":
-        return gen.config3(n_each=4096, seed=5, n_halos=20)
-    return _mass_box()
 
 
 @pytest.mark.parametrize("case", sorted(OPTION_CASES))
