@@ -13,14 +13,22 @@ nBucket as tuned, tau = 0.16 N_box^(-1/3) per box.  Three forms are timed:
   value        batched + adjoint + compensated: every box in the same launches
                (skg_load_batch), fp32 two-sum sums, and each step also forms
                the nSmooth sensitivity of every density (d rho / d ln k, the
-               dh adjoint direction) and copies it to the host; launch shapes
-               from the autotuner.  The membership report built from it (host
-               post-processing) is formed once after the timed steps;
+               dh adjoint direction) and the dx/dh adjoint of the total
+               density, and copies both to the host; launch shapes from the
+               autotuner.  The membership report built from d rho / d ln k
+               (host post-processing) is formed once after the timed steps.
+               Particles are resident in HBM when a step starts; the same step
+               with the upload from host memory is reported beside it
+               (config.host_to_ids_particles_per_s);
   plain_gpu    one context per box run in turn, fp32 running sums (the
                reference's arithmetic), no adjoint, default shapes; timed on
                every 4th box in size order (a rate, particles/s);
   cpu_baseline the reference SKID (oracle/_ref) on one core over four boxes
                spread across the size range.
+
+The default run also times the 2^24 gas + 2^24 dark box (nSmooth 64, -gd) for
+a few steps and reports it under "two_species" with its own roofline and CPU
+baseline (--c3-each 0 skips it).
 
 --workload two-species runs the 2^24 gas + 2^24 dark box instead (one box).
 --workload unbind runs config 5: 2^27 particles, 60,000 groups with membership
@@ -80,6 +88,16 @@ class quiet_stdout:
         return False
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_run(boxes, kws):
     """The reference (or, without it, the oracle's restatement) on one core."""
     from oracle import oracle as O
@@ -97,17 +115,13 @@ def cpu_run(boxes, kws):
 
 def copy_calibration():
     """A plain device copy of known size (256 MiB read, 256 MiB written, three
-    dispatches): what FETCH_SIZE and WRITE_SIZE read for it gives the factor
-    from counter KiB to HBM bytes."""
-    import torch
-    n = 64 << 20
-    a = torch.ones(n, device="cuda")
-    b = torch.empty_like(a)
-    torch.cuda.synchronize()
-    for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    log(f"copied {4 * n} bytes x3")
+    dispatches of the library's k_copy_calib): what FETCH_SIZE and WRITE_SIZE
+    read for it gives the factor from counter KiB to HBM bytes."""
+    from skid_amd import gen, skg
+    ctx = skg.Context(gen.config1(n=4096, seed=1), skg.make_params(0.01, 32, period=1.0))
+    ctx.device_copy(256 << 20, 3)
+    ctx.close()
+    log(f"copied {256 << 20} bytes x3")
 
 
 def measured_traffic(name):
@@ -143,11 +157,16 @@ def roofline(kst, traffic=None):
     return out
 
 
-def run_path(ctx, adjoint=False):
-    """main.c:374-471 on a context; with adjoint, also d rho / d ln nSmooth.
-    Returns (trace, nGroup, (rho, drho, groups))."""
+def run_path(ctx, adjoint=False, gbar=None):
+    """main.c:374-471 on a context; with adjoint, also d rho / d ln nSmooth
+    (the dh direction, for the membership report) and the vector-Jacobian
+    product sum_j gbar_j d rho_j / d x_i, d h_i (the dx adjoint) with the
+    given weights.  Returns (trace, nGroup, (rho, drho, groups))."""
     b2, rho, _ = ctx.density()
-    drho = ctx.density_dlnk() if adjoint else None
+    drho = None
+    if adjoint:
+        drho = ctx.density_dlnk()
+        ctx.adjoint(gbar, 3)
     tr = ctx.move_loop()
     ctx.fof()
     ctx.microsteps()
@@ -178,9 +197,10 @@ def bench_batched(args):
     t = time.time()
     tuned = ctx.autotune(0 if args.shapes else skg.SKG_TUNE_ALL)
     log(f"autotune {time.time() - t:.1f}s: {tuned}")
+    gbar = np.ones(N, np.float32)                    # dx adjoint of the total density sum_j rho_j
     for w in range(args.warmup):
         t0 = time.perf_counter()
-        tr, ng, last = run_path(ctx, adjoint=True)
+        tr, ng, last = run_path(ctx, adjoint=True, gbar=gbar)
         log(f"warmup {w}: {time.perf_counter() - t0:.3f}s groups={ng - 1}")
     tuned = ctx.autotune(0)                          # the tile chosen on the warmup's groups
     ctx.set_profiling(True)
@@ -188,17 +208,29 @@ def bench_batched(args):
     t0 = time.perf_counter()
     for s in range(args.steps):
         ts = time.perf_counter()
-        tr, ng, last = run_path(ctx, adjoint=True)
+        tr, ng, last = run_path(ctx, adjoint=True, gbar=gbar)
         log(f"step {s}: {time.perf_counter() - ts:.3f}s groups={ng - 1}")
     wall = time.perf_counter() - t0
+    ctx.kernel_stats_last = ctx.kernel_stats()
+    ctx.stage_times_last = ctx.stage_times()
+    ctx.set_profiling(False)
+    # the same step from particles in host memory: the upload (skg_load_batch,
+    # host SoA to HBM over PCIe) inside the bracket; reported beside value
+    pcie = []
+    for s in range(args.pcie_steps):
+        ts = time.perf_counter()
+        ctx.reload()
+        run_path(ctx, adjoint=True, gbar=gbar)
+        pcie.append(time.perf_counter() - ts)
+        log(f"host-to-ids step {s}: {pcie[-1]:.3f}s")
     rho, drho, g = last
     rp = skg.nsmooth_report(rho, drho, g, dens_min=float(np.quantile(rho[rho > 0][::64], 0.25)), rel_k=0.25)
     report = {"dlnrho_dlnk": rp["dlnrho_dlnk"], "dens_min_probe": rp["dens_min"], "rel_k": rp["rel_k"],
               "at_risk": rp["at_risk"], "groups_with_risk": int((rp["group_frac_at_risk"][1:] > 0).sum())}
     log("nSmooth sensitivity report:", report)
-    kst = ctx.kernel_stats()
+    kst = ctx.kernel_stats_last
     dev_peak = skg.device_memory()[1]
-    st = {k: (v - st0[k]) / args.steps for k, v in ctx.stage_times().items()}
+    st = {k: (v - st0[k]) / args.steps for k, v in ctx.stage_times_last.items()}
     log("stage ms per step:", {k: round(v, 1) for k, v in st.items()})
     log("kernels:", json.dumps({k: {kk: (round(vv, 3) if isinstance(vv, float) else vv) for kk, vv in v.items()}
                                  for k, v in kst.items()}))
@@ -233,7 +265,7 @@ def bench_batched(args):
         kws = [dict(nSmooth=32, fTau=float(taus[i]), fPeriod=1.0) for i in pick]
         dt, kind = cpu_run(sb, kws)
         ncpu = int(sum(b.n for b in sb))
-        cpu = dict(value=ncpu / dt, unit="particles/s", cores=1, kind=kind,
+        cpu = dict(value=ncpu / dt, unit="particles/s", cores=1, kind=kind, cpu_model=cpu_model(),
                    sample=f"{len(sb)} of the 256 boxes ({', '.join(str(b.n) for b in sb)} particles; 5th-65th "
                           f"size percentiles), each run end-to-end as one skid process would, {dt:.2f}s on 1 core")
         log("cpu baseline:", cpu)
@@ -245,11 +277,17 @@ def bench_batched(args):
         forms["batched_over_plain"] = value / plain["value"]
     if cpu:
         forms["gpu_over_cpu"] = value / cpu["value"]
+    pcie_rate = N / float(np.median(pcie)) if pcie else None
     config = {"workload": f"{len(boxes)} independent periodic unit boxes, N log-uniform in [2048, 524288] "
                           f"(seeded, {N} particles), NFW halos (N/4000 per box) + 25% background, nSmooth 32, "
-                          f"tau 0.16 N^-1/3 per box, end-to-end group finding of all boxes per step, plus "
-                          f"d rho / d ln nSmooth per particle (the membership report from it is formed once, untimed)",
+                          f"tau 0.16 N^-1/3 per box. A step runs from particles resident in HBM to group ids in "
+                          f"host memory: density, move loop, FOF, micro-steps, centres, unbinding, kdTooSmall, "
+                          f"plus per particle d rho / d ln nSmooth and the dx/dh adjoint of sum_j rho_j, both "
+                          f"copied to the host (the membership report built from d rho / d ln nSmooth is host "
+                          f"post-processing, formed once after the timed steps). host_to_ids_particles_per_s "
+                          f"adds the upload of the particles from host memory (skg_load_batch) to each step",
               "n_particles": N, "boxes": len(boxes), "form": "batched+adjoint+compensated",
+              "host_to_ids_particles_per_s": pcie_rate,
               "autotune": tuned, "knn_queries_per_s": knn_qps, "device_peak_bytes": dev_peak,
               "forms": forms, "plain_gpu": plain,
               "nsmooth_report": report}
@@ -289,7 +327,7 @@ def bench_two_species(args):
         sb = gen.config3(n_each=ce, seed=3, n_halos=max(1, int(8000 * ce / (1 << 24))))
         tc = 0.001 * (2.0 ** ((24 - 16) / 3.0))
         dt, kind = cpu_run([sb], [dict(nSmooth=64, fTau=tc, fPeriod=1.0, bGasAndDark=1)])
-        cpu = dict(value=sb.n / dt, unit="particles/s", cores=1, kind=kind,
+        cpu = dict(value=sb.n / dt, unit="particles/s", cores=1, kind=kind, cpu_model=cpu_model(),
                    sample=f"same recipe at 2^16+2^16 particles, tau {tc:.4g}, {dt:.2f}s on 1 core")
     config = {"workload": f"two-species box 2^{le} gas + 2^{le} dark (8000 NFW halos, dN/dM~M^-1.9), nSmooth 64, "
                           f"nBucket 16, tau {tau:.4g}, -gd, periodic unit box, end-to-end group finding",
@@ -368,7 +406,7 @@ def bench_unbind(args):
                 O.ref_unbind(b2, g2, **kw)
             dt = time.perf_counter() - t1
             m2 = int((g2 > 0).sum())
-            cpu = dict(value=m2 / dt, unit="group members/s", cores=1, kind="reference",
+            cpu = dict(value=m2 / dt, unit="group members/s", cores=1, kind="reference", cpu_model=cpu_model(),
                        sample=f"reference -unbind path on a 2^20 draw of the same recipe with 500 groups of "
                               f"8..2e4 members ({m2} members), {dt:.2f}s on 1 core")
     value = members / (wall / args.steps)
@@ -377,6 +415,20 @@ def bench_unbind(args):
                           f"FP32 compensated potentials, FP64 energies and sign test",
               "n_particles": n, "members": members, "form": "unbind", "device_peak_bytes": dev_peak}
     return value, wall, config, roof, cpu
+
+
+def c3_record(args):
+    """The two-species box (2^24 gas + 2^24 dark, nSmooth 64, tau 0.001, -gd) as
+    a record of its own inside the default run's line: its particles/s over a
+    few steps, the roofline of its dominant kernel, and the reference on one
+    core at 2^16 + 2^16 particles."""
+    a = argparse.Namespace(**vars(args))
+    a.log2_each, a.steps, a.warmup, a.cpu_boxes = args.c3_each, args.c3_steps, 1, args.cpu_boxes
+    value, wall, config, roof, cpu = bench_two_species(a)
+    return {"metric": "end-to-end particles/s, two-species box", "value": value, "unit": "particles/s",
+            "steps": a.steps, "warmup": a.warmup, "ms_per_step": 1000.0 * wall / a.steps, "config": config,
+            "roofline": roof, "cpu_baseline": cpu,
+            "gpu_over_cpu": (value / cpu["value"]) if cpu else None}
 
 
 def main():
@@ -391,6 +443,11 @@ def main():
     ap.add_argument("--log2-each", type=int, default=24, help="two-species: particles per species = 2^this")
     ap.add_argument("--plain-every", type=int, default=4, help="plain GPU form on every n-th box (0 skips)")
     ap.add_argument("--cpu-boxes", type=int, default=4, help="boxes in the CPU sample (0 skips)")
+    ap.add_argument("--c3-each", type=int, default=24,
+                    help="batched: also time the two-species box at 2^this per species (0 skips)")
+    ap.add_argument("--c3-steps", type=int, default=2, help="timed steps of the two-species record")
+    ap.add_argument("--pcie-steps", type=int, default=1,
+                    help="batched: steps timed from particles in host memory (the upload included)")
     ap.add_argument("--shapes", default="", help="nBucket,subMax,potTile: skip the autotuner and use these")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 256 MiB on the device three times (the reference kernel for scaling "
@@ -403,8 +460,11 @@ def main():
         print(f"bench.py: the group-finding path runs on one GPU; --gpus {args.gpus} is not supported "
               f"(sharding across GPUs is not implemented)", file=sys.stderr)
         sys.exit(2)
+    c3 = None
     if args.workload == "batched":
         value, wall, config, roof, cpu = bench_batched(args)
+        if args.c3_each > 0:
+            c3 = c3_record(args)
     elif args.workload == "unbind":
         value, wall, config, roof, cpu = bench_unbind(args)
     else:
@@ -420,6 +480,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic", "config": config,
         "roofline": roof, "cpu_baseline": cpu,
     }
+    if c3:
+        out["two_species"] = c3
     print(json.dumps(out), flush=True)
 
 
