@@ -214,7 +214,9 @@ int skg_density_adjoint(skg_ctx *ctx, const float *gbar, int parts, float *dx, f
    derivative of the density of smooth1.c:249-263 along its smoothing lengths):
    with the neighbour sets frozen and every h_j = sqrt(fBall2_j)/2 scaling as
    nSmooth^(1/3), drho[i] = d rho_i / d ln nSmooth (by iOrder; 0 for
-   non-scatterers).  Forward product of d rho/dh along dh_j = h_j/3. */
+   non-scatterers).  Forward product of d rho/dh along dh_j = h_j/3.  Once it
+   has been called on a context, every later skg_density forms both halves in
+   its own kernels and this call only combines them. */
 int skg_density_dlnk(skg_ctx *ctx, float *drho);
 
 /* Frozen-field gradient a(x) = sum_s (r_s - x) m_s g(q)/(pi h_s^5) at nq

@@ -20,19 +20,6 @@
 
 static inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
 
-__device__ __forceinline__ void m4_derivs(float q, float &w, float &wp) {
-	if (q < 1.0f) {
-		w = fmaf(fmaf(0.75f, q, -1.5f), q * q, 1.0f);
-		wp = q * fmaf(2.25f, q, -3.0f);
-	} else if (q < 2.0f) {
-		const float t = 2.0f - q;
-		w = 0.25f * t * t * t;
-		wp = -0.75f * t * t;
-	} else {
-		w = 0.f; wp = 0.f;
-	}
-}
-
 // exact periodic separation a - b under the nearest-image rule
 __device__ __forceinline__ float per_delta(float a, float b, float L, int per) {
 	float d = fsub(a, b);
@@ -57,7 +44,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(const unsigned long long *__
 	Acc ax, ay, az, ahh;
 	ax.init(); ay.init(); az.init(); ahh.init();
 	for (int s = 1; s < k; ++s) {
-		const unsigned long long key = heap[(size_t)i * k + s];
+		const unsigned long long key = heap[(size_t)s * nS + i];     // slot-major: coalesced
 		const int j = (int)(unsigned)key;
 		float dxv = 0.f, dyv = 0.f, dzv = 0.f, r = 0.f;
 		if (j != oi) {
@@ -95,7 +82,7 @@ __global__ __launch_bounds__(256) void k_dlnk_gather(const unsigned long long *_
 	Acc a;
 	a.init();
 	for (int s = 1; s < k; ++s) {
-		const int j = (int)(unsigned)heap[(size_t)i * k + s];
+		const int j = (int)(unsigned)heap[(size_t)s * nS + i];
 		float r = 0.f;
 		if (j != oi) {
 			const float dx = per_delta(xi, pos[3 * j], L, per), dy = per_delta(yi, pos[3 * j + 1], L, per);
@@ -213,12 +200,17 @@ __global__ void k_add3_ord(const float *__restrict__ in, const int *__restrict__
 	for (int c = 0; c < 3; ++c) dx[3 * o + c] = fadd(dx[3 * o + c], in[3 * i + c]);
 }
 
+__global__ void k_dlnk_combine(const float *__restrict__ g, const float *__restrict__ s, const int *__restrict__ ord,
+		int n, float *out) {
+	int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) out[ord[i]] = fadd(g[i], s[i]);
+}
+
 // The entity tree with every radius as built (independent of later ScatterCuts):
 // the move tree's leaves, refitted with the radii as built.
 static int full_tree(skg_ctx *c) {
 	if (c->tfull.lo && c->tfull.n == c->nE) return 0;
 	DevTree *t = &c->tfull;
-	skg_free_tree(t);
 	if (dalloc(&t->lo, 2 * (size_t)c->te.nLeaf2) || dalloc(&t->hi, 2 * (size_t)c->te.nLeaf2)) return SKG_E_NOMEM;
 	int rc;
 	if ((rc = skg_copy_tree_leaves(c, t, &c->te))) return rc;
@@ -242,14 +234,12 @@ int skg_gradient_points(skg_ctx *c, const float4 *q, int nq, float *out) {
 	int *perm = nullptr;
 	float4 *qs = nullptr;
 	float *tmp = nullptr;
-	if (dalloc(&perm, nq) || dalloc(&qs, nq) || dalloc(&tmp, 3 * (size_t)nq)) return SKG_E_NOMEM;
+	if (sget(c, 2, &perm, nq) || sget(c, 3, &qs, nq) || sget(c, 4, &tmp, 3 * (size_t)nq)) return SKG_E_NOMEM;
 	if ((rc = skg_morton_sort(c, q, nullptr, nq, perm, nullptr))) return rc;
 	k_gather_pts<<<nblk(nq, 256), 256, 0, c->st>>>(q, perm, nq, qs);
 	launch_cover<1>(c, qs, nq, nullptr, nullptr, 0, nullptr, tmp);
 	k_unpermute3<<<nblk(nq, 256), 256, 0, c->st>>>(tmp, perm, nq, out);
 	SKG_HIP(hipGetLastError());
-	SKG_HIP(hipStreamSynchronize(c->st));
-	dfree(&perm); dfree(&qs); dfree(&tmp);
 	return 0;
 }
 
@@ -260,7 +250,7 @@ extern "C" int skg_density_adjoint(skg_ctx *c, const float *gbar, int parts, flo
 	int rc;
 	const int n = c->n, nS = c->nS;
 	float *dg = nullptr, *ddx = nullptr, *ddh = nullptr, *part2 = nullptr;
-	if (dalloc(&dg, n) || dalloc(&ddx, 3 * (size_t)n) || dalloc(&ddh, n) || dalloc(&part2, 3 * (size_t)nS))
+	if (sget(c, 0, &dg, n) || sget(c, 1, &ddx, 3 * (size_t)n) || sget(c, 2, &ddh, n) || sget(c, 3, &part2, 3 * (size_t)nS))
 		return SKG_E_NOMEM;
 	SKG_HIP(hipMemcpyAsync(dg, gbar, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
 	SKG_HIP(hipMemsetAsync(ddx, 0, 12 * (size_t)n, c->st));
@@ -275,7 +265,6 @@ extern "C" int skg_density_adjoint(skg_ctx *c, const float *gbar, int parts, flo
 	SKG_HIP(hipMemcpyAsync(dx, ddx, 12 * (size_t)n, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipMemcpyAsync(dh, ddh, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
-	dfree(&dg); dfree(&ddx); dfree(&ddh); dfree(&part2);
 	return SKG_OK;
 }
 
@@ -286,17 +275,22 @@ extern "C" int skg_density_dlnk(skg_ctx *c, float *drho) {
 	int rc;
 	const int n = c->n, nS = c->nS;
 	float *d = nullptr, *part = nullptr;
-	if (dalloc(&d, n) || dalloc(&part, nS)) return SKG_E_NOMEM;
+	if (sget(c, 0, &d, n) || sget(c, 1, &part, nS)) return SKG_E_NOMEM;
 	SKG_HIP(hipMemsetAsync(d, 0, 4 * (size_t)n, c->st));
-	const bool per = c->P.bPeriodic != 0;
-	k_dlnk_gather<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_heap, c->P.nSmooth, c->s_ord, c->s_ball2, nS, c->d_pos,
-			c->d_mass, per ? c->P.fPeriod[0] : 0.f, per, d);
-	if ((rc = full_tree(c))) return rc;
-	launch_cover<2>(c, c->s_pos, nS, c->s_ord, nullptr, 0, c->s_box, part);
-	k_add_ord<<<nblk(nS, 256), 256, 0, c->st>>>(part, c->s_ord, nS, d);
+	if (c->dlnkReady) {
+		// both halves were formed by the last skg_density's kernels
+		k_dlnk_combine<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_dlnkG, c->s_dlnkS, c->s_ord, nS, d);
+	} else {
+		const bool per = c->P.bPeriodic != 0;
+		k_dlnk_gather<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_heap, c->P.nSmooth, c->s_ord, c->s_ball2, nS, c->d_pos,
+				c->d_mass, per ? c->P.fPeriod[0] : 0.f, per, d);
+		if ((rc = full_tree(c))) return rc;
+		launch_cover<2>(c, c->s_pos, nS, c->s_ord, nullptr, 0, c->s_box, part);
+		k_add_ord<<<nblk(nS, 256), 256, 0, c->st>>>(part, c->s_ord, nS, d);
+		c->wantDlnk = 1;                          // later skg_density calls form the halves themselves
+	}
 	SKG_HIP(hipGetLastError());
 	SKG_HIP(hipMemcpyAsync(drho, d, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
-	dfree(&d); dfree(&part);
 	return SKG_OK;
 }

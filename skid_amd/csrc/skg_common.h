@@ -22,6 +22,8 @@
 #define SKG_TILE_T 256
 // records allocated past the last entity of e_pos / e_aux (reads in blocks of 4)
 #define SKG_EPAD 4
+// scratch slots per context (skg_slot)
+#define SKG_NSLOT 24
 
 // Per-box constants of a batched context (box b = particles [off, off+n) by
 // iOrder).  A single-box context has one entry holding the parameter block's
@@ -78,8 +80,13 @@ struct skg_ctx {
 	float4 *s_pos = nullptr;
 	int *s_ord = nullptr;
 	float *s_ball2 = nullptr, *s_rhoG = nullptr, *s_rhoS = nullptr;
+	// d rho / d ln nSmooth halves formed by the density kernels themselves once
+	// skg_density_dlnk has been asked for (wantDlnk); dlnkReady: they belong
+	// to the last skg_density
+	float *s_dlnkG = nullptr, *s_dlnkS = nullptr;
+	int wantDlnk = 0, dlnkReady = 0;
 	int *s_box = nullptr;
-	unsigned long long *s_heap = nullptr;   // n_S * k keys of the last kNN (optional)
+	unsigned long long *s_heap = nullptr;   // k x n_S keys of the last kNN, slot-major (slot s of query i at s n_S + i)
 	DevTree ts;
 
 	// entities S' = S + replicas
@@ -96,6 +103,7 @@ struct skg_ctx {
 	                             // filled up with dead entities (fBall2 -1, position NaN)
 	DevTree te;
 	DevTree tfull;               // entity tree with every radius as built (adjoint, field gradient)
+	DevTree tfof;                // FOF tree (skg_fof, skg_fof_points)
 
 	// movers
 	int nMove = 0, nActive = 0, step = 0;
@@ -124,13 +132,17 @@ struct skg_ctx {
 	int *v32a = nullptr, *v32b = nullptr;
 	size_t vcap = 0;
 	float *part = nullptr; size_t partCap = 0;   // per-block reduction partials
+	void *slot[SKG_NSLOT] = {nullptr};           // stage temporaries kept between calls (skg_slot)
+	size_t slotBytes[SKG_NSLOT] = {0};
 
 	// launch shapes (set at create; SKG_COVER_SUB_MAX overrides): point sets of
 	// at most subMax points use the sub-wave cover kernel
 	long subMax = 0;
 	long s64Max = 0;            // at most this many active movers: 64 lanes (a wavefront) per mover
-	// per-tile leaf lists of the current 5-step block (SKG_TILE_LISTS=0: off)
-	int useLists = 1, tlCap = 512;
+	// per-tile lists of the current 5-step block: entities (SKG_TILE_LISTS=1,
+	// default), leaves (2) or none (0: every step walks the tree); tlCap
+	// entries per tile (a multiple of 4)
+	int useLists = 1, tlCap = 1024;
 	int *tl_list = nullptr, *tl_cnt = nullptr;
 	// device-resident move loop: ml[0] step stamp, ml[1] active, ml[2] next
 	int *d_ml = nullptr;
@@ -317,6 +329,20 @@ __device__ __forceinline__ float m4_term(float d2, float ih2, float fNorm) {
 	else rs = (float)(0.25 * (double)rs * (double)rs * (double)rs);
 	return fmul(rs, fNorm);
 }
+// The same M4 term in fp32 for the compensated mode (no double operations in
+// the per-pair code): q from the hardware square root (about 1 ulp), then
+// 1 - 0.75 (2 - q) q^2 inside the unit radius and 0.25 (2 - q)^3 outside,
+// where 2 - q is exact near q = 2.  A few ulp per term.
+__device__ __forceinline__ float m4_term_f32(float d2, float ih2, float fNorm) {
+	const float r2 = fmul(d2, ih2);
+	const float t = fsub(2.0f, __builtin_amdgcn_sqrtf(r2));
+	const float w = r2 < 1.0f ? fmaf(fmul(-0.75f, t), r2, 1.0f) : fmul(fmul(0.25f, t), fmul(t, t));
+	return fmul(w, fNorm);
+}
+template <bool COMP> __device__ __forceinline__ float m4_w(float d2, float ih2, float fNorm) {
+	return COMP ? m4_term_f32(d2, ih2, fNorm) : m4_term(d2, ih2, fNorm);
+}
+
 // Gradient factor g(q)*norm in the reference's float/double mix
 // (smooth1.c:450-454): sqrt of a float in double rounds like a correctly
 // rounded fp32 sqrt, and 2.25*rs is exact in double, so the fma is the
@@ -344,6 +370,29 @@ __device__ __forceinline__ float grad_factor_f32(float d2, float ih2, float gnor
 	const float gin = fmaf(2.25f, q, -3.0f);
 	const float gout = fmul(fmul(-0.75f, fmul(t, t)), __builtin_amdgcn_rcpf(q));
 	return fmul(r2 < 1.0f ? gin : gout, gnorm);
+}
+
+// M4 kernel w(q) and its derivative w'(q) (Monaghan & Lattanzio 1985), fp32.
+__device__ __forceinline__ void m4_derivs(float q, float &w, float &wp) {
+	if (q < 1.0f) {
+		w = fmaf(fmaf(0.75f, q, -1.5f), q * q, 1.0f);
+		wp = q * fmaf(2.25f, q, -3.0f);
+	} else if (q < 2.0f) {
+		const float t = 2.0f - q;
+		w = 0.25f * t * t * t;
+		wp = -0.75f * t * t;
+	} else {
+		w = 0.f; wp = 0.f;
+	}
+}
+// One pair's term of d rho / d ln nSmooth before the factor -ih^3/(6 pi): the
+// smoothing lengths scale as nSmooth^(1/3), so h dW/dh = -(3w + q w')/(pi h^3)
+// and d/d ln k = (1/3) h d/dh; ih = 1/h of the ball the term is taken with.
+__device__ __forceinline__ float dlnk_term(float r, float ih, float m) {
+	const float q = r * ih;
+	float w, wp;
+	m4_derivs(q, w, wp);
+	return m * (3.0f * w + q * wp);
 }
 
 // ---------------------------------------------------------------- host helpers (skg_core.hip)
@@ -388,3 +437,10 @@ template <typename T> static inline int dalloc(T **p, size_t count) {
 	return skg_dmalloc((void **)p, count * sizeof(T));
 }
 template <typename T> static inline void dfree(T **p) { skg_dfree((void **)p); }
+// Stage temporaries: slot s of the context, grown to count items when needed
+// and kept for the next call.  Within one entry point every temporary takes
+// its own slot; entry points never overlap on a context.
+int skg_slot(skg_ctx *c, int s, void **p, size_t bytes);
+template <typename T> static inline int sget(skg_ctx *c, int s, T **p, size_t count) {
+	return skg_slot(c, s, (void **)p, count * sizeof(T));
+}

@@ -26,13 +26,37 @@ static std::mutex g_mem_mu;
 static std::unordered_map<void *, size_t> g_mem_live;
 static long long g_mem_cur = 0, g_mem_peak = 0;
 
+// An array that already holds an allocation keeps it when that is large
+// enough and at most twice the request (plus 64 MiB): the stages re-size their
+// arrays on every call, and a hipFree synchronises the device.
 int skg_dmalloc(void **p, size_t bytes) {
+	if (*p) {
+		std::lock_guard<std::mutex> lk(g_mem_mu);
+		auto it = g_mem_live.find(*p);
+		if (it != g_mem_live.end() && it->second >= bytes && it->second <= 2 * bytes + ((size_t)64 << 20)) return 0;
+	}
 	skg_dfree(p);
 	if (hipMalloc(p, bytes) != hipSuccess) { *p = nullptr; return SKG_E_NOMEM; }
 	std::lock_guard<std::mutex> lk(g_mem_mu);
 	g_mem_live[*p] = bytes;
 	g_mem_cur += (long long)bytes;
 	g_mem_peak = std::max(g_mem_peak, g_mem_cur);
+	return 0;
+}
+
+// Per-context scratch slots: a stage's temporaries live in numbered slots that
+// keep their allocation from call to call (freed by skg_destroy), so a stage
+// entry point allocates only when an input outgrows the last call's.
+int skg_slot(skg_ctx *c, int s, void **p, size_t bytes) {
+	if (s < 0 || s >= SKG_NSLOT) return SKG_E_PARAM;
+	if (bytes == 0) bytes = 1;
+	if (c->slotBytes[s] < bytes || !c->slot[s]) {
+		skg_dfree(&c->slot[s]);
+		c->slotBytes[s] = 0;
+		if (skg_dmalloc(&c->slot[s], bytes)) return SKG_E_NOMEM;
+		c->slotBytes[s] = bytes;
+	}
+	*p = c->slot[s];
 	return 0;
 }
 
@@ -538,7 +562,7 @@ int skg_build_tree_prefix(skg_ctx *c, DevTree *t, const float4 *pos, const unsig
 	int rc;
 	if (n <= 0) return skg_build_tree(c, t, pos, n, B, b2, ibox);
 	int *depth = nullptr, *flag = nullptr, *off = nullptr;
-	if (dalloc(&depth, n) || dalloc(&flag, n) || dalloc(&off, n)) return SKG_E_NOMEM;
+	if (sget(c, 17, &depth, n) || sget(c, 18, &flag, n) || sget(c, 19, &off, n)) return SKG_E_NOMEM;
 	k_leaf_depth<<<nblk(n, 256), 256, 0, c->st>>>(keys, n, B, minDepth, depth);
 	k_leaf_flags<<<nblk(n, 256), 256, 0, c->st>>>(keys, depth, n, flag);
 	if ((rc = skg_exclusive_scan(c, flag, off, n))) return rc;
@@ -555,15 +579,11 @@ int skg_build_tree_prefix(skg_ctx *c, DevTree *t, const float4 *pos, const unsig
 	if (dalloc(&t->lstart, (size_t)nLeaf2 + 1)) return SKG_E_NOMEM;
 	k_leaf_starts<<<nblk(std::max(n, nLeaf2 + 1), 256), 256, 0, c->st>>>(flag, off, n, nLeaf, nLeaf2, t->lstart);
 	t->n = n; t->B = B; t->nLeaf = nLeaf; t->nLeaf2 = nLeaf2;
-	rc = tree_fill(c, t, pos, b2, ibox);
-	SKG_HIP(hipStreamSynchronize(c->st));
-	dfree(&depth); dfree(&flag); dfree(&off);
-	return rc;
+	return tree_fill(c, t, pos, b2, ibox);
 }
 
 int skg_copy_tree_leaves(skg_ctx *c, DevTree *dst, const DevTree *src) {
-	dfree(&dst->lstart);
-	if (!src->lstart) return 0;
+	if (!src->lstart) { dfree(&dst->lstart); return 0; }
 	if (dalloc(&dst->lstart, (size_t)src->nLeaf2 + 1)) return SKG_E_NOMEM;
 	SKG_HIP(hipMemcpyAsync(dst->lstart, src->lstart, 4 * ((size_t)src->nLeaf2 + 1), hipMemcpyDeviceToDevice, c->st));
 	return 0;
@@ -619,7 +639,7 @@ __global__ __launch_bounds__(64) void k_knn(
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		float L, int k, const float *__restrict__ mass_ord, const float *__restrict__ pos_ord,
 		float *__restrict__ ball2_out, float *__restrict__ rhoG, unsigned long long *__restrict__ heap_out,
-		const int *__restrict__ sbox, const DevBox *__restrict__ bt) {
+		const int *__restrict__ sbox, const DevBox *__restrict__ bt, float *__restrict__ dlnk_out) {
 	__shared__ unsigned long long heap[(KMAX + 1) * 64];   // k slots + the zero sentinel row
 	__shared__ int stk[64];
 	__shared__ float stkd[64];
@@ -714,13 +734,19 @@ __global__ __launch_bounds__(64) void k_knn(
 	const float h2 = key_d2(top);
 	const float ih2 = (float)(4.0 / (double)h2);
 	const float fNorm = (float)(0.5 * M_1_PI * sqrt((double)ih2) * (double)ih2);
+	// with dlnk_out, the gather half of d rho / d ln nSmooth over the same
+	// pairs, from exact periodic separations, h = sqrt(fBall2)/2
+	const bool dl = dlnk_out != nullptr;
+	const float ih = dl ? 1.0f / (0.5f * sqrtf(h2)) : 0.f;
 	Sum<COMP> acc;
 	acc.init();
+	Acc dacc;
+	dacc.init();
 	for (int s = 1; s < k; ++s) {
 		const unsigned long long key = heap[s * 64 + lane];
 		const unsigned j = (unsigned)key;
-		float d2 = key_d2(key);
-		if (COMP && PER) {
+		float d2 = key_d2(key), d2x = d2;
+		if (PER && (COMP || dl)) {
 			// weights from exact periodic separations (membership stays the
 			// reference's d2 test, so the neighbour set is unchanged)
 			const float px = pos_ord[3 * j], py = pos_ord[3 * j + 1], pz = pos_ord[3 * j + 2];
@@ -729,15 +755,19 @@ __global__ __launch_bounds__(64) void k_knn(
 				dx = exact_delta(q.x, px, dx > hL ? -L : (dx < -hL ? L : 0.f));
 				dy = exact_delta(q.y, py, dy > hL ? -L : (dy < -hL ? L : 0.f));
 				dz = exact_delta(q.z, pz, dz > hL ? -L : (dz < -hL ? L : 0.f));
-				d2 = sq3(dx, dy, dz);
+				d2x = sq3(dx, dy, dz);
 			}
+			if (COMP) d2 = d2x;
 		}
-		acc.add(fmul(m4_term(d2, ih2, fNorm), mass_ord[j]));
+		const float mj = mass_ord[j];
+		acc.add(fmul(m4_w<COMP>(d2, ih2, fNorm), mj));
+		if (dl) dacc.add(dlnk_term(sqrtf(d2x), ih, mj));
 	}
 	ball2_out[qi] = h2;
 	rhoG[qi] = acc.val();
-	if (heap_out)
-		for (int s = 0; s < k; ++s) heap_out[(size_t)qi * k + s] = heap[s * 64 + lane];
+	if (dl) dlnk_out[qi] = dacc.val() * (float)(-M_1_PI / 6.0) * ih * ih * ih;
+	if (heap_out)                                  // slot-major: one slot of 64 queries is one 512-byte store
+		for (int s = 0; s < k; ++s) heap_out[(size_t)s * n + qi] = heap[s * 64 + lane];
 }
 
 // =====================================================================
@@ -752,11 +782,48 @@ __global__ __launch_bounds__(64) void k_knn(
 // smallest positive density among its hits (hmin; during the move loop the
 // record's w holds the entity's density).  Records are read four at a time
 // (one wait each).
+// One entity e (record A, X, image code K) against the lane's point when want.
+// MODE 0 with dl also sums the scatter half of d rho / d ln nSmooth into s1
+// (exact separations; the factor -1/(6 pi) is applied by the caller).
+template <int MODE, bool COMP>
+__device__ __forceinline__ void cover_hit(int e, const float4 &A, const float4 &X, int K, bool want,
+		const float4 &p, const float4 *__restrict__ eorg, int *__restrict__ ehit, int stamp, float &hmin,
+		float Lx, float Ly, float Lz, Sum<COMP> &s0, Sum<COMP> &s1, Sum<COMP> &s2, bool dl = false) {
+	float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
+	float d2 = sq3(dx, dy, dz);
+	if (want && d2 < X.x) {
+		if (COMP || (MODE == 0 && dl)) {
+			// replica: separation from the source position plus the exact shift;
+			// K < 0: the code is read here, from the source record's w
+			float4 O = make_float4(0.f, 0.f, 0.f, 0.f);
+			if (K < 0) { O = eorg[e]; K = (int)O.w; }
+			if (K != 13) {                       // 9(ix+1) + 3(iy+1) + (iz+1)
+				O = eorg[e];
+				dx = exact_delta(O.x, p.x, (float)(K / 9 - 1) * Lx);
+				dy = exact_delta(O.y, p.y, (float)((K / 3) % 3 - 1) * Ly);
+				dz = exact_delta(O.z, p.z, (float)(K % 3 - 1) * Lz);
+			}
+		}
+		if (MODE == 0) {
+			s0.add(fmul(m4_w<COMP>(COMP ? sq3(dx, dy, dz) : d2, X.y, X.w), A.w));
+			if (dl) {
+				const float ih = __fsqrt_rn(X.y);      // X.y = 4 / fBall2 = 1 / h^2
+				s1.add(dlnk_term(sqrtf(sq3(dx, dy, dz)), ih, A.w) * (ih * ih * ih));
+			}
+		} else {
+			const float f = COMP ? grad_factor_f32(sq3(dx, dy, dz), X.y, X.z) : grad_factor(d2, X.y, X.z);
+			s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
+			if (ehit) ehit[e] = stamp;
+			else hmin = fminf(hmin, X.w > 0.f ? X.w : INFINITY);
+		}
+	}
+}
+
 template <int MODE, bool COMP>
 __device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const float4 &p,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
 		const int *__restrict__ ecode, int *__restrict__ ehit, int stamp, float &hmin, float Lx, float Ly,
-		float Lz, Sum<COMP> &s0, Sum<COMP> &s1, Sum<COMP> &s2) {
+		float Lz, Sum<COMP> &s0, Sum<COMP> &s1, Sum<COMP> &s2, bool dl = false) {
 	if (cnt <= 0 || !want) return;
 	for (int t0 = 0; t0 < cnt; t0 += 4) {
 		// four consecutive records in one scalar load each; the entity arrays
@@ -764,36 +831,17 @@ __device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const 
 		// leaf's end stay in the allocation (their values are never used)
 		const float4 *ap = epos + start + t0, *xp = eaux + start + t0;
 		float4 A4[4], X4[4];
-		int K4[4];                                         // image codes: a hit waits on no load
+		int K4[4] = {13, 13, 13, 13};                       // image codes: a hit waits on no load
 #pragma unroll
-		for (int u = 0; u < 4; ++u) { A4[u] = ap[u]; X4[u] = xp[u]; if (COMP) K4[u] = ecode[start + t0 + u]; }
+		for (int u = 0; u < 4; ++u) {
+			A4[u] = ap[u]; X4[u] = xp[u];
+			if (COMP || (MODE == 0 && dl)) K4[u] = ecode[start + t0 + u];
+		}
 #pragma unroll
 		for (int u = 0; u < 4; ++u) {
 			if (t0 + u >= cnt) break;
-			const int e = start + t0 + u;
-			const float4 A = A4[u], X = X4[u];
-			float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
-			float d2 = sq3(dx, dy, dz);
-			if (d2 < X.x) {
-				if (COMP) {
-					// replica: separation from the source position plus the exact shift
-					const int code = K4[u];        // 9(ix+1) + 3(iy+1) + (iz+1)
-					if (code != 13) {
-						const float4 O = eorg[e];
-						dx = exact_delta(O.x, p.x, (float)(code / 9 - 1) * Lx);
-						dy = exact_delta(O.y, p.y, (float)((code / 3) % 3 - 1) * Ly);
-						dz = exact_delta(O.z, p.z, (float)(code % 3 - 1) * Lz);
-					}
-				}
-				if (MODE == 0) {
-					s0.add(fmul(m4_term(COMP ? sq3(dx, dy, dz) : d2, X.y, X.w), A.w));
-				} else {
-					const float f = COMP ? grad_factor_f32(sq3(dx, dy, dz), X.y, X.z) : grad_factor(d2, X.y, X.z);
-					s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
-					if (ehit) ehit[e] = stamp;
-					else hmin = fminf(hmin, X.w > 0.f ? X.w : INFINITY);
-				}
-			}
+			cover_hit<MODE, COMP>(start + t0 + u, A4[u], X4[u], K4[u], true, p, eorg, ehit, stamp, hmin, Lx, Ly, Lz,
+					s0, s1, s2, dl);
 		}
 	}
 }
@@ -829,11 +877,20 @@ __device__ __forceinline__ void box_min_atomic(float hmin, bool valid, int pb, i
 // lane's point now.  One walk per block then replaces one walk per step.  A
 // tile with a point within the margin of a periodic face (it may wrap to the
 // far side) or with more than cap leaves gets count -1 and walks the tree.
+//
+// ELIST: the list holds entities instead of leaves -- every live entity whose
+// ball, widened by the margin, holds a lane's point now, in tree order (leaf
+// by leaf, entities in index order, as the walk meets them), so a lane's hits
+// and its sums keep the walk's order.  A lane tests only leaves of its own
+// box, so every listed entity is of some lane's box.
+template <bool ELIST>
 __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__restrict__ ppos,
 		const int *__restrict__ plist, int np, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi,
 		int nLeaf2, const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro, float msteps,
 		float mabs, int per, float cx, float cy, float cz, float Lx, float Ly, float Lz, int cap,
-		int *__restrict__ tlist, int *__restrict__ tcnt, const int *__restrict__ npDev) {
+		int *__restrict__ tlist, int *__restrict__ tcnt, const int *__restrict__ npDev,
+		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const int *__restrict__ lstart, int B,
+		int ne) {
 	if (npDev) np = *npDev;
 	const int lane = threadIdx.x & 63;
 	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
@@ -852,6 +909,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__rest
 				fabsf(p.z - cz) > 0.5f * Lz - margin;
 	if (__any(nearFace)) { if (lane == 0) tcnt[tile] = -1; return; }
 	int n = 0, node = 1;
+	int *out = tlist + (size_t)tile * cap;
 	while (node) {
 		node = uniform_int(node);
 		const float4 lo = nlo[node], hi = nhi[node];
@@ -862,8 +920,31 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__rest
 		}
 		if (__any(need)) {
 			if (node < nLeaf2) { node = 2 * node; continue; }
-			if (lane == 0 && n < cap) tlist[(size_t)tile * cap + n] = node - nLeaf2;
-			++n;
+			if (!ELIST) {
+				if (lane == 0 && n < cap) out[n] = node - nLeaf2;
+				++n;
+			} else if (n <= cap) {
+				const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
+				const int start = uniform_int(rg.x), cnt = uniform_int(rg.y - rg.x);
+				for (int t0 = 0; t0 < cnt; t0 += 4) {
+					// four records per scalar load (the arrays carry SKG_EPAD spare records)
+					float4 A4[4], X4[4];
+#pragma unroll
+					for (int u = 0; u < 4; ++u) { A4[u] = epos[start + t0 + u]; X4[u] = eaux[start + t0 + u]; }
+#pragma unroll
+					for (int u = 0; u < 4; ++u) {
+						if (t0 + u >= cnt) break;
+						if (X4[u].x < 0.f) continue;          // cut (or filler): never hit again
+						const float r = __fsqrt_rn(X4[u].x) + margin;
+						const float dx = fsub(A4[u].x, p.x), dy = fsub(A4[u].y, p.y), dz = fsub(A4[u].z, p.z);
+						const bool hit = need && sq3(dx, dy, dz) < r * r * 1.000001f;
+						if (__any(hit)) {
+							if (lane == 0 && n < cap) out[n] = start + t0 + u;
+							++n;
+						}
+					}
+				}
+			}
 		}
 		node = (node >> __builtin_ctz(~node)) + 1;
 		if (node == 1) node = 0;
@@ -884,7 +965,7 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		int *__restrict__ bthr, const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
 		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart,
 		const int *__restrict__ tlist, const int *__restrict__ tcnt, int tcap, const int *__restrict__ npDev,
-		const int *__restrict__ stampDev) {
+		const int *__restrict__ stampDev, int elist, const int *__restrict__ eblk, float *__restrict__ out_dl) {
 	if (npDev) np = *npDev;                       // device-resident active count and step stamp
 	if (stampDev) stamp = *stampDev;
 	const int lane = threadIdx.x & 63;
@@ -898,6 +979,7 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 	Sum<COMP> s0, s1, s2;
 	s0.init(); s1.init(); s2.init();
 	float hmin = INFINITY;
+	const bool dl = MODE == 0 && out_dl != nullptr;
 	// With a leaf list for this tile (built at the start of the block, see
 	// k_cover_lists) the wavefront scans those leaves and walks no tree.
 	// Otherwise a node is opened when its largest live ball reaches at least
@@ -907,7 +989,33 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 	const int tile = base >> 6;
 	const int nl = tcnt ? uniform_int(tcnt[tile]) : -1;
 	int node = nl >= 0 ? 0 : 1;
-	for (int k = 0; k < nl; ++k) {
+	if (elist && nl > 0) {
+		// entity list (k_cover_lists<true>): the listed entities in tree
+		// order; a tile that straddles two boxes lists entities of both, and a
+		// lane takes only its own box's range of the entity array
+		const int *L = tlist + (size_t)tile * tcap;
+		int elo = 0, ehi = 0x7FFFFFFF;
+		if (pbox) { elo = eblk[pb] * 256; ehi = eblk[pb + 1] * 256; }
+		for (int k = 0; k < nl; k += 4) {
+			int E4[4];
+#pragma unroll
+			for (int u = 0; u < 4; ++u) E4[u] = L[k + u];      // tcap is a multiple of 4
+#pragma unroll
+			for (int u = 0; u < 4; ++u) if (k + u >= nl) E4[u] = E4[0];
+			float4 A4[4], X4[4];
+			int K4[4] = {13, 13, 13, 13};
+#pragma unroll
+			for (int u = 0; u < 4; ++u) { A4[u] = epos[E4[u]]; X4[u] = eaux[E4[u]]; if (COMP) K4[u] = ecode[E4[u]]; }
+#pragma unroll
+			for (int u = 0; u < 4; ++u) {
+				if (k + u >= nl) break;
+				const int e = E4[u];
+				const bool mine = valid && e >= elo && e < ehi;
+				cover_hit<MODE, COMP>(e, A4[u], X4[u], K4[u], mine, p, eorg, ehit, stamp, hmin, Lx, Ly, Lz, s0, s1, s2);
+			}
+		}
+	}
+	for (int k = 0; !elist && k < nl; ++k) {
 		const int leaf = uniform_int(tlist[(size_t)tile * tcap + k]);
 		// the list holds what any of the block's steps can reach; this step
 		// scans a leaf only where the walk below would open it (leaves hold
@@ -918,7 +1026,7 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		if (!__any(want)) continue;
 		const int2 rg = leaf_range(lstart, leaf, B, ne);
 		cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), want, p, epos, eaux, eorg, ecode, ehit,
-				stamp, hmin, Lx, Ly, Lz, s0, s1, s2);
+				stamp, hmin, Lx, Ly, Lz, s0, s1, s2, dl);
 	}
 	while (node) {
 		node = uniform_int(node);
@@ -929,7 +1037,7 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		if (visit) {
 			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
 			cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), need, p, epos, eaux, eorg, ecode,
-					ehit, stamp, hmin, Lx, Ly, Lz, s0, s1, s2);
+					ehit, stamp, hmin, Lx, Ly, Lz, s0, s1, s2, dl);
 		}
 		// next node in pre-order after node's subtree (strip trailing ones, step)
 		node = (node >> __builtin_ctz(~node)) + 1;
@@ -937,7 +1045,11 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 	}
 	if (MODE == 1 && bthr) box_min_atomic(hmin, valid, pb, bthr);
 	if (!valid) return;
-	if (MODE == 0) { out_sum[ii] = s0.val(); return; }
+	if (MODE == 0) {
+		out_sum[ii] = s0.val();
+		if (dl) out_dl[ii] = s1.val() * (float)(-M_1_PI / 6.0);
+		return;
+	}
 	// kdMoveParticles (kd.c:702-732)
 	const float fStep = micro ? bt[pb].fMicro : bt[pb].fStep;
 	const float ax = s0.val(), ay = s1.val(), az = s2.val();
@@ -965,11 +1077,11 @@ template <int MODE, bool COMP, int S>
 __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 		float4 *__restrict__ ppos, const int *__restrict__ plist, int np,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
-		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
-		float *__restrict__ out_sum, int *__restrict__ ehit, int stamp, int *__restrict__ bthr,
+		int ne, const float4 *__restrict__ nlo, const float4 *__restrict__ nhi,
+		int nLeaf2, int B, float *__restrict__ out_sum, int *__restrict__ ehit, int stamp, int *__restrict__ bthr,
 		const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
 		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart,
-		const int *__restrict__ npDev, const int *__restrict__ stampDev) {
+		const int *__restrict__ npDev, const int *__restrict__ stampDev, float *__restrict__ out_dl) {
 	if (npDev) np = *npDev;
 	if (stampDev) stamp = *stampDev;
 	const int blk = xcd_tile(blockIdx.x, gridDim.x);
@@ -984,6 +1096,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 	Sum<COMP> s0, s1, s2;
 	s0.init(); s1.init(); s2.init();
 	float hmin = INFINITY;
+	const bool dl = MODE == 0 && out_dl != nullptr;
 	int node = valid ? 1 : 0;
 	while (node) {                                // the S lanes of a point move in lockstep
 		if (S == 64) node = uniform_int(node);    // one point per wavefront: scalar node reads
@@ -993,28 +1106,8 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 		if (need) {
 			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
 			for (int e = rg.x + sub; e < rg.y; e += S) {
-				const float4 A = epos[e], X = eaux[e];
-				float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
-				float d2 = sq3(dx, dy, dz);
-				if (d2 < X.x) {
-					if (COMP) {
-						const float4 O = eorg[e];
-						const int code = (int)O.w;     // 9(ix+1) + 3(iy+1) + (iz+1)
-						if (code != 13) {
-							dx = exact_delta(O.x, p.x, (float)(code / 9 - 1) * Lx);
-							dy = exact_delta(O.y, p.y, (float)((code / 3) % 3 - 1) * Ly);
-							dz = exact_delta(O.z, p.z, (float)(code % 3 - 1) * Lz);
-						}
-					}
-					if (MODE == 0) {
-						s0.add(fmul(m4_term(COMP ? sq3(dx, dy, dz) : d2, X.y, X.w), A.w));
-					} else {
-						const float f = COMP ? grad_factor_f32(sq3(dx, dy, dz), X.y, X.z) : grad_factor(d2, X.y, X.z);
-						s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
-						if (ehit) ehit[e] = stamp;
-						else hmin = fminf(hmin, X.w > 0.f ? X.w : INFINITY);
-					}
-				}
+				cover_hit<MODE, COMP>(e, epos[e], eaux[e], -1, true, p, eorg, ehit, stamp, hmin, Lx, Ly, Lz, s0, s1, s2,
+						dl);
 			}
 		}
 		int nx = node;
@@ -1026,14 +1119,16 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 #pragma unroll
 	for (int o = S / 2; o > 0; o >>= 1) {
 		s0.merge(__shfl_xor(s0.s, o, 64), __shfl_xor(s0.cc(), o, 64));
-		if (MODE == 1) {
-			s1.merge(__shfl_xor(s1.s, o, 64), __shfl_xor(s1.cc(), o, 64));
-			s2.merge(__shfl_xor(s2.s, o, 64), __shfl_xor(s2.cc(), o, 64));
-		}
+		if (MODE == 1 || dl) s1.merge(__shfl_xor(s1.s, o, 64), __shfl_xor(s1.cc(), o, 64));
+		if (MODE == 1) s2.merge(__shfl_xor(s2.s, o, 64), __shfl_xor(s2.cc(), o, 64));
 	}
 	if (MODE == 1 && bthr) box_min_atomic(hmin, valid, pb, bthr);
 	if (!valid || sub != 0) return;
-	if (MODE == 0) { out_sum[ii] = s0.val(); return; }
+	if (MODE == 0) {
+		out_sum[ii] = s0.val();
+		if (dl) out_dl[ii] = s1.val() * (float)(-M_1_PI / 6.0);
+		return;
+	}
 	const float fStep = micro ? bt[pb].fMicro : bt[pb].fStep;
 	const float ax = s0.val(), ay = s1.val(), az = s2.val();
 	const float ss = sq3(ax, ay, az);
@@ -1603,6 +1698,8 @@ extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
 	c->s64Max = 1 << 14;
 	if (const char *e = getenv("SKG_COVER_S64_MAX")) c->s64Max = atol(e);
 	if (const char *e = getenv("SKG_TILE_LISTS")) c->useLists = atoi(e);
+	c->tlCap = c->useLists == 2 ? 512 : 1024;
+	if (const char *e = getenv("SKG_TILE_CAP")) c->tlCap = std::max(4, atoi(e) & ~3);
 	if (const char *e = getenv("SKG_GRAPH")) c->useGraph = atoi(e);
 	if (const char *e = getenv("SKG_GRAPH_SHRINK")) c->gShrink = std::max(1, atoi(e));
 	if (const char *e = getenv("SKG_POT_TILE")) {
@@ -1633,8 +1730,8 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->d_pos); dfree(&c->d_vel); dfree(&c->d_mass); dfree(&c->d_soft); dfree(&c->d_temp);
 	dfree(&c->d_rho); dfree(&c->d_ball2); dfree(&c->d_grp);
 	dfree(&c->s_pos); dfree(&c->s_ord); dfree(&c->s_ball2); dfree(&c->s_rhoG); dfree(&c->s_rhoS);
-	dfree(&c->s_heap); dfree(&c->s_box);
-	skg_free_tree(&c->ts); skg_free_tree(&c->te); skg_free_tree(&c->tfull);
+	dfree(&c->s_heap); dfree(&c->s_box); dfree(&c->s_dlnkG); dfree(&c->s_dlnkS);
+	skg_free_tree(&c->ts); skg_free_tree(&c->te); skg_free_tree(&c->tfull); skg_free_tree(&c->tfof);
 	dfree(&c->e_b2); dfree(&c->e_pos); dfree(&c->e_aux); dfree(&c->e_org); dfree(&c->e_code); dfree(&c->e_rho); dfree(&c->e_src); dfree(&c->e_live); dfree(&c->e_hit);
 	dfree(&c->e_box);
 	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2); dfree(&c->m_box);
@@ -1650,6 +1747,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	dfree(&c->d_thr); dfree(&c->d_cnt);
 	skg_dfree(&c->tmp);
 	dfree(&c->k64a); dfree(&c->k64b); dfree(&c->v32a); dfree(&c->v32b); dfree(&c->part);
+	for (int i = 0; i < SKG_NSLOT; ++i) { skg_dfree(&c->slot[i]); c->slotBytes[i] = 0; }
 	for (int i = 0; i < c->caprec; ++i) { (void)hipEventDestroy(c->prec[i].a); (void)hipEventDestroy(c->prec[i].b); }
 	free(c->prec);
 	if (c->e0) (void)hipEventDestroy(c->e0);
@@ -1762,7 +1860,7 @@ static void launch_knn(skg_ctx *c, bool per, bool comp) {
 #define KL(P_, C_)                                                                           \
 	k_knn<KMAX, P_, C_, BATCH><<<nb, 64, 0, c->st>>>(c->s_pos, c->s_ord, c->nS, c->ts.lo, c->ts.hi, \
 			c->ts.nLeaf2, c->ts.B, L, c->P.nSmooth, c->d_mass, c->d_pos, c->s_ball2, c->s_rhoG, c->s_heap, \
-			c->s_box, c->d_bt)
+			c->s_box, c->d_bt, c->wantDlnk ? c->s_dlnkG : nullptr)
 	if (per) { if (comp) KL(true, true); else KL(true, false); }
 	else { if (comp) KL(false, true); else KL(false, false); }
 #undef KL
@@ -1774,7 +1872,7 @@ static void launch_knn(skg_ctx *c, bool per, bool comp) {
 }
 
 static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, int np, const int *pbox,
-		float *out, int stamp, int micro, double work) {
+		float *out, int stamp, int micro, double work, float *out_dl = nullptr) {
 	if (np <= 0) return 0;
 	const int pslot = skg_prof_begin(c);
 	const int nb = nblk(np, SKG_TILE_T);
@@ -1783,12 +1881,13 @@ static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, in
 #define CL(M_, C_)                                                                             \
 	k_cover<M_, C_><<<nb, SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->e_org, c->e_code, c->nE, c->te.lo, \
 			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, nullptr, c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, \
-			C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart, nullptr, nullptr, c->tlCap, nullptr, nullptr)
+			C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart, nullptr, nullptr, c->tlCap, nullptr, nullptr, 0, \
+			nullptr, out_dl)
 #define CS(M_, C_)                                                                             \
 	k_cover_sub<M_, C_, 16><<<nblk(16L * np, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, \
 			c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp,       \
 			nullptr, c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart, \
-			nullptr, nullptr)
+			nullptr, nullptr, out_dl)
 	if (np <= c->subMax) {
 		if (mode == 0) { if (comp) CS(0, true); else CS(0, false); }
 		else { if (comp) CS(1, true); else CS(1, false); }
@@ -1814,6 +1913,7 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		return SKG_E_PARAM;   // -p sets one length for all axes (main.c:273-281)
 	if (256 % B != 0 && nBox > 1) return SKG_E_PARAM;   // boxes start on whole leaves
 	const int smask = scatter_species(c);
+	c->dlnkReady = 0;
 	int nS = 0;
 	for (int b = 0; b < nBox; ++b) {
 		const HostBox &h = c->hbox[b];
@@ -1830,9 +1930,10 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		// scatterers (kdScatterActive) in (box, Morton) order
 		float4 *tmp4 = nullptr;
 		int *tmpo = nullptr, *tmpb = nullptr, *perm = nullptr, *flag = nullptr, *off = nullptr;
-		if (dalloc(&c->s_pos, nS + SKG_EPAD) || dalloc(&c->s_ord, nS + SKG_EPAD) || dalloc(&c->s_box, nS) || dalloc(&c->s_ball2, nS) ||
-		    dalloc(&c->s_rhoG, nS) || dalloc(&c->s_rhoS, nS) || dalloc(&tmp4, nS) || dalloc(&tmpo, nS) ||
-		    dalloc(&tmpb, nS) || dalloc(&perm, nS) || dalloc(&flag, n) || dalloc(&off, n))
+		if ((c->wantDlnk && (dalloc(&c->s_dlnkG, nS) || dalloc(&c->s_dlnkS, nS))) ||
+		    dalloc(&c->s_pos, nS + SKG_EPAD) || dalloc(&c->s_ord, nS + SKG_EPAD) || dalloc(&c->s_box, nS) || dalloc(&c->s_ball2, nS) ||
+		    dalloc(&c->s_rhoG, nS) || dalloc(&c->s_rhoS, nS) || sget(c, 0, &tmp4, nS) || sget(c, 1, &tmpo, nS) ||
+		    sget(c, 2, &tmpb, nS) || sget(c, 3, &perm, nS) || sget(c, 4, &flag, n) || sget(c, 5, &off, n))
 			return SKG_E_NOMEM;
 		k_species_flags<<<nblk(n, 256), 256, 0, c->st>>>(c->d_type, n, smask, flag);
 		if ((rc = skg_exclusive_scan(c, flag, off, n))) return rc;
@@ -1843,8 +1944,6 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		if ((rc = skg_hilbert_sort(c, tmp4, nullptr, nS, perm, nBox > 1 ? tmpb : nullptr))) return rc;
 		k_permute_s<<<nblk(nS, 256), 256, 0, c->st>>>(tmp4, tmpo, tmpb, perm, nS, c->s_pos, c->s_ord, c->s_box);
 		if ((rc = skg_build_tree(c, &c->ts, c->s_pos, nS, B, nullptr, nBox > 1 ? c->s_box : nullptr))) return rc;
-		SKG_HIP(hipStreamSynchronize(c->st));
-		dfree(&tmp4); dfree(&tmpo); dfree(&tmpb); dfree(&perm); dfree(&flag); dfree(&off);
 		skg_timer_stop(c, SKG_T_TREE);
 
 		// kNN radii + gather half of the density
@@ -1864,7 +1963,7 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		skg_timer_start(c);
 		unsigned *mask = nullptr;
 		int *cnt = nullptr, *roff = nullptr;
-		if (dalloc(&mask, nS) || dalloc(&cnt, nS) || dalloc(&roff, nS)) return SKG_E_NOMEM;
+		if (sget(c, 6, &mask, nS) || sget(c, 7, &cnt, nS) || sget(c, 8, &roff, nS)) return SKG_E_NOMEM;
 		int nRep = 0;
 		if (per) {
 			const float *L = c->P.fPeriod, *C = c->P.fCenter;
@@ -1883,14 +1982,14 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 			SKG_HIP(hipMemsetAsync(roff, 0, 4 * (size_t)nS, c->st));
 		}
 		c->nRep = nRep;
-		skg_free_tree(&c->tfull);
+		c->tfull.n = -1;                        // rebuilt on demand (full_tree), arrays kept
 		const int nEu = nS + nRep;
 		c->nLive = nEu;
 		float4 *e4u = nullptr;
 		float *eb2u = nullptr;
 		int *esrcu = nullptr, *perm2 = nullptr, *ebu = nullptr, *bfirst = nullptr;
-		if (dalloc(&e4u, nEu) || dalloc(&eb2u, nEu) || dalloc(&esrcu, nEu) || dalloc(&perm2, nEu) ||
-		    dalloc(&ebu, nEu) || dalloc(&bfirst, nBox))
+		if (sget(c, 9, &e4u, nEu) || sget(c, 10, &eb2u, nEu) || sget(c, 11, &esrcu, nEu) || sget(c, 12, &perm2, nEu) ||
+		    sget(c, 13, &ebu, nEu) || sget(c, 14, &bfirst, nBox))
 			return SKG_E_NOMEM;
 		const float *L = c->P.fPeriod;
 		k_rep_write<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_pos, c->s_ball2, nS, mask, roff,
@@ -1920,38 +2019,33 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		}
 		const int nE = 256 * c->h_eblk[nBox];
 		c->nE = nE;
-		float *eb2s = nullptr;
 		unsigned long long *ekey = nullptr;
-		if (dalloc(&ekey, nE) || dalloc(&eb2s, nE) || dalloc(&c->e_pos, nE + SKG_EPAD) || dalloc(&c->e_aux, nE + SKG_EPAD) || dalloc(&c->e_rho, nE) ||
+		if (sget(c, 15, &ekey, nE) || dalloc(&c->e_b2, nE) || dalloc(&c->e_pos, nE + SKG_EPAD) || dalloc(&c->e_aux, nE + SKG_EPAD) || dalloc(&c->e_rho, nE) ||
 		    dalloc(&c->e_org, nE) || dalloc(&c->e_code, nE + SKG_EPAD) || dalloc(&c->e_src, nE) || dalloc(&c->e_live, nE) || dalloc(&c->e_hit, nE) ||
 		    dalloc(&c->e_box, nE) || dalloc(&c->d_eblk, nBox + 1))
 			return SKG_E_NOMEM;
 		SKG_HIP(hipMemcpyAsync(c->d_eblk, c->h_eblk, 4 * (size_t)(nBox + 1), hipMemcpyHostToDevice, c->st));
 		const int bbits = nBox > 1 ? skg_box_bits(c) : 0;
 		k_entity_fill<<<nblk(nE, 256), 256, 0, c->st>>>(nE, c->d_eblk, nBox, c->e_pos, c->e_aux, c->e_org, c->e_code, c->e_rho,
-				c->e_src, c->e_live, c->e_hit, eb2s, c->e_box, ekey, bbits);
+				c->e_src, c->e_live, c->e_hit, c->e_b2, c->e_box, ekey, bbits);
 		k_entity_setup<<<nblk(nEu, 256), 256, 0, c->st>>>(e4u, eb2u, esrcu, perm2, nEu, ebu, bfirst, c->d_eblk,
 				nullptr, c->s_pos, per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, c->e_pos, c->e_aux,
-				c->e_org, c->e_code, c->e_rho, c->e_src, c->e_live, c->e_hit, eb2s, c->k64b, ekey);
+				c->e_org, c->e_code, c->e_rho, c->e_src, c->e_live, c->e_hit, c->e_b2, c->k64b, ekey);
 		// leaves: maximal key cells of at most B entities, never across boxes
-		if ((rc = skg_build_tree_prefix(c, &c->te, c->e_pos, ekey, nE, B, 1 + bbits, eb2s, c->e_box))) return rc;
-		SKG_HIP(hipStreamSynchronize(c->st));
-		dfree(&mask); dfree(&cnt); dfree(&roff); dfree(&e4u); dfree(&eb2u); dfree(&esrcu); dfree(&perm2);
-		dfree(&ebu); dfree(&bfirst); dfree(&ekey);
-		dfree(&c->e_b2);
-		c->e_b2 = eb2s;
+		if ((rc = skg_build_tree_prefix(c, &c->te, c->e_pos, ekey, nE, B, 1 + bbits, c->e_b2, c->e_box))) return rc;
 
 		// scatter half of the density: balls of others holding each scatterer
-		if ((rc = cover_launch(c, 0, c->s_pos, nullptr, nS, c->s_box, c->s_rhoS, 0, 0, 24.0 * nS))) return rc;
+		// (and of d rho / d ln nSmooth once that has been asked for)
+		if ((rc = cover_launch(c, 0, c->s_pos, nullptr, nS, c->s_box, c->s_rhoS, 0, 0, 24.0 * nS,
+				c->wantDlnk ? c->s_dlnkS : nullptr))) return rc;
+		c->dlnkReady = c->wantDlnk;
 		float *rhoS = nullptr;
-		if (dalloc(&rhoS, nS)) return SKG_E_NOMEM;
+		if (sget(c, 16, &rhoS, nS)) return SKG_E_NOMEM;
 		k_combine_rho<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_rhoG, c->s_rhoS, c->s_ball2, c->s_ord, nS, rhoS,
 				c->d_rho, c->d_ball2);
 		// entity densities for ScatterCut: each entity carries its source's density
 		k_entity_rho<<<nblk(nE, 256), 256, 0, c->st>>>(c->e_src, nE, rhoS, c->e_rho);
 		SKG_HIP(hipGetLastError());
-		SKG_HIP(hipStreamSynchronize(c->st));
-		dfree(&rhoS);
 		skg_timer_stop(c, SKG_T_DENSITY);
 	}
 	if (fBall2) SKG_HIP(hipMemcpyAsync(fBall2, c->d_ball2, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
@@ -1974,8 +2068,10 @@ extern "C" int skg_neighbors(skg_ctx *c, int *nbr, float *d2) {
 	SKG_HIP(hipMemcpyAsync(ord.data(), c->s_ord, 4 * (size_t)c->nS, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
 	for (size_t i = 0; i < (size_t)c->n * k; ++i) { nbr[i] = -1; if (d2) d2[i] = -1.f; }
+	std::vector<unsigned long long> rowv(k);
 	for (int i = 0; i < c->nS; ++i) {
-		unsigned long long *row = &h[(size_t)i * k];
+		unsigned long long *row = rowv.data();
+		for (int s = 0; s < k; ++s) row[s] = h[(size_t)s * c->nS + i];   // the heap is stored slot-major
 		std::sort(row, row + k);
 		for (int s = 0; s < k; ++s) {
 			nbr[(size_t)ord[i] * k + s] = (int)(unsigned)row[s];
@@ -2012,10 +2108,13 @@ static int count_active(skg_ctx *c, long cap) {
 static void enqueue_lists(skg_ctx *c, long cap, int nsteps, int micro) {
 	const float mabs = 1e-6f * c->P.fPeriod[0] * (c->P.bPeriodic ? 1.f : 0.f);
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
-	k_cover_lists<<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap,
-			c->te.lo, c->te.hi, c->te.nLeaf2, c->nBox > 1 ? c->m_box : nullptr, c->d_bt, micro, 1.02f * nsteps,
-			mabs, c->P.bPeriodic, C[0], C[1], C[2], L[0], L[1], L[2], c->tlCap, c->tl_list, c->tl_cnt,
-			c->d_ml + 1);
+#define LL(E_) k_cover_lists<E_><<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap,    \
+			c->te.lo, c->te.hi, c->te.nLeaf2, c->nBox > 1 ? c->m_box : nullptr, c->d_bt, micro, 1.02f * nsteps,   \
+			mabs, c->P.bPeriodic, C[0], C[1], C[2], L[0], L[1], L[2], c->tlCap, c->tl_list, c->tl_cnt,          \
+			c->d_ml + 1, c->e_pos, c->e_aux, c->te.lstart, c->te.B, c->nE)
+	if (c->useLists == 2) LL(false);
+	else LL(true);
+#undef LL
 }
 
 // nsteps Denmax steps over the active list (at most cap movers).
@@ -2048,15 +2147,16 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 		if (sub && cap <= c->s64Max) k_cover_sub<1, C_, 64><<<nblk(64L * cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>( \
 				c->m_pos, c->m_list, (int)cap, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi,          \
 				c->te.nLeaf2, c->te.B, nullptr, ehit, 0, bthr, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0],     \
-				L[1], L[2], c->te.lstart, c->d_ml + 1, c->d_ml);                                                 \
+				L[1], L[2], c->te.lstart, c->d_ml + 1, c->d_ml, nullptr);                                        \
 		else if (sub) k_cover_sub<1, C_, 16><<<nblk(16L * cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos,  \
 				c->m_list, (int)cap, c->e_pos, c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2,      \
 				c->te.B, nullptr, ehit, 0, bthr, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1],             \
-				L[2], c->te.lstart, c->d_ml + 1, c->d_ml);                                                       \
+				L[2], c->te.lstart, c->d_ml + 1, c->d_ml, nullptr);                                              \
 		else k_cover<1, C_><<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap,    \
 				c->e_pos, c->e_aux, c->e_org, c->e_code, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, nullptr, \
 				ehit, 0, bthr, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart,           \
-				lists ? c->tl_list : nullptr, lists ? c->tl_cnt : nullptr, c->tlCap, c->d_ml + 1, c->d_ml)
+				lists ? c->tl_list : nullptr, lists ? c->tl_cnt : nullptr, c->tlCap, c->d_ml + 1, c->d_ml,  \
+				c->useLists != 2, c->d_eblk, nullptr)
 		if (comp) { MV(true); } else { MV(false); }
 #undef MV
 		if (pslot >= 0) skg_prof_end(c, pslot, SKG_K_COVER_MOVE, 24.0 * c->nActive + 24.0 * c->nLive);
@@ -2140,7 +2240,7 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	(void)hipSetDevice(c->dev);
 	int rc, n = c->n;
 	int *flag = nullptr, *off = nullptr;
-	if (dalloc(&flag, n) || dalloc(&off, n)) return SKG_E_NOMEM;
+	if (sget(c, 4, &flag, n) || sget(c, 5, &off, n)) return SKG_E_NOMEM;
 	k_mover_flags<<<nblk(n, 256), 256, 0, c->st>>>(n, c->d_type, c->inType, c->d_rho, c->d_mass,
 			c->d_temp, c->P.fDensMin, c->P.fTempMax, c->P.fMassMax, c->P.bGasAndDark, c->P.bGasOnly, flag);
 	if ((rc = skg_exclusive_scan(c, flag, off, n))) return rc;
@@ -2173,7 +2273,6 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	ml_set(c, 1, nm);
 	if ((rc = count_active(c, nm))) return rc;
 	SKG_HIP(hipStreamSynchronize(c->st));
-	dfree(&flag); dfree(&off);
 	if (nMove) *nMove = nm;
 	c->stage = 3;
 	return SKG_OK;
@@ -2189,11 +2288,8 @@ static int build_sorted_cut(skg_ctx *c) {
 	unsigned long long *k0 = nullptr, *k1 = nullptr;
 	int *v0 = nullptr, *v1 = nullptr, rc;
 	const DevTree *t = &c->te;
-	auto done = [&](int status) {
-		dfree(&k0); dfree(&k1); dfree(&v0); dfree(&v1);
-		return status;
-	};
-	if (dalloc(&k0, ne) || dalloc(&k1, ne) || dalloc(&v0, ne) || dalloc(&v1, ne) || dalloc(&c->cs_perm, ne) ||
+	auto done = [&](int status) { return status; };
+	if (sget(c, 6, &k0, ne) || sget(c, 7, &k1, ne) || sget(c, 8, &v0, ne) || sget(c, 9, &v1, ne) || dalloc(&c->cs_perm, ne) ||
 	    dalloc(&c->cs_rho, ne) || dalloc(&c->cs_off, nBox + 1) || dalloc(&c->cs_pos, nBox) ||
 	    dalloc(&c->e_leaf, ne) || dalloc(&c->leafDirty, t->nLeaf2))
 		return done(SKG_E_NOMEM);
@@ -2313,12 +2409,11 @@ extern "C" int skg_get_moved(skg_ctx *c, float *r) {
 	if (c->stage < 3) return SKG_E_STATE;
 	(void)hipSetDevice(c->dev);
 	float *d = nullptr;
-	if (dalloc(&d, 3 * (size_t)c->n)) return SKG_E_NOMEM;
+	if (sget(c, 0, &d, 3 * (size_t)c->n)) return SKG_E_NOMEM;
 	k_fill_f<<<nblk(3 * c->n, 256), 256, 0, c->st>>>(d, 3 * c->n, NAN);
 	if (c->nMove > 0) k_scatter_moved<<<nblk(c->nMove, 256), 256, 0, c->st>>>(c->m_ord, c->nMove, c->m_pos, d);
 	SKG_HIP(hipMemcpyAsync(r, d, 12 * (size_t)c->n, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
-	dfree(&d);
 	return SKG_OK;
 }
 
@@ -2330,7 +2425,7 @@ extern "C" int skg_field_gradient(skg_ctx *c, int nq, const float *qpos, float *
 	(void)hipSetDevice(c->dev);
 	float4 *q4 = nullptr;
 	float *out = nullptr;
-	if (dalloc(&q4, nq) || dalloc(&out, 3 * (size_t)nq)) return SKG_E_NOMEM;
+	if (sget(c, 0, &q4, nq) || sget(c, 1, &out, 3 * (size_t)nq)) return SKG_E_NOMEM;
 	std::vector<float4> h(nq);
 	for (int i = 0; i < nq; ++i) h[i] = make_float4(qpos[3 * i], qpos[3 * i + 1], qpos[3 * i + 2], 0.f);
 	SKG_HIP(hipMemcpyAsync(q4, h.data(), 16 * (size_t)nq, hipMemcpyHostToDevice, c->st));
@@ -2338,7 +2433,6 @@ extern "C" int skg_field_gradient(skg_ctx *c, int nq, const float *qpos, float *
 	if (rc) return rc;
 	SKG_HIP(hipMemcpyAsync(acc, out, 12 * (size_t)nq, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));
-	dfree(&q4); dfree(&out);
 	return SKG_OK;
 }
 

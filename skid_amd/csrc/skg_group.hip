@@ -1539,8 +1539,9 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		int *dgs = nullptr, *dgm = nullptr;
 		double *part = nullptr, *cmv = nullptr;
 		unsigned long long *npair = nullptr;
-		if (dalloc(&dt, std::max<size_t>(1, tiles.size())) || dalloc(&dgs, std::max<size_t>(1, gsmall.size())) ||
-		    dalloc(&dgm, std::max<size_t>(1, gmed.size())) || dalloc(&cmv, 8) || dalloc(&npair, 3))
+		// temporaries in the context's scratch slots (kept between calls, freed with the context)
+		if (sget(c, 0, &dt, std::max<size_t>(1, tiles.size())) || sget(c, 1, &dgs, std::max<size_t>(1, gsmall.size())) ||
+		    sget(c, 2, &dgm, std::max<size_t>(1, gmed.size())) || sget(c, 3, &cmv, 8) || sget(c, 4, &npair, 3))
 			return SKG_E_NOMEM;
 		SKG_HIP(hipMemsetAsync(npair, 0, 24, c->st));
 		if (!tiles.empty()) SKG_HIP(hipMemcpyAsync(dt, tiles.data(), 8 * tiles.size(), hipMemcpyHostToDevice, c->st));
@@ -1587,10 +1588,10 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 			float4 *zp = nullptr, *zs = nullptr;
 			float *ze = nullptr, *zes = nullptr;
 			int *perm = nullptr, *zb = nullptr, *zbs = nullptr;
-			DevTree t;
+			DevTree &t = c->tfof;                  // FOF is done: its tree's arrays serve the scoop tree
 			const bool batch = c->nBox > 1;
-			if (dalloc(&zp, n0) || dalloc(&zs, n0) || dalloc(&perm, n0) || dalloc(&ze, n0) || dalloc(&zes, n0) ||
-			    dalloc(&zb, n0) || dalloc(&zbs, n0))
+			if (sget(c, 6, &zp, n0) || sget(c, 7, &zs, n0) || sget(c, 8, &perm, n0) || sget(c, 9, &ze, n0) ||
+			    sget(c, 10, &zes, n0) || sget(c, 11, &zb, n0) || sget(c, 12, &zbs, n0))
 				return SKG_E_NOMEM;
 			k_gather_pos_members<<<nblk(n0, 256), 256, 0, c->st>>>(G->mem + hs[0], n0, c->d_pos, c->d_mass,
 					c->d_soft, c->d_boxOf, zp, ze, zb);
@@ -1601,9 +1602,6 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 			k_unb_scoop<<<nG - 1, 64, 0, c->st>>>(G->gstart, G->gend, nG, G->cen, G->rel, zs, zes, n0, t.lo, t.hi,
 					t.nLeaf2, t.B, c->P.bPeriodic ? L[0] : FLT_MAX, c->P.bPeriodic, sc2, G->uq, G->uv, G->uphi,
 					c->P.G, c->P.iSoftType, nMax, batch ? zbs : nullptr, G->mem, c->d_boxOf, c->d_bt);
-			SKG_HIP(hipStreamSynchronize(c->st));
-			dfree(&zp); dfree(&zs); dfree(&perm); dfree(&ze); dfree(&zes); dfree(&zb); dfree(&zbs);
-			skg_free_tree(&t);
 		}
 		double pairs = 0;
 		for (int gg : gmed) pairs += (double)(he[gg] - hs[gg]) * (he[gg] - hs[gg] - 1);
@@ -1652,8 +1650,8 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 			for (int i = 0, acc = 0; i < nb; ++i) { w0[i] = acc; acc += wn[i]; for (int j = 0; j < wn[i]; ++j) wmap.push_back(i); }
 			int *dbg = nullptr, *dw0 = nullptr, *dwn = nullptr, *dwm = nullptr, *dact = nullptr;
 			double *cmAll = nullptr;
-			if (dalloc(&dbg, nb) || dalloc(&dw0, nb) || dalloc(&dwn, nb) || dalloc(&dwm, nwgAll) || dalloc(&dact, 1) ||
-			    dalloc(&cmAll, 8 * (size_t)nb) || dalloc(&part, 8 * (size_t)nwgAll))
+			if (sget(c, 13, &dbg, nb) || sget(c, 14, &dw0, nb) || sget(c, 15, &dwn, nb) || sget(c, 16, &dwm, nwgAll) ||
+			    sget(c, 17, &dact, 1) || sget(c, 18, &cmAll, 8 * (size_t)nb) || sget(c, 5, &part, 8 * (size_t)nwgAll))
 				return SKG_E_NOMEM;
 			SKG_HIP(hipMemcpyAsync(dbg, gbig.data(), 4 * nb, hipMemcpyHostToDevice, c->st));
 			SKG_HIP(hipMemcpyAsync(dw0, w0.data(), 4 * nb, hipMemcpyHostToDevice, c->st));
@@ -1692,7 +1690,7 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 				int *dcw = nullptr;
 				CoopCand *pubAll = nullptr;
 				CoopHdr *hdrAll = nullptr;
-				if (dalloc(&dcw, nb) || dalloc(&pubAll, (size_t)nb * COOP_T * CPC) || dalloc(&hdrAll, nwgAll))
+				if (sget(c, 19, &dcw, nb) || sget(c, 20, &pubAll, (size_t)nb * COOP_T * CPC) || sget(c, 21, &hdrAll, nwgAll))
 					return SKG_E_NOMEM;
 				SKG_HIP(hipMemcpyAsync(dcw, cwv.data(), 4 * nb, hipMemcpyHostToDevice, c->st));
 				void *args[] = {&dbg, &dw0, &dwn, &dwm, &dcw, &G->gstart, &G->gend, &G->uq, &G->uv, &G->uphi,
@@ -1701,29 +1699,28 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 				const hipError_t le = hipLaunchCooperativeKernel((const void *)k_unb_coopc, dim3(nwgAll), dim3(COOP_T),
 						args, 0, c->st);
 				if (le != hipSuccess) fallback(le);
-				SKG_HIP(hipStreamSynchronize(c->st));
-				dfree(&dcw); dfree(&pubAll); dfree(&hdrAll);
 			}
-			SKG_HIP(hipStreamSynchronize(c->st));
-			dfree(&dbg); dfree(&dw0); dfree(&dwn); dfree(&dwm); dfree(&dact); dfree(&cmAll);
 		}
 		SKG_HIP(hipGetLastError());
 		SKG_HIP(hipMemcpyAsync(&hp[1], npair + 1, 8, hipMemcpyDeviceToHost, c->st));
 		SKG_HIP(hipStreamSynchronize(c->st));
 		skg_prof_end(c, bslot, SKG_K_UNB_BIG, 20.0 * (double)hp[1]);
 		c->nBigGroups = (int)gbig.size();
-		SKG_HIP(hipMemcpy(&hp[2], npair + 2, 8, hipMemcpyDeviceToHost));
+		SKG_HIP(hipMemcpyAsync(&hp[2], npair + 2, 8, hipMemcpyDeviceToHost, c->st));
+		SKG_HIP(hipStreamSynchronize(c->st));
 		c->nFullPasses = (long long)hp[2];
-		dfree(&dt); dfree(&dgs); dfree(&dgm); dfree(&part); dfree(&cmv); dfree(&npair);
 	}
-	int nu = 0;
-	SKG_HIP(hipMemcpy(&nu, G->d_nunb, 4, hipMemcpyDeviceToHost));
+	// read on the context's stream: its (non-blocking) work is not ordered
+	// with the null stream
+	int nu = 0, c0 = 0;
+	SKG_HIP(hipMemcpyAsync(&nu, G->d_nunb, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipMemcpyAsync(&c0, G->count, 4, hipMemcpyDeviceToHost, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
 	G->unbound = nu;
 	// group 0 gains the unbound members
-	int c0 = 0;
-	SKG_HIP(hipMemcpy(&c0, G->count, 4, hipMemcpyDeviceToHost));
 	c0 += nu;
-	SKG_HIP(hipMemcpy(G->count, &c0, 4, hipMemcpyHostToDevice));
+	SKG_HIP(hipMemcpyAsync(G->count, &c0, 4, hipMemcpyHostToDevice, c->st));
+	SKG_HIP(hipStreamSynchronize(c->st));
 	skg_timer_stop(c, SKG_T_UNBIND);
 	if (nUnbound) *nUnbound = nu;
 	c->stage = 8;
@@ -1745,7 +1742,7 @@ extern "C" int skg_too_small(skg_ctx *c) {
 	GroupState *G = c->gs;
 	const int nG = G->nG, n = c->n;
 	int rc, *keep = nullptr, *off = nullptr;
-	if (dalloc(&keep, nG) || dalloc(&off, nG)) return SKG_E_NOMEM;
+	if (sget(c, 0, &keep, nG) || sget(c, 1, &off, nG)) return SKG_E_NOMEM;
 	k_keep_flags<<<nblk(nG, 256), 256, 0, c->st>>>(G->count, nG, c->P.nMembers, keep);
 	if ((rc = skg_exclusive_scan(c, keep, off, nG))) return rc;
 	int last[2];
@@ -1772,7 +1769,7 @@ extern "C" int skg_too_small(skg_ctx *c) {
 	SKG_HIP(hipStreamSynchronize(c->st));
 	std::swap(G->mass, m2); std::swap(G->rel, r2); std::swap(G->cen, c2); std::swap(G->vcm, v2);
 	std::swap(G->rbound, b2);
-	dfree(&m2); dfree(&r2); dfree(&c2); dfree(&v2); dfree(&b2); dfree(&keep); dfree(&off);
+	dfree(&m2); dfree(&r2); dfree(&c2); dfree(&v2); dfree(&b2);
 	G->nG = nN;
 	c->nGroup = nN;
 	dfree(&G->gstart); dfree(&G->gend); dfree(&G->count);

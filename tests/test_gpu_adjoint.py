@@ -211,6 +211,15 @@ def test_density_dlnk_central_differences():
     err = np.abs(fd - d) / r0
     print(f"d rho/d ln k: max |FD - GPU|/rho = {err.max():.2e}; d ln rho/d ln k median {np.median(d / r0):.3f}")
     assert err.max() <= 1e-5
+    # once asked for, the next skg_density forms both halves in its own
+    # kernels (kNN epilogue, scatter cover) and skg_density_dlnk combines them
+    b2b, rhob, _ = ctx.density()
+    assert np.array_equal(b2b, b2) and np.array_equal(rhob, rho)
+    d2 = ctx.density_dlnk()
+    err2 = np.abs(fd - d2) / r0
+    print(f"fused: max |FD - GPU|/rho = {err2.max():.2e}; max |fused - separate|/rho = {np.max(np.abs(d2 - d) / r0):.2e}")
+    assert err2.max() <= 1e-5
+    assert np.max(np.abs(d2 - d) / r0) <= 1e-6
     # densities fall as the smoothing volume grows: the sensitivity is
     # negative for most particles
     assert np.median(d / r0) < 0
