@@ -22,8 +22,14 @@ static const unsigned long long KEY_SENT = 0x7F800000FFFFFFFFull;   // d2 = +inf
 // =====================================================================
 // scratch, timers, sorting
 // =====================================================================
-static std::mutex g_mem_mu;
-static std::unordered_map<void *, size_t> g_mem_live;
+// The tracker lives on the heap and is never destroyed: a context released
+// late in process exit (after this library's static destructors) still finds
+// it intact.
+static std::mutex &mem_mu() { static std::mutex *m = new std::mutex; return *m; }
+static std::unordered_map<void *, size_t> &mem_live() {
+	static std::unordered_map<void *, size_t> *m = new std::unordered_map<void *, size_t>;
+	return *m;
+}
 static long long g_mem_cur = 0, g_mem_peak = 0;
 
 // An array that already holds an allocation keeps it when that is large
@@ -31,14 +37,14 @@ static long long g_mem_cur = 0, g_mem_peak = 0;
 // arrays on every call, and a hipFree synchronises the device.
 int skg_dmalloc(void **p, size_t bytes) {
 	if (*p) {
-		std::lock_guard<std::mutex> lk(g_mem_mu);
-		auto it = g_mem_live.find(*p);
-		if (it != g_mem_live.end() && it->second >= bytes && it->second <= 2 * bytes + ((size_t)64 << 20)) return 0;
+		std::lock_guard<std::mutex> lk(mem_mu());
+		auto it = mem_live().find(*p);
+		if (it != mem_live().end() && it->second >= bytes && it->second <= 2 * bytes + ((size_t)64 << 20)) return 0;
 	}
 	skg_dfree(p);
 	if (hipMalloc(p, bytes) != hipSuccess) { *p = nullptr; return SKG_E_NOMEM; }
-	std::lock_guard<std::mutex> lk(g_mem_mu);
-	g_mem_live[*p] = bytes;
+	std::lock_guard<std::mutex> lk(mem_mu());
+	mem_live()[*p] = bytes;
 	g_mem_cur += (long long)bytes;
 	g_mem_peak = std::max(g_mem_peak, g_mem_cur);
 	return 0;
@@ -63,16 +69,16 @@ int skg_slot(skg_ctx *c, int s, void **p, size_t bytes) {
 void skg_dfree(void **p) {
 	if (!*p) return;
 	{
-		std::lock_guard<std::mutex> lk(g_mem_mu);
-		auto it = g_mem_live.find(*p);
-		if (it != g_mem_live.end()) { g_mem_cur -= (long long)it->second; g_mem_live.erase(it); }
+		std::lock_guard<std::mutex> lk(mem_mu());
+		auto it = mem_live().find(*p);
+		if (it != mem_live().end()) { g_mem_cur -= (long long)it->second; mem_live().erase(it); }
 	}
 	(void)hipFree(*p);
 	*p = nullptr;
 }
 
 extern "C" int skg_device_memory(long long *current, long long *peak, int reset_peak) {
-	std::lock_guard<std::mutex> lk(g_mem_mu);
+	std::lock_guard<std::mutex> lk(mem_mu());
 	if (current) *current = g_mem_cur;
 	if (peak) *peak = g_mem_peak;
 	if (reset_peak) g_mem_peak = g_mem_cur;
@@ -665,9 +671,14 @@ __global__ __launch_bounds__(64) void k_knn(
 	// One candidate (position cc, iOrder oj) against the lane's query.
 	auto try_cand = [&](const float4 cc, unsigned oj) {
 		if (!BATCH || oj - boff < bn) {
-			const float dx = img_delta<PER>(q.x, qxm, qxp, cc.x, hL);
-			const float dy = img_delta<PER>(q.y, qym, qyp, cc.y, hL);
-			const float dz = img_delta<PER>(q.z, qzm, qzp, cc.z, hL);
+			float dx = fsub(q.x, cc.x), dy = fsub(q.y, cc.y), dz = fsub(q.z, cc.z);
+			// the query-side image only where a delta exceeds half a period (the
+			// same values img_delta gives; most wavefronts never take it)
+			if (PER && fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz)) > hL) {
+				dx = img_delta<PER>(q.x, qxm, qxp, cc.x, hL);
+				dy = img_delta<PER>(q.y, qym, qyp, cc.y, hL);
+				dz = img_delta<PER>(q.z, qzm, qzp, cc.z, hL);
+			}
 			const unsigned long long key = ((unsigned long long)__float_as_uint(sq3(dx, dy, dz)) << 32) | oj;
 			if (key < top) top = heap_replace<KMAX>(heap, lane, k, key);
 		}
@@ -776,15 +787,13 @@ __global__ __launch_bounds__(64) void k_knn(
 // reference's smx->pp).  MODE 0 accumulates the scatter half of the density,
 // MODE 1 the density gradient and then moves the point (kdMoveParticles).
 // =====================================================================
-// Test the cnt entities from start against the lane's point (lanes with want)
-// and accumulate: MODE 0 the scatter density, MODE 1 the gradient.  For
-// ScatterCut, MODE 1 either stamps each hit entity (ehit) or keeps the lane's
-// smallest positive density among its hits (hmin; during the move loop the
-// record's w holds the entity's density).  Records are read four at a time
-// (one wait each).
 // One entity e (record A, X, image code K) against the lane's point when want.
-// MODE 0 with dl also sums the scatter half of d rho / d ln nSmooth into s1
-// (exact separations; the factor -1/(6 pi) is applied by the caller).
+// MODE 0 accumulates the scatter density (with dl also the scatter half of
+// d rho / d ln nSmooth into s1, exact separations, the factor -1/(6 pi)
+// applied by the caller); MODE 1 the gradient, and for ScatterCut either
+// stamps the hit entity (ehit) or keeps the lane's smallest positive density
+// among its hits (hmin; during the move loop the record's w holds the
+// entity's density).
 template <int MODE, bool COMP>
 __device__ __forceinline__ void cover_hit(int e, const float4 &A, const float4 &X, int K, bool want,
 		const float4 &p, const float4 *__restrict__ eorg, int *__restrict__ ehit, int stamp, float &hmin,
@@ -819,6 +828,8 @@ __device__ __forceinline__ void cover_hit(int e, const float4 &A, const float4 &
 	}
 }
 
+// The cnt entities from start against the lane's point (lanes with want),
+// through cover_hit.  Records are read four at a time (one wait each).
 template <int MODE, bool COMP>
 __device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const float4 &p,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
@@ -935,7 +946,8 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__rest
 					for (int u = 0; u < 4; ++u) {
 						if (t0 + u >= cnt) break;
 						if (X4[u].x < 0.f) continue;          // cut (or filler): never hit again
-						const float r = __fsqrt_rn(X4[u].x) + margin;
+						// hardware square root (about 1 ulp): covered by the factor below
+						const float r = __builtin_amdgcn_sqrtf(X4[u].x) + margin;
 						const float dx = fsub(A4[u].x, p.x), dy = fsub(A4[u].y, p.y), dz = fsub(A4[u].z, p.z);
 						const bool hit = need && sq3(dx, dy, dz) < r * r * 1.000001f;
 						if (__any(hit)) {
@@ -996,10 +1008,12 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		const int *L = tlist + (size_t)tile * tcap;
 		int elo = 0, ehi = 0x7FFFFFFF;
 		if (pbox) { elo = eblk[pb] * 256; ehi = eblk[pb + 1] * 256; }
+		// indices and records four at a time (tcap is a multiple of 4, so a
+		// group never reads past the tile's list)
 		for (int k = 0; k < nl; k += 4) {
 			int E4[4];
 #pragma unroll
-			for (int u = 0; u < 4; ++u) E4[u] = L[k + u];      // tcap is a multiple of 4
+			for (int u = 0; u < 4; ++u) E4[u] = L[k + u];
 #pragma unroll
 			for (int u = 0; u < 4; ++u) if (k + u >= nl) E4[u] = E4[0];
 			float4 A4[4], X4[4];

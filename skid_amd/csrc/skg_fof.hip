@@ -106,11 +106,18 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restri
 					for (int j = max(first, i + 1); j < last; ++j) {
 						if (fbox && fbox[j] != qb) continue;
 						const float4 c = fp[j];
-						float dx = img_delta<PER>(q.x, qxm, qxp, c.x, hL);
-						float dy = img_delta<PER>(q.y, qym, qyp, c.y, hL);
-						float dz = img_delta<PER>(q.z, qzm, qzp, c.z, hL);
+						float dx = fsub(q.x, c.x), dy = fsub(q.y, c.y), dz = fsub(q.z, c.z);
+						// images only where a delta exceeds half a period (the values
+						// img_delta gives); otherwise the pair seen from j is the same
+						// d2 (c - q = -(q - c) exactly), so only then is it tested again
+						const bool far = PER && fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz)) > hL;
+						if (far) {
+							dx = img_delta<PER>(q.x, qxm, qxp, c.x, hL);
+							dy = img_delta<PER>(q.y, qym, qyp, c.y, hL);
+							dz = img_delta<PER>(q.z, qzm, qzp, c.z, hL);
+						}
 						bool link = sq3(dx, dy, dz) < tau2;
-						if (PER && !link) {
+						if (far && !link) {
 							// the pair seen from j: shift applied on j's side
 							float ex = img_delta<PER>(c.x, fsub(c.x, L), fadd(c.x, L), q.x, hL);
 							float ey = img_delta<PER>(c.y, fsub(c.y, L), fadd(c.y, L), q.y, hL);
