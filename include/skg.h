@@ -88,6 +88,15 @@ const char *skg_last_error(void);
    sizing runs against one GPU's HBM. */
 int skg_device_memory(long long *current, long long *peak, int reset_peak);
 
+/* Page-locked host memory (hipHostMalloc) for the host arrays the stage calls
+   read and write: their copies then run as DMA at the link's rate, with no
+   staging through a bounce buffer and no page faults on first touch.  Any host
+   pointer works for those calls; these are for callers that repeat a pass and
+   keep their output arrays.  No reference counterpart (SKID's arrays are
+   malloc'd host memory, kd.c:16-28). */
+int skg_host_alloc(void **p, long long bytes);
+int skg_host_free(void *p);
+
 /* Particle input in tipsy species order, n = nGas + nDark + nStar; replaces the
    array fill of kdReadTipsy (kd.c:122-222).  pos/vel are n*3 floats; vel, temp
    may be NULL. */

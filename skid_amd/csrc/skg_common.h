@@ -147,6 +147,7 @@ struct skg_ctx {
 	// device-resident move loop: ml[0] step stamp, ml[1] active, ml[2] next
 	int *d_ml = nullptr;
 	int *m_flag = nullptr, *m_bcnt = nullptr;   // prune flags; block counts then offsets
+	int *m_wrap = nullptr;                       // per mover: wrapped across a periodic face since its tile's list was built
 	long lastSort = 0;                          // active count at the last mover sort
 	// ScatterCut over density-sorted entities (after the initial step): each
 	// box's live entities in ascending density, its first still-live position

@@ -15,6 +15,7 @@
 #include <vector>
 #include <chrono>
 #include <mutex>
+#include <type_traits>
 #include <unordered_map>
 
 static const unsigned long long KEY_SENT = 0x7F800000FFFFFFFFull;   // d2 = +inf, index = max
@@ -82,6 +83,20 @@ extern "C" int skg_device_memory(long long *current, long long *peak, int reset_
 	if (current) *current = g_mem_cur;
 	if (peak) *peak = g_mem_peak;
 	if (reset_peak) g_mem_peak = g_mem_cur;
+	return SKG_OK;
+}
+
+extern "C" int skg_host_alloc(void **p, long long bytes) {
+	if (!p || bytes < 0) return SKG_E_PARAM;
+	*p = nullptr;
+	if (bytes == 0) return SKG_OK;
+	const hipError_t e = hipHostMalloc(p, (size_t)bytes, hipHostMallocDefault);
+	if (e != hipSuccess) { *p = nullptr; return e == hipErrorNoDevice ? SKG_E_NODEVICE : SKG_E_NOMEM; }
+	return SKG_OK;
+}
+
+extern "C" int skg_host_free(void *p) {
+	if (p) (void)hipHostFree(p);
 	return SKG_OK;
 }
 
@@ -793,7 +808,7 @@ __global__ __launch_bounds__(64) void k_knn(
 // applied by the caller); MODE 1 the gradient, and for ScatterCut either
 // stamps the hit entity (ehit) or keeps the lane's smallest positive density
 // among its hits (hmin; during the move loop the record's w holds the
-// entity's density).
+// entity's density, +inf for 0).
 template <int MODE, bool COMP>
 __device__ __forceinline__ void cover_hit(int e, const float4 &A, const float4 &X, int K, bool want,
 		const float4 &p, const float4 *__restrict__ eorg, int *__restrict__ ehit, int stamp, float &hmin,
@@ -823,7 +838,7 @@ __device__ __forceinline__ void cover_hit(int e, const float4 &A, const float4 &
 			const float f = COMP ? grad_factor_f32(sq3(dx, dy, dz), X.y, X.z) : grad_factor(d2, X.y, X.z);
 			s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
 			if (ehit) ehit[e] = stamp;
-			else hmin = fminf(hmin, X.w > 0.f ? X.w : INFINITY);
+			else hmin = X.w < hmin ? X.w : hmin;     // w is +inf for a density of 0
 		}
 	}
 }
@@ -886,8 +901,11 @@ __device__ __forceinline__ void box_min_atomic(float hmin, bool valid, int pb, i
 // movers) is unchanged, the leaves any step can need are those whose ball
 // reach, widened by the distance a mover can still travel (margin), holds a
 // lane's point now.  One walk per block then replaces one walk per step.  A
-// tile with a point within the margin of a periodic face (it may wrap to the
-// far side) or with more than cap leaves gets count -1 and walks the tree.
+// tile with more than cap entries gets count -1 and walks the tree.  Near a
+// periodic face the listed entities include the replicas there; a mover that
+// crosses the face during the block jumps to the far side, which its list
+// does not cover, so the move step flags it (mwrap, cleared here) and its
+// wavefront walks the tree for the rest of the block.
 //
 // ELIST: the list holds entities instead of leaves -- every live entity whose
 // ball, widened by the margin, holds a lane's point now, in tree order (leaf
@@ -901,7 +919,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__rest
 		float mabs, int per, float cx, float cy, float cz, float Lx, float Ly, float Lz, int cap,
 		int *__restrict__ tlist, int *__restrict__ tcnt, const int *__restrict__ npDev,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const int *__restrict__ lstart, int B,
-		int ne) {
+		int ne, int *__restrict__ mwrap) {
 	if (npDev) np = *npDev;
 	const int lane = threadIdx.x & 63;
 	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
@@ -914,11 +932,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__rest
 	const int pb = pbox ? pbox[pi] : 0;
 	// each lane's margin: msteps (1.02 x the block's steps) of its own box's step
 	const float margin = fmaf(msteps, micro ? bt[pb].fMicro : bt[pb].fStep, mabs);
-	bool nearFace = false;
-	if (per && valid)
-		nearFace = fabsf(p.x - cx) > 0.5f * Lx - margin || fabsf(p.y - cy) > 0.5f * Ly - margin ||
-				fabsf(p.z - cz) > 0.5f * Lz - margin;
-	if (__any(nearFace)) { if (lane == 0) tcnt[tile] = -1; return; }
+	if (valid && mwrap) mwrap[pi] = 0;
 	int n = 0, node = 1;
 	int *out = tlist + (size_t)tile * cap;
 	while (node) {
@@ -977,9 +991,13 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		int *__restrict__ bthr, const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
 		float cx, float cy, float cz, float Lx, float Ly, float Lz, const int *__restrict__ lstart,
 		const int *__restrict__ tlist, const int *__restrict__ tcnt, int tcap, const int *__restrict__ npDev,
-		const int *__restrict__ stampDev, int elist, const int *__restrict__ eblk, float *__restrict__ out_dl) {
+		const int *__restrict__ stampDev, int elist, const int *__restrict__ eblk, float *__restrict__ out_dl,
+		int *__restrict__ mwrap) {
 	if (npDev) np = *npDev;                       // device-resident active count and step stamp
 	if (stampDev) stamp = *stampDev;
+	// staging slots for the entity-list scan, 64 per wavefront
+	__shared__ float4 s_eA[SKG_TILE_T], s_eX[SKG_TILE_T];
+	__shared__ int s_eK[SKG_TILE_T], s_eE[SKG_TILE_T];
 	const int lane = threadIdx.x & 63;
 	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
 	if (base >= np) return;                       // whole wavefront past the end
@@ -999,35 +1017,64 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 	// only the union of their lanes' paths).  The node index is wave-uniform,
 	// so node bounds and entity records are read through the scalar cache.
 	const int tile = base >> 6;
-	const int nl = tcnt ? uniform_int(tcnt[tile]) : -1;
+	int nl = tcnt ? uniform_int(tcnt[tile]) : -1;
+	// a lane that crossed a periodic face since the list was built is on the
+	// far side, which the list does not cover: the wavefront walks the tree
+	if (nl >= 0 && mwrap && __any(valid && mwrap[pi] != 0)) nl = -1;
 	int node = nl >= 0 ? 0 : 1;
 	if (elist && nl > 0) {
 		// entity list (k_cover_lists<true>): the listed entities in tree
 		// order; a tile that straddles two boxes lists entities of both, and a
-		// lane takes only its own box's range of the entity array
+		// lane takes only its own box's range of the entity array (that test
+		// only in such a tile).  The list is staged 64 entries at a time in
+		// this wavefront's slots of LDS: lane j gathers entry j's records
+		// (one coalesced index read, record reads near one another in tree
+		// order), then every lane reads each staged record at one address, a
+		// broadcast.  The wavefront reads back only what it wrote itself, and
+		// LDS operations of one wavefront complete in order.
 		const int *L = tlist + (size_t)tile * tcap;
 		int elo = 0, ehi = 0x7FFFFFFF;
 		if (pbox) { elo = eblk[pb] * 256; ehi = eblk[pb + 1] * 256; }
-		// indices and records four at a time (tcap is a multiple of 4, so a
-		// group never reads past the tile's list)
-		for (int k = 0; k < nl; k += 4) {
-			int E4[4];
+		const int wb = threadIdx.x & ~63;
+		auto scan_list = [&](auto straddle) {
+			for (int c = 0; c < nl; c += 64) {
+				const int m = min(64, nl - c);
+				if (lane < m) {
+					const int e = L[c + lane];
+					s_eA[threadIdx.x] = epos[e];
+					s_eX[threadIdx.x] = eaux[e];
+					s_eK[threadIdx.x] = COMP ? ecode[e] : 13;
+					s_eE[threadIdx.x] = e;
+				}
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+				for (int j = 0; j < m; j += 2) {
+					float4 A2[2], X2[2];
+					int K2[2], E2[2];
 #pragma unroll
-			for (int u = 0; u < 4; ++u) E4[u] = L[k + u];
+					for (int u = 0; u < 2; ++u) {
+						A2[u] = s_eA[wb + j + u]; X2[u] = s_eX[wb + j + u];
+						K2[u] = s_eK[wb + j + u]; E2[u] = s_eE[wb + j + u];
+					}
 #pragma unroll
-			for (int u = 0; u < 4; ++u) if (k + u >= nl) E4[u] = E4[0];
-			float4 A4[4], X4[4];
-			int K4[4] = {13, 13, 13, 13};
-#pragma unroll
-			for (int u = 0; u < 4; ++u) { A4[u] = epos[E4[u]]; X4[u] = eaux[E4[u]]; if (COMP) K4[u] = ecode[E4[u]]; }
-#pragma unroll
-			for (int u = 0; u < 4; ++u) {
-				if (k + u >= nl) break;
-				const int e = E4[u];
-				const bool mine = valid && e >= elo && e < ehi;
-				cover_hit<MODE, COMP>(e, A4[u], X4[u], K4[u], mine, p, eorg, ehit, stamp, hmin, Lx, Ly, Lz, s0, s1, s2);
+					for (int u = 0; u < 2; ++u) {
+						if (j + u >= m) break;
+						const int e = E2[u];
+						const bool mine = decltype(straddle)::value ? (valid && e >= elo && e < ehi) : valid;
+						cover_hit<MODE, COMP>(e, A2[u], X2[u], K2[u], mine, p, eorg, ehit, stamp, hmin, Lx, Ly, Lz,
+								s0, s1, s2);
+					}
+				}
+				// every lane has read this chunk before the next overwrites it
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 			}
-		}
+		};
+		const int pb0 = uniform_int(pb);
+		if (__any(valid && pb != pb0)) scan_list(std::true_type());
+		else scan_list(std::false_type());
 	}
 	for (int k = 0; !elist && k < nl; ++k) {
 		const int leaf = uniform_int(tlist[(size_t)tile * tcap + k]);
@@ -1073,11 +1120,13 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 	else ai = 0.0f;
 	float r[3] = {fsub(p.x, fmul(ai, ax)), fsub(p.y, fmul(ai, ay)), fsub(p.z, fmul(ai, az))};
 	const float C[3] = {cx, cy, cz}, LL[3] = {Lx, Ly, Lz};
+	bool wrapped = false;
 	for (int j = 0; j < 3; ++j) {
-		if ((double)r[j] > (double)C[j] + 0.5 * (double)LL[j]) r[j] = fsub(r[j], LL[j]);
-		if ((double)r[j] <= (double)C[j] - 0.5 * (double)LL[j]) r[j] = fadd(r[j], LL[j]);
+		if ((double)r[j] > (double)C[j] + 0.5 * (double)LL[j]) { r[j] = fsub(r[j], LL[j]); wrapped = true; }
+		if ((double)r[j] <= (double)C[j] - 0.5 * (double)LL[j]) { r[j] = fadd(r[j], LL[j]); wrapped = true; }
 	}
 	ppos[pi] = make_float4(r[0], r[1], r[2], p.w);
+	if (wrapped && mwrap) mwrap[pi] = 1;
 }
 
 // Sub-wave variant: S lanes per point (64/S points per wavefront), each lane
@@ -1370,7 +1419,7 @@ __global__ __launch_bounds__(256) void k_scat_thr(const int *__restrict__ elive,
 	float v = INFINITY;
 	if (e < ne && elive[e]) {
 		if (ehit[e] == stamp) { float r = erho[e]; if (r > 0.f) v = r; }
-		else if (bInitial) { erho[e] = 0.f; eaux[e].w = 0.f; }
+		else if (bInitial) { erho[e] = 0.f; eaux[e].w = INFINITY; }
 	}
 	v = wave_min(v);
 	if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = v;
@@ -1401,10 +1450,11 @@ __global__ __launch_bounds__(64) void k_box_min(const float *__restrict__ part, 
 // From the move loop on, an entity record's w (the scatter density's kernel
 // normalisation, used only by skg_density, which rebuilds the records) holds
 // the entity's density, which the move step reads with the record for the
-// ScatterCut threshold.
+// ScatterCut threshold (smooth1.c:460-461 takes the smallest positive one), or
+// +inf for a density of 0, so that a hit takes a plain minimum.
 __global__ void k_aux_rho(float4 *eaux, const float *__restrict__ erho, int ne) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < ne) eaux[i].w = erho[i];
+	if (i < ne) eaux[i].w = erho[i] > 0.f ? erho[i] : INFINITY;
 }
 
 // Per-step reset for the move kernel's threshold atomics: every box's slots
@@ -1748,7 +1798,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	skg_free_tree(&c->ts); skg_free_tree(&c->te); skg_free_tree(&c->tfull); skg_free_tree(&c->tfof);
 	dfree(&c->e_b2); dfree(&c->e_pos); dfree(&c->e_aux); dfree(&c->e_org); dfree(&c->e_code); dfree(&c->e_rho); dfree(&c->e_src); dfree(&c->e_live); dfree(&c->e_hit);
 	dfree(&c->e_box);
-	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2); dfree(&c->m_box);
+	dfree(&c->m_pos); dfree(&c->m_old); dfree(&c->m_ord); dfree(&c->m_list); dfree(&c->m_list2); dfree(&c->m_box); dfree(&c->m_wrap);
 	dfree(&c->tl_list); dfree(&c->tl_cnt);
 	skg_drop_graph(c);
 	dfree(&c->d_ml); dfree(&c->m_flag); dfree(&c->m_bcnt);
@@ -1896,7 +1946,7 @@ static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, in
 	k_cover<M_, C_><<<nb, SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, c->e_aux, c->e_org, c->e_code, c->nE, c->te.lo, \
 			c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp, nullptr, c->nBox > 1 ? pbox : nullptr, c->d_bt, micro, \
 			C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart, nullptr, nullptr, c->tlCap, nullptr, nullptr, 0, \
-			nullptr, out_dl)
+			nullptr, out_dl, nullptr)
 #define CS(M_, C_)                                                                             \
 	k_cover_sub<M_, C_, 16><<<nblk(16L * np, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(ppos, plist, np, c->e_pos, \
 			c->e_aux, c->e_org, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, out, c->e_hit, stamp,       \
@@ -2125,7 +2175,7 @@ static void enqueue_lists(skg_ctx *c, long cap, int nsteps, int micro) {
 #define LL(E_) k_cover_lists<E_><<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap,    \
 			c->te.lo, c->te.hi, c->te.nLeaf2, c->nBox > 1 ? c->m_box : nullptr, c->d_bt, micro, 1.02f * nsteps,   \
 			mabs, c->P.bPeriodic, C[0], C[1], C[2], L[0], L[1], L[2], c->tlCap, c->tl_list, c->tl_cnt,          \
-			c->d_ml + 1, c->e_pos, c->e_aux, c->te.lstart, c->te.B, c->nE)
+			c->d_ml + 1, c->e_pos, c->e_aux, c->te.lstart, c->te.B, c->nE, c->m_wrap)
 	if (c->useLists == 2) LL(false);
 	else LL(true);
 #undef LL
@@ -2170,7 +2220,7 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 				c->e_pos, c->e_aux, c->e_org, c->e_code, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, nullptr, \
 				ehit, 0, bthr, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart,           \
 				lists ? c->tl_list : nullptr, lists ? c->tl_cnt : nullptr, c->tlCap, c->d_ml + 1, c->d_ml,  \
-				c->useLists != 2, c->d_eblk, nullptr)
+				c->useLists != 2, c->d_eblk, nullptr, lists ? c->m_wrap : nullptr)
 		if (comp) { MV(true); } else { MV(false); }
 #undef MV
 		if (pslot >= 0) skg_prof_end(c, pslot, SKG_K_COVER_MOVE, 24.0 * c->nActive + 24.0 * c->nLive);
@@ -2268,6 +2318,7 @@ extern "C" int skg_init_move(skg_ctx *c, int *nMove) {
 	drop_graph(c);
 	const size_t nbm = (size_t)nblk(std::max(nm, 1), 256);
 	if (dalloc(&c->m_ord, nm) || dalloc(&c->m_pos, nm) || dalloc(&c->m_old, nm) || dalloc(&c->m_list, nm) ||
+	    dalloc(&c->m_wrap, nm) ||
 	    dalloc(&c->m_list2, nm) || dalloc(&c->m_box, nm) || dalloc(&c->m_flag, nm) || dalloc(&c->m_bcnt, 2 * nbm))
 		return SKG_E_NOMEM;
 	if (!c->d_ml && dalloc(&c->d_ml, 4)) return SKG_E_NOMEM;

@@ -96,3 +96,27 @@ def test_device_memory_is_accounted_and_released():
     assert cur - cur0 >= 40 * n and peak >= cur
     ctx.close()
     assert skg.device_memory()[0] == cur0
+
+
+def test_keep_outputs_pinned_arrays_match():
+    """keep_outputs(): per-pass outputs in page-locked arrays the context
+    reuses; their values equal those of freshly allocated arrays, pass by pass
+    (the second pass forms d rho / d ln k inside the density, as a context does
+    once it has been asked for it)."""
+    box = gen.config1(n=4096, seed=3)
+    P = skg.make_params(0.01, 32, period=1.0)
+    a = skg.Context(box, P)
+    ref = []
+    for _ in range(2):
+        b2, rho, _ = a.density()
+        ref.append((b2, rho, a.density_dlnk()))
+    a.close()
+    c = skg.Context(box, P)
+    c.keep_outputs(True)
+    for b2a, rhoa, dla in ref:
+        b2c, rhoc, _ = c.density()
+        dlc = c.density_dlnk()
+        assert np.array_equal(b2a, b2c) and np.array_equal(rhoa, rhoc) and np.array_equal(dla, dlc)
+    b2d, _, _ = c.density()
+    assert b2d is b2c                     # the same kept array, overwritten in place
+    c.close()
