@@ -935,20 +935,38 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__rest
 	if (valid && mwrap) mwrap[pi] = 0;
 	int n = 0, node = 1;
 	int *out = tlist + (size_t)tile * cap;
+	// entries collect in one VGPR, lane n % 64 holding entry n, and go out as
+	// one coalesced store per 64 (cap is a multiple of 64)
+	int buf = 0;
+	auto emit = [&](int idx) {
+		if (lane == (n & 63)) buf = idx;
+		++n;
+		if ((n & 63) == 0 && n <= cap) out[n - 64 + lane] = buf;
+	};
 	while (node) {
 		node = uniform_int(node);
 		const float4 lo = nlo[node], hi = nhi[node];
 		bool need = false;
+		float c = 0.f;
 		if (valid && lo.w >= 0.f && brange_has(hi.w, pb)) {
-			const float r = __fsqrt_rn(lo.w) + margin;
+			// hardware square root (about 1 ulp): covered by the factor below
+			c = __builtin_amdgcn_sqrtf(lo.w);
+			const float r = c + margin;
 			need = point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < r * r * 1.000001f;
 		}
 		if (__any(need)) {
 			if (node < nLeaf2) { node = 2 * node; continue; }
 			if (!ELIST) {
-				if (lane == 0 && n < cap) out[n] = node - nLeaf2;
-				++n;
+				emit(node - nLeaf2);
 			} else if (n <= cap) {
+				// An entity of ball r is listed when its reach r + margin holds
+				// the lane's point.  With c the leaf's largest live ball radius,
+				// (r + m)^2 <= r^2 (1 + m/c) + m (c + m) for any c > 0 (since
+				// r^2/c + c >= 2r), so one fma per entity replaces a square
+				// root; the bound is exact for the leaf's largest ball.
+				c = fmaxf(c, 1e-30f);
+				const float k1 = fmaf(margin, 1.f / c, 1.f) * 1.000002f;
+				const float k2 = margin * (c + margin) * 1.000002f;
 				const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
 				const int start = uniform_int(rg.x), cnt = uniform_int(rg.y - rg.x);
 				for (int t0 = 0; t0 < cnt; t0 += 4) {
@@ -960,14 +978,9 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__rest
 					for (int u = 0; u < 4; ++u) {
 						if (t0 + u >= cnt) break;
 						if (X4[u].x < 0.f) continue;          // cut (or filler): never hit again
-						// hardware square root (about 1 ulp): covered by the factor below
-						const float r = __builtin_amdgcn_sqrtf(X4[u].x) + margin;
 						const float dx = fsub(A4[u].x, p.x), dy = fsub(A4[u].y, p.y), dz = fsub(A4[u].z, p.z);
-						const bool hit = need && sq3(dx, dy, dz) < r * r * 1.000001f;
-						if (__any(hit)) {
-							if (lane == 0 && n < cap) out[n] = start + t0 + u;
-							++n;
-						}
+						const bool hit = need && sq3(dx, dy, dz) < fmaf(X4[u].x, k1, k2);
+						if (__any(hit)) emit(start + t0 + u);
 					}
 				}
 			}
@@ -975,6 +988,8 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_lists(const float4 *__rest
 		node = (node >> __builtin_ctz(~node)) + 1;
 		if (node == 1) node = 0;
 	}
+	const int rem = n & 63;
+	if (n <= cap && rem && lane < rem) out[n - rem + lane] = buf;
 	if (lane == 0) tcnt[tile] = n <= cap ? n : -1;
 }
 
@@ -2168,12 +2183,14 @@ static int count_active(skg_ctx *c, long cap) {
 }
 
 // Leaf lists for the next nsteps steps (margin: the distance a mover can
-// travel in nsteps steps of its own box's step, a lane at a time).
+// travel before the last of them, nsteps - 1 steps of its own box's step, a
+// lane at a time: step i takes its gradient at the point i moves away).
 static void enqueue_lists(skg_ctx *c, long cap, int nsteps, int micro) {
 	const float mabs = 1e-6f * c->P.fPeriod[0] * (c->P.bPeriodic ? 1.f : 0.f);
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
 #define LL(E_) k_cover_lists<E_><<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap,    \
-			c->te.lo, c->te.hi, c->te.nLeaf2, c->nBox > 1 ? c->m_box : nullptr, c->d_bt, micro, 1.02f * nsteps,   \
+			c->te.lo, c->te.hi, c->te.nLeaf2, c->nBox > 1 ? c->m_box : nullptr, c->d_bt, micro,                    \
+			1.02f * (nsteps - 1),                                                                                \
 			mabs, c->P.bPeriodic, C[0], C[1], C[2], L[0], L[1], L[2], c->tlCap, c->tl_list, c->tl_cnt,          \
 			c->d_ml + 1, c->e_pos, c->e_aux, c->te.lstart, c->te.B, c->nE, c->m_wrap)
 	if (c->useLists == 2) LL(false);
