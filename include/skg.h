@@ -265,7 +265,12 @@ int skg_fof_points_noalloc(skg_ctx *ctx, int n, const float *d_pos, float fTau, 
                         the latter wins becomes the switch point;
      SKG_TUNE_POT_TILE  the unbinding potential's workgroup / LDS tile in
                         {64, 128, 256}, timed on the groups of the next
-                        skg_unbind.
+                        skg_unbind;
+     SKG_TUNE_KNN       the kNN's shape: its tree's leaf size in
+                        {8, 16, 32, 64} (independent of nBucket) and 1 or 2
+                        wavefronts per kNN workgroup (each wavefront with its
+                        own 64 queries and LDS heaps; 2 only for nSmooth <= 64),
+                        timed by the kNN stage of skg_density.
    Needs loaded particles; leaves the context loaded (stage of skg_load), so
    the caller runs the path again from skg_density.  Outputs (any may be
    NULL): the chosen nBucket, switch size and tile (the tile as chosen so
@@ -273,8 +278,14 @@ int skg_fof_points_noalloc(skg_ctx *ctx, int n, const float *d_pos, float fTau, 
 #define SKG_TUNE_BUCKET   1
 #define SKG_TUNE_MAPPING  2
 #define SKG_TUNE_POT_TILE 4
-#define SKG_TUNE_ALL      7
+#define SKG_TUNE_KNN      8
+#define SKG_TUNE_ALL      15
 int skg_autotune(skg_ctx *ctx, int what, int *nBucket, long long *subMax, int *potTile);
+/* The kNN shape in use: its tree's leaf size and wavefronts per kNN
+   workgroup (set by SKG_TUNE_KNN, or SKG_KNN_BUCKET / SKG_KNN_WAVES in the
+   environment at skg_create; default nBucket and 1).  No reference
+   counterpart. */
+int skg_knn_shape(skg_ctx *ctx, int *knnBucket, int *knnWaves);
 
 /* Per-stage device time in ms since skg_create, indexed by SKG_T_*. */
 #define SKG_T_TREE 0

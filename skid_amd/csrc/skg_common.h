@@ -138,6 +138,7 @@ struct skg_ctx {
 	// launch shapes (set at create; SKG_COVER_SUB_MAX overrides): point sets of
 	// at most subMax points use the sub-wave cover kernel
 	long subMax = 0;
+	int knnBucket = 0, knnWaves = 1;             // kNN tree leaf size (0: nBucket), wavefronts per kNN workgroup
 	long s64Max = 0;            // at most this many active movers: 64 lanes (a wavefront) per mover
 	// per-tile lists of the current 5-step block: entities (SKG_TILE_LISTS=1,
 	// default), leaves (2) or none (0: every step walks the tree); tlCap

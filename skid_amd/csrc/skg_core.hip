@@ -654,18 +654,24 @@ __device__ __forceinline__ float key_d2(unsigned long long k) { return __uint_as
 // BATCH: several boxes share the arrays; a lane accepts only candidates whose
 // iOrder lies in its own box's range, and the walk skips subtrees holding none
 // of the block's boxes.
-template <int KMAX, bool PER, bool COMP, bool BATCH>
-__global__ __launch_bounds__(64) void k_knn(
+template <int KMAX, bool PER, bool COMP, bool BATCH, int W>
+__global__ __launch_bounds__(64 * W) void k_knn(
 		const float4 *__restrict__ spos, const int *__restrict__ sord, int n,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		float L, int k, const float *__restrict__ mass_ord, const float *__restrict__ pos_ord,
 		float *__restrict__ ball2_out, float *__restrict__ rhoG, unsigned long long *__restrict__ heap_out,
 		const int *__restrict__ sbox, const DevBox *__restrict__ bt, float *__restrict__ dlnk_out) {
-	__shared__ unsigned long long heap[(KMAX + 1) * 64];   // k slots + the zero sentinel row
-	__shared__ int stk[64];
-	__shared__ float stkd[64];
-	const int lane = threadIdx.x;
-	const int base = xcd_tile(blockIdx.x, gridDim.x) * 64;
+	// W wavefronts per workgroup, each with its own 64 queries, heaps and stack
+	__shared__ unsigned long long heapW[W][(KMAX + 1) * 64];   // k slots + the zero sentinel row
+	__shared__ int stkW[W][64];
+	__shared__ float stkdW[W][64];
+	const int wv = W > 1 ? (int)(threadIdx.x >> 6) : 0;
+	unsigned long long *heap = heapW[wv];
+	int *stk = stkW[wv];
+	float *stkd = stkdW[wv];
+	const int lane = threadIdx.x & 63;
+	const int base = (xcd_tile(blockIdx.x, gridDim.x) * W + wv) * 64;
+	if (base >= n) return;                        // a wavefront past the end (no workgroup barrier here)
 	const int qi = base + lane;
 	const bool valid = qi < n;
 	const float4 q = spos[valid ? qi : base];
@@ -1776,6 +1782,8 @@ extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
 	if (const char *e = getenv("SKG_COVER_SUB_MAX")) c->subMax = atol(e);
 	c->s64Max = 1 << 14;
 	if (const char *e = getenv("SKG_COVER_S64_MAX")) c->s64Max = atol(e);
+	if (const char *e = getenv("SKG_KNN_BUCKET")) c->knnBucket = std::max(0, std::min(1024, atoi(e)));
+	if (const char *e = getenv("SKG_KNN_WAVES")) c->knnWaves = atoi(e) == 2 ? 2 : 1;
 	if (const char *e = getenv("SKG_TILE_LISTS")) c->useLists = atoi(e);
 	c->tlCap = c->useLists == 2 ? 512 : 1024;
 	if (const char *e = getenv("SKG_TILE_CAP")) c->tlCap = std::max(4, atoi(e) & ~3);
@@ -1932,22 +1940,30 @@ static int scatter_species(const skg_ctx *c) {
 	return 0;
 }
 
-template <int KMAX, bool BATCH>
+template <int KMAX, bool BATCH, int W>
 static void launch_knn(skg_ctx *c, bool per, bool comp) {
-	const int nb = nblk(c->nS, 64);
+	const int nb = nblk(c->nS, 64 * W);
 	const float L = per ? c->P.fPeriod[0] : FLT_MAX;
 #define KL(P_, C_)                                                                           \
-	k_knn<KMAX, P_, C_, BATCH><<<nb, 64, 0, c->st>>>(c->s_pos, c->s_ord, c->nS, c->ts.lo, c->ts.hi, \
+	k_knn<KMAX, P_, C_, BATCH, W><<<nb, 64 * W, 0, c->st>>>(c->s_pos, c->s_ord, c->nS, c->ts.lo, c->ts.hi, \
 			c->ts.nLeaf2, c->ts.B, L, c->P.nSmooth, c->d_mass, c->d_pos, c->s_ball2, c->s_rhoG, c->s_heap, \
 			c->s_box, c->d_bt, c->wantDlnk ? c->s_dlnkG : nullptr)
 	if (per) { if (comp) KL(true, true); else KL(true, false); }
 	else { if (comp) KL(false, true); else KL(false, false); }
 #undef KL
 }
+// knnWaves: wavefronts per kNN workgroup (1 or 2; 2 only up to k = 64, where
+// two heaps fit a workgroup's LDS at useful occupancy)
 template <int KMAX>
 static void launch_knn(skg_ctx *c, bool per, bool comp) {
-	if (c->nBox > 1) launch_knn<KMAX, true>(c, per, comp);
-	else launch_knn<KMAX, false>(c, per, comp);
+	const bool two = KMAX <= 64 && c->knnWaves == 2;
+	if (c->nBox > 1) {
+		if (two) launch_knn<KMAX, true, (KMAX <= 64 ? 2 : 1)>(c, per, comp);
+		else launch_knn<KMAX, true, 1>(c, per, comp);
+	} else {
+		if (two) launch_knn<KMAX, false, (KMAX <= 64 ? 2 : 1)>(c, per, comp);
+		else launch_knn<KMAX, false, 1>(c, per, comp);
+	}
 }
 
 static int cover_launch(skg_ctx *c, int mode, float4 *ppos, const int *plist, int np, const int *pbox,
@@ -2022,7 +2038,9 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		// wavefronts' runs of 64 queries are compact (no Morton jumps)
 		if ((rc = skg_hilbert_sort(c, tmp4, nullptr, nS, perm, nBox > 1 ? tmpb : nullptr))) return rc;
 		k_permute_s<<<nblk(nS, 256), 256, 0, c->st>>>(tmp4, tmpo, tmpb, perm, nS, c->s_pos, c->s_ord, c->s_box);
-		if ((rc = skg_build_tree(c, &c->ts, c->s_pos, nS, B, nullptr, nBox > 1 ? c->s_box : nullptr))) return rc;
+		// the kNN tree's own leaf size (knnBucket; 0: nBucket)
+		const int Bk = c->knnBucket > 0 ? c->knnBucket : B;
+		if ((rc = skg_build_tree(c, &c->ts, c->s_pos, nS, Bk, nullptr, nBox > 1 ? c->s_box : nullptr))) return rc;
 		skg_timer_stop(c, SKG_T_TREE);
 
 		// kNN radii + gather half of the density
@@ -2621,6 +2639,28 @@ extern "C" int skg_autotune(skg_ctx *c, int what, int *nBucket, long long *subMa
 	}
 	// 3. unbinding potential tile: timed on the groups of the next skg_unbind
 	if (what & SKG_TUNE_POT_TILE) c->tunePot = 1;
+	// 4. kNN shape: the kNN tree's leaf size and 1 or 2 wavefronts per kNN
+	// workgroup, timed by the tree and kNN stages of skg_density (best of two
+	// runs each, after one untimed run)
+	if (what & SKG_TUNE_KNN) {
+		double best = 1e300;
+		int bestB = c->knnBucket, bestW = c->knnWaves;
+		if ((rc = skg_density(c, nullptr, nullptr, nullptr))) return rc;
+		for (int B : {8, 16, 32, 64})
+			for (int W = 1; W <= (c->P.nSmooth <= 64 ? 2 : 1); ++W) {
+				c->knnBucket = B;
+				c->knnWaves = W;
+				double t = 1e300;
+				for (int r = 0; r < 2; ++r) {
+					const double k0 = c->tms[SKG_T_TREE] + c->tms[SKG_T_KNN];
+					if ((rc = skg_density(c, nullptr, nullptr, nullptr))) return rc;
+					t = std::min(t, c->tms[SKG_T_TREE] + c->tms[SKG_T_KNN] - k0);
+				}
+				if (t < best) { best = t; bestB = B; bestW = W; }
+			}
+		c->knnBucket = bestB;
+		c->knnWaves = bestW;
+	}
 	c->prof = prof;
 	// the caller reruns the path from skg_density with the chosen shapes
 	for (int i = 0; i < SKG_T_N; ++i) c->tms[i] = 0;
@@ -2628,5 +2668,12 @@ extern "C" int skg_autotune(skg_ctx *c, int what, int *nBucket, long long *subMa
 	if (nBucket) *nBucket = c->P.nBucket;
 	if (subMax) *subMax = c->subMax;
 	if (potTile) *potTile = c->potTile;
+	return SKG_OK;
+}
+
+extern "C" int skg_knn_shape(skg_ctx *c, int *knnBucket, int *knnWaves) {
+	if (!c) return SKG_E_PARAM;
+	if (knnBucket) *knnBucket = c->knnBucket > 0 ? c->knnBucket : c->P.nBucket;
+	if (knnWaves) *knnWaves = c->knnWaves;
 	return SKG_OK;
 }

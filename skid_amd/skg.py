@@ -22,7 +22,7 @@ SKG_E_NODEVICE, SKG_E_PARAM, SKG_E_NOMEM, SKG_E_STATE, SKG_E_HIP, SKG_E_NSMOOTH 
 SKG_DARK, SKG_GAS, SKG_STAR = 1, 2, 4
 SKG_PLUMMER, SKG_SPLINE = 1, 2
 SKG_ACC_PLAIN, SKG_ACC_COMPENSATED = 0, 1
-SKG_TUNE_BUCKET, SKG_TUNE_MAPPING, SKG_TUNE_POT_TILE, SKG_TUNE_ALL = 1, 2, 4, 7
+SKG_TUNE_BUCKET, SKG_TUNE_MAPPING, SKG_TUNE_POT_TILE, SKG_TUNE_KNN, SKG_TUNE_ALL = 1, 2, 4, 8, 15
 SKG_NOALLOC_MAX_POINTS, SKG_NOALLOC_SCRATCH_BYTES = 524288, 64 << 20
 STAGES = ("tree", "knn", "density", "move", "fof", "group", "unbind", "spare")
 KERNELS = ("knn", "cover_density", "cover_move", "fof", "unbind_potential", "unbind_loop", "unbind_big")
@@ -87,6 +87,7 @@ def lib():
         L.skg_set_groups.argtypes = [vp, fp, fp, fp]
         L.skg_group_energies.argtypes = [vp, fp, fp]
         L.skg_autotune.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_longlong), C.POINTER(C.c_int)]
+        L.skg_knn_shape.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
         L.skg_fof_points.argtypes = [vp, C.c_int, fp, C.c_float, fp, ip]
         L.skg_fof_points_noalloc.argtypes = [vp, C.c_int, vp, C.c_float, vp, ip]
         L.skg_stage_times.argtypes = [vp, C.POINTER(C.c_double)]
@@ -349,7 +350,10 @@ class Context:
         must then be run again from density()."""
         b, m, t = C.c_int(), C.c_longlong(), C.c_int()
         _check(self.L.skg_autotune(self.h, what, C.byref(b), C.byref(m), C.byref(t)), "skg_autotune")
-        return {"nBucket": b.value, "sub_max": m.value, "pot_tile": t.value}
+        kb, kw = C.c_int(), C.c_int()
+        _check(self.L.skg_knn_shape(self.h, C.byref(kb), C.byref(kw)), "skg_knn_shape")
+        return {"nBucket": b.value, "sub_max": m.value, "pot_tile": t.value, "knn_bucket": kb.value,
+                "knn_waves": kw.value}
 
     def stage_times(self) -> dict:
         t = (C.c_double * 8)()

@@ -98,3 +98,38 @@ def test_edge_cases():
         assert np.array_equal(b2, bb)
     with pytest.raises(skg.SkgError):
         gpu_density(b, 9)
+
+
+def test_knn_shapes_agree():
+    """The kNN's tree leaf size and wavefronts per workgroup (SKG_KNN_BUCKET,
+    SKG_KNN_WAVES: the autotuner's kNN axis) change no result: fBall2, the
+    density, d rho / d ln k and the neighbour lists are identical to the
+    default shape's, on one box (k = 32 and 64) and on a batch of uneven
+    boxes."""
+    import os
+    import subprocess
+    import sys
+    code = (
+        "import sys, numpy as np; sys.path.insert(0, '.');"
+        "from skid_amd import gen, skg;"
+        "k = int(sys.argv[2]);"
+        "sizes = [2500, 9000, 30000];"
+        "P = skg.make_params(0.01, k, period=1.0);"
+        "c1 = skg.Context(gen.config1(n=8192, seed=4), P);"
+        "b2, rho, _ = c1.density(); dl = c1.density_dlnk(); nb, _ = c1.neighbors();"
+        "boxes = [gen.config4_box(n, 50 + i) for i, n in enumerate(sizes)];"
+        "taus = np.array([gen.config4_tau(n) for n in sizes], np.float32);"
+        "bc = skg.BatchContext(boxes, P, taus); bb2, brho, _ = bc.density();"
+        "np.save(sys.argv[1], np.concatenate([b2, rho, dl, nb.reshape(-1).astype(np.float32), bb2, brho]))"
+    )
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    for k in (32, 64):
+        outs = []
+        for bucket, waves in (("0", "1"), ("8", "1"), ("16", "2"), ("64", "2")):
+            env = dict(os.environ)
+            env.update({"SKG_KNN_BUCKET": bucket, "SKG_KNN_WAVES": waves})
+            path = f"/tmp/skg_knn_{k}_{bucket}_{waves}.npy"
+            subprocess.run([sys.executable, "-c", code, path, str(k)], check=True, env=env, timeout=240, cwd=root)
+            outs.append(np.load(path))
+        for o in outs[1:]:
+            assert np.array_equal(outs[0], o), k

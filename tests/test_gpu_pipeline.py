@@ -190,6 +190,7 @@ def test_autotune_keeps_results(box1):
     t = ctx.autotune(skg.SKG_TUNE_ALL)
     print("tuned", t)
     assert t["nBucket"] in (8, 16, 32, 64) and t["sub_max"] >= 0
+    assert t["knn_bucket"] in (8, 16, 32, 64) and t["knn_waves"] in (1, 2)
     b2, _, _ = ctx.density()
     assert np.array_equal(b2, rb2)
     tr, g, ng = ctx.run()
