@@ -620,24 +620,41 @@ int skg_copy_tree_leaves(skg_ctx *c, DevTree *dst, const DevTree *src) {
 // through the lanes as a uniform (scalar-load) candidate tile.
 // =====================================================================
 // Slot k holds 0, below every key, so a node's right child is always
-// readable: both children come in one LDS read (adjacent rows).
+// readable: both children come in one LDS read (rows 512 bytes apart).  The
+// sift-down is unrolled to the heap's depth (no loop bookkeeping; a lane that
+// has placed its key idles through the remaining levels), and the new top is
+// known at the first level -- the larger of the key and the root's larger
+// child -- so no LDS read follows the sift.
 template <int KMAX>
 __device__ __forceinline__ unsigned long long heap_replace(unsigned long long *heap, int lane, int k,
 		unsigned long long key) {
+	constexpr int LEVELS = KMAX <= 32 ? 5 : KMAX <= 64 ? 6 : KMAX <= 128 ? 7 : 8;
+	unsigned long long *h = heap + lane;
 	int i = 0;
-	for (;;) {
+	bool done = false;
+	unsigned long long top = key;
+#pragma unroll
+	for (int d = 0; d < LEVELS; ++d) {
 		const int l = 2 * i + 1;
-		if (l >= k) break;
-		unsigned long long cv = heap[l * 64 + lane];
-		const unsigned long long cr = heap[(l + 1) * 64 + lane];
-		int ci = l;
-		if (cr > cv) { cv = cr; ci = l + 1; }
-		if (cv <= key) break;
-		heap[i * 64 + lane] = cv;
-		i = ci;
+		if (!done && l < k) {
+			const unsigned long long *row = h + l * 64;
+			unsigned long long cv = row[0];
+			const unsigned long long cr = row[64];
+			int ci = l;
+			if (cr > cv) { cv = cr; ci = l + 1; }
+			if (cv <= key) {
+				done = true;
+			} else {
+				h[i * 64] = cv;
+				if (d == 0) top = cv;
+				i = ci;
+			}
+		} else {
+			done = true;
+		}
 	}
-	heap[i * 64 + lane] = key;
-	return heap[lane];
+	h[i * 64] = key;
+	return top;
 }
 
 template <bool PER>
@@ -689,13 +706,16 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 	heap[k * 64 + lane] = 0ull;
 	unsigned long long top = KEY_SENT;
 
-	// One candidate (position cc, iOrder oj) against the lane's query.
-	auto try_cand = [&](const float4 cc, unsigned oj) {
-		if (!BATCH || oj - boff < bn) {
+	// One candidate (position cc, iOrder oj) against the lane's query.  CB:
+	// test that the candidate is of the lane's box; CI: look for a delta over
+	// half a period (a leaf that is all the lane's box, or that no delta of
+	// the wavefront's can take that far, skips the test).
+	auto try_cand = [&](const float4 cc, unsigned oj, auto CB, auto CI) {
+		if (!(BATCH && decltype(CB)::value) || oj - boff < bn) {
 			float dx = fsub(q.x, cc.x), dy = fsub(q.y, cc.y), dz = fsub(q.z, cc.z);
 			// the query-side image only where a delta exceeds half a period (the
 			// same values img_delta gives; most wavefronts never take it)
-			if (PER && fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz)) > hL) {
+			if (PER && decltype(CI)::value && fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz)) > hL) {
 				dx = img_delta<PER>(q.x, qxm, qxp, cc.x, hL);
 				dy = img_delta<PER>(q.y, qym, qyp, cc.y, hL);
 				dz = img_delta<PER>(q.z, qzm, qzp, cc.z, hL);
@@ -708,7 +728,7 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 	// at a time through the scalar cache, one wait per four (the arrays carry
 	// SKG_EPAD entries past the end, so the reads past a range's end stay in
 	// the allocation; their values are never used).  skip: the seeded tile.
-	auto scan = [&](int start, int cnt, bool skip) {
+	auto scan = [&](int start, int cnt, bool skip, auto CB, auto CI) {
 		for (int t0 = 0; t0 < cnt; t0 += 4) {
 			float4 C4[4];
 			unsigned O4[4];
@@ -719,14 +739,16 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 				if (t0 + u >= cnt) break;
 				const int j = start + t0 + u;
 				if (skip && j >= base && j < base + 64) continue;
-				try_cand(C4[u], O4[u]);
+				try_cand(C4[u], O4[u], CB, CI);
 			}
 		}
 	};
+	using yes = std::true_type;
+	using no = std::false_type;
 
 	// Seed with the block's own 64 points (the nearest tile of candidates).
 	const int nseed = min(64, n - base);
-	if (valid) scan(base, nseed, false);
+	if (valid) scan(base, nseed, false, yes(), yes());
 	float R2 = wave_max(valid ? key_d2(top) : 0.f);
 
 	int sp = 1;
@@ -757,7 +779,17 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 		const bool need = valid && (!BATCH || brange_has(hi.w, qb)) &&
 				point_box_d2<PER>(lo, hi, q.x, q.y, q.z, L) <= key_d2(top);
 		if (!__any(need)) continue;
-		if (need) scan(start, cnt, true);
+		// the leaf is all of every needing lane's box: no box test; every
+		// delta of the wavefront's queries to the leaf's box under 0.999 of
+		// half a period: no image test
+		const bool cb = BATCH && __any(need && !brange_is(hi.w, qb));
+		const bool ci = PER && uniform_int(!(fmaxf(bhi.x - lo.x, hi.x - blo.x) < 0.999f * hL &&
+				fmaxf(bhi.y - lo.y, hi.y - blo.y) < 0.999f * hL &&
+				fmaxf(bhi.z - lo.z, hi.z - blo.z) < 0.999f * hL)) != 0;
+		if (need) {
+			if (cb) { if (ci) scan(start, cnt, true, yes(), yes()); else scan(start, cnt, true, yes(), no()); }
+			else { if (ci) scan(start, cnt, true, no(), yes()); else scan(start, cnt, true, no(), no()); }
+		}
 		R2 = wave_max(valid ? key_d2(top) : 0.f);
 	}
 	if (!valid) return;
