@@ -671,6 +671,7 @@ __device__ __forceinline__ float key_d2(unsigned long long k) { return __uint_as
 // BATCH: several boxes share the arrays; a lane accepts only candidates whose
 // iOrder lies in its own box's range, and the walk skips subtrees holding none
 // of the block's boxes.
+#define SKG_KNN_SEED 64
 template <int KMAX, bool PER, bool COMP, bool BATCH, int W>
 __global__ __launch_bounds__(64 * W) void k_knn(
 		const float4 *__restrict__ spos, const int *__restrict__ sord, int n,
@@ -689,6 +690,7 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 	const int lane = threadIdx.x & 63;
 	const int base = (xcd_tile(blockIdx.x, gridDim.x) * W + wv) * 64;
 	if (base >= n) return;                        // a wavefront past the end (no workgroup barrier here)
+	const int sd0 = max(0, base - SKG_KNN_SEED), sd1 = min(n, base + 64 + SKG_KNN_SEED);   // seeded range
 	const int qi = base + lane;
 	const bool valid = qi < n;
 	const float4 q = spos[valid ? qi : base];
@@ -727,7 +729,7 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 	// The cnt candidates from start (a uniform range): records are read four
 	// at a time through the scalar cache, one wait per four (the arrays carry
 	// SKG_EPAD entries past the end, so the reads past a range's end stay in
-	// the allocation; their values are never used).  skip: the seeded tile.
+	// the allocation; their values are never used).  skip: the seeded range.
 	auto scan = [&](int start, int cnt, bool skip, auto CB, auto CI) {
 		for (int t0 = 0; t0 < cnt; t0 += 4) {
 			float4 C4[4];
@@ -738,7 +740,7 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 			for (int u = 0; u < 4; ++u) {
 				if (t0 + u >= cnt) break;
 				const int j = start + t0 + u;
-				if (skip && j >= base && j < base + 64) continue;
+				if (skip && j >= sd0 && j < sd1) continue;
 				try_cand(C4[u], O4[u], CB, CI);
 			}
 		}
@@ -746,9 +748,10 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 	using yes = std::true_type;
 	using no = std::false_type;
 
-	// Seed with the block's own 64 points (the nearest tile of candidates).
-	const int nseed = min(64, n - base);
-	if (valid) scan(base, nseed, false, yes(), yes());
+	// Seed with the block's own 64 points and the SKG_KNN_SEED on each side
+	// along the curve (the nearest candidates: the radii start small, so the
+	// walk prunes from its first node).
+	if (valid) scan(sd0, sd1 - sd0, false, yes(), yes());
 	float R2 = wave_max(valid ? key_d2(top) : 0.f);
 
 	int sp = 1;
@@ -775,7 +778,7 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 		const int start = (node - nLeaf2) * B;
 		if (start >= n) continue;
 		const int cnt = min(B, n - start);
-		if (start >= base && start + cnt <= base + 64) continue;     // seeded tile
+		if (start >= sd0 && start + cnt <= sd1) continue;            // seeded range
 		const bool need = valid && (!BATCH || brange_has(hi.w, qb)) &&
 				point_box_d2<PER>(lo, hi, q.x, q.y, q.z, L) <= key_d2(top);
 		if (!__any(need)) continue;
