@@ -311,7 +311,8 @@ int skg_stage_times(skg_ctx *ctx, double *ms);
 #define SKG_K_UNB_POT 4      /* 20 flops per ordered pair */
 #define SKG_K_UNB_LOOP 5     /* removal loops of groups < 32768 members: 20 flops per kdSubPot pair */
 #define SKG_K_UNB_BIG 6      /* removal loops of larger groups (cooperative): 20 flops per kdSubPot pair */
-#define SKG_K_N 7
+#define SKG_K_COVER_BUILD 7  /* a block's first move step building its entity lists: as SKG_K_COVER_MOVE */
+#define SKG_K_N 8
 int skg_set_profiling(skg_ctx *ctx, int on);
 /* Calibration copy (no reference counterpart): allocates two device buffers of
    bytes bytes (rounded down to 16) and copies one into the other reps times

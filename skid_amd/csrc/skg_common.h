@@ -139,6 +139,7 @@ struct skg_ctx {
 	// at most subMax points use the sub-wave cover kernel
 	long subMax = 0;
 	int knnBucket = 0, knnWaves = 1;             // kNN tree leaf size (0: nBucket), wavefronts per kNN workgroup
+	int fusedBuild = 1;                          // entity lists built by each block's first step
 	long s64Max = 0;            // at most this many active movers: 64 lanes (a wavefront) per mover
 	// per-tile lists of the current 5-step block: entities (SKG_TILE_LISTS=1,
 	// default), leaves (2) or none (0: every step walks the tree); tlCap

@@ -937,6 +937,27 @@ __device__ __forceinline__ void box_min_atomic(float hmin, bool valid, int pb, i
 	}
 }
 
+// kdMoveParticles (kd.c:702-732) for one lane: a step of fStep against the
+// gradient (ax, ay, az), then the periodic wrap into the box; a wrap sets the
+// mover's flag (mwrap) that sends its wavefront back to the tree walk for the
+// rest of the block.
+__device__ __forceinline__ void move_point(float4 *__restrict__ ppos, int pi, const float4 &p, float ax, float ay,
+		float az, float fStep, float cx, float cy, float cz, float Lx, float Ly, float Lz, int *__restrict__ mwrap) {
+	const float ss = sq3(ax, ay, az);
+	float ai = (float)sqrt((double)ss);
+	if (ai > 0.0f) ai = (float)((double)fStep / sqrt((double)ss));
+	else ai = 0.0f;
+	float r[3] = {fsub(p.x, fmul(ai, ax)), fsub(p.y, fmul(ai, ay)), fsub(p.z, fmul(ai, az))};
+	const float C[3] = {cx, cy, cz}, LL[3] = {Lx, Ly, Lz};
+	bool wrapped = false;
+	for (int j = 0; j < 3; ++j) {
+		if ((double)r[j] > (double)C[j] + 0.5 * (double)LL[j]) { r[j] = fsub(r[j], LL[j]); wrapped = true; }
+		if ((double)r[j] <= (double)C[j] - 0.5 * (double)LL[j]) { r[j] = fadd(r[j], LL[j]); wrapped = true; }
+	}
+	ppos[pi] = make_float4(r[0], r[1], r[2], p.w);
+	if (wrapped && mwrap) mwrap[pi] = 1;
+}
+
 // Leaf lists for a block of move steps.  Entities do not move and ScatterCut
 // only removes balls, so while the active list (and with it every tile of 64
 // movers) is unchanged, the leaves any step can need are those whose ball
@@ -1167,22 +1188,103 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		if (dl) out_dl[ii] = s1.val() * (float)(-M_1_PI / 6.0);
 		return;
 	}
-	// kdMoveParticles (kd.c:702-732)
+	move_point(ppos, pi, p, s0.val(), s1.val(), s2.val(), micro ? bt[pb].fMicro : bt[pb].fStep, cx, cy, cz, Lx,
+			Ly, Lz, mwrap);
+}
+
+// The first step of a 5-step block with its tile's entity list built on the
+// way: one walk with the block's margin does what k_cover_lists<true> and
+// k_cover<1> would do in two.  A leaf is opened when its largest live ball,
+// widened by the lane's margin, reaches a lane's point; each live entity in it
+// is listed when its own widened reach holds a lane's point (the same fma
+// bound as k_cover_lists<true>), and it adds to a lane's gradient when its
+// ball holds the lane's point.  The widened test opens every node the plain
+// walk opens (and more, whose entities hold no lane's point), and both visit
+// leaves in tree order, so the hits and the order of the sums are k_cover's.
+// Then the lanes' points move.  The list and count go to tlist / tcnt as
+// k_cover_lists<true> writes them.
+template <bool COMP>
+__global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8))) void k_cover_build(
+		float4 *__restrict__ ppos, const int *__restrict__ plist, int np, const float4 *__restrict__ epos,
+		const float4 *__restrict__ eaux, const float4 *__restrict__ eorg, const int *__restrict__ ecode, int ne,
+		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B, int *__restrict__ ehit,
+		int *__restrict__ bthr, const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
+		float msteps, float mabs, float cx, float cy, float cz, float Lx, float Ly, float Lz,
+		const int *__restrict__ lstart, int cap, int *__restrict__ tlist, int *__restrict__ tcnt,
+		const int *__restrict__ npDev, const int *__restrict__ stampDev, int *__restrict__ mwrap) {
+	if (npDev) np = *npDev;
+	const int stamp = stampDev ? *stampDev : 0;
+	const int lane = threadIdx.x & 63;
+	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
+	if (base >= np) return;                       // whole wavefront past the end
+	const int tile = base >> 6;
+	const int ii = base + lane;
+	const bool valid = ii < np;
+	const int pi = plist ? plist[valid ? ii : base] : (valid ? ii : base);
+	const float4 p = ppos[pi];
+	const int pb = pbox ? pbox[pi] : 0;
 	const float fStep = micro ? bt[pb].fMicro : bt[pb].fStep;
-	const float ax = s0.val(), ay = s1.val(), az = s2.val();
-	const float ss = sq3(ax, ay, az);
-	float ai = (float)sqrt((double)ss);
-	if (ai > 0.0f) ai = (float)((double)fStep / sqrt((double)ss));
-	else ai = 0.0f;
-	float r[3] = {fsub(p.x, fmul(ai, ax)), fsub(p.y, fmul(ai, ay)), fsub(p.z, fmul(ai, az))};
-	const float C[3] = {cx, cy, cz}, LL[3] = {Lx, Ly, Lz};
-	bool wrapped = false;
-	for (int j = 0; j < 3; ++j) {
-		if ((double)r[j] > (double)C[j] + 0.5 * (double)LL[j]) { r[j] = fsub(r[j], LL[j]); wrapped = true; }
-		if ((double)r[j] <= (double)C[j] - 0.5 * (double)LL[j]) { r[j] = fadd(r[j], LL[j]); wrapped = true; }
+	const float margin = fmaf(msteps, fStep, mabs);
+	if (valid && mwrap) mwrap[pi] = 0;
+	Sum<COMP> s0, s1, s2;
+	s0.init(); s1.init(); s2.init();
+	float hmin = INFINITY;
+	int n = 0, node = 1;
+	int *out = tlist + (size_t)tile * cap;
+	int buf = 0;                                   // entry n in lane n % 64, stored 64 at a time
+	auto emit = [&](int idx) {
+		if (lane == (n & 63)) buf = idx;
+		++n;
+		if ((n & 63) == 0 && n <= cap) out[n - 64 + lane] = buf;
+	};
+	while (node) {
+		node = uniform_int(node);
+		const float4 lo = nlo[node], hi = nhi[node];
+		bool need = false;
+		float c = 0.f;
+		if (valid && lo.w >= 0.f && brange_has(hi.w, pb)) {
+			// hardware square root (about 1 ulp): covered by the factor below
+			c = __builtin_amdgcn_sqrtf(lo.w);
+			const float r = c + margin;
+			need = point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < r * r * 1.000001f;
+		}
+		if (__any(need)) {
+			if (node < nLeaf2) { node = 2 * node; continue; }
+			c = fmaxf(c, 1e-30f);
+			const float k1 = fmaf(margin, 1.f / c, 1.f) * 1.000002f;
+			const float k2 = margin * (c + margin) * 1.000002f;
+			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
+			const int start = uniform_int(rg.x), cnt = uniform_int(rg.y - rg.x);
+			for (int t0 = 0; t0 < cnt; t0 += 4) {
+				// four records per scalar load (the arrays carry SKG_EPAD spare records)
+				float4 A4[4], X4[4];
+				int K4[4] = {13, 13, 13, 13};
+#pragma unroll
+				for (int u = 0; u < 4; ++u) {
+					A4[u] = epos[start + t0 + u]; X4[u] = eaux[start + t0 + u];
+					if (COMP) K4[u] = ecode[start + t0 + u];
+				}
+#pragma unroll
+				for (int u = 0; u < 4; ++u) {
+					if (t0 + u >= cnt) break;
+					if (X4[u].x < 0.f) continue;          // cut (or filler): neither listed nor hit
+					const float dx = fsub(A4[u].x, p.x), dy = fsub(A4[u].y, p.y), dz = fsub(A4[u].z, p.z);
+					const bool listed = need && sq3(dx, dy, dz) < fmaf(X4[u].x, k1, k2);
+					if (__any(listed)) emit(start + t0 + u);
+					cover_hit<1, COMP>(start + t0 + u, A4[u], X4[u], K4[u], need, p, eorg, ehit, stamp, hmin, Lx, Ly,
+							Lz, s0, s1, s2);
+				}
+			}
+		}
+		node = (node >> __builtin_ctz(~node)) + 1;
+		if (node == 1) node = 0;
 	}
-	ppos[pi] = make_float4(r[0], r[1], r[2], p.w);
-	if (wrapped && mwrap) mwrap[pi] = 1;
+	const int rem = n & 63;
+	if (n <= cap && rem && lane < rem) out[n - rem + lane] = buf;
+	if (lane == 0) tcnt[tile] = n <= cap ? n : -1;
+	if (bthr) box_min_atomic(hmin, valid, pb, bthr);
+	if (!valid) return;
+	move_point(ppos, pi, p, s0.val(), s1.val(), s2.val(), fStep, cx, cy, cz, Lx, Ly, Lz, mwrap);
 }
 
 // Sub-wave variant: S lanes per point (64/S points per wavefront), each lane
@@ -1819,6 +1921,7 @@ extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
 	if (const char *e = getenv("SKG_COVER_S64_MAX")) c->s64Max = atol(e);
 	if (const char *e = getenv("SKG_KNN_BUCKET")) c->knnBucket = std::max(0, std::min(1024, atoi(e)));
 	if (const char *e = getenv("SKG_KNN_WAVES")) c->knnWaves = atoi(e) == 2 ? 2 : 1;
+	if (const char *e = getenv("SKG_FUSED_BUILD")) c->fusedBuild = atoi(e) != 0;
 	if (const char *e = getenv("SKG_TILE_LISTS")) c->useLists = atoi(e);
 	c->tlCap = c->useLists == 2 ? 512 : 1024;
 	if (const char *e = getenv("SKG_TILE_CAP")) c->tlCap = std::max(4, atoi(e) & ~3);
@@ -2262,9 +2365,13 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 	const float *C = c->P.fCenter, *L = c->P.fPeriod;
 	const int *pbox = c->nBox > 1 ? c->m_box : nullptr;
 	const int nb = nblk(c->nE, 256);
+	const float mabs = 1e-6f * c->P.fPeriod[0] * (c->P.bPeriodic ? 1.f : 0.f);
 	const char *ft = getenv("SKG_FUSED_THR");
 	const bool fusedThr = !(ft && atoi(ft) == 0);
-	if (lists) enqueue_lists(c, cap, nsteps, micro);
+	// entity lists: built by the block's first step itself (k_cover_build), or
+	// by a pass of their own before it (SKG_FUSED_BUILD=0, and leaf lists)
+	const bool fusedBuild = lists && c->useLists != 2 && c->fusedBuild;
+	if (lists && !fusedBuild) enqueue_lists(c, cap, nsteps, micro);
 	for (int i = 0; i < nsteps; ++i) {
 		k_bump<<<1, 1, 0, c->st>>>(c->d_ml);
 		// the initial step stamps its hits (unhit entities get density 0);
@@ -2291,9 +2398,18 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 				ehit, 0, bthr, pbox, c->d_bt, micro, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart,           \
 				lists ? c->tl_list : nullptr, lists ? c->tl_cnt : nullptr, c->tlCap, c->d_ml + 1, c->d_ml,  \
 				c->useLists != 2, c->d_eblk, nullptr, lists ? c->m_wrap : nullptr)
-		if (comp) { MV(true); } else { MV(false); }
+#define MB(C_)                                                                                                \
+		k_cover_build<C_><<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap, c->e_pos, \
+				c->e_aux, c->e_org, c->e_code, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, ehit, bthr, pbox,   \
+				c->d_bt, micro, 1.02f * (nsteps - 1), mabs, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart,        \
+				c->tlCap, c->tl_list, c->tl_cnt, c->d_ml + 1, c->d_ml, c->m_wrap)
+		if (fusedBuild && i == 0) { if (comp) { MB(true); } else { MB(false); } }
+		else if (comp) { MV(true); } else { MV(false); }
+#undef MB
 #undef MV
-		if (pslot >= 0) skg_prof_end(c, pslot, SKG_K_COVER_MOVE, 24.0 * c->nActive + 24.0 * c->nLive);
+		if (pslot >= 0)
+			skg_prof_end(c, pslot, fusedBuild && i == 0 ? SKG_K_COVER_BUILD : SKG_K_COVER_MOVE,
+					24.0 * c->nActive + 24.0 * c->nLive);
 		if (stamps) {
 			k_scat_thr<<<nb, 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->e_aux, c->nE, c->d_ml, bInitial,
 					c->part);

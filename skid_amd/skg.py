@@ -25,7 +25,7 @@ SKG_ACC_PLAIN, SKG_ACC_COMPENSATED = 0, 1
 SKG_TUNE_BUCKET, SKG_TUNE_MAPPING, SKG_TUNE_POT_TILE, SKG_TUNE_KNN, SKG_TUNE_ALL = 1, 2, 4, 8, 15
 SKG_NOALLOC_MAX_POINTS, SKG_NOALLOC_SCRATCH_BYTES = 524288, 64 << 20
 STAGES = ("tree", "knn", "density", "move", "fof", "group", "unbind", "spare")
-KERNELS = ("knn", "cover_density", "cover_move", "fof", "unbind_potential", "unbind_loop", "unbind_big")
+KERNELS = ("knn", "cover_density", "cover_move", "fof", "unbind_potential", "unbind_loop", "unbind_big", "cover_build")
 FLT_MAX = 3.4028234663852886e38
 
 

@@ -116,8 +116,9 @@ def test_two_species_catalogue():
 def test_move_loop_launch_paths_agree():
     """The move loop gives the same trace and groups however its blocks are
     launched: direct launches (default), one hipGraph per block, with per-tile
-    entity lists (default), leaf lists or none, and with a list capacity so
-    small that most tiles fall back to the walk.  A low SKG_COVER_SUB_MAX makes the dense steps
+    entity lists built by each block's first step (default) or by a pass of
+    their own, leaf lists or none, and with a list capacity so small that most
+    tiles fall back to the walk.  A low SKG_COVER_SUB_MAX makes the dense steps
     use the lane-per-mover kernel (and the lists) and the sparse tail the
     sub-wave kernel."""
     import os
@@ -134,6 +135,8 @@ def test_move_loop_launch_paths_agree():
     base = {"SKG_COVER_SUB_MAX": "2000"}
     for tag, env_add in (("direct", {}), ("graph", {"SKG_GRAPH": "1"}), ("nolists", {"SKG_TILE_LISTS": "0"}),
                          ("leaflists", {"SKG_TILE_LISTS": "2"}), ("smallcap", {"SKG_TILE_CAP": "64"}),
+                         ("sepbuild", {"SKG_FUSED_BUILD": "0"}),
+                         ("sepbuild_smallcap", {"SKG_FUSED_BUILD": "0", "SKG_TILE_CAP": "64"}),
                          ("graph_nolists", {"SKG_GRAPH": "1", "SKG_TILE_LISTS": "0"}),
                          ("stamped", {"SKG_FUSED_THR": "0"}), ("scan_cut", {"SKG_SORTED_CUT": "0"})):
         env = dict(os.environ)
