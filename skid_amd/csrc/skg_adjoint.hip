@@ -32,29 +32,34 @@ __device__ __forceinline__ float per_delta(float a, float b, float L, int per) {
 }
 
 // Pass 1: per query i, over its kNN list (heap keys; slot 0 is the head).
+// The lists hold iOrder; each neighbour's position, mass and weight are read in
+// scatterer (curve) order through sinv (iOrder -> scatterer index), so that the
+// reads of a wavefront's neighbours fall on nearby lines.
 __global__ __launch_bounds__(256) void k_adj_gather(const unsigned long long *__restrict__ heap, int k,
-		const int *__restrict__ sord, const float *__restrict__ sb2, int nS, const float *__restrict__ pos,
-		const float *__restrict__ mass, const float *__restrict__ gbar, int parts, float L, int per,
+		const int *__restrict__ sord, const int *__restrict__ sinv, const float4 *__restrict__ spos,
+		const float *__restrict__ gs, const float *__restrict__ sb2, int nS, int parts, float L, int per,
 		float *dx, float *dh) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= nS) return;
 	const int oi = sord[i];
-	const float xi = pos[3 * oi], yi = pos[3 * oi + 1], zi = pos[3 * oi + 2], mi = mass[oi], gi = gbar[oi];
+	const float4 pi = spos[i];
+	const float xi = pi.x, yi = pi.y, zi = pi.z, mi = pi.w, gi = gs[i];
 	const float h = 0.5f * sqrtf(sb2[i]), ih = 1.0f / h, norm = (float)(M_1_PI) * ih * ih * ih * ih;
 	Acc ax, ay, az, ahh;
 	ax.init(); ay.init(); az.init(); ahh.init();
 	for (int s = 1; s < k; ++s) {
 		const unsigned long long key = heap[(size_t)s * nS + i];     // slot-major: coalesced
-		const int j = (int)(unsigned)key;
+		const int sj = sinv[(int)(unsigned)key];
+		const float4 pj = spos[sj];
 		float dxv = 0.f, dyv = 0.f, dzv = 0.f, r = 0.f;
-		if (j != oi) {
-			dxv = per_delta(xi, pos[3 * j], L, per); dyv = per_delta(yi, pos[3 * j + 1], L, per);
-			dzv = per_delta(zi, pos[3 * j + 2], L, per);
+		if (sj != i) {
+			dxv = per_delta(xi, pj.x, L, per); dyv = per_delta(yi, pj.y, L, per);
+			dzv = per_delta(zi, pj.z, L, per);
 			r = sqrtf(sq3(dxv, dyv, dzv));
 		}
 		float w, wp;
 		m4_derivs(r * ih, w, wp);
-		const float coef = ((parts & 1) ? gi * mass[j] : 0.f) + ((parts & 2) ? gbar[j] * mi : 0.f);
+		const float coef = ((parts & 1) ? gi * pj.w : 0.f) + ((parts & 2) ? gs[sj] * mi : 0.f);
 		// the self pair (r = 0, W = 1/(pi h^3)) moves with h but not with x
 		ahh.add(-0.5f * (3.0f * w + r * ih * wp) * norm * coef);
 		if (!(r > 0.f)) continue;
@@ -63,6 +68,17 @@ __global__ __launch_bounds__(256) void k_adj_gather(const unsigned long long *__
 	}
 	dx[3 * oi] = ax.val(); dx[3 * oi + 1] = ay.val(); dx[3 * oi + 2] = az.val();
 	dh[oi] = ahh.val();
+}
+
+// The inverse of the scatterer order (iOrder -> scatterer index, scatterers
+// only) and the weights in scatterer order.
+__global__ void k_scat_inverse(const int *__restrict__ sord, int nS, const float *__restrict__ g, int *sinv,
+		float *gs) {
+	const int s = blockIdx.x * blockDim.x + threadIdx.x;
+	if (s >= nS) return;
+	const int o = sord[s];
+	sinv[o] = s;
+	gs[s] = g[o];
 }
 
 // Sensitivity of the density to nSmooth with the neighbour sets frozen: all
@@ -249,15 +265,18 @@ extern "C" int skg_density_adjoint(skg_ctx *c, const float *gbar, int parts, flo
 	(void)hipSetDevice(c->dev);
 	int rc;
 	const int n = c->n, nS = c->nS;
-	float *dg = nullptr, *ddx = nullptr, *ddh = nullptr, *part2 = nullptr;
-	if (sget(c, 0, &dg, n) || sget(c, 1, &ddx, 3 * (size_t)n) || sget(c, 2, &ddh, n) || sget(c, 3, &part2, 3 * (size_t)nS))
+	float *dg = nullptr, *ddx = nullptr, *ddh = nullptr, *part2 = nullptr, *gs = nullptr;
+	int *sinv = nullptr;
+	if (sget(c, 0, &dg, n) || sget(c, 1, &ddx, 3 * (size_t)n) || sget(c, 2, &ddh, n) || sget(c, 3, &part2, 3 * (size_t)nS) ||
+	    sget(c, 4, &sinv, n) || sget(c, 5, &gs, nS))
 		return SKG_E_NOMEM;
 	SKG_HIP(hipMemcpyAsync(dg, gbar, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
 	SKG_HIP(hipMemsetAsync(ddx, 0, 12 * (size_t)n, c->st));
 	SKG_HIP(hipMemsetAsync(ddh, 0, 4 * (size_t)n, c->st));
 	const bool per = c->P.bPeriodic != 0;
-	k_adj_gather<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_heap, c->P.nSmooth, c->s_ord, c->s_ball2, nS, c->d_pos,
-			c->d_mass, dg, parts, per ? c->P.fPeriod[0] : 0.f, per, ddx, ddh);
+	k_scat_inverse<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_ord, nS, dg, sinv, gs);
+	k_adj_gather<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_heap, c->P.nSmooth, c->s_ord, sinv, c->s_pos, gs, c->s_ball2,
+			nS, parts, per ? c->P.fPeriod[0] : 0.f, per, ddx, ddh);
 	if ((rc = full_tree(c))) return rc;
 	launch_cover<0>(c, c->s_pos, nS, c->s_ord, dg, parts, c->s_box, part2);
 	k_add3_ord<<<nblk(nS, 256), 256, 0, c->st>>>(part2, c->s_ord, nS, ddx);

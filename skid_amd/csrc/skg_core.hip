@@ -1803,19 +1803,24 @@ __global__ void k_compact(const int *__restrict__ list, const int *__restrict__ 
 }
 
 // Active movers per box (lanes of one box add once per wavefront).
+// Active movers per box.  The active list is ordered by box (the movers are
+// sorted by box, then along the curve, and every prune compacts the list in
+// order), so the thread at a box's last mover finds the box's first by binary
+// search and writes the count; no atomics.  Boxes with no mover keep 0.
 __global__ void k_box_count(const int *__restrict__ list, const int *__restrict__ nDev, const int *__restrict__ mbox,
 		int *cnt) {
 	const int n = *nDev;
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	const int b = i < n ? mbox[list[i]] : -1;
-	const int b0 = __shfl(b, 0, 64);
-	const bool same = __all(b == b0 || b < 0);
-	if (same) {
-		const int c = __popcll(__ballot(b >= 0));
-		if ((threadIdx.x & 63) == 0 && b0 >= 0 && c) atomicAdd(&cnt[b0], c);
-	} else if (b >= 0) {
-		atomicAdd(&cnt[b], 1);
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i >= n) return;
+	const int b = mbox[list[i]];
+	if (i + 1 < n && mbox[list[i + 1]] == b) return;
+	int lo = 0, hi = i;                            // the first index whose box is >= b
+	while (lo < hi) {
+		const int m = (lo + hi) >> 1;
+		if (mbox[list[m]] < b) lo = m + 1;
+		else hi = m;
 	}
+	cnt[b] = i + 1 - lo;
 }
 
 // CutCriterion (kd.c:555-597) per particle by iOrder.
