@@ -70,15 +70,10 @@ __global__ __launch_bounds__(256) void k_adj_gather(const unsigned long long *__
 	dh[oi] = ahh.val();
 }
 
-// The inverse of the scatterer order (iOrder -> scatterer index, scatterers
-// only) and the weights in scatterer order.
-__global__ void k_scat_inverse(const int *__restrict__ sord, int nS, const float *__restrict__ g, int *sinv,
-		float *gs) {
+// The weights in scatterer order.
+__global__ void k_scat_weights(const int *__restrict__ sord, int nS, const float *__restrict__ g, float *gs) {
 	const int s = blockIdx.x * blockDim.x + threadIdx.x;
-	if (s >= nS) return;
-	const int o = sord[s];
-	sinv[o] = s;
-	gs[s] = g[o];
+	if (s < nS) gs[s] = g[sord[s]];
 }
 
 // Sensitivity of the density to nSmooth with the neighbour sets frozen: all
@@ -266,16 +261,15 @@ extern "C" int skg_density_adjoint(skg_ctx *c, const float *gbar, int parts, flo
 	int rc;
 	const int n = c->n, nS = c->nS;
 	float *dg = nullptr, *ddx = nullptr, *ddh = nullptr, *part2 = nullptr, *gs = nullptr;
-	int *sinv = nullptr;
 	if (sget(c, 0, &dg, n) || sget(c, 1, &ddx, 3 * (size_t)n) || sget(c, 2, &ddh, n) || sget(c, 3, &part2, 3 * (size_t)nS) ||
-	    sget(c, 4, &sinv, n) || sget(c, 5, &gs, nS))
+	    sget(c, 5, &gs, nS))
 		return SKG_E_NOMEM;
 	SKG_HIP(hipMemcpyAsync(dg, gbar, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
 	SKG_HIP(hipMemsetAsync(ddx, 0, 12 * (size_t)n, c->st));
 	SKG_HIP(hipMemsetAsync(ddh, 0, 4 * (size_t)n, c->st));
 	const bool per = c->P.bPeriodic != 0;
-	k_scat_inverse<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_ord, nS, dg, sinv, gs);
-	k_adj_gather<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_heap, c->P.nSmooth, c->s_ord, sinv, c->s_pos, gs, c->s_ball2,
+	k_scat_weights<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_ord, nS, dg, gs);
+	k_adj_gather<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_heap, c->P.nSmooth, c->s_ord, c->s_inv, c->s_pos, gs, c->s_ball2,
 			nS, parts, per ? c->P.fPeriod[0] : 0.f, per, ddx, ddh);
 	if ((rc = full_tree(c))) return rc;
 	launch_cover<0>(c, c->s_pos, nS, c->s_ord, dg, parts, c->s_box, part2);

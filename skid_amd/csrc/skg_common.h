@@ -79,6 +79,7 @@ struct skg_ctx {
 	int nS = 0;
 	float4 *s_pos = nullptr;
 	int *s_ord = nullptr;
+	int *s_inv = nullptr;        // iOrder -> scatterer index (scatterers only)
 	float *s_ball2 = nullptr, *s_rhoG = nullptr, *s_rhoS = nullptr;
 	// d rho / d ln nSmooth halves formed by the density kernels themselves once
 	// skg_density_dlnk has been asked for (wantDlnk); dlnkReady: they belong

@@ -676,7 +676,7 @@ template <int KMAX, bool PER, bool COMP, bool BATCH, int W>
 __global__ __launch_bounds__(64 * W) void k_knn(
 		const float4 *__restrict__ spos, const int *__restrict__ sord, int n,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
-		float L, int k, const float *__restrict__ mass_ord, const float *__restrict__ pos_ord,
+		float L, int k, const int *__restrict__ sinv,
 		float *__restrict__ ball2_out, float *__restrict__ rhoG, unsigned long long *__restrict__ heap_out,
 		const int *__restrict__ sbox, const DevBox *__restrict__ bt, float *__restrict__ dlnk_out) {
 	// W wavefronts per workgroup, each with its own 64 queries, heaps and stack
@@ -811,12 +811,13 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 	dacc.init();
 	for (int s = 1; s < k; ++s) {
 		const unsigned long long key = heap[s * 64 + lane];
-		const unsigned j = (unsigned)key;
+		// the neighbour's record in scatterer order (near the query's own)
+		const float4 pj = spos[sinv[(unsigned)key]];
 		float d2 = key_d2(key), d2x = d2;
 		if (PER && (COMP || dl)) {
 			// weights from exact periodic separations (membership stays the
 			// reference's d2 test, so the neighbour set is unchanged)
-			const float px = pos_ord[3 * j], py = pos_ord[3 * j + 1], pz = pos_ord[3 * j + 2];
+			const float px = pj.x, py = pj.y, pz = pj.z;
 			float dx = fsub(q.x, px), dy = fsub(q.y, py), dz = fsub(q.z, pz);
 			if (fabsf(dx) > hL || fabsf(dy) > hL || fabsf(dz) > hL) {
 				dx = exact_delta(q.x, px, dx > hL ? -L : (dx < -hL ? L : 0.f));
@@ -826,7 +827,7 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 			}
 			if (COMP) d2 = d2x;
 		}
-		const float mj = mass_ord[j];
+		const float mj = pj.w;
 		acc.add(fmul(m4_w<COMP>(d2, ih2, fNorm), mj));
 		if (dl) dacc.add(dlnk_term(sqrtf(d2x), ih, mj));
 	}
@@ -1382,6 +1383,12 @@ __global__ void k_gather_scat(const float *__restrict__ pos, const float *__rest
 	const int o = ord[i];
 	out4[i] = make_float4(pos[3 * o], pos[3 * o + 1], pos[3 * o + 2], mass[o]);
 	obox[i] = boxOf[o];
+}
+
+// inv[ord[i]] = i: the inverse of an order over its n entries.
+__global__ void k_inverse(const int *__restrict__ ord, int n, int *inv) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) inv[ord[i]] = i;
 }
 
 __global__ void k_permute_s(const float4 *__restrict__ in4, const int *__restrict__ inord,
@@ -1959,7 +1966,7 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	skg_group_free(c);
 	dfree(&c->d_pos); dfree(&c->d_vel); dfree(&c->d_mass); dfree(&c->d_soft); dfree(&c->d_temp);
 	dfree(&c->d_rho); dfree(&c->d_ball2); dfree(&c->d_grp);
-	dfree(&c->s_pos); dfree(&c->s_ord); dfree(&c->s_ball2); dfree(&c->s_rhoG); dfree(&c->s_rhoS);
+	dfree(&c->s_pos); dfree(&c->s_ord); dfree(&c->s_inv); dfree(&c->s_ball2); dfree(&c->s_rhoG); dfree(&c->s_rhoS);
 	dfree(&c->s_heap); dfree(&c->s_box); dfree(&c->s_dlnkG); dfree(&c->s_dlnkS);
 	skg_free_tree(&c->ts); skg_free_tree(&c->te); skg_free_tree(&c->tfull); skg_free_tree(&c->tfof);
 	dfree(&c->e_b2); dfree(&c->e_pos); dfree(&c->e_aux); dfree(&c->e_org); dfree(&c->e_code); dfree(&c->e_rho); dfree(&c->e_src); dfree(&c->e_live); dfree(&c->e_hit);
@@ -2089,7 +2096,7 @@ static void launch_knn(skg_ctx *c, bool per, bool comp) {
 	const float L = per ? c->P.fPeriod[0] : FLT_MAX;
 #define KL(P_, C_)                                                                           \
 	k_knn<KMAX, P_, C_, BATCH, W><<<nb, 64 * W, 0, c->st>>>(c->s_pos, c->s_ord, c->nS, c->ts.lo, c->ts.hi, \
-			c->ts.nLeaf2, c->ts.B, L, c->P.nSmooth, c->d_mass, c->d_pos, c->s_ball2, c->s_rhoG, c->s_heap, \
+			c->ts.nLeaf2, c->ts.B, L, c->P.nSmooth, c->s_inv, c->s_ball2, c->s_rhoG, c->s_heap, \
 			c->s_box, c->d_bt, c->wantDlnk ? c->s_dlnkG : nullptr)
 	if (per) { if (comp) KL(true, true); else KL(true, false); }
 	else { if (comp) KL(false, true); else KL(false, false); }
@@ -2169,7 +2176,8 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		float4 *tmp4 = nullptr;
 		int *tmpo = nullptr, *tmpb = nullptr, *perm = nullptr, *flag = nullptr, *off = nullptr;
 		if ((c->wantDlnk && (dalloc(&c->s_dlnkG, nS) || dalloc(&c->s_dlnkS, nS))) ||
-		    dalloc(&c->s_pos, nS + SKG_EPAD) || dalloc(&c->s_ord, nS + SKG_EPAD) || dalloc(&c->s_box, nS) || dalloc(&c->s_ball2, nS) ||
+		    dalloc(&c->s_pos, nS + SKG_EPAD) || dalloc(&c->s_ord, nS + SKG_EPAD) || dalloc(&c->s_inv, n) ||
+		    dalloc(&c->s_box, nS) || dalloc(&c->s_ball2, nS) ||
 		    dalloc(&c->s_rhoG, nS) || dalloc(&c->s_rhoS, nS) || sget(c, 0, &tmp4, nS) || sget(c, 1, &tmpo, nS) ||
 		    sget(c, 2, &tmpb, nS) || sget(c, 3, &perm, nS) || sget(c, 4, &flag, n) || sget(c, 5, &off, n))
 			return SKG_E_NOMEM;
@@ -2181,6 +2189,7 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		// wavefronts' runs of 64 queries are compact (no Morton jumps)
 		if ((rc = skg_hilbert_sort(c, tmp4, nullptr, nS, perm, nBox > 1 ? tmpb : nullptr))) return rc;
 		k_permute_s<<<nblk(nS, 256), 256, 0, c->st>>>(tmp4, tmpo, tmpb, perm, nS, c->s_pos, c->s_ord, c->s_box);
+		k_inverse<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_ord, nS, c->s_inv);
 		// the kNN tree's own leaf size (knnBucket; 0: nBucket)
 		const int Bk = c->knnBucket > 0 ? c->knnBucket : B;
 		if ((rc = skg_build_tree(c, &c->ts, c->s_pos, nS, Bk, nullptr, nBox > 1 ? c->s_box : nullptr))) return rc;
