@@ -851,7 +851,7 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 // stamps the hit entity (ehit) or keeps the lane's smallest positive density
 // among its hits (hmin; during the move loop the record's w holds the
 // entity's density, +inf for 0).
-template <int MODE, bool COMP>
+template <int MODE, bool COMP, bool LAZYK = false>
 __device__ __forceinline__ void cover_hit(int e, const float4 &A, const float4 &X, int K, bool want,
 		const float4 &p, const float4 *__restrict__ eorg, int *__restrict__ ehit, int stamp, float &hmin,
 		float Lx, float Ly, float Lz, Sum<COMP> &s0, Sum<COMP> &s1, Sum<COMP> &s2, bool dl = false) {
@@ -860,9 +860,10 @@ __device__ __forceinline__ void cover_hit(int e, const float4 &A, const float4 &
 	if (want && d2 < X.x) {
 		if (COMP || (MODE == 0 && dl)) {
 			// replica: separation from the source position plus the exact shift;
-			// K < 0: the code is read here, from the source record's w
+			// LAZYK: the caller passes K < 0 and the code is read here, from the
+			// source record's w
 			float4 O = make_float4(0.f, 0.f, 0.f, 0.f);
-			if (K < 0) { O = eorg[e]; K = (int)O.w; }
+			if (LAZYK && K < 0) { O = eorg[e]; K = (int)O.w; }
 			if (K != 13) {                       // 9(ix+1) + 3(iy+1) + (iz+1)
 				O = eorg[e];
 				dx = exact_delta(O.x, p.x, (float)(K / 9 - 1) * Lx);
@@ -880,7 +881,7 @@ __device__ __forceinline__ void cover_hit(int e, const float4 &A, const float4 &
 			const float f = COMP ? grad_factor_f32(sq3(dx, dy, dz), X.y, X.z) : grad_factor(d2, X.y, X.z);
 			s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
 			if (ehit) ehit[e] = stamp;
-			else hmin = X.w < hmin ? X.w : hmin;     // w is +inf for a density of 0
+			else hmin = __builtin_amdgcn_fmed3f(X.w, -INFINITY, hmin);   // min; w is +inf for a density of 0
 		}
 	}
 }
@@ -1328,7 +1329,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 		if (need) {
 			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
 			for (int e = rg.x + sub; e < rg.y; e += S) {
-				cover_hit<MODE, COMP>(e, epos[e], eaux[e], -1, true, p, eorg, ehit, stamp, hmin, Lx, Ly, Lz, s0, s1, s2,
+				cover_hit<MODE, COMP, true>(e, epos[e], eaux[e], -1, true, p, eorg, ehit, stamp, hmin, Lx, Ly, Lz, s0, s1, s2,
 						dl);
 			}
 		}
