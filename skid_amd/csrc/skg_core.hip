@@ -551,19 +551,35 @@ __device__ __forceinline__ int key_lcp(const unsigned long long *__restrict__ k,
 	const unsigned long long a = k[i], b = k[j];
 	return a != b ? __clzll((long long)(a ^ b)) : 64 + __clz(i ^ j);
 }
-__global__ void k_leaf_depth(const unsigned long long *__restrict__ keys, int n, int B, int minDepth, int *depth) {
+// lcp of each item with the next (the last item's entry is not used).
+__global__ void k_adj_lcp(const unsigned long long *__restrict__ keys, int n, int *adj) {
+	const int i = blockIdx.x * blockDim.x + threadIdx.x;
+	if (i < n) adj[i] = i + 1 < n ? key_lcp(keys, i, i + 1) : 0;
+}
+// The depth d of item i's leaf: the smallest d with at most B items (itself
+// included) of lcp >= d with it in [i-B, i+B].  Over sorted keys lcp(i, j) is
+// the minimum of the adjacent lcps between i and j, so it falls as j moves
+// away from i on either side; the B-th largest v of the two runs, found by
+// merging them, gives d = v + 1 (0 when the window holds fewer than B others).
+__global__ void k_leaf_depth(const int *__restrict__ adj, int n, int B, int minDepth, int *depth) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
-	const int j0 = max(0, i - B), j1 = min(n - 1, i + B);
-	int lo = 0, hi = 97;                  // count(hi) = 1 <= B
-	while (lo < hi) {
-		const int d = (lo + hi) >> 1;
-		int cnt = 1;
-		for (int j = j0; j <= j1; ++j)
-			if (j != i && key_lcp(keys, i, j) >= d) ++cnt;
-		if (cnt <= B) hi = d; else lo = d + 1;
+	const int ra = min(B, n - 1 - i), la = min(B, i);
+	if (ra + la < B) { depth[i] = max(0, minDepth); return; }
+	int a = 0, b = 0, v = 0;
+	int rv = ra > 0 ? adj[i] : -1, lv = la > 0 ? adj[i - 1] : -1;
+	for (int t = 0; t < B; ++t) {
+		if (a < ra && (b >= la || rv >= lv)) {
+			v = rv;
+			++a;
+			rv = a < ra ? min(rv, adj[i + a]) : -1;
+		} else {
+			v = lv;
+			++b;
+			lv = b < la ? min(lv, adj[i - 1 - b]) : -1;
+		}
 	}
-	depth[i] = max(lo, minDepth);
+	depth[i] = max(v + 1, minDepth);
 }
 __global__ void k_leaf_flags(const unsigned long long *__restrict__ keys, const int *__restrict__ depth, int n,
 		int *flag) {
@@ -584,7 +600,8 @@ int skg_build_tree_prefix(skg_ctx *c, DevTree *t, const float4 *pos, const unsig
 	if (n <= 0) return skg_build_tree(c, t, pos, n, B, b2, ibox);
 	int *depth = nullptr, *flag = nullptr, *off = nullptr;
 	if (sget(c, 17, &depth, n) || sget(c, 18, &flag, n) || sget(c, 19, &off, n)) return SKG_E_NOMEM;
-	k_leaf_depth<<<nblk(n, 256), 256, 0, c->st>>>(keys, n, B, minDepth, depth);
+	k_adj_lcp<<<nblk(n, 256), 256, 0, c->st>>>(keys, n, off);          // off: free until the scan
+	k_leaf_depth<<<nblk(n, 256), 256, 0, c->st>>>(off, n, B, minDepth, depth);
 	k_leaf_flags<<<nblk(n, 256), 256, 0, c->st>>>(keys, depth, n, flag);
 	if ((rc = skg_exclusive_scan(c, flag, off, n))) return rc;
 	int last[2];
