@@ -116,7 +116,7 @@ __global__ __launch_bounds__(256) void k_dlnk_gather(const unsigned long long *_
 template <int MODE>
 __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restrict__ pts, int np,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float *__restrict__ eb2,
-		const float4 *__restrict__ eorg, const int *__restrict__ esrc, const int *__restrict__ sord, int ne,
+		const float4 *__restrict__ eorg, const int *__restrict__ ecode, const float *__restrict__ ew, int ne,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B,
 		const float *__restrict__ mass, const float *__restrict__ gbar, const int *__restrict__ pord,
 		int parts, float Lx, float Ly, float Lz, const int *__restrict__ pbox, const int *__restrict__ lstart,
@@ -131,6 +131,40 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restri
 	if (MODE == 0 && valid) { const int oj = pord[ii]; mj = mass[oj]; gj = gbar[oj]; }
 	Acc s0, s1, s2;
 	s0.init(); s1.init(); s2.init();
+	// One entity e (position and mass A, radius as built b2, image code K)
+	// against the lane's point.
+	auto hit = [&](int e, const float4 &A, float b2, int K) {
+		float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
+		if (!(sq3(dx, dy, dz) < b2)) return;
+		if (K != 13) {                            // replica: source position plus the exact shift
+			const float4 O = eorg[e];
+			dx = exact_delta(O.x, p.x, (float)(K / 9 - 1) * Lx);
+			dy = exact_delta(O.y, p.y, (float)((K / 3) % 3 - 1) * Ly);
+			dz = exact_delta(O.z, p.z, (float)(K % 3 - 1) * Lz);
+		}
+		if (MODE == 1) {
+			const float4 X = eaux[e];
+			const float f = grad_factor(sq3(dx, dy, dz), X.y, X.z);
+			s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
+		} else if (MODE == 2) {
+			const float ih = 2.0f / sqrtf(b2), q = sqrtf(sq3(dx, dy, dz)) * ih;
+			float w, wp;
+			m4_derivs(q, w, wp);
+			s0.add(A.w * (3.0f * w + q * wp) * ih * ih * ih);
+		} else {
+			// hardware square root, reciprocal square root and reciprocal (about
+			// 1 ulp each) for r, 1/h and 1/r
+			const float r = __builtin_amdgcn_sqrtf(sq3(dx, dy, dz));
+			if (!(r > 0.f)) return;
+			const float ih = 2.0f * __builtin_amdgcn_rsqf(b2);
+			float w, wp;
+			m4_derivs(r * ih, w, wp);
+			const float coef = ((parts & 1) ? ew[e] * mj : 0.f) + ((parts & 2) ? gj * A.w : 0.f);
+			const float c = 0.5f * (float)M_1_PI * wp * ih * ih * ih * ih * coef * __builtin_amdgcn_rcpf(r);
+			// dx_j -= c (r_i - r_j)/r with (r_i - r_j) = (dx, dy, dz)
+			s0.add(-c * dx); s1.add(-c * dy); s2.add(-c * dz);
+		}
+	};
 	int node = 1;
 	for (;;) {
 		node = uniform_int(node);
@@ -140,39 +174,20 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restri
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit && need) {
 			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
-			const int start = uniform_int(rg.x), stop = uniform_int(rg.y);
-			for (int e = start; e < stop; ++e) {
-				const float4 A = epos[e];
-				float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
-				const float d2 = sq3(dx, dy, dz);
-				if (!(d2 < eb2[e])) continue;
-				const float4 O = eorg[e];
-				const int code = (int)O.w;
-				if (code != 13) {
-					dx = exact_delta(O.x, p.x, (float)(code / 9 - 1) * Lx);
-					dy = exact_delta(O.y, p.y, (float)((code / 3) % 3 - 1) * Ly);
-					dz = exact_delta(O.z, p.z, (float)(code % 3 - 1) * Lz);
+			const int start = uniform_int(rg.x), cnt = uniform_int(rg.y - rg.x);
+			// four records per scalar load (the arrays carry SKG_EPAD spare records)
+			for (int t0 = 0; t0 < cnt; t0 += 4) {
+				float4 A4[4];
+				float B4[4];
+				int K4[4];
+#pragma unroll
+				for (int u = 0; u < 4; ++u) {
+					A4[u] = epos[start + t0 + u]; B4[u] = eb2[start + t0 + u]; K4[u] = ecode[start + t0 + u];
 				}
-				const float4 X = eaux[e];
-				if (MODE == 1) {
-					const float f = grad_factor(sq3(dx, dy, dz), X.y, X.z);
-					s0.add(fmul(dx, f)); s1.add(fmul(dy, f)); s2.add(fmul(dz, f));
-				} else if (MODE == 2) {
-					const float ih = 2.0f / sqrtf(eb2[e]), q = sqrtf(sq3(dx, dy, dz)) * ih;
-					float w, wp;
-					m4_derivs(q, w, wp);
-					s0.add(A.w * (3.0f * w + q * wp) * ih * ih * ih);
-				} else {
-					const float r = sqrtf(sq3(dx, dy, dz));
-					if (!(r > 0.f)) continue;
-					const float h = 0.5f * sqrtf(eb2[e]), ih = 1.0f / h;
-					float w, wp;
-					m4_derivs(r * ih, w, wp);
-					const int oi = sord[esrc[e]];
-					const float coef = ((parts & 1) ? gbar[oi] * mj : 0.f) + ((parts & 2) ? gj * A.w : 0.f);
-					const float c = 0.5f * wp * (float)M_1_PI * ih * ih * ih * ih * coef / r;
-					// dx_j -= c (r_i - r_j)/r with (r_i - r_j) = (dx, dy, dz)
-					s0.add(-c * dx); s1.add(-c * dy); s2.add(-c * dz);
+#pragma unroll
+				for (int u = 0; u < 4; ++u) {
+					if (t0 + u >= cnt) break;
+					hit(start + t0 + u, A4[u], B4[u], K4[u]);
 				}
 			}
 		}
@@ -184,6 +199,14 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restri
 	if (!valid) return;
 	if (MODE == 2) { out[ii] = s0.val() * (float)(-M_1_PI / 6.0); return; }
 	out[3 * ii] = s0.val(); out[3 * ii + 1] = s1.val(); out[3 * ii + 2] = s2.val();
+}
+
+// Each entity's weight for the dx adjoint: gbar of its source scatterer (0 for
+// fillers).
+__global__ void k_entity_weights(const int *__restrict__ esrc, const int *__restrict__ sord, int ne,
+		const float *__restrict__ gbar, float *ew) {
+	const int e = blockIdx.x * blockDim.x + threadIdx.x;
+	if (e < ne) ew[e] = esrc[e] >= 0 ? gbar[sord[esrc[e]]] : 0.f;
 }
 
 __global__ void k_full_b2(const float *__restrict__ eb2, int ne, float *b2) {
@@ -231,11 +254,11 @@ static int full_tree(skg_ctx *c) {
 
 template <int MODE>
 static void launch_cover(skg_ctx *c, const float4 *pts, int np, const int *pord, const float *gbar, int parts,
-		const int *pbox, float *out) {
+		const int *pbox, float *out, const float *ew = nullptr) {
 	const float *L = c->P.fPeriod;
 	const bool per = c->P.bPeriodic != 0;
-	k_adj_cover<MODE><<<nblk(np, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(pts, np, c->e_pos, c->e_aux, c->e_b2, c->e_org, c->e_src,
-			c->s_ord, c->nE, c->tfull.lo, c->tfull.hi, c->tfull.nLeaf2, c->tfull.B, c->d_mass, gbar, pord, parts,
+	k_adj_cover<MODE><<<nblk(np, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(pts, np, c->e_pos, c->e_aux, c->e_b2, c->e_org,
+			c->e_code, ew, c->nE, c->tfull.lo, c->tfull.hi, c->tfull.nLeaf2, c->tfull.B, c->d_mass, gbar, pord, parts,
 			per ? L[0] : 0.f, per ? L[1] : 0.f, per ? L[2] : 0.f, c->nBox > 1 ? pbox : nullptr, c->tfull.lstart, out);
 }
 
@@ -260,9 +283,9 @@ extern "C" int skg_density_adjoint(skg_ctx *c, const float *gbar, int parts, flo
 	(void)hipSetDevice(c->dev);
 	int rc;
 	const int n = c->n, nS = c->nS;
-	float *dg = nullptr, *ddx = nullptr, *ddh = nullptr, *part2 = nullptr, *gs = nullptr;
+	float *dg = nullptr, *ddx = nullptr, *ddh = nullptr, *part2 = nullptr, *gs = nullptr, *ew = nullptr;
 	if (sget(c, 0, &dg, n) || sget(c, 1, &ddx, 3 * (size_t)n) || sget(c, 2, &ddh, n) || sget(c, 3, &part2, 3 * (size_t)nS) ||
-	    sget(c, 5, &gs, nS))
+	    sget(c, 5, &gs, nS) || sget(c, 6, &ew, (size_t)c->nE + SKG_EPAD))
 		return SKG_E_NOMEM;
 	SKG_HIP(hipMemcpyAsync(dg, gbar, 4 * (size_t)n, hipMemcpyHostToDevice, c->st));
 	SKG_HIP(hipMemsetAsync(ddx, 0, 12 * (size_t)n, c->st));
@@ -272,7 +295,8 @@ extern "C" int skg_density_adjoint(skg_ctx *c, const float *gbar, int parts, flo
 	k_adj_gather<<<nblk(nS, 256), 256, 0, c->st>>>(c->s_heap, c->P.nSmooth, c->s_ord, c->s_inv, c->s_pos, gs, c->s_ball2,
 			nS, parts, per ? c->P.fPeriod[0] : 0.f, per, ddx, ddh);
 	if ((rc = full_tree(c))) return rc;
-	launch_cover<0>(c, c->s_pos, nS, c->s_ord, dg, parts, c->s_box, part2);
+	k_entity_weights<<<nblk(c->nE, 256), 256, 0, c->st>>>(c->e_src, c->s_ord, c->nE, dg, ew);
+	launch_cover<0>(c, c->s_pos, nS, c->s_ord, dg, parts, c->s_box, part2, ew);
 	k_add3_ord<<<nblk(nS, 256), 256, 0, c->st>>>(part2, c->s_ord, nS, ddx);
 	SKG_HIP(hipGetLastError());
 	SKG_HIP(hipMemcpyAsync(dx, ddx, 12 * (size_t)n, hipMemcpyDeviceToHost, c->st));

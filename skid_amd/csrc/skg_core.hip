@@ -2287,7 +2287,7 @@ extern "C" int skg_density(skg_ctx *c, float *fBall2, float *rho, int *nExtraSca
 		const int nE = 256 * c->h_eblk[nBox];
 		c->nE = nE;
 		unsigned long long *ekey = nullptr;
-		if (sget(c, 15, &ekey, nE) || dalloc(&c->e_b2, nE) || dalloc(&c->e_pos, nE + SKG_EPAD) || dalloc(&c->e_aux, nE + SKG_EPAD) || dalloc(&c->e_rho, nE) ||
+		if (sget(c, 15, &ekey, nE) || dalloc(&c->e_b2, nE + SKG_EPAD) || dalloc(&c->e_pos, nE + SKG_EPAD) || dalloc(&c->e_aux, nE + SKG_EPAD) || dalloc(&c->e_rho, nE) ||
 		    dalloc(&c->e_org, nE) || dalloc(&c->e_code, nE + SKG_EPAD) || dalloc(&c->e_src, nE) || dalloc(&c->e_live, nE) || dalloc(&c->e_hit, nE) ||
 		    dalloc(&c->e_box, nE) || dalloc(&c->d_eblk, nBox + 1))
 			return SKG_E_NOMEM;
