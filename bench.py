@@ -11,7 +11,7 @@ NFW halos (N/4000 of them) plus a 25% uniform background, nSmooth 32,
 nBucket as tuned, tau = 0.16 N_box^(-1/3) per box.  Three forms are timed:
 
   value        batched + adjoint + compensated: every box in the same launches
-               (skg_load_batch), fp32 two-sum sums, and each step also forms
+               (skg_load_batch), fp32 terms summed in FP64, and each step also forms
                the nSmooth sensitivity of every density (d rho / d ln k, the
                dh adjoint direction) and the dx/dh adjoint of the total
                density, and copies both to the host; launch shapes from the

@@ -40,7 +40,7 @@ extern "C" {
 
 /* Accumulation modes of the density / gradient sums. */
 #define SKG_ACC_PLAIN        0   /* fp32 running sums (the reference's arithmetic) */
-#define SKG_ACC_COMPENSATED  1   /* fp32 two-sum compensated sums (default) */
+#define SKG_ACC_COMPENSATED  1   /* fp32 terms summed in FP64 (default) */
 
 typedef struct skg_params {
 	int nSmooth;            /* -s, default 64 (main.c:103) */

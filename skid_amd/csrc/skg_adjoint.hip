@@ -45,7 +45,7 @@ __global__ __launch_bounds__(256) void k_adj_gather(const unsigned long long *__
 	const float4 pi = spos[i];
 	const float xi = pi.x, yi = pi.y, zi = pi.z, mi = pi.w, gi = gs[i];
 	const float h = 0.5f * sqrtf(sb2[i]), ih = 1.0f / h, norm = (float)(M_1_PI) * ih * ih * ih * ih;
-	Acc ax, ay, az, ahh;
+	Sum<true> ax, ay, az, ahh;
 	ax.init(); ay.init(); az.init(); ahh.init();
 	for (int s = 1; s < k; ++s) {
 		const unsigned long long key = heap[(size_t)s * nS + i];     // slot-major: coalesced
@@ -90,7 +90,7 @@ __global__ __launch_bounds__(256) void k_dlnk_gather(const unsigned long long *_
 	const int oi = sord[i];
 	const float xi = pos[3 * oi], yi = pos[3 * oi + 1], zi = pos[3 * oi + 2];
 	const float h = 0.5f * sqrtf(sb2[i]), ih = 1.0f / h;
-	Acc a;
+	Sum<true> a;
 	a.init();
 	for (int s = 1; s < k; ++s) {
 		const int j = (int)(unsigned)heap[(size_t)s * nS + i];
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restri
 	const int pb = pbox ? pbox[valid ? ii : base] : 0;
 	float mj = 0.f, gj = 0.f;
 	if (MODE == 0 && valid) { const int oj = pord[ii]; mj = mass[oj]; gj = gbar[oj]; }
-	Acc s0, s1, s2;
+	Sum<true> s0, s1, s2;
 	s0.init(); s1.init(); s2.init();
 	// One entity e (position and mass A, radius as built b2, image code K)
 	// against the lane's point.

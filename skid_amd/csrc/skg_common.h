@@ -247,7 +247,7 @@ __device__ __forceinline__ float sq3(float dx, float dy, float dz) {
 	return fadd(fadd(fmul(dx, dx), fmul(dy, dy)), fmul(dz, dz));
 }
 
-// Compensated (two-sum) accumulator.
+// Compensated (two-sum) accumulator: the group sums (centres, potentials).
 struct Acc {
 	float s, c;
 	__device__ __forceinline__ void init() { s = 0.f; c = 0.f; }
@@ -263,15 +263,28 @@ struct Acc {
 	// fold in another partial (its sum and its carried error)
 	__device__ __forceinline__ void merge(float os, float oc) { add(os); c = fadd(c, oc); }
 };
+// The per-point sums of the cover kernels.  Sum<true> (the compensated mode)
+// carries the sum of the fp32 terms in FP64: a term converts exactly and each
+// add rounds once at 2^-53 relative, so the fp32 result is the terms' exact sum
+// rounded once, up to n 2^-53 of the sum of their magnitudes (a two-sum
+// accumulator leaves (n u)^2 of it, u = 2^-24).  It costs two full-rate
+// instructions per term (convert, FP64 add) against a two-sum's seven.
+// Sum<false> is the plain fp32 sum.  xor_merge folds in the partial of the
+// lane o positions away (butterfly reductions over sub-wave groups).
 template <bool COMP> struct Sum;
-template <> struct Sum<true> : Acc {};
+template <> struct Sum<true> {
+	double s;
+	__device__ __forceinline__ void init() { s = 0.0; }
+	__device__ __forceinline__ void add(float x) { s = __dadd_rn(s, (double)x); }
+	__device__ __forceinline__ float val() const { return (float)s; }
+	__device__ __forceinline__ void xor_merge(int o) { s = __dadd_rn(s, __shfl_xor(s, o, 64)); }
+};
 template <> struct Sum<false> {
 	float s;
 	__device__ __forceinline__ void init() { s = 0.f; }
 	__device__ __forceinline__ void add(float x) { s = fadd(s, x); }
 	__device__ __forceinline__ float val() const { return s; }
-	__device__ __forceinline__ float cc() const { return 0.f; }
-	__device__ __forceinline__ void merge(float os, float) { add(os); }
+	__device__ __forceinline__ void xor_merge(int o) { s = fadd(s, __shfl_xor(s, o, 64)); }
 };
 
 // ---------------------------------------------------------------- box distances

@@ -824,7 +824,7 @@ __global__ __launch_bounds__(64 * W) void k_knn(
 	const float ih = dl ? 1.0f / (0.5f * sqrtf(h2)) : 0.f;
 	Sum<COMP> acc;
 	acc.init();
-	Acc dacc;
+	Sum<true> dacc;
 	dacc.init();
 	for (int s = 1; s < k; ++s) {
 		const unsigned long long key = heap[s * 64 + lane];
@@ -1358,9 +1358,9 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_cover_sub(
 	// fold the S partial sums of each point (xor partners stay within the group)
 #pragma unroll
 	for (int o = S / 2; o > 0; o >>= 1) {
-		s0.merge(__shfl_xor(s0.s, o, 64), __shfl_xor(s0.cc(), o, 64));
-		if (MODE == 1 || dl) s1.merge(__shfl_xor(s1.s, o, 64), __shfl_xor(s1.cc(), o, 64));
-		if (MODE == 1) s2.merge(__shfl_xor(s2.s, o, 64), __shfl_xor(s2.cc(), o, 64));
+		s0.xor_merge(o);
+		if (MODE == 1 || dl) s1.xor_merge(o);
+		if (MODE == 1) s2.xor_merge(o);
 	}
 	if (MODE == 1 && bthr) box_min_atomic(hmin, valid, pb, bthr);
 	if (!valid || sub != 0) return;

@@ -22,7 +22,7 @@ def gpu_density(box, k, period=1.0, **kw):
 def test_knn_density_config1(box1, k, per):
     """fBall2 bit-exact in both modes; the plain mode (the reference's per-term
     arithmetic) within 1e-5 of the reference, the compensated mode (exact
-    periodic separations, two-sum sums) within 1e-5 of the FP64 oracle."""
+    periodic separations, fp32 terms summed in FP64) within 1e-5 of the FP64 oracle."""
     kw = dict(nSmooth=k, fTau=0.005)
     if per:
         kw["fPeriod"] = 1.0
