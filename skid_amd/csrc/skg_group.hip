@@ -619,17 +619,49 @@ __global__ __launch_bounds__(COOP_T) void k_unb_coop(const int *__restrict__ bg,
 __device__ __forceinline__ float kernel_max(int soft, float twoh) {
 	return soft == SKG_PLUMMER ? 2.0f / twoh : 2.8f / twoh;     // K(0): spline 1.4*2/twoh, Plummer 2/twoh
 }
+// Block-wide reductions of k_unb_cand: a butterfly within each wavefront, then
+// the CAND_T / 64 wavefront results through LDS.  Larger E wins, ties to the
+// smaller index (argmax); argmin the same with smaller E.
+__device__ __forceinline__ void wave_argmax_d(double &E, int &i) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) {
+		const double oE = __shfl_xor(E, o, 64);
+		const int oi = __shfl_xor(i, o, 64);
+		if (oE > E || (oE == E && oi < i)) { E = oE; i = oi; }
+	}
+}
+__device__ __forceinline__ void wave_argmin_d(double &E, int &i) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) {
+		const double oE = __shfl_xor(E, o, 64);
+		const int oi = __shfl_xor(i, o, 64);
+		if (oE < E || (oE == E && oi < i)) { E = oE; i = oi; }
+	}
+}
+__device__ __forceinline__ double wave_max_d(double v) {
+#pragma unroll
+	for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+	return v;
+}
+
 __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ glist, const int *__restrict__ gs,
 		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv, float2 *uphi,
 		int *uord, int *grp, int *gcount, float *gmass, float4 *gvcm, float4 *grbound,
 		const float4 *__restrict__ grel, float fShift, float fCosmo, float z, float G, int soft, int bSub,
 		int bNoUnbind, float hx, int *nunb, unsigned long long *npairs, unsigned long long *nfull) {
-	__shared__ double rA[CAND_T], rB[CAND_T];
-	__shared__ int rI[CAND_T], rJ[CAND_T];
+	constexpr int NW = CAND_T / 64;
+	__shared__ double rA[CAND_T];
 	__shared__ int pend[CAND_T * CT];
-	__shared__ float4 bq, bv;
 	__shared__ double cmS[7];
-	const int g = glist[blockIdx.x], tid = threadIdx.x;
+	// full-pass results per wavefront
+	__shared__ double fCut[NW], fMin[NW], fR2[NW];
+	__shared__ int fIdx[NW], fEps[NW];
+	// removal rounds: each wavefront's best candidate, two buffers used in turn
+	// (a round's barrier keeps the next round's writes off the slots still read)
+	__shared__ double bE_s[2][NW];
+	__shared__ int bI_s[2][NW];
+	__shared__ float4 bQ_s[2][NW], bV_s[2][NW];
+	const int g = glist[blockIdx.x], tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
 	const int s = gs[g], n0 = ge[g] - s;
 	const double zf = 1.0 + (double)z;
 	// fp64 mass, centre of mass and velocity (kd.c:1360-1375)
@@ -656,7 +688,7 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 	int cI[CT];
 	float4 cQ[CT], cV[CT];
 	double cP[CT];
-	int n = n0, removed = 0, npend = 0, imin = 0;
+	int n = n0, removed = 0, npend = 0, imin = 0, rb = 0;
 	unsigned long long pairs = 0, fulls = 0;
 	double w0x = 0, w0y = 0, w0z = 0, Ecut = -INFINITY, Rmax = 0, grow = 0;
 	float epsMin = 0.f;
@@ -664,20 +696,21 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 	while (!stop) {
 		if (needFull) {
 			++fulls;
+			__syncthreads();                          // the last removal's pend entry and member marks
 			// potentials catch up with the removals since the last pass (kdSubPot, grav.c:39-60)
 			const double wx = (double)fShift * cm[4] + (double)fCosmo * cm[1];
 			const double wy = (double)fShift * cm[5] + (double)fCosmo * cm[2];
 			const double wz = (double)fShift * cm[6] + (double)fCosmo * cm[3];
-			double eCut = -INFINITY, rMax = 0.0, mE = INFINITY;
+			double eCut = -INFINITY, r2Max = 0.0, mE = INFINITY;
 			int mi = INT_MAX;
 			float eMin = INFINITY;
 			double cE[CT];
 			#pragma unroll
 			for (int c = 0; c < CT; ++c) { cI[c] = -1; cE[c] = -INFINITY; }
-			for (int i = tid; i < n0; i += CAND_T) {
-				if (uord[s + i] < 0) continue;
-				const float4 q = uq[s + i], v = uv[s + i];
-				float2 ph = uphi[s + i];
+			int lo = 0;                               // the slot holding the smallest kept energy
+			double loE = -INFINITY;
+			// one member (its loads issued with the next three members')
+			auto visit = [&](int i, const float4 &q, const float4 &v, float2 ph) {
 				if (bSub && npend) {
 					Acc ac; ac.s = ph.x; ac.c = ph.y;
 					for (int k = 0; k < npend; ++k) {
@@ -698,12 +731,10 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 				const double E = 0.5 * dd - phi * zf;
 				if (E < mE || (E == mE && i < mi)) { mE = E; mi = i; }
 				eMin = fminf(eMin, v.w);
-				// keep the CT largest; what is displaced feeds the bound
-				int lo = 0;
-				double loE = cE[0];
-				#pragma unroll
-				for (int c = 1; c < CT; ++c) if (cE[c] < loE) { loE = cE[c]; lo = c; }
-				double outE = E, outR = sqrt(dd);
+				// keep the CT largest; what is displaced feeds the bound (R as
+				// its square: the square root of the largest square is the
+				// largest root)
+				double outE = E, outR2 = dd;
 				if (E > loE) {
 					#pragma unroll
 					for (int c = 0; c < CT; ++c) {
@@ -712,33 +743,48 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 								const double ox = (double)fShift * (double)cV[c].x + (double)fCosmo * (double)cQ[c].x - wx;
 								const double oy = (double)fShift * (double)cV[c].y + (double)fCosmo * (double)cQ[c].y - wy;
 								const double oz = (double)fShift * (double)cV[c].z + (double)fCosmo * (double)cQ[c].z - wz;
-								outE = cE[c]; outR = sqrt(ox * ox + oy * oy + oz * oz);
-							} else { outE = -INFINITY; outR = 0.0; }
+								outE = cE[c]; outR2 = ox * ox + oy * oy + oz * oz;
+							} else { outE = -INFINITY; outR2 = 0.0; }
 							cI[c] = i; cQ[c] = q; cV[c] = v; cP[c] = phi; cE[c] = E;
 						}
 					}
+					lo = 0; loE = cE[0];
+					#pragma unroll
+					for (int c = 1; c < CT; ++c) if (cE[c] < loE) { loE = cE[c]; lo = c; }
 				}
 				if (outE > eCut) eCut = outE;
-				if (outR > rMax) rMax = outR;
-			}
-			// block reductions: E_cut, R, smallest softening, argmin E
-			rA[tid] = eCut; rB[tid] = mE; rI[tid] = mi; rJ[tid] = __float_as_int(eMin);
-			__syncthreads();
-			for (int o = CAND_T / 2; o > 0; o >>= 1) {
-				if (tid < o) {
-					rA[tid] = fmax(rA[tid], rA[tid + o]);
-					if (rB[tid + o] < rB[tid] || (rB[tid + o] == rB[tid] && rI[tid + o] < rI[tid])) { rB[tid] = rB[tid + o]; rI[tid] = rI[tid + o]; }
-					rJ[tid] = __float_as_int(fminf(__int_as_float(rJ[tid]), __int_as_float(rJ[tid + o])));
+				if (outR2 > r2Max) r2Max = outR2;
+			};
+			for (int i0 = tid; i0 < n0; i0 += 4 * CAND_T) {
+				int o4[4];
+				float4 q4[4], v4[4];
+				float2 p4[4];
+				#pragma unroll
+				for (int u = 0; u < 4; ++u) {
+					const int i = i0 + u * CAND_T;
+					o4[u] = -1;
+					if (i < n0) { o4[u] = uord[s + i]; q4[u] = uq[s + i]; v4[u] = uv[s + i]; p4[u] = uphi[s + i]; }
 				}
-				__syncthreads();
+				#pragma unroll
+				for (int u = 0; u < 4; ++u)
+					if (o4[u] >= 0) visit(i0 + u * CAND_T, q4[u], v4[u], p4[u]);
 			}
-			Ecut = rA[0]; imin = rI[0]; epsMin = __int_as_float(rJ[0]);
+			// block reductions: E_cut, R^2, smallest softening, argmin E
+			eCut = wave_max_d(eCut);
+			r2Max = wave_max_d(r2Max);
+			wave_argmin_d(mE, mi);
+			eMin = wave_min(eMin);
+			if (lane == 0) { fCut[wv] = eCut; fR2[wv] = r2Max; fMin[wv] = mE; fIdx[wv] = mi; fEps[wv] = __float_as_int(eMin); }
 			__syncthreads();
-			rA[tid] = rMax;
-			__syncthreads();
-			for (int o = CAND_T / 2; o > 0; o >>= 1) { if (tid < o) rA[tid] = fmax(rA[tid], rA[tid + o]); __syncthreads(); }
-			Rmax = rA[0];
-			__syncthreads();
+			eCut = fCut[0]; r2Max = fR2[0]; mE = fMin[0]; mi = fIdx[0]; eMin = __int_as_float(fEps[0]);
+			#pragma unroll
+			for (int w = 1; w < NW; ++w) {
+				eCut = fmax(eCut, fCut[w]);
+				r2Max = fmax(r2Max, fR2[w]);
+				if (fMin[w] < mE || (fMin[w] == mE && fIdx[w] < mi)) { mE = fMin[w]; mi = fIdx[w]; }
+				eMin = fminf(eMin, __int_as_float(fEps[w]));
+			}
+			Ecut = eCut; imin = mi; epsMin = eMin; Rmax = sqrt(r2Max);
 			w0x = wx; w0y = wy; w0z = wz;
 			grow = 0.0; npend = 0; needFull = false; lastWasFull = true;
 			if (bNoUnbind || n == 0) break;
@@ -755,17 +801,27 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 			const double E = 0.5 * (d0 * d0 + d1 * d1 + d2 * d2) - cP[c] * zf;
 			if (E > bE || (E == bE && cI[c] < bi)) { bE = E; bi = cI[c]; bc = c; }
 		}
-		rA[tid] = bE; rI[tid] = bi;
-		__syncthreads();
-		for (int o = CAND_T / 2; o > 0; o >>= 1) {
-			if (tid < o && (rA[tid + o] > rA[tid] || (rA[tid + o] == rA[tid] && rI[tid + o] < rI[tid]))) {
-				rA[tid] = rA[tid + o]; rI[tid] = rI[tid + o];
-			}
-			__syncthreads();
+		// the wavefront's best: its owner (or lane 0, with no candidate in the
+		// wavefront) publishes it with the member's records
+		double wE = bE;
+		int wI = bi;
+		wave_argmax_d(wE, wI);
+		if (wI == INT_MAX ? lane == 0 : (bc >= 0 && bi == wI)) {
+			bE_s[rb][wv] = wE; bI_s[rb][wv] = wI;
+			#pragma unroll
+			for (int c = 0; c < CT; ++c) if (c == bc) { bQ_s[rb][wv] = cQ[c]; bV_s[rb][wv] = cV[c]; }
 		}
-		const double BE = rA[0];
-		const int BI = rI[0];
 		__syncthreads();
+		double BE = bE_s[rb][0];
+		int BI = bI_s[rb][0], BW = 0;
+		#pragma unroll
+		for (int w = 1; w < NW; ++w) {
+			const double e = bE_s[rb][w];
+			const int j = bI_s[rb][w];
+			if (e > BE || (e == BE && j < BI)) { BE = e; BI = j; BW = w; }
+		}
+		const int rbw = rb;
+		rb ^= 1;
 		const double wx = (double)fShift * cm[4] + (double)fCosmo * cm[1];
 		const double wy = (double)fShift * cm[5] + (double)fCosmo * cm[2];
 		const double wz = (double)fShift * cm[6] + (double)fCosmo * cm[3];
@@ -784,13 +840,12 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 		// remove BI (owned by one thread's candidate slot)
 		if (bc >= 0 && bi == BI) {
 			#pragma unroll
-			for (int c = 0; c < CT; ++c) if (c == bc) { bq = cQ[c]; bv = cV[c]; cI[c] = -1; }
+			for (int c = 0; c < CT; ++c) if (c == bc) cI[c] = -1;
 			const int ob = uord[s + BI];
 			grp[ob] = 0;
 			uord[s + BI] = ~ob;
 		}
-		__syncthreads();
-		const float4 qb = bq, vb = bv;
+		const float4 qb = bQ_s[rbw][BW], vb = bV_s[rbw][BW];
 		--n; ++removed;
 		if (n == 0) { cm[0] = 0.0; cm[4] = cm[5] = cm[6] = 0.0; stop = true; break; }
 		cm[0] -= (double)qb.w;
@@ -810,10 +865,10 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 			++npend;
 			if (npend >= CAND_T * CT) needFull = true;
 		}
-		__syncthreads();
 	}
 	// iMin over all members in the final state (kd.c:1400-1408 of the last pass)
 	if (!lastWasFull && n > 0) {
+		__syncthreads();                              // the last removals' member marks
 		double mE = INFINITY;
 		int mi = INT_MAX;
 		for (int i = tid; i < n0; i += CAND_T) {
@@ -834,15 +889,14 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 			const double E = 0.5 * (d0 * d0 + d1 * d1 + d2 * d2) - phi * zf;
 			if (E < mE || (E == mE && i < mi)) { mE = E; mi = i; }
 		}
-		rB[tid] = mE; rI[tid] = mi;
+		wave_argmin_d(mE, mi);
+		if (lane == 0) { fMin[wv] = mE; fIdx[wv] = mi; }
 		__syncthreads();
-		for (int o = CAND_T / 2; o > 0; o >>= 1) {
-			if (tid < o && (rB[tid + o] < rB[tid] || (rB[tid + o] == rB[tid] && rI[tid + o] < rI[tid]))) {
-				rB[tid] = rB[tid + o]; rI[tid] = rI[tid + o];
-			}
-			__syncthreads();
-		}
-		imin = rI[0];
+		mE = fMin[0]; mi = fIdx[0];
+		#pragma unroll
+		for (int w = 1; w < NW; ++w)
+			if (fMin[w] < mE || (fMin[w] == mE && fIdx[w] < mi)) { mE = fMin[w]; mi = fIdx[w]; }
+		imin = mi;
 	}
 	if (tid == 0) {
 		gcount[g] = n;
