@@ -144,8 +144,9 @@ struct skg_ctx {
 	long s64Max = 0;            // at most this many active movers: 64 lanes (a wavefront) per mover
 	// per-tile lists of the current 5-step block: entities (SKG_TILE_LISTS=1,
 	// default), leaves (2) or none (0: every step walks the tree); tlCap
-	// entries per tile (a multiple of 4)
-	int useLists = 1, tlCap = 1024;
+	// entries per tile (a multiple of 4; for entity lists set from the mover
+	// count when the move loop starts, unless tlCapFixed)
+	int useLists = 1, tlCap = 1024, tlCapFixed = 0;
 	int *tl_list = nullptr, *tl_cnt = nullptr;
 	// device-resident move loop: ml[0] step stamp, ml[1] active, ml[2] next
 	int *d_ml = nullptr;
