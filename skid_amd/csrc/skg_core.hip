@@ -960,6 +960,7 @@ __device__ __forceinline__ void box_min_atomic(float hmin, bool valid, int pb, i
 // gradient (ax, ay, az), then the periodic wrap into the box; a wrap sets the
 // mover's flag (mwrap) that sends its wavefront back to the tree walk for the
 // rest of the block.
+#define MWRAP_OUT 8
 __device__ __forceinline__ void move_point(float4 *__restrict__ ppos, int pi, const float4 &p, float ax, float ay,
 		float az, float fStep, float cx, float cy, float cz, float Lx, float Ly, float Lz, int *__restrict__ mwrap) {
 	const float ss = sq3(ax, ay, az);
@@ -968,13 +969,18 @@ __device__ __forceinline__ void move_point(float4 *__restrict__ ppos, int pi, co
 	else ai = 0.0f;
 	float r[3] = {fsub(p.x, fmul(ai, ax)), fsub(p.y, fmul(ai, ay)), fsub(p.z, fmul(ai, az))};
 	const float C[3] = {cx, cy, cz}, LL[3] = {Lx, Ly, Lz};
-	bool wrapped = false;
+	int wrapped = 0;                                 // axes crossed, one bit each
 	for (int j = 0; j < 3; ++j) {
-		if ((double)r[j] > (double)C[j] + 0.5 * (double)LL[j]) { r[j] = fsub(r[j], LL[j]); wrapped = true; }
-		if ((double)r[j] <= (double)C[j] - 0.5 * (double)LL[j]) { r[j] = fadd(r[j], LL[j]); wrapped = true; }
+		if ((double)r[j] > (double)C[j] + 0.5 * (double)LL[j]) { r[j] = fsub(r[j], LL[j]); wrapped |= 1 << j; }
+		if ((double)r[j] <= (double)C[j] - 0.5 * (double)LL[j]) { r[j] = fadd(r[j], LL[j]); wrapped |= 1 << j; }
 	}
 	ppos[pi] = make_float4(r[0], r[1], r[2], p.w);
-	if (wrapped && mwrap) mwrap[pi] = 1;
+	// mwrap holds the axis whose far side the block's list covers (1 + axis,
+	// 0 for none) and gains MWRAP_OUT when the mover crosses another face
+	if (wrapped && mwrap) {
+		const int code = mwrap[pi] & 7;
+		if (code == 0 || code > 3 || (wrapped & ~(1 << (code - 1)))) mwrap[pi] = code | MWRAP_OUT;
+	}
 }
 
 // Leaf lists for a block of move steps.  Entities do not move and ScatterCut
@@ -984,9 +990,11 @@ __device__ __forceinline__ void move_point(float4 *__restrict__ ppos, int pi, co
 // lane's point now.  One walk per block then replaces one walk per step.  A
 // tile with more than cap entries gets count -1 and walks the tree.  Near a
 // periodic face the listed entities include the replicas there; a mover that
-// crosses the face during the block jumps to the far side, which its list
-// does not cover, so the move step flags it (mwrap, cleared here) and its
-// wavefront walks the tree for the rest of the block.
+// crosses the face during the block jumps to the far side, which this list
+// does not cover, so the move step flags it (mwrap, cleared here: no face
+// covered) and its wavefront walks the tree for the rest of the block.  The
+// fused build (k_cover_build) also lists the far side of a lane's one near
+// face.
 //
 // ELIST: the list holds entities instead of leaves -- every live entity whose
 // ball, widened by the margin, holds a lane's point now, in tree order (leaf
@@ -1116,7 +1124,7 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 	int nl = tcnt ? uniform_int(tcnt[tile]) : -1;
 	// a lane that crossed a periodic face since the list was built is on the
 	// far side, which the list does not cover: the wavefront walks the tree
-	if (nl >= 0 && mwrap && __any(valid && mwrap[pi] != 0)) nl = -1;
+	if (nl >= 0 && mwrap && __any(valid && (mwrap[pi] & MWRAP_OUT) != 0)) nl = -1;
 	int node = nl >= 0 ? 0 : 1;
 	if (elist && nl > 0) {
 		// entity list (k_cover_lists<true>): the listed entities in tree
@@ -1228,8 +1236,8 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		const float4 *__restrict__ eaux, const float4 *__restrict__ eorg, const int *__restrict__ ecode, int ne,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int B, int *__restrict__ ehit,
 		int *__restrict__ bthr, const int *__restrict__ pbox, const DevBox *__restrict__ bt, int micro,
-		float msteps, float mabs, float cx, float cy, float cz, float Lx, float Ly, float Lz,
-		const int *__restrict__ lstart, int cap, int *__restrict__ tlist, int *__restrict__ tcnt,
+		float msteps, float mabs, int per, float wsteps, float cx, float cy, float cz, float Lx, float Ly,
+		float Lz, const int *__restrict__ lstart, int cap, int *__restrict__ tlist, int *__restrict__ tcnt,
 		const int *__restrict__ npDev, const int *__restrict__ stampDev, int *__restrict__ mwrap) {
 	if (npDev) np = *npDev;
 	const int stamp = stampDev ? *stampDev : 0;
@@ -1244,7 +1252,31 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 	const int pb = pbox ? pbox[pi] : 0;
 	const float fStep = micro ? bt[pb].fMicro : bt[pb].fStep;
 	const float margin = fmaf(msteps, fStep, mabs);
-	if (valid && mwrap) mwrap[pi] = 0;
+	// A lane within wsteps steps of exactly one face (periodic) may cross it
+	// during the block: the list then also holds what reaches the lane's image
+	// q across that face (with the full margin), and mwrap records the axis
+	// (1 + axis), so that a crossing of that face leaves the list valid.  A
+	// lane near two or more faces gets code 4, one near none 0: then any
+	// crossing sends its wavefront to the tree walk.
+	float4 q = p;
+	int code = 0;
+	if (per) {
+		const float cc[3] = {cx, cy, cz}, LL[3] = {Lx, Ly, Lz}, pp[3] = {p.x, p.y, p.z};
+		const float wm = fmaf(wsteps, fStep, mabs);
+		float qq[3] = {p.x, p.y, p.z};
+		int nf = 0;
+#pragma unroll
+		for (int j = 0; j < 3; ++j) {
+			const float h = 0.5f * LL[j];
+			if (pp[j] + wm > cc[j] + h) { qq[j] = pp[j] - LL[j]; code = 1 + j; ++nf; }
+			if (pp[j] - wm <= cc[j] - h) { qq[j] = pp[j] + LL[j]; code = 1 + j; ++nf; }
+		}
+		if (nf > 1) code = 4;
+		if (code >= 1 && code <= 3) q = make_float4(qq[0], qq[1], qq[2], p.w);
+	}
+	const bool img = valid && code >= 1 && code <= 3;
+	const bool wimg = __any(img);
+	if (valid && mwrap) mwrap[pi] = code;
 	Sum<COMP> s0, s1, s2;
 	s0.init(); s1.init(); s2.init();
 	float hmin = INFINITY;
@@ -1264,8 +1296,9 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		if (valid && lo.w >= 0.f && brange_has(hi.w, pb)) {
 			// hardware square root (about 1 ulp): covered by the factor below
 			c = __builtin_amdgcn_sqrtf(lo.w);
-			const float r = c + margin;
-			need = point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < r * r * 1.000001f;
+			const float r = c + margin, r2 = r * r * 1.000001f;
+			need = point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < r2;
+			if (wimg && img && !need) need = point_box_d2<false>(lo, hi, q.x, q.y, q.z, 0.f) < r2;
 		}
 		if (__any(need)) {
 			if (node < nLeaf2) { node = 2 * node; continue; }
@@ -1288,7 +1321,10 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 					if (t0 + u >= cnt) break;
 					if (X4[u].x < 0.f) continue;          // cut (or filler): neither listed nor hit
 					const float dx = fsub(A4[u].x, p.x), dy = fsub(A4[u].y, p.y), dz = fsub(A4[u].z, p.z);
-					const bool listed = need && sq3(dx, dy, dz) < fmaf(X4[u].x, k1, k2);
+					const float reach = fmaf(X4[u].x, k1, k2);
+					bool listed = need && sq3(dx, dy, dz) < reach;
+					if (wimg && img && need && !listed)
+						listed = sq3(fsub(A4[u].x, q.x), fsub(A4[u].y, q.y), fsub(A4[u].z, q.z)) < reach;
 					if (__any(listed)) emit(start + t0 + u);
 					cover_hit<1, COMP>(start + t0 + u, A4[u], X4[u], K4[u], need, p, eorg, ehit, stamp, hmin, Lx, Ly,
 							Lz, s0, s1, s2);
@@ -2403,6 +2439,10 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 	// entity lists: built by the block's first step itself (k_cover_build), or
 	// by a pass of their own before it (SKG_FUSED_BUILD=0, and leaf lists)
 	const bool fusedBuild = lists && c->useLists != 2 && c->fusedBuild;
+	// lanes within this many steps of a face get their image listed too: they
+	// can cross it in time for two or more of the block's list steps (closer
+	// to the margin the build walks more than the list steps save)
+	const float wsteps = 1.02f * (nsteps - 2);
 	if (lists && !fusedBuild) enqueue_lists(c, cap, nsteps, micro);
 	for (int i = 0; i < nsteps; ++i) {
 		k_bump<<<1, 1, 0, c->st>>>(c->d_ml);
@@ -2433,7 +2473,8 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 #define MB(C_)                                                                                                \
 		k_cover_build<C_><<<nblk(cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>(c->m_pos, c->m_list, (int)cap, c->e_pos, \
 				c->e_aux, c->e_org, c->e_code, c->nE, c->te.lo, c->te.hi, c->te.nLeaf2, c->te.B, ehit, bthr, pbox,   \
-				c->d_bt, micro, 1.02f * (nsteps - 1), mabs, C[0], C[1], C[2], L[0], L[1], L[2], c->te.lstart,        \
+				c->d_bt, micro, 1.02f * (nsteps - 1), mabs, c->P.bPeriodic, wsteps, C[0], C[1], C[2], L[0], L[1], L[2], \
+				c->te.lstart,                                                                                        \
 				c->tlCap, c->tl_list, c->tl_cnt, c->d_ml + 1, c->d_ml, c->m_wrap)
 		if (fusedBuild && i == 0) { if (comp) { MB(true); } else { MB(false); } }
 		else if (comp) { MV(true); } else { MV(false); }
