@@ -29,13 +29,15 @@ __device__ __forceinline__ int uf_find(int *par, int x) {
 	}
 }
 
-__device__ __forceinline__ void uf_union(int *par, int a, int b) {
+// Returns the root the two components share once merged (a root of that
+// component at the time: other merges may hook it under a smaller one later).
+__device__ __forceinline__ int uf_union(int *par, int a, int b) {
 	for (;;) {
 		a = uf_find(par, a);
 		b = uf_find(par, b);
-		if (a == b) return;
+		if (a == b) return a;
 		if (a > b) { int t = a; a = b; b = t; }
-		if (atomicCAS(&par[b], b, a) == b) return;
+		if (atomicCAS(&par[b], b, a) == b) return a;
 	}
 }
 
@@ -70,6 +72,10 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restri
 	const float3 blo = make_float3(wave_min(q.x), wave_min(q.y), wave_min(q.z));
 	const float3 bhi = make_float3(wave_max(q.x), wave_max(q.y), wave_max(q.z));
 	int node = 1, lev = 0;
+	// a node of the lane's own component (components only ever merge, so one
+	// seen earlier stays one): a point whose parent is that node is already
+	// linked to the lane's point, and its union is skipped
+	int mine = i;
 	for (;;) {
 		node = uniform_int(node);
 		lev = uniform_int(lev);
@@ -95,7 +101,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restri
 			if (contained) {
 				// every member is within tau of q: merge q with the node, and the
 				// node's members with each other once.
-				uf_union(par, i, first);
+				mine = uf_union(par, i, first);
 				if (atomicCAS(&united[node], 0, 1) == 0)
 					for (int j = first + 1; j < last; ++j) uf_union(par, first, j);
 			}
@@ -106,6 +112,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restri
 					for (int j = max(first, i + 1); j < last; ++j) {
 						if (fbox && fbox[j] != qb) continue;
 						const float4 c = fp[j];
+						const int pj = __hip_atomic_load(&par[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 						float dx = fsub(q.x, c.x), dy = fsub(q.y, c.y), dz = fsub(q.z, c.z);
 						// images only where a delta exceeds half a period (the values
 						// img_delta gives); otherwise the pair seen from j is the same
@@ -124,7 +131,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restri
 							float ez = img_delta<PER>(c.z, fsub(c.z, L), fadd(c.z, L), q.z, hL);
 							link = sq3(ex, ey, ez) < tau2;
 						}
-						if (link) uf_union(par, i, j);
+						if (link && pj != mine) mine = uf_union(par, i, j);
 					}
 				}
 			}
