@@ -109,8 +109,9 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restri
 			if (__any(want)) {
 				if (node < nLeaf2) descend = true;
 				else if (want) {
+					const bool oneBox = !fbox || brange_is(hi.w, qb);      // the leaf holds only the lane's box
 					for (int j = max(first, i + 1); j < last; ++j) {
-						if (fbox && fbox[j] != qb) continue;
+						if (!oneBox && fbox[j] != qb) continue;
 						const float4 c = fp[j];
 						const int pj = __hip_atomic_load(&par[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 						float dx = fsub(q.x, c.x), dy = fsub(q.y, c.y), dz = fsub(q.z, c.z);
@@ -131,7 +132,7 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restri
 							float ez = img_delta<PER>(c.z, fsub(c.z, L), fadd(c.z, L), q.z, hL);
 							link = sq3(ex, ey, ez) < tau2;
 						}
-						if (link && pj != mine) mine = uf_union(par, i, j);
+						if (link && pj != mine) mine = uf_union(par, mine, j);
 					}
 				}
 			}
