@@ -493,6 +493,23 @@ __global__ void k_level_ball(float4 *lo, int first, int count) {
 	lo[i].w = fmaxf(lo[2 * i].w, lo[2 * i + 1].w);
 }
 
+// The levels of at most mtop nodes in one workgroup, level by level (one
+// launch instead of one per level).  A level reads what the one below wrote
+// in the same launch: stores complete to L2 (agent fence) before the barrier,
+// and the reads bypass the CU's L1.
+__global__ __launch_bounds__(1024) void k_level_ball_top(float4 *lo, int mtop) {
+	for (int m = mtop; m >= 1; m >>= 1) {
+		for (int t = threadIdx.x; t < m; t += blockDim.x) {
+			const int i = m + t;
+			const float a = __hip_atomic_load(&lo[2 * i].w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			const float b = __hip_atomic_load(&lo[2 * i + 1].w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			lo[i].w = fmaxf(a, b);
+		}
+		__threadfence();
+		__syncthreads();
+	}
+}
+
 void skg_free_tree(DevTree *t) { dfree(&t->lo); dfree(&t->hi); dfree(&t->lstart); t->n = t->nLeaf = t->nLeaf2 = 0; }
 
 static int tree_fill(skg_ctx *c, DevTree *t, const float4 *pos, const float *b2, const int *ibox) {
@@ -1678,10 +1695,11 @@ __global__ void k_aux_rho(float4 *eaux, const float *__restrict__ erho, int ne) 
 
 // Per-step reset for the move kernel's threshold atomics: every box's slots
 // to +inf, and the live total k_box_sum adds into to 0.
-__global__ void k_thr_reset(float *slots, int n, int *liveTotal) {
+// ml (optional): also advances the step stamp (ml[0]), one launch fewer per step
+__global__ void k_thr_reset(float *slots, int n, int *liveTotal, int *ml) {
 	const int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i < n) slots[i] = INFINITY;
-	if (i == 0) liveTotal[0] = 0;
+	if (i == 0) { liveTotal[0] = 0; if (ml) ml[0] += 1; }
 }
 // thr[b] = the minimum over box b's slots (one wavefront per box).
 __global__ __launch_bounds__(64) void k_thr_fold(const float *__restrict__ slots, float *thr) {
@@ -1745,14 +1763,19 @@ __global__ void k_cut_bounds(const unsigned long long *__restrict__ key, int ne,
 		if (i < nBox) pos[i] = a;
 	}
 }
-__global__ __launch_bounds__(256) void k_cut_sorted(const float *__restrict__ thr, const int *__restrict__ off,
+__global__ __launch_bounds__(256) void k_cut_sorted(float *thr, const int *__restrict__ off,
 		int *pos, const float *__restrict__ srho, const int *__restrict__ perm, int *elive, float4 *eaux,
-		int *bscat, int *liveTotal, const int *__restrict__ eleaf, int *dirty) {
+		int *bscat, int *liveTotal, const int *__restrict__ eleaf, int *dirty, const float *__restrict__ slots) {
 	__shared__ int snew;
 	const int b = blockIdx.x;
 	const int lo = pos[b], hi = off[b + 1];
+	// slots (optional): the box's SKG_THR_SLOTS partial minima, folded here
+	// (k_thr_fold's work, one launch fewer per step)
+	float tf = 0.f;
+	if (slots && threadIdx.x < 64) tf = wave_min(slots[(size_t)b * SKG_THR_SLOTS + threadIdx.x]);
 	if (threadIdx.x == 0) {
-		const float t0 = thr[b];
+		if (slots) thr[b] = tf;
+		const float t0 = slots ? tf : thr[b];
 		const float t = (t0 == INFINITY) ? 0.f : t0;
 		int a = lo, z = hi;
 		while (a < z) { const int m = (a + z) >> 1; if (srho[m] >= t) z = m; else a = m + 1; }
@@ -2445,7 +2468,6 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 	const float wsteps = 1.02f * (nsteps - 2);
 	if (lists && !fusedBuild) enqueue_lists(c, cap, nsteps, micro);
 	for (int i = 0; i < nsteps; ++i) {
-		k_bump<<<1, 1, 0, c->st>>>(c->d_ml);
 		// the initial step stamps its hits (unhit entities get density 0);
 		// later steps take the threshold from the move kernel itself
 		// (SKG_FUSED_THR=0: stamps and a pass over the entities every step)
@@ -2454,7 +2476,9 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 		int *bthr = stamps ? nullptr : (int *)c->d_bslot;
 		if (!stamps)
 			k_thr_reset<<<nblk((long)c->nBox * SKG_THR_SLOTS, 256), 256, 0, c->st>>>(c->d_bslot,
-					c->nBox * SKG_THR_SLOTS, c->d_cnt);
+					c->nBox * SKG_THR_SLOTS, c->d_cnt, c->d_ml);
+		else
+			k_bump<<<1, 1, 0, c->st>>>(c->d_ml);
 		const int pslot = capturing ? -1 : skg_prof_begin(c);
 #define MV(C_)                                                                                                \
 		if (sub && cap <= c->s64Max) k_cover_sub<1, C_, 64><<<nblk(64L * cap, SKG_TILE_T), SKG_TILE_T, 0, c->st>>>( \
@@ -2487,12 +2511,13 @@ static int enqueue_steps(skg_ctx *c, long cap, bool sub, bool lists, int nsteps,
 			k_scat_thr<<<nb, 256, 0, c->st>>>(c->e_live, c->e_hit, c->e_rho, c->e_aux, c->nE, c->d_ml, bInitial,
 					c->part);
 			k_box_min<<<c->nBox, 64, 0, c->st>>>(c->part, c->d_eblk, c->d_bthr, c->d_cnt);
-		} else {
+		} else if (!(c->cutReady && !bInitial)) {
 			k_thr_fold<<<c->nBox, SKG_THR_SLOTS, 0, c->st>>>(c->d_bslot, c->d_bthr);
 		}
 		if (c->cutReady && !bInitial) {
 			k_cut_sorted<<<c->nBox, 256, 0, c->st>>>(c->d_bthr, c->cs_off, c->cs_pos, c->cs_rho, c->cs_perm,
-					c->e_live, c->e_aux, c->d_bscat, c->d_cnt, c->e_leaf, c->leafDirty);
+					c->e_live, c->e_aux, c->d_bscat, c->d_cnt, c->e_leaf, c->leafDirty,
+					stamps ? nullptr : c->d_bslot);
 		} else {
 			k_scat_cut<<<nb, 256, 0, c->st>>>(c->e_live, c->e_aux, c->e_rho, c->nE, c->e_box, c->d_bthr,
 					(int *)c->part);
@@ -2533,8 +2558,9 @@ static int refit_entities(skg_ctx *c) {
 	else
 		k_leaf_ball<<<nblk(t->nLeaf2, 256), 256, 0, c->st>>>(c->e_aux, t->n, t->B, t->nLeaf, t->nLeaf2, t->lstart,
 				t->lo);
-	for (int m = t->nLeaf2 >> 1; m >= 1; m >>= 1)
-		k_level_ball<<<nblk(m, 256), 256, 0, c->st>>>(t->lo, m, m);
+	int m = t->nLeaf2 >> 1;
+	for (; m > 2048; m >>= 1) k_level_ball<<<nblk(m, 256), 256, 0, c->st>>>(t->lo, m, m);
+	if (m >= 1) k_level_ball_top<<<1, 1024, 0, c->st>>>(t->lo, m);
 	SKG_HIP(hipGetLastError());
 	return 0;
 }
