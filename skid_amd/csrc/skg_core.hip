@@ -1258,10 +1258,14 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		const int *__restrict__ npDev, const int *__restrict__ stampDev, int *__restrict__ mwrap) {
 	if (npDev) np = *npDev;
 	const int stamp = stampDev ? *stampDev : 0;
+	// staging slots for an opened leaf's entity records, 64 per wavefront
+	__shared__ float4 s_bA[SKG_TILE_T], s_bX[SKG_TILE_T];
+	__shared__ int s_bK[SKG_TILE_T];
 	const int lane = threadIdx.x & 63;
 	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
 	if (base >= np) return;                       // whole wavefront past the end
 	const int tile = base >> 6;
+	const int wb = threadIdx.x & ~63;
 	const int ii = base + lane;
 	const bool valid = ii < np;
 	const int pi = plist ? plist[valid ? ii : base] : (valid ? ii : base);
@@ -1324,28 +1328,37 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 			const float k2 = margin * (c + margin) * 1.000002f;
 			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
 			const int start = uniform_int(rg.x), cnt = uniform_int(rg.y - rg.x);
-			for (int t0 = 0; t0 < cnt; t0 += 4) {
-				// four records per scalar load (the arrays carry SKG_EPAD spare records)
-				float4 A4[4], X4[4];
-				int K4[4] = {13, 13, 13, 13};
-#pragma unroll
-				for (int u = 0; u < 4; ++u) {
-					A4[u] = epos[start + t0 + u]; X4[u] = eaux[start + t0 + u];
-					if (COMP) K4[u] = ecode[start + t0 + u];
+			// the leaf's records staged in this wavefront's LDS slots, 64 at a
+			// time (lane j reads record j: one round trip), then read by every
+			// lane at one address; the wavefront reads back only what it wrote,
+			// and its LDS operations complete in order
+			for (int c0 = 0; c0 < cnt; c0 += 64) {
+				const int m = min(64, cnt - c0);
+				if (lane < m) {
+					s_bA[threadIdx.x] = epos[start + c0 + lane];
+					s_bX[threadIdx.x] = eaux[start + c0 + lane];
+					s_bK[threadIdx.x] = COMP ? ecode[start + c0 + lane] : 13;
 				}
-#pragma unroll
-				for (int u = 0; u < 4; ++u) {
-					if (t0 + u >= cnt) break;
-					if (X4[u].x < 0.f) continue;          // cut (or filler): neither listed nor hit
-					const float dx = fsub(A4[u].x, p.x), dy = fsub(A4[u].y, p.y), dz = fsub(A4[u].z, p.z);
-					const float reach = fmaf(X4[u].x, k1, k2);
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+				for (int t = 0; t < m; ++t) {
+					const float4 A = s_bA[wb + t], X = s_bX[wb + t];
+					if (X.x < 0.f) continue;              // cut (or filler): neither listed nor hit
+					const int e = start + c0 + t;
+					const float dx = fsub(A.x, p.x), dy = fsub(A.y, p.y), dz = fsub(A.z, p.z);
+					const float reach = fmaf(X.x, k1, k2);
 					bool listed = need && sq3(dx, dy, dz) < reach;
 					if (wimg && img && need && !listed)
-						listed = sq3(fsub(A4[u].x, q.x), fsub(A4[u].y, q.y), fsub(A4[u].z, q.z)) < reach;
-					if (__any(listed)) emit(start + t0 + u);
-					cover_hit<1, COMP>(start + t0 + u, A4[u], X4[u], K4[u], need, p, eorg, ehit, stamp, hmin, Lx, Ly,
-							Lz, s0, s1, s2);
+						listed = sq3(fsub(A.x, q.x), fsub(A.y, q.y), fsub(A.z, q.z)) < reach;
+					if (__any(listed)) emit(e);
+					cover_hit<1, COMP>(e, A, X, s_bK[wb + t], need, p, eorg, ehit, stamp, hmin, Lx, Ly, Lz, s0, s1,
+							s2);
 				}
+				// every lane has read these records before the next chunk overwrites them
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 			}
 		}
 		node = (node >> __builtin_ctz(~node)) + 1;
