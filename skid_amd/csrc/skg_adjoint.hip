@@ -123,6 +123,11 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restri
 		float *out) {
 	const int lane = threadIdx.x & 63, base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63),
 			ii = base + lane;
+	// staging slots for an opened leaf's entity records, 64 per wavefront
+	__shared__ float4 sA[SKG_TILE_T];
+	__shared__ float sB[SKG_TILE_T];
+	__shared__ int sK[SKG_TILE_T];
+	const int wb = threadIdx.x & ~63;
 	if (base >= np) return;                       // whole wavefront past the end
 	const bool valid = ii < np;
 	const float4 p = pts[valid ? ii : base];
@@ -172,23 +177,29 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_adj_cover(const float4 *__restri
 		const bool need = valid && brange_has(hi.w, pb) && point_box_d2_np(lo, hi, p) < lo.w;
 		const bool visit = __any(need);
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
-		if (visit && need) {
+		if (visit) {
 			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
 			const int start = uniform_int(rg.x), cnt = uniform_int(rg.y - rg.x);
-			// four records per scalar load (the arrays carry SKG_EPAD spare records)
-			for (int t0 = 0; t0 < cnt; t0 += 4) {
-				float4 A4[4];
-				float B4[4];
-				int K4[4];
-#pragma unroll
-				for (int u = 0; u < 4; ++u) {
-					A4[u] = epos[start + t0 + u]; B4[u] = eb2[start + t0 + u]; K4[u] = ecode[start + t0 + u];
+			// the leaf's records staged in this wavefront's LDS slots, 64 at a
+			// time (lane j reads record j: one round trip), then read by every
+			// lane with need at one address; the wavefront reads back only what
+			// it wrote, and its LDS operations complete in order
+			for (int c0 = 0; c0 < cnt; c0 += 64) {
+				const int m = min(64, cnt - c0);
+				if (lane < m) {
+					sA[threadIdx.x] = epos[start + c0 + lane];
+					sB[threadIdx.x] = eb2[start + c0 + lane];
+					sK[threadIdx.x] = ecode[start + c0 + lane];
 				}
-#pragma unroll
-				for (int u = 0; u < 4; ++u) {
-					if (t0 + u >= cnt) break;
-					hit(start + t0 + u, A4[u], B4[u], K4[u]);
-				}
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+				if (need)
+					for (int t = 0; t < m; ++t) hit(start + c0 + t, sA[wb + t], sB[wb + t], sK[wb + t]);
+				// every lane has read these records before the next chunk overwrites them
+				__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+				__builtin_amdgcn_wave_barrier();
+				__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 			}
 		}
 		int nx = node;

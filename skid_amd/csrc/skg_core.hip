@@ -920,32 +920,35 @@ __device__ __forceinline__ void cover_hit(int e, const float4 &A, const float4 &
 	}
 }
 
-// The cnt entities from start against the lane's point (lanes with want),
-// through cover_hit.  Records are read four at a time (one wait each).
+// The same scan called by the whole wavefront (want per lane): the leaf's
+// records are staged 64 at a time in the wavefront's LDS slots sA/sX/sK (lane
+// j reads record j, one round trip), then every lane with want reads each
+// staged record at one address.  The wavefront reads back only what it wrote,
+// and its LDS operations complete in order.
 template <int MODE, bool COMP>
-__device__ __forceinline__ void cover_scan(int start, int cnt, bool want, const float4 &p,
+__device__ __forceinline__ void cover_scan_staged(int start, int cnt, bool want, const float4 &p,
 		const float4 *__restrict__ epos, const float4 *__restrict__ eaux, const float4 *__restrict__ eorg,
 		const int *__restrict__ ecode, int *__restrict__ ehit, int stamp, float &hmin, float Lx, float Ly,
-		float Lz, Sum<COMP> &s0, Sum<COMP> &s1, Sum<COMP> &s2, bool dl = false) {
-	if (cnt <= 0 || !want) return;
-	for (int t0 = 0; t0 < cnt; t0 += 4) {
-		// four consecutive records in one scalar load each; the entity arrays
-		// carry SKG_EPAD records past the last entity, so the reads past a
-		// leaf's end stay in the allocation (their values are never used)
-		const float4 *ap = epos + start + t0, *xp = eaux + start + t0;
-		float4 A4[4], X4[4];
-		int K4[4] = {13, 13, 13, 13};                       // image codes: a hit waits on no load
-#pragma unroll
-		for (int u = 0; u < 4; ++u) {
-			A4[u] = ap[u]; X4[u] = xp[u];
-			if (COMP || (MODE == 0 && dl)) K4[u] = ecode[start + t0 + u];
+		float Lz, Sum<COMP> &s0, Sum<COMP> &s1, Sum<COMP> &s2, bool dl, float4 *sA, float4 *sX, int *sK,
+		int lane) {
+	for (int c0 = 0; c0 < cnt; c0 += 64) {
+		const int m = min(64, cnt - c0);
+		if (lane < m) {
+			sA[lane] = epos[start + c0 + lane];
+			sX[lane] = eaux[start + c0 + lane];
+			sK[lane] = (COMP || (MODE == 0 && dl)) ? ecode[start + c0 + lane] : 13;
 		}
-#pragma unroll
-		for (int u = 0; u < 4; ++u) {
-			if (t0 + u >= cnt) break;
-			cover_hit<MODE, COMP>(start + t0 + u, A4[u], X4[u], K4[u], true, p, eorg, ehit, stamp, hmin, Lx, Ly, Lz,
-					s0, s1, s2, dl);
-		}
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+		if (want)
+			for (int t = 0; t < m; ++t)
+				cover_hit<MODE, COMP>(start + c0 + t, sA[t], sX[t], sK[t], true, p, eorg, ehit, stamp, hmin, Lx, Ly,
+						Lz, s0, s1, s2, dl);
+		// every lane has read these records before the next chunk overwrites them
+		__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+		__builtin_amdgcn_wave_barrier();
+		__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 	}
 }
 
@@ -1207,8 +1210,9 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 				point_box_d2<false>(lo, hi, p.x, p.y, p.z, 0.f) < lo.w;
 		if (!__any(want)) continue;
 		const int2 rg = leaf_range(lstart, leaf, B, ne);
-		cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), want, p, epos, eaux, eorg, ecode, ehit,
-				stamp, hmin, Lx, Ly, Lz, s0, s1, s2, dl);
+		cover_scan_staged<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), want, p, epos, eaux, eorg, ecode,
+				ehit, stamp, hmin, Lx, Ly, Lz, s0, s1, s2, dl, s_eA + (threadIdx.x & ~63), s_eX + (threadIdx.x & ~63),
+				s_eK + (threadIdx.x & ~63), lane);
 	}
 	while (node) {
 		node = uniform_int(node);
@@ -1218,8 +1222,9 @@ __global__ __launch_bounds__(SKG_TILE_T) __attribute__((amdgpu_waves_per_eu(8, 8
 		if (visit && node < nLeaf2) { node = 2 * node; continue; }
 		if (visit) {
 			const int2 rg = leaf_range(lstart, node - nLeaf2, B, ne);
-			cover_scan<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), need, p, epos, eaux, eorg, ecode,
-					ehit, stamp, hmin, Lx, Ly, Lz, s0, s1, s2, dl);
+			cover_scan_staged<MODE, COMP>(uniform_int(rg.x), uniform_int(rg.y - rg.x), need, p, epos, eaux, eorg,
+					ecode, ehit, stamp, hmin, Lx, Ly, Lz, s0, s1, s2, dl, s_eA + (threadIdx.x & ~63),
+					s_eX + (threadIdx.x & ~63), s_eK + (threadIdx.x & ~63), lane);
 		}
 		// next node in pre-order after node's subtree (strip trailing ones, step)
 		node = (node >> __builtin_ctz(~node)) + 1;
