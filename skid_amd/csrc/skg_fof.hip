@@ -57,7 +57,8 @@ template <bool PER>
 __global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restrict__ fp, int n,
 		const float4 *__restrict__ nlo, const float4 *__restrict__ nhi, int nLeaf2, int depth, int B,
 		float L, float tau2c, const int *__restrict__ fbox, const DevBox *__restrict__ bt, int *par, int *united) {
-	const int lane = threadIdx.x & 63;
+	__shared__ float4 sP[SKG_TILE_T];               // staged leaf points, 64 per wavefront
+	const int lane = threadIdx.x & 63, wb = threadIdx.x & ~63;
 	const int base = xcd_tile(blockIdx.x, gridDim.x) * SKG_TILE_T + (threadIdx.x & ~63);
 	if (base >= n) return;                        // whole wavefront past the end
 	const int i = base + lane;
@@ -108,31 +109,47 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restri
 			const bool want = overlap && !contained && (last - 1 > i);
 			if (__any(want)) {
 				if (node < nLeaf2) descend = true;
-				else if (want) {
+				else {
+					// the leaf's points staged 64 at a time in this wavefront's LDS
+					// slots (lane t reads point c0 + t: one round trip), then each
+					// lane with want reads its own range there; the wavefront reads
+					// back only what it wrote, and its LDS operations complete in
+					// order
 					const bool oneBox = !fbox || brange_is(hi.w, qb);      // the leaf holds only the lane's box
-					for (int j = max(first, i + 1); j < last; ++j) {
-						if (!oneBox && fbox[j] != qb) continue;
-						const float4 c = fp[j];
-						const int pj = __hip_atomic_load(&par[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-						float dx = fsub(q.x, c.x), dy = fsub(q.y, c.y), dz = fsub(q.z, c.z);
-						// images only where a delta exceeds half a period (the values
-						// img_delta gives); otherwise the pair seen from j is the same
-						// d2 (c - q = -(q - c) exactly), so only then is it tested again
-						const bool far = PER && fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz)) > hL;
-						if (far) {
-							dx = img_delta<PER>(q.x, qxm, qxp, c.x, hL);
-							dy = img_delta<PER>(q.y, qym, qyp, c.y, hL);
-							dz = img_delta<PER>(q.z, qzm, qzp, c.z, hL);
-						}
-						bool link = sq3(dx, dy, dz) < tau2;
-						if (far && !link) {
-							// the pair seen from j: shift applied on j's side
-							float ex = img_delta<PER>(c.x, fsub(c.x, L), fadd(c.x, L), q.x, hL);
-							float ey = img_delta<PER>(c.y, fsub(c.y, L), fadd(c.y, L), q.y, hL);
-							float ez = img_delta<PER>(c.z, fsub(c.z, L), fadd(c.z, L), q.z, hL);
-							link = sq3(ex, ey, ez) < tau2;
-						}
-						if (link && pj != mine) mine = uf_union(par, mine, j);
+					for (int c0 = first; c0 < last; c0 += 64) {
+						const int c1 = min(c0 + 64, last);
+						if (c0 + lane < c1) sP[threadIdx.x] = fp[c0 + lane];
+						__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+						__builtin_amdgcn_wave_barrier();
+						__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+						for (int j = max(c0, i + 1); want && j < c1; ++j) {
+							if (!oneBox && fbox[j] != qb) continue;
+							const float4 c = sP[wb + j - c0];
+							const int pj = __hip_atomic_load(&par[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+							float dx = fsub(q.x, c.x), dy = fsub(q.y, c.y), dz = fsub(q.z, c.z);
+							// images only where a delta exceeds half a period (the values
+							// img_delta gives); otherwise the pair seen from j is the same
+							// d2 (c - q = -(q - c) exactly), so only then is it tested again
+							const bool far = PER && fmaxf(fmaxf(fabsf(dx), fabsf(dy)), fabsf(dz)) > hL;
+							if (far) {
+								dx = img_delta<PER>(q.x, qxm, qxp, c.x, hL);
+								dy = img_delta<PER>(q.y, qym, qyp, c.y, hL);
+								dz = img_delta<PER>(q.z, qzm, qzp, c.z, hL);
+							}
+							bool link = sq3(dx, dy, dz) < tau2;
+							if (far && !link) {
+								// the pair seen from j: shift applied on j's side
+								float ex = img_delta<PER>(c.x, fsub(c.x, L), fadd(c.x, L), q.x, hL);
+								float ey = img_delta<PER>(c.y, fsub(c.y, L), fadd(c.y, L), q.y, hL);
+								float ez = img_delta<PER>(c.z, fsub(c.z, L), fadd(c.z, L), q.z, hL);
+								link = sq3(ex, ey, ez) < tau2;
+							}
+							if (link && pj != mine) mine = uf_union(par, mine, j);
+					}
+					// every lane has read these points before the next chunk overwrites them
+					__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+					__builtin_amdgcn_wave_barrier();
+					__builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 					}
 				}
 			}
