@@ -258,11 +258,12 @@ int skg_fof_points_noalloc(skg_ctx *ctx, int n, const float *d_pos, float fTau, 
 /* Autotuner (no reference counterpart): times launch shapes on the loaded
    particles and keeps the fastest for this context.  what is a mask of
      SKG_TUNE_BUCKET    leaf size nBucket in {8, 16, 32, 64} (skg_density plus
-                        the first 6 move steps per candidate);
-     SKG_TUNE_MAPPING   the mover-to-lane mapping: for active sets of 4^k * 4096
-                        strided movers, one lane per mover (64 per wavefront)
-                        against 16 lanes per mover; the largest size at which
-                        the latter wins becomes the switch point;
+                        the whole move loop per candidate);
+     SKG_TUNE_MAPPING   the mover-to-lane mapping: the active count at and
+                        below which a block runs 16 lanes per mover instead of
+                        one lane per mover with its tiles' lists, from
+                        {2^14, 2^16, 2^18, 2^20}, each timed over the whole
+                        move loop;
      SKG_TUNE_POT_TILE  the unbinding potential's workgroup / LDS tile in
                         {64, 128, 256}, timed on the groups of the next
                         skg_unbind;

@@ -2845,10 +2845,6 @@ extern "C" int skg_stage_times(skg_ctx *c, double *ms) {
 // =====================================================================
 // Autotuner: time launch shapes on the loaded particles and keep the fastest.
 // =====================================================================
-__global__ void k_stride_list(const int *__restrict__ full, int n, int stride, int m, int *out) {
-	int i = blockIdx.x * blockDim.x + threadIdx.x;
-	if (i < m) out[i] = full[(int)min((long)n - 1, (long)i * stride)];
-}
 
 static float elapsed_ms(skg_ctx *c) {
 	float ms = 0.f;
@@ -2891,31 +2887,30 @@ extern "C" int skg_autotune(skg_ctx *c, int what, int *nBucket, long long *subMa
 		}
 		c->P.nBucket = bestB;
 	}
-	// 2. mover mapping: per active-set size, one lane per mover (64 per
-	// wavefront) against 16 lanes per mover, on strided (scattered) subsets
+	// 2. mover mapping: the active count at and below which a block runs 16
+	// lanes per mover instead of one lane per mover (with its tiles' lists),
+	// timed over the whole move loop for each candidate (a block's cost with
+	// lists includes its list build, which a single step on a subset misses)
 	if (what & SKG_TUNE_MAPPING) {
-		if ((rc = skg_density(c, nullptr, nullptr, nullptr))) return rc;
-		if ((rc = skg_init_move(c, nullptr))) return rc;
-		const int nm = c->nMove;
-		long cross = 0;
-		int *full = nullptr;
-		if (nm > 0 && dalloc(&full, nm)) return SKG_E_NOMEM;
-		if (nm > 0) SKG_HIP(hipMemcpyAsync(full, c->m_list, 4 * (size_t)nm, hipMemcpyDeviceToDevice, c->st));
-		for (long m = 1 << 12; m <= nm / 2; m <<= 2) {
-			const int stride = (int)(nm / m);
-			k_stride_list<<<nblk(m, 256), 256, 0, c->st>>>(full, nm, stride, (int)m, c->m_list);
-			ml_set(c, 1, (int)m);
-			c->nActive = (int)m;
-			float t[2];
-			for (int sub = 0; sub < 2; ++sub) {
-				(void)hipEventRecord(c->e0, c->st);
-				if ((rc = enqueue_steps(c, m, sub != 0, false, 1, 0, 0, false))) return rc;
-				t[sub] = elapsed_ms(c);
-			}
-			if (t[1] < t[0]) cross = m;
+		auto now_ms = [&]() {
+			(void)hipStreamSynchronize(c->st);
+			return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch())
+					.count();
+		};
+		double best = 1e300;
+		long bestM = c->subMax;
+		for (long m : {1L << 14, 1L << 16, 1L << 18, 1L << 20}) {
+			c->subMax = m;
+			if ((rc = skg_density(c, nullptr, nullptr, nullptr))) return rc;   // the cuts restart from here
+			if ((rc = skg_init_move(c, nullptr))) return rc;
+			const double t0 = now_ms();
+			if ((rc = skg_move_first(c, &na, &ns))) return rc;
+			for (int blk = 0; na > 0 && blk < 1000; ++blk)
+				if ((rc = skg_move_block(c, &na, &ns))) return rc;
+			const double cost = now_ms() - t0;
+			if (cost < best) { best = cost; bestM = m; }
 		}
-		dfree(&full);
-		c->subMax = cross > 0 ? cross : 0;
+		c->subMax = bestM;
 	}
 	// 3. unbinding potential tile: timed on the groups of the next skg_unbind
 	if (what & SKG_TUNE_POT_TILE) c->tunePot = 1;
