@@ -638,6 +638,30 @@ __device__ __forceinline__ void wave_argmin_d(double &E, int &i) {
 		if (oE < E || (oE == E && oi < i)) { E = oE; i = oi; }
 	}
 }
+// The same argmax with DPP: row steps within quads, half-rows and rows, then
+// the two row broadcasts, after which lane 63 holds the wavefront's (E, i);
+// every lane takes lane 63's.  A lane whose source is not written keeps its
+// own value.  Needs every lane of the wavefront active.
+template <int CTRL, int RMASK>
+__device__ __forceinline__ void dpp_argmax_step(double &E, int &i) {
+	const int lo = __double2loint(E), hi = __double2hiint(E);
+	const int olo = __builtin_amdgcn_update_dpp(lo, lo, CTRL, RMASK, 0xF, false);
+	const int ohi = __builtin_amdgcn_update_dpp(hi, hi, CTRL, RMASK, 0xF, false);
+	const int oi = __builtin_amdgcn_update_dpp(i, i, CTRL, RMASK, 0xF, false);
+	const double oE = __hiloint2double(ohi, olo);
+	if (oE > E || (oE == E && oi < i)) { E = oE; i = oi; }
+}
+__device__ __forceinline__ void wave_argmax_dpp(double &E, int &i) {
+	dpp_argmax_step<0xB1, 0xF>(E, i);            // quad_perm [1,0,3,2]
+	dpp_argmax_step<0x4E, 0xF>(E, i);            // quad_perm [2,3,0,1]
+	dpp_argmax_step<0x141, 0xF>(E, i);           // row_half_mirror
+	dpp_argmax_step<0x140, 0xF>(E, i);           // row_mirror
+	dpp_argmax_step<0x142, 0xA>(E, i);           // row_bcast:15 into rows 1 and 3
+	dpp_argmax_step<0x143, 0xC>(E, i);           // row_bcast:31 into rows 2 and 3
+	E = __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(E), 63),
+			__builtin_amdgcn_readlane(__double2loint(E), 63));
+	i = __builtin_amdgcn_readlane(i, 63);
+}
 __device__ __forceinline__ double wave_max_d(double v) {
 #pragma unroll
 	for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
@@ -805,7 +829,7 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 		// wavefront) publishes it with the member's records
 		double wE = bE;
 		int wI = bi;
-		wave_argmax_d(wE, wI);
+		wave_argmax_dpp(wE, wI);
 		if (wI == INT_MAX ? lane == 0 : (bc >= 0 && bi == wI)) {
 			bE_s[rb][wv] = wE; bI_s[rb][wv] = wI;
 			#pragma unroll
