@@ -54,6 +54,10 @@ struct skg_ctx {
 	int dev = 0;
 	hipStream_t st = nullptr;
 	hipEvent_t e0 = nullptr, e1 = nullptr;
+	// a second stream (default priority) for work that overlaps the main
+	// stream's inside one entry point, forked and joined with evFork / evJoin
+	hipStream_t st2 = nullptr;
+	hipEvent_t evFork = nullptr, evJoin = nullptr;
 	double tms[SKG_T_N] = {0};
 	int stage = 0;            // 0 created, 1 loaded, 2 density, 3 movers, 4 moving done, 5 fof, 6 micro, 7 centers, 8 unbound, 9 final
 	int n = 0, nGas = 0, nDark = 0, nStar = 0, inType = 0;

@@ -2039,6 +2039,9 @@ extern "C" int skg_create(const skg_params *P, skg_ctx **out) {
 		if (t == 64 || t == 128 || t == 256) c->potTile = t;
 	}
 	if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+	    hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
+	    hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming) != hipSuccess ||
+	    hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming) != hipSuccess ||
 	    hipEventCreate(&c->e0) != hipSuccess || hipEventCreate(&c->e1) != hipSuccess) {
 		delete c;
 		return SKG_E_NODEVICE;
@@ -2084,6 +2087,9 @@ extern "C" int skg_destroy(skg_ctx *c) {
 	free(c->prec);
 	if (c->e0) (void)hipEventDestroy(c->e0);
 	if (c->e1) (void)hipEventDestroy(c->e1);
+	if (c->evFork) (void)hipEventDestroy(c->evFork);
+	if (c->evJoin) (void)hipEventDestroy(c->evJoin);
+	if (c->st2) (void)hipStreamDestroy(c->st2);
 	if (c->st) (void)hipStreamDestroy(c->st);
 	delete c;
 	return SKG_OK;

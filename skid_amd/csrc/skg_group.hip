@@ -616,6 +616,7 @@ __global__ __launch_bounds__(COOP_T) void k_unb_coop(const int *__restrict__ bg,
 // of removals is therefore the reference's (ties aside).
 #define CAND_T 256
 #define CT 8
+#define CAND_WAVE 2048        // groups below this many members: one wavefront each
 __device__ __forceinline__ float kernel_max(int soft, float twoh) {
 	return soft == SKG_PLUMMER ? 2.0f / twoh : 2.8f / twoh;     // K(0): spline 1.4*2/twoh, Plummer 2/twoh
 }
@@ -668,14 +669,15 @@ __device__ __forceinline__ double wave_max_d(double v) {
 	return v;
 }
 
-__global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ glist, const int *__restrict__ gs,
+template <int T>
+__global__ __launch_bounds__(T) void k_unb_cand(const int *__restrict__ glist, const int *__restrict__ gs,
 		const int *__restrict__ ge, const float4 *__restrict__ uq, const float4 *__restrict__ uv, float2 *uphi,
 		int *uord, int *grp, int *gcount, float *gmass, float4 *gvcm, float4 *grbound,
 		const float4 *__restrict__ grel, float fShift, float fCosmo, float z, float G, int soft, int bSub,
 		int bNoUnbind, float hx, int *nunb, unsigned long long *npairs, unsigned long long *nfull) {
-	constexpr int NW = CAND_T / 64;
-	__shared__ double rA[CAND_T];
-	__shared__ int pend[CAND_T * CT];
+	constexpr int NW = T / 64;
+	__shared__ double rA[T];
+	__shared__ int pend[T * CT];
 	__shared__ double cmS[7];
 	// full-pass results per wavefront
 	__shared__ double fCut[NW], fMin[NW], fR2[NW];
@@ -691,7 +693,7 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 	// fp64 mass, centre of mass and velocity (kd.c:1360-1375)
 	{
 		double a[7] = {0, 0, 0, 0, 0, 0, 0};
-		for (int i = tid; i < n0; i += CAND_T) {
+		for (int i = tid; i < n0; i += T) {
 			const float4 q = uq[s + i], v = uv[s + i];
 			a[0] += (double)q.w;
 			a[1] += (double)fmul(q.w, q.x); a[2] += (double)fmul(q.w, q.y); a[3] += (double)fmul(q.w, q.z);
@@ -700,7 +702,7 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 		for (int k = 0; k < 7; ++k) {
 			rA[tid] = a[k];
 			__syncthreads();
-			for (int o = CAND_T / 2; o > 0; o >>= 1) { if (tid < o) rA[tid] += rA[tid + o]; __syncthreads(); }
+			for (int o = T / 2; o > 0; o >>= 1) { if (tid < o) rA[tid] += rA[tid + o]; __syncthreads(); }
 			if (tid == 0) cmS[k] = rA[0];
 			__syncthreads();
 		}
@@ -779,19 +781,19 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 				if (outE > eCut) eCut = outE;
 				if (outR2 > r2Max) r2Max = outR2;
 			};
-			for (int i0 = tid; i0 < n0; i0 += 4 * CAND_T) {
+			for (int i0 = tid; i0 < n0; i0 += 4 * T) {
 				int o4[4];
 				float4 q4[4], v4[4];
 				float2 p4[4];
 				#pragma unroll
 				for (int u = 0; u < 4; ++u) {
-					const int i = i0 + u * CAND_T;
+					const int i = i0 + u * T;
 					o4[u] = -1;
 					if (i < n0) { o4[u] = uord[s + i]; q4[u] = uq[s + i]; v4[u] = uv[s + i]; p4[u] = uphi[s + i]; }
 				}
 				#pragma unroll
 				for (int u = 0; u < 4; ++u)
-					if (o4[u] >= 0) visit(i0 + u * CAND_T, q4[u], v4[u], p4[u]);
+					if (o4[u] >= 0) visit(i0 + u * T, q4[u], v4[u], p4[u]);
 			}
 			// block reductions: E_cut, R^2, smallest softening, argmin E
 			eCut = wave_max_d(eCut);
@@ -887,7 +889,7 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 			grow += zf * (double)fmul(fmul(G, qb.w), kernel_max(soft, fadd(vb.w, epsMin)));
 			if (tid == 0) pend[npend] = BI;
 			++npend;
-			if (npend >= CAND_T * CT) needFull = true;
+			if (npend >= T * CT) needFull = true;
 		}
 	}
 	// iMin over all members in the final state (kd.c:1400-1408 of the last pass)
@@ -895,7 +897,7 @@ __global__ __launch_bounds__(CAND_T) void k_unb_cand(const int *__restrict__ gli
 		__syncthreads();                              // the last removals' member marks
 		double mE = INFINITY;
 		int mi = INT_MAX;
-		for (int i = tid; i < n0; i += CAND_T) {
+		for (int i = tid; i < n0; i += T) {
 			if (uord[s + i] < 0) continue;
 			const float4 q = uq[s + i], v = uv[s + i];
 			const float2 ph = uphi[s + i];
@@ -1610,6 +1612,11 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		gmed.insert(gmed.end(), gbig.begin() + 128, gbig.end());
 		gbig.resize(128);
 	}
+	// groups below CAND_WAVE members run one wavefront each (512 candidates,
+	// no workgroup barriers), the rest 256 threads (2048 candidates); the
+	// small ones first in the list
+	const int nMedW = (int)(std::stable_partition(gmed.begin(), gmed.end(),
+			[&](int g) { return he[g] - hs[g] < CAND_WAVE; }) - gmed.begin());
 	SKG_HIP(hipMemsetAsync(G->d_nunb, 0, 4, c->st));
 	const bool any = !gsmall.empty() || !gmed.empty() || !gbig.empty();
 	if (any) {
@@ -1693,10 +1700,27 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 			k_unb_small<<<(int)gsmall.size(), 64, 0, c->st>>>(dgs, G->gstart, G->gend, G->uq, G->uv, G->uphi, G->uord,
 					c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel, c->P.fShift, c->P.fCosmo, c->P.z,
 					c->P.G, c->P.iSoftType, bSub, c->P.bNoUnbind, hx, G->d_nunb, npair);
-		if (!gmed.empty())
-			k_unb_cand<<<(int)gmed.size(), CAND_T, 0, c->st>>>(dgm, G->gstart, G->gend, G->uq, G->uv, G->uphi, G->uord,
-					c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel, c->P.fShift, c->P.fCosmo, c->P.z,
+		// the two sizes run side by side: the wavefront-sized groups on the
+		// context's second stream, forked from and joined back to the main
+		// one (the groups are disjoint; the counters are atomic)
+		const bool fork = nMedW > 0 && (int)gmed.size() > nMedW;
+		if (fork) {
+			SKG_HIP(hipEventRecord(c->evFork, c->st));
+			SKG_HIP(hipStreamWaitEvent(c->st2, c->evFork, 0));
+		}
+		if ((int)gmed.size() > nMedW)
+			k_unb_cand<CAND_T><<<(int)gmed.size() - nMedW, CAND_T, 0, c->st>>>(dgm + nMedW, G->gstart, G->gend,
+					G->uq, G->uv, G->uphi, G->uord, c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel,
+					c->P.fShift, c->P.fCosmo, c->P.z, c->P.G, c->P.iSoftType, bSub, c->P.bNoUnbind, hx, G->d_nunb,
+					npair, npair + 2);
+		if (nMedW > 0)
+			k_unb_cand<64><<<nMedW, 64, 0, fork ? c->st2 : c->st>>>(dgm, G->gstart, G->gend, G->uq, G->uv, G->uphi,
+					G->uord, c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel, c->P.fShift, c->P.fCosmo, c->P.z,
 					c->P.G, c->P.iSoftType, bSub, c->P.bNoUnbind, hx, G->d_nunb, npair, npair + 2);
+		if (fork) {
+			SKG_HIP(hipEventRecord(c->evJoin, c->st2));
+			SKG_HIP(hipStreamWaitEvent(c->st, c->evJoin, 0));
+		}
 		unsigned long long hp[3] = {0, 0, 0};
 		SKG_HIP(hipMemcpyAsync(&hp[0], npair, 8, hipMemcpyDeviceToHost, c->st));
 		SKG_HIP(hipMemcpyAsync(&hp[2], npair + 2, 8, hipMemcpyDeviceToHost, c->st));
@@ -1748,7 +1772,7 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 			auto fallback = [&](hipError_t e) -> int {
 				(void)hipGetLastError();
 				c->nCoopFallbacks++;
-				k_unb_cand<<<nb, CAND_T, 0, c->st>>>(dbg, G->gstart, G->gend, G->uq, G->uv, G->uphi, G->uord,
+				k_unb_cand<CAND_T><<<nb, CAND_T, 0, c->st>>>(dbg, G->gstart, G->gend, G->uq, G->uv, G->uphi, G->uord,
 						c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel, c->P.fShift, c->P.fCosmo, c->P.z,
 						c->P.G, c->P.iSoftType, bSub, c->P.bNoUnbind, hx, G->d_nunb, npair1, npair + 2);
 				(void)e;
