@@ -1696,14 +1696,11 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 		skg_prof_end(c, pslot, SKG_K_UNB_POT, 20.0 * pairs);
 		const int bSub = (c->inType == SKG_DARK || c->inType == SKG_STAR);   // kd.c:1441
 		const int lslot = skg_prof_begin(c);
-		if (!gsmall.empty())
-			k_unb_small<<<(int)gsmall.size(), 64, 0, c->st>>>(dgs, G->gstart, G->gend, G->uq, G->uv, G->uphi, G->uord,
-					c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel, c->P.fShift, c->P.fCosmo, c->P.z,
-					c->P.G, c->P.iSoftType, bSub, c->P.bNoUnbind, hx, G->d_nunb, npair);
-		// the two sizes run side by side: the wavefront-sized groups on the
-		// context's second stream, forked from and joined back to the main
-		// one (the groups are disjoint; the counters are atomic)
-		const bool fork = nMedW > 0 && (int)gmed.size() > nMedW;
+		// the sizes run side by side: groups of at most 64 members and the
+		// wavefront-sized ones on the context's second stream, forked from and
+		// joined back to the main one, the 256-thread groups on the main one
+		// (the groups are disjoint; the counters are atomic)
+		const bool fork = (nMedW > 0 || !gsmall.empty()) && (int)gmed.size() > nMedW;
 		if (fork) {
 			SKG_HIP(hipEventRecord(c->evFork, c->st));
 			SKG_HIP(hipStreamWaitEvent(c->st2, c->evFork, 0));
@@ -1713,6 +1710,10 @@ extern "C" int skg_unbind(skg_ctx *c, int *nUnbound) {
 					G->uq, G->uv, G->uphi, G->uord, c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel,
 					c->P.fShift, c->P.fCosmo, c->P.z, c->P.G, c->P.iSoftType, bSub, c->P.bNoUnbind, hx, G->d_nunb,
 					npair, npair + 2);
+		if (!gsmall.empty())
+			k_unb_small<<<(int)gsmall.size(), 64, 0, fork ? c->st2 : c->st>>>(dgs, G->gstart, G->gend, G->uq, G->uv,
+					G->uphi, G->uord, c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel, c->P.fShift,
+					c->P.fCosmo, c->P.z, c->P.G, c->P.iSoftType, bSub, c->P.bNoUnbind, hx, G->d_nunb, npair);
 		if (nMedW > 0)
 			k_unb_cand<64><<<nMedW, 64, 0, fork ? c->st2 : c->st>>>(dgm, G->gstart, G->gend, G->uq, G->uv, G->uphi,
 					G->uord, c->d_grp, G->count, G->mass, G->vcm, G->rbound, G->rel, c->P.fShift, c->P.fCosmo, c->P.z,
