@@ -2748,9 +2748,10 @@ extern "C" int skg_move_block(skg_ctx *c, int *nActive, int *nScatter) {
 		// prune does not change it)
 		if ((rc = read_counts(c, &ns))) return rc;
 		// Pruning thins the active list in its old order, so runs of 64 active
-		// movers spread out; once half are gone, re-sort the rest by where they
-		// are now (per-mover sums do not depend on the order).
-		if (c->nActive > 0 && (double)c->nActive < mover_resort_frac() * (double)c->lastSort) {
+		// movers spread out; once a tenth are gone, re-sort the rest by where
+		// they are now (per-mover sums do not depend on the order).  At most
+		// s64Max movers run a wavefront each, where the order does not matter.
+		if (c->nActive > c->s64Max && (double)c->nActive < mover_resort_frac() * (double)c->lastSort) {
 			if ((rc = mover_resort(c))) return rc;
 		}
 	} else {
