@@ -202,3 +202,39 @@ def test_autotune_keeps_results(box1):
     frac = O.match_fraction(g, rg)
     print(f"tuned {t2}: groups {ng} vs {rng}, match {frac:.5f}")
     assert frac >= 0.999
+
+
+def test_leaves_of_128():
+    """Leaves of 128 points (nBucket 128): the walks that stage a leaf's records
+    through LDS do so in chunks of 64, which smaller leaves never reach.  FOF
+    on such a tree equals brute force; the whole path's groups match the oracle
+    run with the same leaf size; the dx adjoint agrees with the one on leaves of
+    16 (the same sums in another order)."""
+    b = gen.config1(n=8192, seed=21)
+    ctx = skg.Context(b, skg.make_params(0.005, 32, nBucket=128, period=1.0))
+    for tau in (0.002, 0.01):
+        g, ng = ctx.fof_points(b.pos, tau)
+        bg, bng = O.fof_brute(b.pos, tau)
+        assert ng == bng + 1
+        assert np.array_equal(g, bg)
+    box = gen.config1()
+    P = skg.make_params(0.005, 32, nBucket=128, period=1.0)
+    c2 = skg.Context(box, P)
+    tr, g, ng = c2.run()
+    o = O.Oracle(box, nSmooth=32, nBucket=128, fTau=0.005, fPeriod=1.0)
+    o.density()
+    otr = o.move_loop()
+    o.fof()
+    og, ong = o.finish()
+    frac = O.match_fraction(g, og)
+    print(f"nBucket 128: groups gpu {ng - 1} oracle {ong - 1} match {frac:.5f}, trace {len(tr)} vs {len(otr)}")
+    assert frac >= 0.999
+    rng = np.random.default_rng(5)
+    gbar = rng.normal(size=box.n).astype(np.float32)
+    out = []
+    for B in (16, 128):
+        c3 = skg.Context(box, skg.make_params(0.005, 32, nBucket=B, period=1.0))
+        c3.density()
+        out.append(c3.adjoint(gbar, 3))
+    for a, r in zip(out[1], out[0]):
+        assert np.allclose(a, r, rtol=1e-4, atol=1e-5 * np.abs(r).max())
