@@ -125,7 +125,6 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restri
 						for (int j = max(c0, i + 1); want && j < c1; ++j) {
 							if (!oneBox && fbox[j] != qb) continue;
 							const float4 c = sP[wb + j - c0];
-							const int pj = __hip_atomic_load(&par[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 							float dx = fsub(q.x, c.x), dy = fsub(q.y, c.y), dz = fsub(q.z, c.z);
 							// images only where a delta exceeds half a period (the values
 							// img_delta gives); otherwise the pair seen from j is the same
@@ -144,7 +143,8 @@ __global__ __launch_bounds__(SKG_TILE_T) void k_fof_links(const float4 *__restri
 								float ez = img_delta<PER>(c.z, fsub(c.z, L), fadd(c.z, L), q.z, hL);
 								link = sq3(ex, ey, ez) < tau2;
 							}
-							if (link && pj != mine) mine = uf_union(par, mine, j);
+							if (link && __hip_atomic_load(&par[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != mine)
+								mine = uf_union(par, mine, j);
 					}
 					// every lane has read these points before the next chunk overwrites them
 					__builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
