@@ -3,6 +3,7 @@
 
 #include <math.h>
 #include <pthread.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -14,6 +15,7 @@ typedef struct {
 	const int *iv;
 	const float *fv;
 	const char *fmt;
+	int prec;                  /* fmt is "%.<prec>g" or "%g" (prec 6): the formatter above; else 0 */
 	int nan_zero;
 	long lo, hi;               /* lines [lo, hi) */
 	char *buf;
@@ -30,6 +32,89 @@ static size_t fmt_int(char *out, int v)
 	if (v < 0) out[n++] = '-';
 	while (k) out[n++] = tmp[--k];
 	out[n++] = '\n';
+	return n;
+}
+
+/* printf("%.<P>g", (double)f) for a finite, nonzero float f, without printf and
+ * byte for byte (the decimal value rounded once to P significant digits, ties
+ * to even).  f = M 2^E exactly (M < 2^24); R = M 2^E 10^k with k = P - 1 - X
+ * is formed exactly in 128-bit integers, X the decimal exponent (an estimate
+ * from log10, corrected until 10^(P-1) <= R < 10^P: a rounding that carries to
+ * 10^P lands there on the next exponent, as printf's does).  Returns the
+ * characters written, or 0 where R does not fit 128 bits (|f| below about
+ * 1e-29): the caller then uses snprintf. */
+typedef unsigned __int128 u128;
+static size_t fmt_g(char *out, float f, int P)
+{
+	static const uint64_t p10[20] = {1ull, 10ull, 100ull, 1000ull, 10000ull, 100000ull, 1000000ull, 10000000ull,
+		100000000ull, 1000000000ull, 10000000000ull, 100000000000ull, 1000000000000ull, 10000000000000ull,
+		100000000000000ull, 1000000000000000ull, 10000000000000000ull, 100000000000000000ull,
+		1000000000000000000ull, 10000000000000000000ull};
+	uint32_t bits;
+	memcpy(&bits, &f, 4);
+	const int neg = (int)(bits >> 31);
+	const uint32_t ex = (bits >> 23) & 0xFFu, mt = bits & 0x7FFFFFu;
+	const uint64_t M = ex ? (mt | 0x800000u) : mt;
+	const int E = ex ? (int)ex - 150 : -149;
+	int X = (int)floor(log10(ldexp((double)M, E)));
+	uint64_t R = 0;
+	for (int tries = 0;; ++tries) {
+		if (tries > 4) return 0;
+		const int k = P - 1 - X;
+		u128 A, D = 1;
+		if (k >= 0) {
+			if (k > 38) return 0;                        /* 5^38 M < 2^113 */
+			A = M;
+			for (int i = 0; i < k; ++i) A *= 5u;
+			const int s = E + k;                         /* R = A 2^s */
+			if (s >= 0) {
+				if (s > 60 || (A >> (64 - s)) != 0) { ++X; continue; }   /* R far above 10^P */
+				R = (uint64_t)(A << s);
+			} else if (-s >= 120) {
+				R = 0;
+			} else {
+				const int sh = -s;
+				const u128 q = A >> sh, rem = A - (q << sh), half = (u128)1 << (sh - 1);
+				R = (uint64_t)q + ((rem > half || (rem == half && (q & 1u))) ? 1u : 0u);
+			}
+		} else {
+			const int kk = -k;                           /* R = M 2^(E - kk) / 5^kk */
+			if (kk > 38) return 0;
+			for (int i = 0; i < kk; ++i) D *= 5u;
+			const int s = E - kk;
+			if (s >= 0) { A = (u128)M << s; }
+			else { A = M; if (-s > 30) return 0; D <<= -s; }            /* 5^38 2^30 < 2^120 */
+			const u128 q = A / D, rem = A - q * D;
+			R = (uint64_t)q + ((2 * rem > D || (2 * rem == D && (q & 1u))) ? 1u : 0u);
+		}
+		if (R >= p10[P]) { ++X; continue; }
+		if (R < p10[P - 1]) { --X; continue; }
+		break;
+	}
+	char d[20];
+	for (int i = P - 1; i >= 0; --i) { d[i] = (char)('0' + R % 10u); R /= 10u; }
+	int nd = P;                                          /* significant digits left after trailing zeros */
+	while (nd > 1 && d[nd - 1] == '0') --nd;
+	size_t n = 0;
+	if (neg) out[n++] = '-';
+	if (X < -4 || X >= P) {                              /* %e style */
+		out[n++] = d[0];
+		if (nd > 1) { out[n++] = '.'; for (int i = 1; i < nd; ++i) out[n++] = d[i]; }
+		out[n++] = 'e';
+		out[n++] = X < 0 ? '-' : '+';
+		const int ax = X < 0 ? -X : X;
+		if (ax >= 100) out[n++] = (char)('0' + ax / 100);
+		out[n++] = (char)('0' + ax / 10 % 10);
+		out[n++] = (char)('0' + ax % 10);
+	} else if (X >= 0) {                                 /* %f style, X + 1 integer digits */
+		for (int i = 0; i <= X; ++i) out[n++] = d[i];
+		if (nd > X + 1) { out[n++] = '.'; for (int i = X + 1; i < nd; ++i) out[n++] = d[i]; }
+	} else {                                             /* 0.000ddd */
+		out[n++] = '0';
+		out[n++] = '.';
+		for (int i = 0; i < -X - 1; ++i) out[n++] = '0';
+		for (int i = 0; i < nd; ++i) out[n++] = d[i];
+	}
 	return n;
 }
 
@@ -50,14 +135,17 @@ static void *run_job(void *arg)
 			j->len += fmt_int(j->buf + j->len, j->iv[i]);
 		} else {
 			const float f = j->fv[i];
-			int w;
-			if (j->nan_zero && isnan(f)) w = snprintf(j->buf + j->len, 48, "0\n");
-			else {
-				w = snprintf(j->buf + j->len, 46, j->fmt, (double)f);
-				j->buf[j->len + (size_t)w] = '\n';
+			size_t w = 0;
+			if (j->nan_zero && isnan(f)) {
+				memcpy(j->buf + j->len, "0\n", 2);
+				w = 2;
+			} else {
+				if (j->prec > 0 && isfinite(f) && f != 0.f) w = fmt_g(j->buf + j->len, f, j->prec);
+				if (!w) w = (size_t)snprintf(j->buf + j->len, 46, j->fmt, (double)f);
+				j->buf[j->len + w] = '\n';
 				++w;
 			}
-			j->len += (size_t)w;
+			j->len += w;
 		}
 	}
 	return NULL;
@@ -83,6 +171,7 @@ static int write_par(const char *path, long header, long n, const int *iv, const
 		for (t = 0; t < nthreads; ++t) {
 			Job *j = &jobs[t];
 			j->iv = iv; j->fv = fv; j->fmt = fmt; j->nan_zero = nan_zero;
+			j->prec = !fmt ? 0 : !strcmp(fmt, "%.10g") ? 10 : !strcmp(fmt, "%g") ? 6 : 0;
 			j->lo = b + t * per < e ? b + t * per : e;
 			j->hi = j->lo + per < e ? j->lo + per : e;
 			if (t == 0 || nthreads == 1) continue;

@@ -8,8 +8,9 @@
 int tio_write_ints(const char *path, long n, const int *v, int nthreads);
 
 /* "<header>\n" then one line per value (n values) in the printf format fmt
- * ("%.10g" or "%g"); with nan_zero a NaN value prints as "0" (kdOutVector's
- * particles that did not move). */
+ * ("%.10g" or "%g": these two through an exact formatter of the writer's own,
+ * any other through snprintf); with nan_zero a NaN value prints as "0"
+ * (kdOutVector's particles that did not move). */
 int tio_write_floats(const char *path, long header, long n, const float *v, const char *fmt, int nan_zero,
 		int nthreads);
 

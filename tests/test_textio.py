@@ -54,3 +54,26 @@ def test_floats(tio, tmp_path, fmt, nan_zero, nthreads):
     got = open(path, "rb").read()
     lines = ("0\n" if (nan_zero and np.isnan(x)) else (fmt % float(x)) + "\n" for x in v)
     assert got == _ref_text(header, lines)
+
+
+@pytest.mark.parametrize("fmt,nan_zero", [("%.10g", 0), ("%g", 1)])
+def test_floats_exact_rounding(tio, tmp_path, fmt, nan_zero):
+    """The writer's own "%.10g" / "%g" formatter (no printf) against correctly
+    rounded formatting: random bit patterns over all finite floats (denormals
+    and values below ~1e-29 go through snprintf), the float neighbours of every
+    power of ten in range, decimal-looking values and halfway cases."""
+    rng = np.random.default_rng(3)
+    n = 200000
+    v = rng.integers(0, 0x7F800000, n, dtype=np.int64).astype(np.uint32).view(np.float32).copy()
+    v[: n // 2] *= -1
+    near = []
+    for e in range(-30, 39):
+        f = np.float32(10.0 ** e)
+        near += [np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))]
+    dec = (rng.integers(1, 10 ** 7, 20000) * 10.0 ** rng.integers(-20, 20, 20000)).astype(np.float32)
+    ties = np.array([0.5, 1.5, 2.5, 123456.5, 1234567.5, 9999999999.5, 8388607.5, 1e-5, 1e-4, 99999.95], np.float32)
+    v = np.concatenate([v, np.array(near, np.float32), dec, ties])
+    path = str(tmp_path / "e.txt")
+    assert tio.tio_write_floats(path.encode(), 3, len(v), v.ctypes.data, fmt.encode(), nan_zero, 4) == 0
+    got = open(path, "rb").read()
+    assert got == _ref_text(3, ((fmt % float(x)) + "\n" for x in v))
