@@ -158,11 +158,33 @@ static double now_s(void)
 }
 
 /* kdInGroup (kd.c:920-962): an ASCII .grp, the particle count then one group
-   id per particle in input order. */
-static int read_grp(const char *path, int n, int *grp, int *nGroup)
+   id per particle in input order.  Parsed on nIo threads; a file that is not
+   plain whitespace-separated ints goes through one fscanf per value instead, so
+   its error (or its reading) is that of the serial loop. */
+static int read_grp(const char *path, int n, int *grp, int *nGroup, int nIo)
 {
-	FILE *fp = fopen(path, "r");
+	FILE *fp;
 	int i, m = 0, mx = 0;
+	long hdr = 0;
+	const int rc = tio_read_ints(path, n, grp, &hdr, nIo);
+	if (rc == 0) {
+		for (i = 0; i < n; ++i) {
+			if (grp[i] < 0) {
+				fprintf(stderr, "ERROR: bad group id at line %d of %s\n", i + 2, path);
+				return 1;
+			}
+			if (grp[i] > mx) mx = grp[i];
+		}
+		*nGroup = mx + 1;
+		return 0;
+	}
+	if (rc == 1) {
+		fprintf(stderr, "ERROR: Mismatched number of particles\n");
+		fprintf(stderr, "Number in Group file %s: %ld\n", path, hdr);
+		fprintf(stderr, "Number in TIPSY BINARY input file: %d\n", n);
+		return 1;
+	}
+	fp = fopen(path, "r");
 	if (!fp) { fprintf(stderr, "ERROR: Could not open file:%s\n", path); return 1; }
 	if (fscanf(fp, "%d", &m) != 1 || m != n) {
 		fprintf(stderr, "ERROR: Mismatched number of particles\n");
@@ -313,7 +335,7 @@ int main(int argc, char **argv)
 		size_t l = strlen(achGroup);
 		if (l >= 4 && !strcmp(achGroup + l - 4, ".grp")) achGroup[l - 4] = 0;
 		snprintf(achFile, sizeof(achFile), "%s.grp", achGroup);
-		if (read_grp(achFile, S.n, grp, &nGroup)) return 1;
+		if (read_grp(achFile, S.n, grp, &nGroup, nIo)) return 1;
 		snprintf(achFile, sizeof(achFile), "%s.gtp", achGroup);
 		if (read_gtp(achFile, bStandard, nGroup, &S, grp, &gcen, &gvcm)) return 1;
 		if ((rc = skg_set_groups(ctx, grp, gcen, gvcm))) die_skg("skg_set_groups", rc);

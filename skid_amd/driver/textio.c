@@ -212,3 +212,132 @@ int tio_threads(void)
 	if (p > 16) p = 16;
 	return (int)p;
 }
+
+/* ------------------------------------------------------------------ reader */
+/* kdInGroup (kd.c:920-962) reads a .grp with one fscanf("%d") per value; here the
+ * file is read whole and split over threads at whitespace: each thread counts the
+ * tokens that start in its share, a prefix sum gives each share its first value's
+ * index, then each thread converts its tokens.  A token is a run of non-space bytes
+ * (the C locale's isspace set); it must be an optional sign and decimal digits of
+ * a value that fits an int, ended by a space or the end of the file. */
+
+static int is_ws(char c) { return c == ' ' || (c >= '\t' && c <= '\r'); }
+
+/* The token at p (p < end, *p not a space): 0 and *out when it is an int. */
+static int tok_int(const char *p, const char *end, int *out)
+{
+	long long v = 0;
+	int neg = 0;
+	if (*p == '+' || *p == '-') neg = *p++ == '-';
+	if (p >= end || *p < '0' || *p > '9') return -1;
+	for (; p < end && *p >= '0' && *p <= '9'; ++p)
+		if ((v = 10 * v + (*p - '0')) > 2147483648LL) return -1;
+	if (p < end && !is_ws(*p)) return -1;
+	if (!neg && v > 2147483647LL) return -1;
+	*out = (int)(neg ? -v : v);
+	return 0;
+}
+
+typedef struct {
+	const char *buf, *end;
+	size_t base, lo, hi;       /* tokens that start in [lo, hi); base = first byte after the header */
+	long first, count, nexp;
+	int *v;
+	int bad;
+} RJob;
+
+static int tok_start(const RJob *j, size_t i) { return !is_ws(j->buf[i]) && (i == j->base || is_ws(j->buf[i - 1])); }
+
+static void *count_job(void *arg)
+{
+	RJob *j = (RJob *)arg;
+	long c = 0;
+	for (size_t i = j->lo; i < j->hi; ++i) c += tok_start(j, i);
+	j->count = c;
+	return NULL;
+}
+
+static void *parse_job(void *arg)
+{
+	RJob *j = (RJob *)arg;
+	long k = j->first;
+	for (size_t i = j->lo; i < j->hi && k < j->nexp; ++i) {
+		if (!tok_start(j, i)) continue;
+		if (tok_int(j->buf + i, j->end, &j->v[k++])) { j->bad = 1; break; }
+	}
+	return NULL;
+}
+
+/* The whole file (a regular file or a pipe). */
+static char *slurp(const char *path, size_t *len)
+{
+	FILE *fp = fopen(path, "rb");
+	size_t cap = 1 << 20, n = 0, r;
+	char *b;
+	if (!fp) return NULL;
+	if (!fseek(fp, 0, SEEK_END)) {
+		long sz = ftell(fp);
+		if (sz > 0) cap = (size_t)sz + 1;
+		rewind(fp);
+	}
+	if (!(b = malloc(cap))) { fclose(fp); return NULL; }
+	while ((r = fread(b + n, 1, cap - n, fp)) > 0) {
+		n += r;
+		if (n == cap) {
+			char *nb = realloc(b, 2 * cap);
+			if (!nb) { free(b); fclose(fp); return NULL; }
+			b = nb;
+			cap *= 2;
+		}
+	}
+	if (ferror(fp)) { free(b); fclose(fp); return NULL; }
+	fclose(fp);
+	*len = n;
+	return b;
+}
+
+static void run_all(RJob *jobs, int nthreads, void *(*fn)(void *))
+{
+	pthread_t *th = calloc((size_t)nthreads, sizeof(pthread_t));
+	for (int t = 1; t < nthreads; ++t)
+		if (!th || pthread_create(&th[t], NULL, fn, &jobs[t])) { fn(&jobs[t]); if (th) th[t] = 0; }
+	fn(&jobs[0]);
+	for (int t = 1; th && t < nthreads; ++t)
+		if (th[t]) pthread_join(th[t], NULL);
+	free(th);
+}
+
+int tio_read_ints(const char *path, long nexp, int *v, long *header, int nthreads)
+{
+	size_t len = 0, i = 0, base;
+	char *b = slurp(path, &len);
+	RJob *jobs;
+	long total = 0;
+	int h, rc = 0;
+	if (!b) return -1;
+	while (i < len && is_ws(b[i])) ++i;
+	if (i == len || tok_int(b + i, b + len, &h)) { free(b); return 2; }
+	*header = h;
+	if (h != nexp) { free(b); return 1; }
+	while (i < len && !is_ws(b[i])) ++i;
+	base = i;
+	if (nthreads < 1) nthreads = 1;
+	if ((size_t)nthreads > len - base + 1) nthreads = (int)(len - base + 1);
+	if (!(jobs = calloc((size_t)nthreads, sizeof(RJob)))) { free(b); return -1; }
+	for (int t = 0; t < nthreads; ++t) {
+		RJob *j = &jobs[t];
+		j->buf = b; j->end = b + len; j->base = base; j->nexp = nexp; j->v = v;
+		j->lo = base + (len - base) * (size_t)t / (size_t)nthreads;
+		j->hi = base + (len - base) * (size_t)(t + 1) / (size_t)nthreads;
+	}
+	run_all(jobs, nthreads, count_job);
+	for (int t = 0; t < nthreads; ++t) { jobs[t].first = total; total += jobs[t].count; }
+	if (total < nexp) rc = 2;
+	else {
+		run_all(jobs, nthreads, parse_job);
+		for (int t = 0; t < nthreads; ++t) rc |= jobs[t].bad ? 2 : 0;
+	}
+	free(jobs);
+	free(b);
+	return rc;
+}

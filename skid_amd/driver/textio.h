@@ -1,6 +1,7 @@
 /* Text writers for the per-particle output files (.grp, .den, .ray; kd.c:1502-1606)
  * at 10^8 lines: the lines are formatted by several threads into blocks and
- * written in order, so the bytes are those of one fprintf per line. */
+ * written in order, so the bytes are those of one fprintf per line.  The .grp
+ * reader (kdInGroup, kd.c:920-962) parses on several threads likewise. */
 #ifndef SKID_TEXTIO_H
 #define SKID_TEXTIO_H
 
@@ -13,6 +14,15 @@ int tio_write_ints(const char *path, long n, const int *v, int nthreads);
  * (kdOutVector's particles that did not move). */
 int tio_write_floats(const char *path, long header, long n, const float *v, const char *fmt, int nan_zero,
 		int nthreads);
+
+/* Reads a file of that form (a count, then whitespace-separated integers) on
+ * nthreads threads: 0 when the count equals nexp and the next nexp tokens are
+ * ints, stored in v; 1 when the count is not nexp (*header receives it); 2 when
+ * the count or one of the nexp values is missing or is not an int token (a sign
+ * and digits ended by a space or the end of the file, within int range), in which
+ * case the caller's own fscanf loop gives that file's exact behaviour; -1 when the
+ * file cannot be read.  Tokens after the nexp-th are not read. */
+int tio_read_ints(const char *path, long nexp, int *v, long *header, int nthreads);
 
 /* Threads to use: SKID_IO_THREADS if set, else the online processors, at most 16. */
 int tio_threads(void);

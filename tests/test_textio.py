@@ -77,3 +77,91 @@ def test_floats_exact_rounding(tio, tmp_path, fmt, nan_zero):
     assert tio.tio_write_floats(path.encode(), 3, len(v), v.ctypes.data, fmt.encode(), nan_zero, 4) == 0
     got = open(path, "rb").read()
     assert got == _ref_text(3, ((fmt % float(x)) + "\n" for x in v))
+
+
+def _reader(tio):
+    tio.tio_read_ints.argtypes = [C.c_char_p, C.c_long, C.c_void_p, C.POINTER(C.c_long), C.c_int]
+    tio.tio_read_ints.restype = C.c_int
+
+    def read(path, n, nthreads):
+        out = np.full(max(n, 1), -7, np.int32)
+        hdr = C.c_long(-1)
+        rc = tio.tio_read_ints(str(path).encode(), n, out.ctypes.data, C.byref(hdr), nthreads)
+        return rc, hdr.value, out[:n]
+    return read
+
+
+@pytest.mark.parametrize("nthreads", [1, 7, 16])
+def test_read_ints_round_trip(tio, tmp_path, nthreads):
+    """kdInGroup's .grp (a count, then one fscanf("%d") per particle): the
+    parallel reader returns what the writer wrote, across the 2^22-line blocks."""
+    rng = np.random.default_rng(4)
+    n = (1 << 22) + 4321
+    v = rng.integers(0, 60000, n).astype(np.int32)
+    v[:4] = [0, 2147483647, -2147483648, -1]
+    path = tmp_path / "r.grp"
+    assert tio.tio_write_ints(str(path).encode(), n, v.ctypes.data, 3) == 0
+    rc, hdr, got = _reader(tio)(path, n, nthreads)
+    assert (rc, hdr) == (0, n)
+    assert np.array_equal(got, v)
+
+
+def _fscanf_ref(data, n):
+    """What the fscanf("%d") loop reads from data when every one of the first
+    n + 1 tokens is a plain int (the reader's rc 0 / 1 cases); None otherwise."""
+    import re
+    toks = data.split()                           # bytes.split(): the C locale's isspace set
+    if len(toks) < n + 1 or not all(re.fullmatch(rb"[+-]?[0-9]+", t) for t in toks[:n + 1]):
+        return None
+    vals = [int(t) for t in toks[:n + 1]]
+    if any(not -2 ** 31 <= x < 2 ** 31 for x in vals):
+        return None
+    return vals[0], vals[1:]
+
+
+@pytest.mark.parametrize("nthreads", [1, 3, 64])
+def test_read_ints_forms(tio, tmp_path, nthreads):
+    """Whitespace of every kind (tabs, CR LF, runs, none after the last value),
+    signs and leading zeros, tokens past the n-th (read by no fscanf, so
+    ignored even when they are not numbers), a short file, a wrong count, and
+    tokens fscanf("%d") would split or reject (rc 2: the driver then reads the
+    file with its own fscanf loop)."""
+    read = _reader(tio)
+    cases = [
+        (b"3\n1\n2\n3\n", 3, 0),
+        (b"  3\t\t+1\r\n002\v\f-3", 3, 0),
+        (b"3 1 2 3 4 5 junk 0x10", 3, 0),
+        (b"0\n", 0, 0),
+        (b"0", 0, 0),
+        (b"2 -2147483648 2147483647\n", 2, 0),
+        (b"4\n1\n2\n3\n", 3, 1),
+        (b"5\n1\n2\n3\n", 4, 1),
+        (b"3\n1\n2\n", 3, 2),
+        (b"3\n1\n2abc\n3\n", 3, 2),
+        (b"3\n1\n0x10\n3\n", 3, 2),
+        (b"3\n1\n-\n3\n", 3, 2),
+        (b"3\n1\n2147483648\n3\n", 3, 2),
+        (b"3\n1 2-3\n", 3, 2),
+        (b"", 0, 2),
+        (b"   \n", 1, 2),
+        (b"x3\n1\n2\n3\n", 3, 2),
+        (b"3x\n1\n2\n3\n", 3, 2),
+        (b"3\n1\x002\n3\n", 3, 2),
+    ]
+    for k, (data, n, want) in enumerate(cases):
+        path = tmp_path / f"c{k}.grp"
+        path.write_bytes(data)
+        rc, hdr, got = read(path, n, nthreads)
+        assert rc == want, (data, rc)
+        ref = _fscanf_ref(data, n)
+        if want == 0:
+            assert ref is not None and hdr == n and got.tolist() == ref[1], data
+        elif want == 1:
+            assert hdr == int(data.split()[0]), data
+        else:
+            assert ref is None or len(data.split()) < n + 1, data
+
+
+def test_read_ints_missing_file(tio, tmp_path):
+    rc, _, _ = _reader(tio)(tmp_path / "none.grp", 3, 2)
+    assert rc == -1
