@@ -86,11 +86,38 @@ static int rdi(const unsigned char *p, int xdr)
 	return (int)u;
 }
 
-/* kd.c:122-222: gas (12 floats), dark (9), star (11) records after a 32-byte header. */
-static int read_tipsy(FILE *fp, int xdr, Snap *s)
+/* One record type's decode: the records in buf go to particles i0 + [lo, hi). */
+typedef struct {
+	Snap *s;
+	const unsigned char *buf;
+	int nrec, xdr, i0;
+} Decode;
+
+static void decode_range(void *arg, long lo, long hi)
+{
+	static const int nf[3] = {12, 9, 11};
+	const Decode *d = (const Decode *)arg;
+	Snap *s = d->s;
+	long j;
+	for (j = lo; j < hi; ++j) {
+		const unsigned char *r = d->buf + 4 * (size_t)nf[d->nrec] * j;
+		const long i = d->i0 + j;
+		int k;
+		s->mass[i] = rdf(r, d->xdr);
+		for (k = 0; k < 3; ++k) { s->pos[3*i+k] = rdf(r + 4 + 4*k, d->xdr); s->vel[3*i+k] = rdf(r + 16 + 4*k, d->xdr); }
+		if (d->nrec == 0) { s->temp[i] = rdf(r + 32, d->xdr); s->soft[i] = rdf(r + 36, d->xdr); }
+		else if (d->nrec == 1) { s->temp[i] = 0.0f; s->soft[i] = rdf(r + 28, d->xdr); }
+		else { s->temp[i] = 0.0f; s->soft[i] = rdf(r + 36, d->xdr); }
+	}
+}
+
+/* kd.c:122-222: gas (12 floats), dark (9), star (11) records after a 32-byte
+   header.  Each record type is read whole (by nIo threads when the input is a
+   regular file) and decoded on nIo threads. */
+static int read_tipsy(FILE *fp, int xdr, Snap *s, int nIo)
 {
 	unsigned char h[32];
-	int i, j, nrec;
+	int i, nrec;
 	static const int nf[3] = {12, 9, 11};
 	if (read_all(fp, h, 32)) return -1;
 	if (xdr) {
@@ -111,17 +138,12 @@ static int read_tipsy(FILE *fp, int xdr, Snap *s)
 		int cnt = nrec == 0 ? s->nsph : (nrec == 1 ? s->ndark : s->nstar);
 		size_t bytes = 4 * (size_t)nf[nrec] * cnt;
 		unsigned char *buf = malloc(bytes ? bytes : 1);
+		Decode d;
 		if (!buf) return -2;
-		if (read_all(fp, buf, bytes)) { free(buf); return -1; }
-		for (j = 0; j < cnt; ++j, ++i) {
-			const unsigned char *r = buf + 4 * (size_t)nf[nrec] * j;
-			int k;
-			s->mass[i] = rdf(r, xdr);
-			for (k = 0; k < 3; ++k) { s->pos[3*i+k] = rdf(r + 4 + 4*k, xdr); s->vel[3*i+k] = rdf(r + 16 + 4*k, xdr); }
-			if (nrec == 0) { s->temp[i] = rdf(r + 32, xdr); s->soft[i] = rdf(r + 36, xdr); }
-			else if (nrec == 1) { s->temp[i] = 0.0f; s->soft[i] = rdf(r + 28, xdr); }
-			else { s->temp[i] = 0.0f; s->soft[i] = rdf(r + 36, xdr); }
-		}
+		if (tio_fread(fp, buf, bytes, nIo)) { free(buf); return -1; }
+		d.s = s; d.buf = buf; d.nrec = nrec; d.xdr = xdr; d.i0 = i;
+		tio_parallel_for(cnt, nIo, decode_range, &d);
+		i += cnt;
 		free(buf);
 	}
 	return 0;
@@ -249,7 +271,7 @@ static int read_gtp(const char *path, int xdr, int nGroup, const Snap *S, const 
 
 int main(int argc, char **argv)
 {
-	const int nIo = tio_threads();                         /* threads formatting the text outputs */
+	const int nIo = tio_threads();                         /* threads for the text outputs and input decode */
 	skg_params P;
 	skg_ctx *ctx;
 	Snap S;
@@ -320,7 +342,7 @@ int main(int argc, char **argv)
 	P.accumulate = bPlain ? SKG_ACC_PLAIN : SKG_ACC_COMPENSATED;
 
 	memset(&S, 0, sizeof(S));
-	if (read_tipsy(stdin, bStandard, &S)) { fprintf(stderr, "skid: cannot read tipsy input\n"); return 1; }
+	if (read_tipsy(stdin, bStandard, &S, nIo)) { fprintf(stderr, "skid: cannot read tipsy input\n"); return 1; }
 	printf("nDark:%d nGas:%d nStar:%d\n", S.ndark, S.nsph, S.nstar);
 	fflush(stdout);
 	if (bEps) for (i = 0; i < S.n; ++i) S.soft[i] = fEps;  /* kdSetSoft (main.c:464) */

@@ -1,12 +1,15 @@
-/* Parallel, byte-identical line formatting for the per-particle text outputs. */
+/* Parallel, byte-identical line formatting for the per-particle text outputs,
+ * and the parallel readers of the .grp and tipsy inputs. */
 #include "textio.h"
 
+#include <errno.h>
 #include <math.h>
 #include <pthread.h>
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <sys/stat.h>
 #include <unistd.h>
 
 #define TIO_BLOCK (1L << 22)   /* lines per round (split over the threads) */
@@ -340,4 +343,85 @@ int tio_read_ints(const char *path, long nexp, int *v, long *header, int nthread
 	free(jobs);
 	free(b);
 	return rc;
+}
+
+/* ------------------------------------------------------------ binary input */
+typedef struct {
+	long lo, hi;
+	int fd;
+	off_t pos;
+	char *buf;
+	void (*fn)(void *, long, long);
+	void *arg;
+	int bad;
+} PJob;
+
+static void *pread_job(void *arg)
+{
+	PJob *j = (PJob *)arg;
+	long k = j->lo;
+	while (k < j->hi) {
+		const ssize_t r = pread(j->fd, j->buf + k, (size_t)(j->hi - k), j->pos + (off_t)k);
+		if (r < 0 && errno == EINTR) continue;
+		if (r <= 0) { j->bad = 1; break; }
+		k += r;
+	}
+	return NULL;
+}
+
+static void *for_job(void *arg)
+{
+	PJob *j = (PJob *)arg;
+	if (j->lo < j->hi) j->fn(j->arg, j->lo, j->hi);
+	return NULL;
+}
+
+static int run_shares(long n, int nthreads, void *(*body)(void *), PJob *proto)
+{
+	PJob *jobs;
+	pthread_t *th;
+	int bad = 0;
+	if (nthreads < 1) nthreads = 1;
+	jobs = calloc((size_t)nthreads, sizeof(PJob));
+	th = calloc((size_t)nthreads, sizeof(pthread_t));
+	if (!jobs || !th) { free(jobs); free(th); return -1; }
+	for (int t = 0; t < nthreads; ++t) {
+		jobs[t] = *proto;
+		jobs[t].lo = (long)((__int128)n * t / nthreads);
+		jobs[t].hi = (long)((__int128)n * (t + 1) / nthreads);
+		if (t && pthread_create(&th[t], NULL, body, &jobs[t])) { body(&jobs[t]); th[t] = 0; }
+	}
+	body(&jobs[0]);
+	for (int t = 1; t < nthreads; ++t)
+		if (th[t]) pthread_join(th[t], NULL);
+	for (int t = 0; t < nthreads; ++t) bad |= jobs[t].bad;
+	free(jobs);
+	free(th);
+	return bad ? -1 : 0;
+}
+
+int tio_fread(FILE *fp, void *buf, size_t n, int nthreads)
+{
+	struct stat st;
+	off_t pos;
+	PJob p;
+	const int fd = fileno(fp);
+	if (n < ((size_t)1 << 24) || nthreads < 2 || fd < 0 || fstat(fd, &st) || !S_ISREG(st.st_mode)
+			|| (pos = ftello(fp)) < 0 || pos + (off_t)n > st.st_size)
+		return fread(buf, 1, n, fp) == n ? 0 : -1;
+	memset(&p, 0, sizeof(p));
+	p.fd = fd;
+	p.pos = pos;
+	p.buf = (char *)buf;
+	if (run_shares((long)n, nthreads, pread_job, &p)) return -1;
+	return fseeko(fp, pos + (off_t)n, SEEK_SET) ? -1 : 0;
+}
+
+void tio_parallel_for(long n, int nthreads, void (*fn)(void *arg, long lo, long hi), void *arg)
+{
+	PJob p;
+	memset(&p, 0, sizeof(p));
+	p.fn = fn;
+	p.arg = arg;
+	if (n < 65536 || run_shares(n, nthreads, for_job, &p)) fn(arg, 0, n);
 }

@@ -5,6 +5,8 @@
 #ifndef SKID_TEXTIO_H
 #define SKID_TEXTIO_H
 
+#include <stdio.h>
+
 /* "<n>\n" then one "%d\n" line per value. */
 int tio_write_ints(const char *path, long n, const int *v, int nthreads);
 
@@ -23,6 +25,15 @@ int tio_write_floats(const char *path, long header, long n, const float *v, cons
  * case the caller's own fscanf loop gives that file's exact behaviour; -1 when the
  * file cannot be read.  Tokens after the nexp-th are not read. */
 int tio_read_ints(const char *path, long nexp, int *v, long *header, int nthreads);
+
+/* n bytes from fp's position into buf, leaving fp after them: a regular file
+ * of at least 16 MiB is read by nthreads threads (pread at their own offsets),
+ * anything else by one fread.  0, or -1 when fewer than n bytes could be read. */
+int tio_fread(FILE *fp, void *buf, size_t n, int nthreads);
+
+/* fn(arg, lo, hi) over [0, n) in nthreads contiguous shares, one thread each
+ * (one call on the calling thread when n < 65536 or threads cannot start). */
+void tio_parallel_for(long n, int nthreads, void (*fn)(void *arg, long lo, long hi), void *arg);
 
 /* Threads to use: SKID_IO_THREADS if set, else the online processors, at most 16. */
 int tio_threads(void);

@@ -165,3 +165,55 @@ def test_read_ints_forms(tio, tmp_path, nthreads):
 def test_read_ints_missing_file(tio, tmp_path):
     rc, _, _ = _reader(tio)(tmp_path / "none.grp", 3, 2)
     assert rc == -1
+
+
+def _libc():
+    libc = C.CDLL(None)
+    libc.fopen.restype = libc.popen.restype = C.c_void_p
+    libc.fopen.argtypes = libc.popen.argtypes = [C.c_char_p, C.c_char_p]
+    libc.fread.argtypes = [C.c_void_p, C.c_size_t, C.c_size_t, C.c_void_p]
+    libc.fgetc.argtypes = libc.fclose.argtypes = libc.pclose.argtypes = [C.c_void_p]
+    return libc
+
+
+@pytest.mark.parametrize("src,nthreads", [("file", 1), ("file", 7), ("pipe", 7)])
+def test_fread_parallel(tio, tmp_path, src, nthreads):
+    """The tipsy reader's bulk read (kd.c:122-222 reads the records after the
+    32-byte header): from a regular file by threads at their own offsets, from a
+    pipe by fread; the bytes equal the file's and the stream is left just after
+    them (the next byte is the one that follows), and a short file fails."""
+    libc = _libc()
+    tio.tio_fread.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    rng = np.random.default_rng(5)
+    head, n = 32, (20 << 20) + 12345
+    data = rng.integers(0, 256, head + n + 1, dtype=np.uint8)
+    path = tmp_path / "t.bin"
+    path.write_bytes(data.tobytes())
+    fp = libc.fopen(str(path).encode(), b"rb") if src == "file" else libc.popen(f"cat {path}".encode(), b"r")
+    hbuf = np.zeros(head, np.uint8)
+    assert libc.fread(hbuf.ctypes.data, 1, head, fp) == head
+    out = np.zeros(n, np.uint8)
+    assert tio.tio_fread(fp, out.ctypes.data, n, nthreads) == 0
+    assert np.array_equal(out, data[head:head + n])
+    assert libc.fgetc(fp) == int(data[-1])
+    (libc.fclose if src == "file" else libc.pclose)(fp)
+    fp = libc.fopen(str(path).encode(), b"rb")
+    assert libc.fread(hbuf.ctypes.data, 1, head, fp) == head
+    big = np.zeros(n + 2, np.uint8)
+    assert tio.tio_fread(fp, big.ctypes.data, n + 2, nthreads) == -1
+    libc.fclose(fp)
+
+
+def test_parallel_for_shares(tio):
+    """The decode's parallel loop covers [0, n) once, in contiguous shares."""
+    seen = []
+    CB = C.CFUNCTYPE(None, C.c_void_p, C.c_long, C.c_long)
+    cb = CB(lambda arg, lo, hi: seen.append((lo, hi)))
+    tio.tio_parallel_for.argtypes = [C.c_long, C.c_int, CB, C.c_void_p]
+    for n, t in [(1000003, 7), (65536, 16), (100, 8), (0, 4)]:
+        seen.clear()
+        tio.tio_parallel_for(n, t, cb, None)
+        s = sorted(seen)
+        assert (s[0][0] if s else 0) == 0 and (s[-1][1] if s else 0) == n
+        assert all(a[1] == b[0] for a, b in zip(s, s[1:]))
+        assert len(s) <= max(t, 1)
