@@ -54,100 +54,12 @@ static void die_skg(const char *what, int rc)
 }
 
 /* ---------------------------------------------------------------- tipsy I/O */
-typedef struct {
-	double time;
-	int nbodies, ndim, nsph, ndark, nstar;
-	int n;
-	float *pos, *vel, *mass, *soft, *temp;
-} Snap;
+typedef TioSnap Snap;
 
 static uint32_t bswap32(uint32_t x) { return (x >> 24) | ((x >> 8) & 0xff00u) | ((x << 8) & 0xff0000u) | (x << 24); }
 
-static int read_all(FILE *fp, void *buf, size_t n)
-{
-	return fread(buf, 1, n, fp) == n ? 0 : -1;
-}
-
-static float rdf(const unsigned char *p, int xdr)
-{
-	uint32_t u;
-	float f;
-	memcpy(&u, p, 4);
-	if (xdr) u = bswap32(u);
-	memcpy(&f, &u, 4);
-	return f;
-}
-
-static int rdi(const unsigned char *p, int xdr)
-{
-	uint32_t u;
-	memcpy(&u, p, 4);
-	if (xdr) u = bswap32(u);
-	return (int)u;
-}
-
-/* One record type's decode: the records in buf go to particles i0 + [lo, hi). */
-typedef struct {
-	Snap *s;
-	const unsigned char *buf;
-	int nrec, xdr, i0;
-} Decode;
-
-static void decode_range(void *arg, long lo, long hi)
-{
-	static const int nf[3] = {12, 9, 11};
-	const Decode *d = (const Decode *)arg;
-	Snap *s = d->s;
-	long j;
-	for (j = lo; j < hi; ++j) {
-		const unsigned char *r = d->buf + 4 * (size_t)nf[d->nrec] * j;
-		const long i = d->i0 + j;
-		int k;
-		s->mass[i] = rdf(r, d->xdr);
-		for (k = 0; k < 3; ++k) { s->pos[3*i+k] = rdf(r + 4 + 4*k, d->xdr); s->vel[3*i+k] = rdf(r + 16 + 4*k, d->xdr); }
-		if (d->nrec == 0) { s->temp[i] = rdf(r + 32, d->xdr); s->soft[i] = rdf(r + 36, d->xdr); }
-		else if (d->nrec == 1) { s->temp[i] = 0.0f; s->soft[i] = rdf(r + 28, d->xdr); }
-		else { s->temp[i] = 0.0f; s->soft[i] = rdf(r + 36, d->xdr); }
-	}
-}
-
-/* kd.c:122-222: gas (12 floats), dark (9), star (11) records after a 32-byte
-   header.  Each record type is read whole (by nIo threads when the input is a
-   regular file) and decoded on nIo threads. */
-static int read_tipsy(FILE *fp, int xdr, Snap *s, int nIo)
-{
-	unsigned char h[32];
-	int i, nrec;
-	static const int nf[3] = {12, 9, 11};
-	if (read_all(fp, h, 32)) return -1;
-	if (xdr) {
-		uint64_t u;
-		memcpy(&u, h, 8);
-		u = ((uint64_t)bswap32((uint32_t)u) << 32) | bswap32((uint32_t)(u >> 32));
-		memcpy(&s->time, &u, 8);
-	} else memcpy(&s->time, h, 8);
-	s->nbodies = rdi(h + 8, xdr); s->ndim = rdi(h + 12, xdr); s->nsph = rdi(h + 16, xdr);
-	s->ndark = rdi(h + 20, xdr); s->nstar = rdi(h + 24, xdr);
-	if (s->nsph < 0 || s->ndark < 0 || s->nstar < 0) return -1;
-	s->n = s->nsph + s->ndark + s->nstar;
-	s->pos = malloc(12 * (size_t)s->n); s->vel = malloc(12 * (size_t)s->n);
-	s->mass = malloc(4 * (size_t)s->n); s->soft = malloc(4 * (size_t)s->n); s->temp = malloc(4 * (size_t)s->n);
-	if (!s->pos || !s->vel || !s->mass || !s->soft || !s->temp) return -2;
-	i = 0;
-	for (nrec = 0; nrec < 3; ++nrec) {
-		int cnt = nrec == 0 ? s->nsph : (nrec == 1 ? s->ndark : s->nstar);
-		size_t bytes = 4 * (size_t)nf[nrec] * cnt;
-		unsigned char *buf = malloc(bytes ? bytes : 1);
-		Decode d;
-		if (!buf) return -2;
-		if (tio_fread(fp, buf, bytes, nIo)) { free(buf); return -1; }
-		d.s = s; d.buf = buf; d.nrec = nrec; d.xdr = xdr; d.i0 = i;
-		tio_parallel_for(cnt, nIo, decode_range, &d);
-		i += cnt;
-		free(buf);
-	}
-	return 0;
-}
+static float rdf(const unsigned char *p, int xdr) { return tio_get_f32(p, xdr); }
+static int rdi(const unsigned char *p, int xdr) { return tio_get_i32(p, xdr); }
 
 static void wrf(FILE *fp, float f, int xdr)
 {
@@ -342,7 +254,7 @@ int main(int argc, char **argv)
 	P.accumulate = bPlain ? SKG_ACC_PLAIN : SKG_ACC_COMPENSATED;
 
 	memset(&S, 0, sizeof(S));
-	if (read_tipsy(stdin, bStandard, &S, nIo)) { fprintf(stderr, "skid: cannot read tipsy input\n"); return 1; }
+	if (tio_read_tipsy(stdin, bStandard, &S, nIo)) { fprintf(stderr, "skid: cannot read tipsy input\n"); return 1; }
 	printf("nDark:%d nGas:%d nStar:%d\n", S.ndark, S.nsph, S.nstar);
 	fflush(stdout);
 	if (bEps) for (i = 0; i < S.n; ++i) S.soft[i] = fEps;  /* kdSetSoft (main.c:464) */

@@ -425,3 +425,95 @@ void tio_parallel_for(long n, int nthreads, void (*fn)(void *arg, long lo, long 
 	p.arg = arg;
 	if (n < 65536 || run_shares(n, nthreads, for_job, &p)) fn(arg, 0, n);
 }
+
+/* ------------------------------------------------------------ tipsy input */
+static inline float get_f32(const unsigned char *p, int xdr)
+{
+	uint32_t u;
+	float f;
+	memcpy(&u, p, 4);
+	if (xdr) u = __builtin_bswap32(u);
+	memcpy(&f, &u, 4);
+	return f;
+}
+
+static inline int get_i32(const unsigned char *p, int xdr)
+{
+	uint32_t u;
+	memcpy(&u, p, 4);
+	if (xdr) u = __builtin_bswap32(u);
+	return (int)u;
+}
+
+float tio_get_f32(const unsigned char *p, int xdr) { return get_f32(p, xdr); }
+int tio_get_i32(const unsigned char *p, int xdr) { return get_i32(p, xdr); }
+
+/* One record type's decode: the records in buf go to particles i0 + [lo, hi). */
+typedef struct {
+	TioSnap *s;
+	const unsigned char *buf;
+	int nrec, xdr;
+	long i0;
+} Decode;
+
+static const int tipsy_nf[3] = {12, 9, 11};   /* floats per gas, dark, star record */
+
+static void decode_range(void *arg, long lo, long hi)
+{
+	const Decode *d = (const Decode *)arg;
+	TioSnap *s = d->s;
+	const int x = d->xdr;
+	for (long j = lo; j < hi; ++j) {
+		const unsigned char *r = d->buf + 4 * (size_t)tipsy_nf[d->nrec] * j;
+		const long i = d->i0 + j;
+		s->mass[i] = get_f32(r, x);
+		for (int k = 0; k < 3; ++k) {
+			s->pos[3*i+k] = get_f32(r + 4 + 4*k, x);
+			s->vel[3*i+k] = get_f32(r + 16 + 4*k, x);
+		}
+		if (d->nrec == 0) { s->temp[i] = get_f32(r + 32, x); s->soft[i] = get_f32(r + 36, x); }
+		else if (d->nrec == 1) { s->temp[i] = 0.0f; s->soft[i] = get_f32(r + 28, x); }
+		else { s->temp[i] = 0.0f; s->soft[i] = get_f32(r + 36, x); }
+	}
+}
+
+void tio_free_tipsy(TioSnap *s)
+{
+	free(s->pos); free(s->vel); free(s->mass); free(s->soft); free(s->temp);
+	s->pos = s->vel = s->mass = s->soft = s->temp = NULL;
+}
+
+int tio_read_tipsy(FILE *fp, int xdr, TioSnap *s, int nthreads)
+{
+	unsigned char h[32];
+	long i = 0;
+	memset(s, 0, sizeof(*s));
+	if (fread(h, 1, 32, fp) != 32) return -1;
+	if (xdr) {
+		uint64_t u;
+		memcpy(&u, h, 8);
+		u = ((uint64_t)__builtin_bswap32((uint32_t)u) << 32) | __builtin_bswap32((uint32_t)(u >> 32));
+		memcpy(&s->time, &u, 8);
+	} else memcpy(&s->time, h, 8);
+	s->nbodies = get_i32(h + 8, xdr); s->ndim = get_i32(h + 12, xdr);
+	s->nsph = get_i32(h + 16, xdr); s->ndark = get_i32(h + 20, xdr); s->nstar = get_i32(h + 24, xdr);
+	if (s->nsph < 0 || s->ndark < 0 || s->nstar < 0 || (long)s->nsph + s->ndark + s->nstar > 2147483647L)
+		return -1;
+	s->n = s->nsph + s->ndark + s->nstar;
+	s->pos = malloc(12 * (size_t)s->n + 1); s->vel = malloc(12 * (size_t)s->n + 1);
+	s->mass = malloc(4 * (size_t)s->n + 1); s->soft = malloc(4 * (size_t)s->n + 1); s->temp = malloc(4 * (size_t)s->n + 1);
+	if (!s->pos || !s->vel || !s->mass || !s->soft || !s->temp) { tio_free_tipsy(s); return -2; }
+	for (int nrec = 0; nrec < 3; ++nrec) {
+		const long cnt = nrec == 0 ? s->nsph : (nrec == 1 ? s->ndark : s->nstar);
+		const size_t bytes = 4 * (size_t)tipsy_nf[nrec] * (size_t)cnt;
+		unsigned char *buf = malloc(bytes ? bytes : 1);
+		Decode d;
+		if (!buf) { tio_free_tipsy(s); return -2; }
+		if (tio_fread(fp, buf, bytes, nthreads)) { free(buf); tio_free_tipsy(s); return -1; }
+		d.s = s; d.buf = buf; d.nrec = nrec; d.xdr = xdr; d.i0 = i;
+		tio_parallel_for(cnt, nthreads, decode_range, &d);
+		i += cnt;
+		free(buf);
+	}
+	return 0;
+}

@@ -35,6 +35,29 @@ int tio_fread(FILE *fp, void *buf, size_t n, int nthreads);
  * (one call on the calling thread when n < 65536 or threads cannot start). */
 void tio_parallel_for(long n, int nthreads, void (*fn)(void *arg, long lo, long hi), void *arg);
 
+/* A tipsy snapshot's particles in input order (gas, then dark, then star). */
+typedef struct {
+	double time;
+	int nbodies, ndim, nsph, ndark, nstar;
+	int n;                                 /* nsph + ndark + nstar */
+	float *pos, *vel;                      /* 3 per particle */
+	float *mass, *soft, *temp;             /* temp: 0 for dark and star particles */
+} TioSnap;
+
+/* The tipsy reader of kd.c:122-222 (kdReadTipsy): a 32-byte header, then gas
+ * (12 floats: mass, pos, vel, rho, temp, hsmooth, metals, phi), dark (9: mass, pos,
+ * vel, eps, phi) and star (11: mass, pos, vel, metals, tform, eps, phi) records,
+ * native byte order or, with xdr, big-endian.  The records are read with
+ * tio_fread and decoded on nthreads threads; fp is left after the last record.
+ * 0; -1 when the header is bad or the file short; -2 out of memory (the arrays
+ * are then freed). */
+int tio_read_tipsy(FILE *fp, int xdr, TioSnap *s, int nthreads);
+void tio_free_tipsy(TioSnap *s);
+
+/* A 4-byte float / int of a tipsy file (byte-swapped with xdr). */
+float tio_get_f32(const unsigned char *p, int xdr);
+int tio_get_i32(const unsigned char *p, int xdr);
+
 /* Threads to use: SKID_IO_THREADS if set, else the online processors, at most 16. */
 int tio_threads(void);
 

@@ -217,3 +217,61 @@ def test_parallel_for_shares(tio):
         assert (s[0][0] if s else 0) == 0 and (s[-1][1] if s else 0) == n
         assert all(a[1] == b[0] for a, b in zip(s, s[1:]))
         assert len(s) <= max(t, 1)
+
+
+class _Snap(C.Structure):
+    _fields_ = [("time", C.c_double), ("nbodies", C.c_int), ("ndim", C.c_int), ("nsph", C.c_int),
+                ("ndark", C.c_int), ("nstar", C.c_int), ("n", C.c_int),
+                ("pos", C.POINTER(C.c_float)), ("vel", C.POINTER(C.c_float)), ("mass", C.POINTER(C.c_float)),
+                ("soft", C.POINTER(C.c_float)), ("temp", C.POINTER(C.c_float))]
+
+
+def _tipsy_bytes(rng, ns, nd, nst, xdr, t=0.375):
+    """A tipsy file (kd.c:122-222's layout) and the arrays its reader must give."""
+    e = ">" if xdr else "<"
+    head = np.array([(t, ns + nd + nst, 3, ns, nd, nst, 0)],
+                    dtype=[("t", e + "f8")] + [(f, e + "i4") for f in ("n", "d", "s", "k", "r", "p")]).tobytes()
+    recs = [rng.standard_normal((c, w)).astype(np.float32) for c, w in ((ns, 12), (nd, 9), (nst, 11))]
+    body = b"".join(r.astype(e + "f4").tobytes() for r in recs)
+    g, d, s = recs
+    want = {
+        "mass": np.concatenate([g[:, 0], d[:, 0], s[:, 0]]),
+        "pos": np.concatenate([g[:, 1:4], d[:, 1:4], s[:, 1:4]]),
+        "vel": np.concatenate([g[:, 4:7], d[:, 4:7], s[:, 4:7]]),
+        "soft": np.concatenate([g[:, 9], d[:, 7], s[:, 9]]),      # hsmooth, eps, eps
+        "temp": np.concatenate([g[:, 8], np.zeros(nd + nst, np.float32)]),
+    }
+    return head + body, want
+
+
+@pytest.mark.parametrize("xdr,src,sizes", [(0, "file", (3000, 500000, 2000)), (1, "file", (300000, 1, 200000)),
+                                           (1, "pipe", (1000, 2000, 3000)), (0, "file", (0, 7, 0))])
+def test_read_tipsy(tio, tmp_path, xdr, src, sizes):
+    """The driver's tipsy reader against the file's own fields: native and XDR
+    byte order, mixed species, files above and below the threaded-read size,
+    from a file and from a pipe, the stream left after the records; a short
+    file and a negative count are refused."""
+    libc = _libc()
+    tio.tio_read_tipsy.argtypes = [C.c_void_p, C.c_int, C.POINTER(_Snap), C.c_int]
+    tio.tio_free_tipsy.argtypes = [C.POINTER(_Snap)]
+    rng = np.random.default_rng(6 + xdr)
+    data, want = _tipsy_bytes(rng, *sizes, xdr)
+    path = tmp_path / "s.bin"
+    path.write_bytes(data + b"\x5a")
+    fp = libc.fopen(str(path).encode(), b"rb") if src == "file" else libc.popen(f"cat {path}".encode(), b"r")
+    s = _Snap()
+    assert tio.tio_read_tipsy(fp, xdr, C.byref(s), 5) == 0
+    n = sum(sizes)
+    assert (s.n, s.nsph, s.ndark, s.nstar, s.nbodies, s.ndim, s.time) == (n, *sizes, n, 3, 0.375)
+    for f, w in want.items():
+        got = np.ctypeslib.as_array(getattr(s, f), (n * (3 if f in ("pos", "vel") else 1),))
+        assert np.array_equal(got.view(np.uint32), np.ascontiguousarray(w).reshape(-1).view(np.uint32)), f
+    assert libc.fgetc(fp) == 0x5A
+    tio.tio_free_tipsy(C.byref(s))
+    (libc.fclose if src == "file" else libc.pclose)(fp)
+    for bad in (data[:-5], data[:8] + np.array([-1], ">i4" if xdr else "<i4").tobytes() * 6 + data[32:]):
+        path.write_bytes(bad)
+        fp = libc.fopen(str(path).encode(), b"rb")
+        assert tio.tio_read_tipsy(fp, xdr, C.byref(s), 5) == -1
+        assert not s.pos
+        libc.fclose(fp)
