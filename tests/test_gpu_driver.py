@@ -25,6 +25,14 @@ def read_grp(path):
     return g
 
 
+def read_vals(path):
+    """A .den / .ray file: the particle count, then one value per line."""
+    with open(path) as f:
+        n = int(f.readline())
+        v = np.loadtxt(f, dtype=np.float64)
+    return n, np.atleast_1d(v)
+
+
 def run(exe, args, stdin_path, cwd):
     with open(stdin_path, "rb") as f:
         r = subprocess.run([exe] + args, stdin=f, capture_output=True, text=True, timeout=300, cwd=cwd)
@@ -37,7 +45,7 @@ def test_driver_matches_reference_and_unbind_reentry(tmp_path):
     inp = tmp_path / "c1.std"
     gen.write_tipsy(str(inp), box, xdr=True)
     args = ["-tau", "0.005", "-s", "32", "-m", "8", "-p", "1", "-std", "-stats"]
-    out = run(SKID, args + ["-o", "gpu"], inp, tmp_path)
+    out = run(SKID, args + ["-den", "-ray", "-o", "gpu"], inp, tmp_path)
     assert "Number of Groups:" in out
     g = read_grp(tmp_path / "gpu.grp")
     assert os.path.getsize(tmp_path / "gpu.gtp") == 32 + 44 * (g.max())
@@ -45,7 +53,7 @@ def test_driver_matches_reference_and_unbind_reentry(tmp_path):
     assert st.shape == (g.max(), 21)
     assert np.array_equal(st[:, 1], np.bincount(g)[1:])
     if os.path.exists(REF):
-        run(REF, args + ["-o", "ref"], inp, tmp_path)
+        run(REF, args + ["-den", "-ray", "-o", "ref"], inp, tmp_path)
         rg = read_grp(tmp_path / "ref.grp")
         frac = O.match_fraction(g, rg)
         print(f"driver vs reference binary: groups {g.max()} vs {rg.max()}, match {frac:.5f}")
@@ -74,6 +82,32 @@ def test_driver_matches_reference_and_unbind_reentry(tmp_path):
         assert np.percentile(np.abs(x[:, 18:21] - y[:, 18:21]).max(axis=1), 99) <= 2e-6
         # what depends on the centre (radii, circular velocities, dispersion) in 99% of the groups
         assert np.percentile(rel[:, 5:12], 99, axis=0).max() <= 1e-3
+    if os.path.exists(REF):
+        # kdOutDensity (kd.c:1526-1547): every particle's density; here summed
+        # in FP64 (compensated), there in fp32 running sums
+        n, d = read_vals(tmp_path / "gpu.den")
+        rn, rd = read_vals(tmp_path / "ref.den")
+        assert n == rn == g.size and d.shape == rd.shape == (n,)
+        reld = np.abs(d - rd) / np.maximum(np.abs(rd), 1e-30)
+        print(f".den: rel p50 {np.median(reld):.2e} p99 {np.percentile(reld, 99):.2e} max {reld.max():.2e}")
+        assert reld.max() <= 2e-5
+        # kdOutVector (kd.c:1550-1606): each particle's displacement by the
+        # move loop and the micro-steps, all x then all y then all z, 0 for
+        # those that did not move
+        n, v = read_vals(tmp_path / "gpu.ray")
+        rn, rv = read_vals(tmp_path / "ref.ray")
+        assert n == rn == g.size and v.shape == rv.shape == (3 * n,)
+        v, rv = v.reshape(3, n), rv.reshape(3, n)
+        moved, rmoved = (v != 0).any(axis=0), (rv != 0).any(axis=0)
+        both = moved & rmoved
+        dv = np.abs(v - rv).max(axis=0)[both]
+        print(f".ray: moved {moved.sum()} vs {rmoved.sum()}, agree {(moved == rmoved).mean():.5f}; "
+              f"|d| p50 {np.median(dv):.2e} p99 {np.percentile(dv, 99):.2e} max {dv.max():.2e}; "
+              f"|disp| p50 {np.median(np.abs(rv).max(axis=0)[both]):.2e}")
+        assert (moved == rmoved).mean() >= 0.999
+        # the last steps of a trajectory differ at the rounding level between
+        # the two builds, as for the centres above: within 5% of tau in 99%
+        assert np.percentile(dv, 99) <= 0.05 * 0.005
     # -unbind re-entry on our own output: the groups are already bound, so
     # unbinding them again changes nothing
     out = run(SKID, ["-unbind", "gpu.grp", "-tau", "0.005", "-m", "8", "-p", "1", "-std", "-o", "gpu_u"], inp, tmp_path)
