@@ -459,6 +459,15 @@ template <typename T> static inline int dalloc(T **p, size_t count) {
 	return skg_dmalloc((void **)p, count * sizeof(T));
 }
 template <typename T> static inline void dfree(T **p) { skg_dfree((void **)p); }
+// An entry point's own device temporaries, freed on every return path.  Declare
+// it after the pointers it holds, so that it is destroyed before them; dfree
+// skips a pointer already freed (null).
+struct TmpFree {
+	void **p[8];
+	int n = 0;
+	template <typename T> void add(T **q) { if (n < 8) p[n++] = (void **)q; }
+	~TmpFree() { for (int i = n - 1; i >= 0; --i) skg_dfree(p[i]); }
+};
 // Stage temporaries: slot s of the context, grown to count items when needed
 // and kept for the next call.  Within one entry point every temporary takes
 // its own slot; entry points never overlap on a context.

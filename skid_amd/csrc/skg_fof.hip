@@ -306,13 +306,16 @@ extern "C" int skg_fof_points(skg_ctx *c, int n, const float *pos, float fTau, i
 	(void)hipSetDevice(c->dev);
 	float4 *p4 = nullptr;
 	int *g = nullptr, rc, ng = 0;
+	TmpFree tf;
+	tf.add(&p4); tf.add(&g);
 	if (dalloc(&p4, n) || dalloc(&g, n)) return SKG_E_NOMEM;
 	float4 *h = (float4 *)malloc(16 * (size_t)n);
 	if (!h) return SKG_E_NOMEM;
 	for (int i = 0; i < n; ++i) h[i] = make_float4(pos[3 * i], pos[3 * i + 1], pos[3 * i + 2], 0.f);
-	SKG_HIP(hipMemcpyAsync(p4, h, 16 * (size_t)n, hipMemcpyHostToDevice, c->st));
-	SKG_HIP(hipStreamSynchronize(c->st));
+	hipError_t e = hipMemcpyAsync(p4, h, 16 * (size_t)n, hipMemcpyHostToDevice, c->st);
+	if (e == hipSuccess) e = hipStreamSynchronize(c->st);
 	free(h);
+	SKG_HIP(e);
 	if ((rc = fof_run(c, p4, n, fTau, nullptr, g, &ng, nullptr))) return rc;
 	SKG_HIP(hipMemcpyAsync(grp, g, 4 * (size_t)n, hipMemcpyDeviceToHost, c->st));
 	SKG_HIP(hipStreamSynchronize(c->st));

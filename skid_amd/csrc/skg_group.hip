@@ -1856,6 +1856,8 @@ extern "C" int skg_too_small(skg_ctx *c) {
 	k_remap<<<nblk(n, 256), 256, 0, c->st>>>(c->d_grp, n, keep, off);
 	float *m2 = nullptr;
 	float4 *r2 = nullptr, *c2 = nullptr, *v2 = nullptr, *b2 = nullptr;
+	TmpFree tf;
+	tf.add(&m2); tf.add(&r2); tf.add(&c2); tf.add(&v2); tf.add(&b2);
 	const int nN = nKeep + 1;
 	if (dalloc(&m2, nN) || dalloc(&r2, nN) || dalloc(&c2, nN) || dalloc(&v2, nN) || dalloc(&b2, nN))
 		return SKG_E_NOMEM;
@@ -2056,6 +2058,8 @@ extern "C" int skg_group_stats(skg_ctx *c, float *stats) {
 	const float hx = 0.5f * L[0], hy = 0.5f * L[1], hz = 0.5f * L[2];
 	if (skg_ensure_keys(c, n)) return SKG_E_NOMEM;
 	float *dst = nullptr;
+	TmpFree tf;
+	tf.add(&dst);
 	if (dalloc(&dst, (size_t)nG * SKG_STAT_N)) return SKG_E_NOMEM;
 	SKG_HIP(hipMemsetAsync(dst, 0, 4 * (size_t)SKG_STAT_N, c->st));        // group 0's row
 	k_stat_keys<<<nblk(n, 256), 256, 0, c->st>>>(c->d_grp, n, c->d_pos, G->cen, hx, hy, hz, c->k64a, c->v32a);
@@ -2246,6 +2250,8 @@ extern "C" int skg_group_energies(skg_ctx *c, double *W, double *T) {
 		for (int t = hs[g]; t < he[g]; t += TT) tiles.push_back(make_int2(g, t));
 	int2 *dt = nullptr;
 	double *dW = nullptr, *dT = nullptr;
+	TmpFree tf;
+	tf.add(&dt); tf.add(&dW); tf.add(&dT);
 	if (dalloc(&dt, std::max<size_t>(1, tiles.size())) || dalloc(&dW, nG) || dalloc(&dT, nG)) return SKG_E_NOMEM;
 	if (!tiles.empty()) SKG_HIP(hipMemcpyAsync(dt, tiles.data(), 8 * tiles.size(), hipMemcpyHostToDevice, c->st));
 	SKG_HIP(hipMemsetAsync(dW, 0, 8 * (size_t)nG, c->st));
