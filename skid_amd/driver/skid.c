@@ -304,24 +304,12 @@ int main(int argc, char **argv)
 	t1 = now_s(); tMicro = t1 - t0;
 	for (i = 0; i < PRUNE_STEPS; ++i) printf("Microstep:%d nScatter:%d\n", i + 1, nMicro[i]);
 	if (bOutRay) {                                          /* kdOutVector (kd.c:1550-1606) */
-		int j;
 		float *d3;
 		r = malloc(12*(size_t)S.n);
 		d3 = malloc(12*(size_t)S.n);
 		if (!r || !d3) { fprintf(stderr, "skid: out of memory\n"); return 1; }
 		if ((rc = skg_get_moved(ctx, r))) die_skg("skg_get_moved", rc);
-		for (j = 0; j < 3; ++j) {                       /* all x, then all y, then all z */
-			float hj = 0.5f*P.fPeriod[j];
-			for (i = 0; i < S.n; ++i) {
-				float d = r[3*i+j];
-				if (!isnan(d)) {                        /* NaN: did not move, printed as 0 */
-					d = d - S.pos[3*i+j];
-					if (d > hj) d -= 2*hj;
-					if (d <= -hj) d += 2*hj;
-				}
-				d3[(size_t)j*S.n + i] = d;
-			}
-		}
+		tio_ray_vectors(S.n, r, S.pos, P.fPeriod, d3, nIo);   /* NaN: did not move, printed as 0 */
 		snprintf(achFile, sizeof(achFile), "%s.ray", achName);
 		if (tio_write_floats(achFile, S.n, 3L*S.n, d3, "%g", 1, nIo)) { perror(achFile); return 1; }
 		free(d3);

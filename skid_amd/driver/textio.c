@@ -517,3 +517,37 @@ int tio_read_tipsy(FILE *fp, int xdr, TioSnap *s, int nthreads)
 	}
 	return 0;
 }
+
+/* ----------------------------------------------------------- -ray vectors */
+typedef struct {
+	const float *moved, *pos;
+	float h[3];
+	long n;
+	float *d3;
+} Ray;
+
+static void ray_range(void *arg, long lo, long hi)
+{
+	const Ray *r = (const Ray *)arg;
+	for (int j = 0; j < 3; ++j) {
+		const float h = r->h[j];
+		float *out = r->d3 + (size_t)j * r->n;
+		for (long i = lo; i < hi; ++i) {
+			float d = r->moved[3*i+j];
+			if (!isnan(d)) {
+				d = d - r->pos[3*i+j];
+				if (d > h) d -= 2*h;
+				if (d <= -h) d += 2*h;
+			}
+			out[i] = d;
+		}
+	}
+}
+
+void tio_ray_vectors(long n, const float *moved, const float *pos, const float *period, float *d3, int nthreads)
+{
+	Ray r;
+	r.moved = moved; r.pos = pos; r.n = n; r.d3 = d3;
+	for (int j = 0; j < 3; ++j) r.h[j] = 0.5f * period[j];
+	tio_parallel_for(n, nthreads, ray_range, &r);
+}

@@ -58,6 +58,13 @@ void tio_free_tipsy(TioSnap *s);
 float tio_get_f32(const unsigned char *p, int xdr);
 int tio_get_i32(const unsigned char *p, int xdr);
 
+/* kdOutVector's values (kd.c:1550-1606): d3[j n + i] = moved[3i+j] - pos[3i+j]
+ * on axis j, taken once into (-period[j]/2, period[j]/2] (one period subtracted
+ * above half, one added at or below minus half), all x, then all y, then all z;
+ * a NaN in moved (a particle that did not move, printed as 0) stays NaN.  On
+ * nthreads threads. */
+void tio_ray_vectors(long n, const float *moved, const float *pos, const float *period, float *d3, int nthreads);
+
 /* Threads to use: SKID_IO_THREADS if set, else the online processors, at most 16. */
 int tio_threads(void);
 

@@ -275,3 +275,32 @@ def test_read_tipsy(tio, tmp_path, xdr, src, sizes):
         assert tio.tio_read_tipsy(fp, xdr, C.byref(s), 5) == -1
         assert not s.pos
         libc.fclose(fp)
+
+
+@pytest.mark.parametrize("nthreads", [1, 6])
+def test_ray_vectors(tio, nthreads):
+    """kdOutVector's displacements (kd.c:1550-1606) in float32: moved minus
+    initial position on each axis, one period off above half a period, one on
+    at or below minus half, all x then all y then all z; NaN (did not move)
+    stays NaN for the writer to print as 0.  Includes the two boundaries."""
+    tio.tio_ray_vectors.argtypes = [C.c_long, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
+    rng = np.random.default_rng(7)
+    n = 200003
+    per = np.array([1.0, 2.5, 0.75], np.float32)
+    h = np.float32(0.5) * per
+    pos = (rng.random((n, 3), dtype=np.float32) - np.float32(0.5)) * per
+    moved = pos + rng.uniform(-0.9, 0.9, (n, 3)).astype(np.float32) * per
+    moved[::7] = np.nan
+    moved[1, 0], pos[1, 0] = np.float32(0.25), np.float32(-0.25)          # d = +h exactly: kept
+    moved[2, 0], pos[2, 0] = np.float32(-0.25), np.float32(0.25)          # d = -h exactly: one period on
+    out = np.empty(3 * n, np.float32)
+    tio.tio_ray_vectors(n, moved.ctypes.data, pos.ctypes.data, per.ctypes.data, out.ctypes.data, nthreads)
+    want = np.empty((3, n), np.float32)
+    for j in range(3):
+        d = moved[:, j] - pos[:, j]
+        hj, p2 = h[j], np.float32(2) * h[j]
+        d = np.where(d > hj, d - p2, d).astype(np.float32)
+        d = np.where(d <= -hj, d + p2, d).astype(np.float32)
+        want[j] = d
+    assert want[0, 1] == h[0] and want[0, 2] == h[0]
+    assert np.array_equal(out.view(np.uint32), want.reshape(-1).view(np.uint32))
