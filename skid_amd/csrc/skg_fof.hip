@@ -174,11 +174,27 @@ __global__ void k_fill(int *a, int n, int v) {
 	if (i < n) a[i] = v;
 }
 
+// The root of x, reading only.  Every value read is an ancestor of the node it
+// was read from (parents only decrease; a root keeps itself), so the walk ends
+// at x's root.
+__device__ __forceinline__ int uf_root(const int *par, int x) {
+	for (;;) {
+		const int p = __hip_atomic_load(&par[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+		if (p == x) return x;
+		x = p;
+	}
+}
+
+// After the links: par[i] = root of i, and cmin[root] = the component's smallest
+// input index.  The walk here must not write: with uf_find's halving, another
+// thread that had read par[i] before thread i stored the root could store a
+// non-root ancestor over it afterwards, and k_fof_label would then index rank
+// with cmin of a non-root (INT_MAX).  Thread i is the only writer of par[i].
 __global__ void k_fof_root_min(int *par, int n, const int *__restrict__ orig, int *cmin) {
 	int i = blockIdx.x * blockDim.x + threadIdx.x;
 	if (i >= n) return;
-	int r = uf_find(par, i);
-	par[i] = r;
+	const int r = uf_root(par, i);
+	__hip_atomic_store(&par[i], r, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 	atomicMin(&cmin[r], orig[i]);
 }
 __global__ void k_fof_first(const int *__restrict__ par, int n, const int *__restrict__ cmin, int *isFirst) {
